@@ -1,0 +1,1087 @@
+/*
+ * clvisc_oracle.c -- CPU oracle for the CLVisc (3+1)D KT + Israel-Stewart hydro step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file restates, in plain C, the arithmetic of
+ * the reference OpenCL kernels and the RK2 sequencing of the reference Python
+ * drivers, so that the HIP product path can be checked against it.  Only the
+ * test suite, the smoke check and the CPU-baseline leg of bench.py may load it;
+ * the product path never calls, links or falls back to it.
+ *
+ * Reference files restated (paths relative to the reference repository):
+ *   pyvisc/kernel/helper.h        gamma, umu4, t0m, minmod, maxPropagationSpeed,
+ *                                 rootFinding_newton, kt1d, kt1d_real, etaos
+ *   pyvisc/kernel/eos_table.h     eos(), P, S, T, CS2 (uniform row/col table)
+ *   pyvisc/kernel/kernel_ideal.cl kt_src_christoffel, kt_src_along{x,y,z}, update_ev
+ *   pyvisc/kernel/kernel_visc.cl  kt_src_christoffel, kt1d_visc, kt_src_along*, update_ev
+ *   pyvisc/kernel/kernel_IS.cl    visc_initialize, visc_src_christoffel,
+ *                                 visc_src_along*, update_pimn, get_udiff
+ *   pyvisc/kernel/kernel_reduction.cl  reduction_stage1 (max of e)
+ *   pyvisc/ideal.py:211-255, 317-349   CLIdeal.stepUpdate / update_time / evolve
+ *   pyvisc/visc.py:143-149, 232-314, 352-357, 468-560  CLVisc stages and loop
+ *
+ * Each function keeps the reference's operation order (left-to-right, no FMA:
+ * build with -ffp-contract=off) so that FP32 rounding follows the reference as
+ * closely as a different compiler allows.  Compiled twice: ORC_DOUBLE=0 gives the
+ * FP32 oracle, ORC_DOUBLE=1 the FP64 one (same algorithm in double; acu=1e-14
+ * as in pyvisc/kernel/real_type.h:16).
+ *
+ * Extensions beyond the reference (each off unless its flag is set, and each
+ * reduces exactly to the reference path when off):
+ *   eos_kind = 1      log2-spaced (cs2, P, T, s) table instead of the row/col table
+ *   bulk_on           bulk viscous pressure Pi with zeta/s(T) (absent in the reference)
+ *   fp64_root         conserved-to-primitive solve in double where M > vmin*T00
+ *
+ * Layouts are the reference's AoS: ev[N][4] = (e, vx, vy, veta) with veta = tau*deta/dtau,
+ * pi[N][10] in idx() order (00,01,02,03,11,12,13,22,23,33) with eta components
+ * tilde-scaled, cell index I = (i*NY + j)*NZ + k.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef ORC_DOUBLE
+#define ORC_DOUBLE 0
+#endif
+
+#if ORC_DOUBLE
+typedef double real;
+#define ACU 1.0e-14
+#else
+typedef float real;
+#define ACU 1.0e-7f
+#endif
+#define RC(x) ((real)(x))
+
+/* Arithmetic contract shared with the HIP path.  The reference's OpenCL C allows
+ * single-precision division within 2.5 ulp (no -cl-fp32-correctly-rounded-divide-sqrt
+ * in pyvisc/ideal.py:146-171); every quotient a/b is evaluated here as a * RN(1/b),
+ * which lies within 1.5 ulp, with RN(1/b) the correctly rounded reciprocal.  sqrt is
+ * correctly rounded, nothing is contracted into FMA.  The HIP kernels evaluate the
+ * same operations in the same order, so their FP32 results match this oracle bit for
+ * bit on the division-free EOS table (rcp of a run constant is computed once). */
+static real rcp(real b) { return RC(1.0f) / b; }
+#define DIV(a, b) ((a) * rcp(b))
+
+/* THETA and hbarc are float literals in the reference (helper.h:7-8). */
+#define THETA RC(1.1f)
+#define HBARC RC(0.19733f)
+
+/* ------------------------------------------------------------------------- */
+/* Parameters shared with the Python wrapper (oracle/oracle.py).              */
+/* ------------------------------------------------------------------------- */
+typedef struct orc_params {
+    int32_t nx, ny, nz;
+    int32_t riemann_test;      /* -D RIEMANN_TEST */
+    int32_t gubser_visc_test;  /* -D GUBSER_VISC_TEST */
+    int32_t eos_kind;          /* 0: reference row/col table, 1: log2-spaced table */
+    int32_t eos_nrow, eos_ncol, eos_num_ed;   /* kind 0 */
+    int32_t eos_n;                            /* kind 1: number of points */
+    float dt, dx, dy, dz, tau0;
+    float etaos_xmin, etaos_ymin, etaos_left_slop, etaos_right_slop, lam1;
+    float eos_ed_start, eos_ed_step;          /* kind 0 */
+    float eos_log2_emin, eos_inv_dlog2;       /* kind 1 */
+    const float *eos_table;                   /* RGBA = (cs2, P, T, s) */
+    int32_t bulk_on;
+    float zetaos_max, zetaos_tpeak, zetaos_width;
+    int32_t fp64_root;
+    float fp64_root_vmin;
+    int32_t nthreads;          /* OpenMP threads, 0 = runtime default */
+} orc_params;
+
+static int64_t ncells(const orc_params *p) { return (int64_t)p->nx * p->ny * p->nz; }
+
+/* ------------------------------------------------------------------------- */
+/* EOS lookup                                                                 */
+/* ------------------------------------------------------------------------- */
+
+/* Reference row/col table lookup with nearest-texel clamp-to-edge reads and a
+ * manual lerp.  Follows pyvisc/kernel/eos_table.h:39-62. */
+static void eos_rowcol(const orc_params *p, real eps, real out[4]) {
+    const real step = RC(p->eos_ed_step);
+    const real start = RC(p->eos_ed_start);
+    const int ncol = p->eos_ncol, nrow = p->eos_nrow;
+    real ed_per_row = step * ncol;
+    int row = (int)(eps / ed_per_row);
+    int col = (int)((eps - start - row * ed_per_row) / step);
+    real eps_low = start + row * ed_per_row + col * step;
+    int c0 = col < 0 ? 0 : (col > ncol - 1 ? ncol - 1 : col);
+    int r0 = row < 0 ? 0 : (row > nrow - 1 ? nrow - 1 : row);
+    const float *lo = p->eos_table + 4 * ((int64_t)r0 * ncol + c0);
+    col = col + 1;
+    if (col == ncol) { row += 1; col = 0; }
+    int c1 = col < 0 ? 0 : (col > ncol - 1 ? ncol - 1 : col);
+    int r1 = row < 0 ? 0 : (row > nrow - 1 ? nrow - 1 : row);
+    const float *hi = p->eos_table + 4 * ((int64_t)r1 * ncol + c1);
+    real r = (eps - eps_low) / step;
+    for (int c = 0; c < 4; c++) out[c] = (RC(1.0f) - r) * RC(lo[c]) + r * RC(hi[c]);
+}
+
+/* Log2-spaced table (extension): point i sits at log2(e) = log2_emin + i/inv_dlog2.
+ * Between points the four columns are interpolated linearly in log2(e); below the
+ * first point (cs2 held) P, T, s fall linearly to zero at e = 0; above the last
+ * interval the last interval is extrapolated. */
+static void eos_log(const orc_params *p, real eps, real out[4]) {
+    const float *t = p->eos_table;
+    const int n = p->eos_n;
+    const real emin = (real)exp2((double)p->eos_log2_emin);
+    if (!(eps > emin)) {
+        real r = eps > RC(0.0f) ? eps / emin : RC(0.0f);
+        out[0] = RC(t[0]);
+        out[1] = r * RC(t[1]);
+        out[2] = r * RC(t[2]);
+        out[3] = r * RC(t[3]);
+        return;
+    }
+#if ORC_DOUBLE
+    real x = (log2(eps) - RC(p->eos_log2_emin)) * RC(p->eos_inv_dlog2);
+#else
+    real x = (log2f(eps) - p->eos_log2_emin) * p->eos_inv_dlog2;
+#endif
+    int i = (int)x;
+    if (i > n - 2) i = n - 2;
+    real r = x - (real)i;
+    const float *lo = t + 4 * i, *hi = lo + 4;
+    for (int c = 0; c < 4; c++) out[c] = (RC(1.0f) - r) * RC(lo[c]) + r * RC(hi[c]);
+}
+
+static void eos4(const orc_params *p, real eps, real out[4]) {
+    if (p->eos_kind == 1) eos_log(p, eps, out);
+    else eos_rowcol(p, eps, out);
+}
+
+static int in_table(const orc_params *p, real eps) {
+    if (p->eos_kind == 1) return 1;
+    return eps < RC(p->eos_ed_start) + p->eos_num_ed * RC(p->eos_ed_step);
+}
+
+/* P, S, T, CS2 with the analytic fit above the table (eos_table.h:65-100). */
+static real eos_P(const orc_params *p, real eps) {
+    if (in_table(p, eps)) { real o[4]; eos4(p, eps, o); return o[1]; }
+    return RC(0.3327f) * eps - RC(0.3223f) * (real)pow((double)eps, (double)RC(0.4585f))
+           - RC(0.003906f) * eps * (real)exp((double)(RC(-0.05697f) * eps));
+}
+static real eos_S(const orc_params *p, real eps) {
+    if (in_table(p, eps)) { real o[4]; eos4(p, eps, o); return o[3]; }
+    return RC(18.202f) * eps - RC(63.0218f)
+           - RC(4.85479f) * (real)exp((double)(RC(-2.72407E-11f) * (real)pow((double)eps, (double)RC(4.54886f))))
+           + RC(65.1272f) * (real)pow((double)eps, (double)(-RC(0.128012f)))
+                 * (real)exp((double)(RC(-0.00369624f) * (real)pow((double)eps, (double)RC(1.18735f))));
+}
+static real eos_T(const orc_params *p, real eps) {
+    if (in_table(p, eps)) { real o[4]; eos4(p, eps, o); return o[2]; }
+    return (eps + eos_P(p, eps)) / eos_S(p, eps);
+}
+static real eos_CS2(const orc_params *p, real eps) {
+    if (in_table(p, eps)) { real o[4]; eos4(p, eps, o); return o[0]; }
+    return RC(0.333333333f);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Pointwise helpers (helper.h)                                               */
+/* ------------------------------------------------------------------------- */
+static real r_max(real a, real b) { return a > b ? a : b; }
+static real r_min(real a, real b) { return b < a ? b : a; }   /* OpenCL min(): y if y < x */
+static real r_sign(real x) {
+    if (x > RC(0.0f)) return RC(1.0f);
+    if (x < RC(0.0f)) return RC(-1.0f);
+    return x;  /* keeps +-0 */
+}
+
+/* etaos(T) piecewise linear, helper.h:23-26 */
+static real etaos(const orc_params *p, real t) {
+    return (t < RC(p->etaos_xmin)) ? (RC(p->etaos_left_slop) * (t - RC(p->etaos_xmin)) + RC(p->etaos_ymin))
+                                   : (RC(p->etaos_right_slop) * (t - RC(p->etaos_xmin)) + RC(p->etaos_ymin));
+}
+
+/* zeta/s(T) for the bulk extension: Cauchy peak at zetaos_tpeak. */
+static real zetaos(const orc_params *p, real t) {
+    real d = (t - RC(p->zetaos_tpeak)) / RC(p->zetaos_width);
+    return RC(p->zetaos_max) / (RC(1.0f) + d * d);
+}
+
+static real gamma3(real vx, real vy, real vz) {
+    return RC(1.0f) / (real)sqrt((double)r_max(RC(1.0f) - vx * vx - vy * vy - vz * vz, ACU));
+}
+static real gamma_ev(const real *ev) { return gamma3(ev[1], ev[2], ev[3]); }
+
+static void umu4(const real *ev, real u[4]) {
+    real g = gamma_ev(ev);
+    u[0] = g * RC(1.0f); u[1] = g * ev[1]; u[2] = g * ev[2]; u[3] = g * ev[3];
+}
+
+static real minmod(real x, real y) {
+    real res = r_min((real)fabs((double)x), (real)fabs((double)y));
+    return res * (r_sign(x) + r_sign(y)) * RC(0.5f);
+}
+
+/* generalised minmod slope at b from (a, b, c) = cells (i-1, i, i+1), as written in
+ * kt1d / kt1d_visc / kt1d_real (helper.h:213-217, kernel_visc.cl:80-84). */
+static real slope(real a, real b, real c) {
+    return minmod(RC(0.5f) * (c - a), minmod(THETA * (c - b), THETA * (b - a)));
+}
+
+/* T^{tau mu} = (e+P) u^tau u^mu - P g^{tau mu}  (helper.h:172-177) */
+static void t0m(const real *ev, real pr, real out[4]) {
+    real u0 = gamma3(ev[1], ev[2], ev[3]);
+    real ed = ev[0];
+    real u4[4] = {RC(1.0f) * u0, ev[1] * u0, ev[2] * u0, ev[3] * u0};
+    real a = (ed + pr) * u0;
+    out[0] = a * u4[0] - RC(1.0f) * pr;
+    out[1] = a * u4[1] - RC(0.0f) * pr;
+    out[2] = a * u4[2] - RC(0.0f) * pr;
+    out[3] = a * u4[3] - RC(0.0f) * pr;
+}
+
+/* maxPropagationSpeed (helper.h:94-105): returns max(lambda, 1). */
+static real max_prop_speed(const orc_params *p, const real *edv, real vk) {
+    real ut = gamma3(edv[1], edv[2], edv[3]);
+    real uk = ut * vk;
+    real ut2 = ut * ut;
+    real uk2 = uk * uk;
+    real cs2 = eos_CS2(p, edv[0]);
+    real lam = ((real)fabs((double)(ut * uk * (RC(1.0f) - cs2)))
+                + (real)sqrt((double)((ut2 - uk2 - (ut2 - uk2 - RC(1.0f)) * cs2) * cs2)))
+               / (ut2 - (ut2 - RC(1.0f)) * cs2);
+    return r_max(lam, RC(1.0f));
+}
+
+/* rootFinding_newton (helper.h:128-167) with P_eff(e) = P(e) + bulk.  The
+ * optional seed replaces the midpoint start (surrogate-seeded variant). */
+static real root_newton(const orc_params *p, real T00, real M, real bulk, int *iters) {
+    real vl = RC(0.0f), vh = RC(1.0f);
+    real dpe = RC(1.0f) / RC(3.0f);
+    real v = RC(0.5f) * (vl + vh);
+    real ed = T00 - M * v;
+    real pr = eos_P(p, ed) + bulk;
+    real f = (T00 + pr) * v - M;
+    real df = (T00 + pr) - M * v * dpe;
+    real dvold = vh - vl;
+    real dv = dvold;
+    int i = 0;
+    while (1) {
+        if ((f + df * (vh - v)) * (f + df * (vl - v)) > RC(0.0f) ||
+            (real)fabs((double)(RC(2.f) * f)) > (real)fabs((double)(dvold * df))) {
+            dvold = dv;
+            dv = RC(0.5f) * (vh - vl);
+            v = vl + dv;
+        } else {
+            dvold = dv;
+            dv = f / df;
+            v -= dv;
+        }
+        i++;
+        if ((real)fabs((double)dv) < RC(0.00001f) || i > 100) break;
+        ed = T00 - M * v;
+        pr = eos_P(p, ed) + bulk;
+        f = (T00 + pr) * v - M;
+        df = (T00 + pr) - M * v * dpe;
+        if (f > RC(0.0f)) vh = v; else vl = v;
+    }
+    if (iters) *iters = i;
+    return T00 - M * v;
+}
+
+#if !ORC_DOUBLE
+/* FP64 variant of the same iteration for cells with M close to T00 (extension). */
+static double eos_P_d(const orc_params *p, double eps) {
+    /* table values are FP32; interpolation in double */
+    if (p->eos_kind == 1) {
+        const float *t = p->eos_table;
+        double emin = exp2((double)p->eos_log2_emin);
+        if (!(eps > emin)) { double r = eps > 0.0 ? eps / emin : 0.0; return r * t[1]; }
+        double x = (log2(eps) - p->eos_log2_emin) * p->eos_inv_dlog2;
+        int i = (int)x;
+        if (i > p->eos_n - 2) i = p->eos_n - 2;
+        double r = x - i;
+        return (1.0 - r) * t[4 * i + 1] + r * t[4 * i + 5];
+    }
+    if (eps < (double)p->eos_ed_start + p->eos_num_ed * (double)p->eos_ed_step) {
+        double step = p->eos_ed_step, start = p->eos_ed_start;
+        int ncol = p->eos_ncol, nrow = p->eos_nrow;
+        double epr = step * ncol;
+        int row = (int)(eps / epr);
+        int col = (int)((eps - start - row * epr) / step);
+        double elow = start + row * epr + col * step;
+        int c0 = col < 0 ? 0 : (col > ncol - 1 ? ncol - 1 : col);
+        int r0 = row < 0 ? 0 : (row > nrow - 1 ? nrow - 1 : row);
+        double lo = p->eos_table[4 * ((int64_t)r0 * ncol + c0) + 1];
+        col += 1;
+        if (col == ncol) { row += 1; col = 0; }
+        int c1 = col < 0 ? 0 : (col > ncol - 1 ? ncol - 1 : col);
+        int r1 = row < 0 ? 0 : (row > nrow - 1 ? nrow - 1 : row);
+        double hi = p->eos_table[4 * ((int64_t)r1 * ncol + c1) + 1];
+        double r = (eps - elow) / step;
+        return (1.0 - r) * lo + r * hi;
+    }
+    return 0.3327 * eps - 0.3223 * pow(eps, 0.4585) - 0.003906 * eps * exp(-0.05697 * eps);
+}
+
+static double root_newton_d(const orc_params *p, double T00, double M, double bulk, int *iters) {
+    double vl = 0.0, vh = 1.0, dpe = 1.0 / 3.0;
+    double v = 0.5;
+    double ed = T00 - M * v;
+    double pr = eos_P_d(p, ed) + bulk;
+    double f = (T00 + pr) * v - M;
+    double df = (T00 + pr) - M * v * dpe;
+    double dvold = vh - vl, dv = dvold;
+    int i = 0;
+    while (1) {
+        if ((f + df * (vh - v)) * (f + df * (vl - v)) > 0.0 || fabs(2.0 * f) > fabs(dvold * df)) {
+            dvold = dv; dv = 0.5 * (vh - vl); v = vl + dv;
+        } else {
+            dvold = dv; dv = f / df; v -= dv;
+        }
+        i++;
+        if (fabs(dv) < 0.00001 || i > 100) break;
+        ed = T00 - M * v;
+        pr = eos_P_d(p, ed) + bulk;
+        f = (T00 + pr) * v - M;
+        df = (T00 + pr) - M * v * dpe;
+        if (f > 0.0) vh = v; else vl = v;
+    }
+    if (iters) *iters = i;
+    return T00 - M * v;
+}
+#endif
+
+/* solve dispatch: FP64 for fast cells when the extension is on */
+static real solve_ed(const orc_params *p, real T00, real M, real bulk, int *iters) {
+#if !ORC_DOUBLE
+    if (p->fp64_root && M > RC(p->fp64_root_vmin) * T00)
+        return (real)root_newton_d(p, (double)T00, (double)M, (double)bulk, iters);
+#endif
+    return root_newton(p, T00, M, bulk, iters);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Small exported helpers for unit tests                                      */
+/* ------------------------------------------------------------------------- */
+void orc_eos(const orc_params *p, const real *eps, int64_t n, real *out /* n x 4 */) {
+    for (int64_t i = 0; i < n; i++) eos4(p, eps[i], out + 4 * i);
+}
+void orc_eos_funcs(const orc_params *p, const real *eps, int64_t n, real *out /* n x 4: P,S,T,CS2 */) {
+    for (int64_t i = 0; i < n; i++) {
+        out[4 * i + 0] = eos_P(p, eps[i]);
+        out[4 * i + 1] = eos_S(p, eps[i]);
+        out[4 * i + 2] = eos_T(p, eps[i]);
+        out[4 * i + 3] = eos_CS2(p, eps[i]);
+    }
+}
+void orc_minmod4(const real *a, const real *b, real *out) {
+    for (int c = 0; c < 4; c++) out[c] = minmod(a[c], b[c]);
+}
+void orc_root(const orc_params *p, const real *T00, const real *M, int64_t n, real *ed, int32_t *iters) {
+    for (int64_t i = 0; i < n; i++) {
+        int it = 0;
+        ed[i] = solve_ed(p, T00[i], M[i], RC(0.0f), &it);
+        if (iters) iters[i] = it;
+    }
+}
+void orc_max_prop_speed(const orc_params *p, const real *ev, const real *vk, int64_t n, real *out) {
+    for (int64_t i = 0; i < n; i++) out[i] = max_prop_speed(p, ev + 4 * i, vk[i]);
+}
+
+/* reduction_stage1 + host max (kernel_reduction.cl:7-39, ideal.py:257-262) */
+real orc_max_ed(const orc_params *p, const real *ev) {
+    int64_t n = ncells(p);
+    real m = RC(-3.402823466e+38f);
+    for (int64_t i = 0; i < n; i++) if (ev[4 * i] > m) m = ev[4 * i];
+    return m;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Pencil iteration                                                           */
+/* ------------------------------------------------------------------------- */
+/* Cells of the pencil along `axis` through (a, b) of the two other axes:
+ * returns the base index and stride so that cell t of the pencil is base + t*stride. */
+static void pencil(const orc_params *p, int axis, int64_t q, int64_t *base, int64_t *stride, int *len) {
+    int64_t NY = p->ny, NZ = p->nz;
+    if (axis == 0) {        /* q runs over (j,k) */
+        *base = q; *stride = NY * NZ; *len = p->nx;
+    } else if (axis == 1) { /* q runs over (i,k) */
+        int64_t i = q / NZ, k = q % NZ;
+        *base = i * NY * NZ + k; *stride = NZ; *len = p->ny;
+    } else {                /* q runs over (i,j) */
+        *base = q * NZ; *stride = 1; *len = p->nz;
+    }
+}
+static int64_t npencils(const orc_params *p, int axis) {
+    if (axis == 0) return (int64_t)p->ny * p->nz;
+    if (axis == 1) return (int64_t)p->nx * p->nz;
+    return (int64_t)p->nx * p->ny;
+}
+/* 2-ghost constant extrapolation at both ends of a pencil (kernel_ideal.cl:57-62) */
+static int64_t ghost(int64_t base, int64_t stride, int len, int t) {
+    if (t < 0) t = 0;
+    if (t > len - 1) t = len - 1;
+    return base + (int64_t)t * stride;
+}
+
+static void set_threads(const orc_params *p) {
+#ifdef _OPENMP
+    if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
+#else
+    (void)p;
+#endif
+}
+
+/* ------------------------------------------------------------------------- */
+/* Ideal hydro kernels (kernel_ideal.cl)                                      */
+/* ------------------------------------------------------------------------- */
+
+/* kt_src_christoffel, kernel_ideal.cl:3-36 */
+void orc_ideal_christoffel(const orc_params *p, real *Src, const real *ev, real tau, int step) {
+    int64_t n = ncells(p);
+    set_threads(p);
+    #pragma omp parallel for schedule(static)
+    for (int64_t I = 0; I < n; I++) {
+        real *s = Src + 4 * I;
+        if (step == 1) { s[0] = s[1] = s[2] = s[3] = RC(0.0f); }
+        const real *e = ev + 4 * I;
+        real ed = e[0], vx = e[1], vy = e[2], vz = e[3];
+        real pressure = eos_P(p, ed);
+        real u0 = gamma3(vx, vy, vz);
+        real Tzz = (ed + pressure) * u0 * u0 * vz * vz + pressure;
+        real Ttz = (ed + pressure) * u0 * u0 * vz;
+        if (!p->riemann_test) {
+            s[0] = s[0] - Tzz;
+            s[1] = s[1] - RC(0.0f);
+            s[2] = s[2] - RC(0.0f);
+            s[3] = s[3] - Ttz;
+        }
+    }
+}
+
+/* kt1d, helper.h:179-242: H(i+1/2) - H(i-1/2) for 5 stencil cells c[0..4] */
+static void kt1d(const orc_params *p, const real *c[5], int along, real out[4]) {
+    real pr_im1 = eos_P(p, c[1][0]), pr_i = eos_P(p, c[2][0]);
+    real pr_ip1 = eos_P(p, c[3][0]), pr_ip2 = eos_P(p, c[4][0]);
+    real Tm1[4], Ti[4], Tp1[4], Tp2[4], Tm2[4];
+    t0m(c[1], pr_im1, Tm1); t0m(c[2], pr_i, Ti); t0m(c[3], pr_ip1, Tp1); t0m(c[4], pr_ip2, Tp2);
+    real DA0[4], DA1[4], AL[4], AR[4], src[4];
+    for (int q = 0; q < 4; q++) {
+        DA0[q] = slope(Tm1[q], Ti[q], Tp1[q]);
+        DA1[q] = slope(Ti[q], Tp1[q], Tp2[q]);
+        AL[q] = Ti[q] + RC(0.5f) * DA0[q];
+        AR[q] = Tp1[q] - RC(0.5f) * DA1[q];
+    }
+    real pr_half = RC(0.5f) * (pr_ip1 + pr_i);
+    real vi_half = RC(0.5f) * (c[2][1 + along] + c[3][1 + along]);
+    real evh[4];
+    for (int q = 0; q < 4; q++) evh[q] = RC(0.5f) * (c[2][q] + c[3][q]);
+    real lam = max_prop_speed(p, evh, vi_half);
+    for (int q = 0; q < 4; q++) {
+        real g0 = q == 0 ? RC(1.0f) : RC(0.0f);
+        real ga = q == along + 1 ? RC(-1.0f) : RC(0.0f);
+        real Jp = (AR[q] + pr_half * g0) * vi_half - pr_half * ga;
+        real Jm = (AL[q] + pr_half * g0) * vi_half - pr_half * ga;
+        src[q] = RC(0.5f) * (Jp + Jm) - RC(0.5f) * lam * (AR[q] - AL[q]);
+    }
+    real pr_im2 = eos_P(p, c[0][0]);
+    t0m(c[0], pr_im2, Tm2);
+    for (int q = 0; q < 4; q++) {
+        DA1[q] = DA0[q];
+        DA0[q] = slope(Tm2[q], Tm1[q], Ti[q]);
+        AL[q] = Tm1[q] + RC(0.5f) * DA0[q];
+        AR[q] = Ti[q] - RC(0.5f) * DA1[q];
+    }
+    pr_half = RC(0.5f) * (pr_im1 + pr_i);
+    vi_half = RC(0.5f) * (c[1][1 + along] + c[2][1 + along]);
+    for (int q = 0; q < 4; q++) evh[q] = RC(0.5f) * (c[2][q] + c[1][q]);
+    lam = max_prop_speed(p, evh, vi_half);
+    for (int q = 0; q < 4; q++) {
+        real g0 = q == 0 ? RC(1.0f) : RC(0.0f);
+        real ga = q == along + 1 ? RC(-1.0f) : RC(0.0f);
+        real Jp = (AR[q] + pr_half * g0) * vi_half - pr_half * ga;
+        real Jm = (AL[q] + pr_half * g0) * vi_half - pr_half * ga;
+        src[q] -= RC(0.5f) * (Jp + Jm) - RC(0.5f) * lam * (AR[q] - AL[q]);
+        out[q] = src[q];
+    }
+}
+
+/* kt_src_alongx/y/z, kernel_ideal.cl:39-160 */
+void orc_ideal_along(const orc_params *p, int axis, real *Src, const real *ev, real tau) {
+    int64_t np_ = npencils(p, axis);
+    const real DX = RC(axis == 0 ? p->dx : (axis == 1 ? p->dy : p->dz));
+    set_threads(p);
+    #pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < np_; q++) {
+        int64_t base, stride; int len;
+        pencil(p, axis, q, &base, &stride, &len);
+        for (int t = 0; t < len; t++) {
+            const real *c[5];
+            for (int s = 0; s < 5; s++) c[s] = ev + 4 * ghost(base, stride, len, t + s - 2);
+            real k[4];
+            kt1d(p, c, axis, k);
+            real *S = Src + 4 * (base + (int64_t)t * stride);
+            for (int m = 0; m < 4; m++) {
+                if (p->riemann_test) S[m] = S[m] - k[m] / DX;
+                else if (axis < 2) S[m] = S[m] - tau * k[m] / DX;
+                else S[m] = S[m] - tau * k[m] / (tau * DX);
+            }
+        }
+    }
+}
+
+/* update_ev, kernel_ideal.cl:165-231 */
+void orc_ideal_update_ev(const orc_params *p, real *evnew, const real *ev1, const real *Src,
+                         real tau, int step, int32_t *iters) {
+    int64_t n = ncells(p);
+    const real DT = RC(p->dt);
+    set_threads(p);
+    #pragma omp parallel for schedule(static)
+    for (int64_t I = 0; I < n; I++) {
+        const real *e = ev1 + 4 * I;
+        real ed = e[0], vx = e[1], vy = e[2], vz = e[3];
+        real pressure = eos_P(p, ed);
+        real u0 = gamma3(vx, vy, vz);
+        real umu[4] = {u0 * RC(1.0f), u0 * vx, u0 * vy, u0 * vz};
+        real old_time = tau - (step - 1) * DT;
+        real T0m[4];
+        for (int q = 0; q < 4; q++) {
+            real g0 = q == 0 ? RC(1.0f) : RC(0.0f);
+            real t = (ed + pressure) * u0 * umu[q] - pressure * g0;
+            T0m[q] = p->riemann_test ? t : t * old_time;
+        }
+        for (int q = 0; q < 4; q++) T0m[q] = T0m[q] + Src[4 * I + q] * DT / step;
+        real T00, T01, T02, T03;
+        if (p->riemann_test) {
+            T00 = r_max(ACU, T0m[0]);
+            T01 = ((real)fabs((double)T0m[1]) < ACU) ? RC(0.0f) : T0m[1];
+            T02 = ((real)fabs((double)T0m[2]) < ACU) ? RC(0.0f) : T0m[2];
+            T03 = ((real)fabs((double)T0m[3]) < ACU) ? RC(0.0f) : T0m[3];
+        } else {
+            T00 = r_max(ACU, T0m[0]) / tau;
+            T01 = ((real)fabs((double)T0m[1]) < ACU) ? RC(0.0f) : T0m[1] / tau;
+            T02 = ((real)fabs((double)T0m[2]) < ACU) ? RC(0.0f) : T0m[2] / tau;
+            T03 = ((real)fabs((double)T0m[3]) < ACU) ? RC(0.0f) : T0m[3] / tau;
+        }
+        real M = (real)sqrt((double)(T01 * T01 + T02 * T02 + T03 * T03));
+        real SC = RC(0.999f);
+        if (M > T00) {
+            T01 *= SC * T00 / M;
+            T02 *= SC * T00 / M;
+            T03 *= SC * T00 / M;
+            M = SC * T00;
+        }
+        int it = 0;
+        real ed_find = solve_ed(p, T00, M, RC(0.0f), &it);
+        if (iters) iters[I] = it;
+        ed_find = r_max(RC(0.0f), ed_find);
+        real pr = eos_P(p, ed_find);
+        real epv = r_max(ACU, T00 + pr);
+        real *o = evnew + 4 * I;
+        o[0] = ed_find; o[1] = T01 / epv; o[2] = T02 / epv; o[3] = T03 / epv;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Viscous KT kernels (kernel_visc.cl), with the optional bulk Pi extension   */
+/* ------------------------------------------------------------------------- */
+static real bulk_at(const real *bulk, int64_t I) { return bulk ? bulk[I] : RC(0.0f); }
+
+/* kt_src_christoffel (visc), kernel_visc.cl:11-53 */
+void orc_visc_christoffel(const orc_params *p, real *Src, const real *ev, const real *pi,
+                          const real *bulk, real tau, int step) {
+    int64_t n = ncells(p);
+    set_threads(p);
+    (void)tau;
+    #pragma omp parallel for schedule(static)
+    for (int64_t I = 0; I < n; I++) {
+        real *s = Src + 4 * I;
+        if (step == 1) { s[0] = s[1] = s[2] = s[3] = RC(0.0f); }
+        const real *e = ev + 4 * I;
+        real ed = e[0], vx = e[1], vy = e[2], vz = e[3];
+        real u0 = gamma3(vx, vy, vz);
+        real bulkpi = bulk_at(bulk, I);
+        real pressure = eos_P(p, ed) + bulkpi;
+        real Tzz = (ed + pressure) * u0 * u0 * vz * vz + pressure + pi[10 * I + 9];
+        real Ttz = (ed + pressure) * u0 * u0 * vz + pi[10 * I + 3];
+        s[0] = s[0] - Tzz;
+        s[1] = s[1] - RC(0.0f);
+        s[2] = s[2] - RC(0.0f);
+        s[3] = s[3] - Ttz;
+        if (p->riemann_test) {
+            real Ttt = (ed + pressure) * u0 * u0 - pressure;
+            real Ttx = (ed + pressure) * u0 * u0 * vx;
+            real Tty = (ed + pressure) * u0 * u0 * vy;
+            s[0] = s[0] + (Tzz + Ttt);
+            s[1] = s[1] + Ttx;
+            s[2] = s[2] + Tty;
+            s[3] = s[3] + RC(3.0f) * Ttz;
+        }
+    }
+}
+
+/* row mu of pi for the flux along `along`: pi^{a+1, mu} (kernel_visc.cl:151-154) */
+static const int PI_ROW[4][4] = {
+    {0, 1, 2, 3},   /* pi^{0 mu} */
+    {1, 4, 5, 6},   /* pi^{1 mu} */
+    {2, 5, 7, 8},   /* pi^{2 mu} */
+    {3, 6, 8, 9},   /* pi^{3 mu} */
+};
+
+/* kt1d_visc, kernel_visc.cl:60-127 */
+static void kt1d_visc(const orc_params *p, const real *ev[5], const real *pi[5], const real *bk[5],
+                      real vip_half, real vim_half, real tau, int along, real out[4]) {
+    real pr[5], Q[5][4], pim0[5][4], pimi[5][4];
+    for (int i = 0; i < 5; i++) {
+        pr[i] = eos_P(p, ev[i][0]) + (bk[i] ? *bk[i] : RC(0.0f));
+        real T[4];
+        t0m(ev[i], pr[i], T);
+        for (int q = 0; q < 4; q++) {
+            pim0[i][q] = pi[i][PI_ROW[0][q]];
+            pimi[i][q] = pi[i][PI_ROW[along + 1][q]];
+            Q[i][q] = tau * (T[q] + pim0[i][q]);
+        }
+    }
+    real DA0[4], DA1[4], AL[4], AR[4], src[4];
+    for (int q = 0; q < 4; q++) {
+        DA0[q] = slope(Q[1][q], Q[2][q], Q[3][q]);
+        DA1[q] = slope(Q[2][q], Q[3][q], Q[4][q]);
+        AL[q] = Q[2][q] + RC(0.5f) * DA0[q];
+        AR[q] = Q[3][q] - RC(0.5f) * DA1[q];
+    }
+    real pr_half = RC(0.5f) * (pr[2] + pr[3]);
+    real evh[4];
+    for (int q = 0; q < 4; q++) evh[q] = RC(0.5f) * (ev[2][q] + ev[3][q]);
+    real lam = max_prop_speed(p, evh, vip_half);
+    for (int q = 0; q < 4; q++) {
+        real g0 = q == 0 ? RC(1.0f) : RC(0.0f);
+        real ga = q == along + 1 ? RC(-1.0f) : RC(0.0f);
+        real pim0_half = RC(0.5f) * (pim0[2][q] + pim0[3][q]);
+        real pimi_half = RC(0.5f) * (pimi[2][q] + pimi[3][q]);
+        real Jp = (AR[q] + pr_half * tau * g0 - tau * pim0_half) * vip_half - pr_half * tau * ga + tau * pimi_half;
+        real Jm = (AL[q] + pr_half * tau * g0 - tau * pim0_half) * vip_half - pr_half * tau * ga + tau * pimi_half;
+        src[q] = RC(0.5f) * (Jp + Jm) - RC(0.5f) * lam * (AR[q] - AL[q]);
+    }
+    for (int q = 0; q < 4; q++) {
+        DA1[q] = DA0[q];
+        DA0[q] = slope(Q[0][q], Q[1][q], Q[2][q]);
+        AL[q] = Q[1][q] + RC(0.5f) * DA0[q];
+        AR[q] = Q[2][q] - RC(0.5f) * DA1[q];
+    }
+    pr_half = RC(0.5f) * (pr[1] + pr[2]);
+    for (int q = 0; q < 4; q++) evh[q] = RC(0.5f) * (ev[2][q] + ev[1][q]);
+    lam = max_prop_speed(p, evh, vim_half);
+    for (int q = 0; q < 4; q++) {
+        real g0 = q == 0 ? RC(1.0f) : RC(0.0f);
+        real ga = q == along + 1 ? RC(-1.0f) : RC(0.0f);
+        real pim0_half = RC(0.5f) * (pim0[1][q] + pim0[2][q]);
+        real pimi_half = RC(0.5f) * (pimi[1][q] + pimi[2][q]);
+        real Jp = (AR[q] + pr_half * tau * g0 - tau * pim0_half) * vim_half - pr_half * tau * ga + tau * pimi_half;
+        real Jm = (AL[q] + pr_half * tau * g0 - tau * pim0_half) * vim_half - pr_half * tau * ga + tau * pimi_half;
+        src[q] -= RC(0.5f) * (Jp + Jm) - RC(0.5f) * lam * (AR[q] - AL[q]);
+        out[q] = src[q];
+    }
+}
+
+/* kt_src_alongx/y/z (visc), kernel_visc.cl:131-318 */
+void orc_visc_along(const orc_params *p, int axis, real *Src, const real *ev, const real *pi,
+                    const real *bulk, real tau) {
+    int64_t np_ = npencils(p, axis);
+    const real DX = RC(axis == 0 ? p->dx : (axis == 1 ? p->dy : p->dz));
+    set_threads(p);
+    #pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < np_; q++) {
+        int64_t base, stride; int len;
+        pencil(p, axis, q, &base, &stride, &len);
+        for (int t = 0; t < len; t++) {
+            const real *c[5], *pc[5], *bc[5];
+            for (int s = 0; s < 5; s++) {
+                int64_t J = ghost(base, stride, len, t + s - 2);
+                c[s] = ev + 4 * J; pc[s] = pi + 10 * J; bc[s] = bulk ? bulk + J : NULL;
+            }
+            real vip_half = RC(0.5f) * (c[2][1 + axis] + c[3][1 + axis]);
+            real vim_half = RC(0.5f) * (c[1][1 + axis] + c[2][1 + axis]);
+            real k[4];
+            kt1d_visc(p, c, pc, bc, vip_half, vim_half, tau, axis, k);
+            real *S = Src + 4 * (base + (int64_t)t * stride);
+            for (int m = 0; m < 4; m++) {
+                if (axis < 2) S[m] = S[m] - k[m] / DX;
+                else S[m] = S[m] - k[m] / (tau * DX);
+            }
+        }
+    }
+}
+
+/* update_ev (visc), kernel_visc.cl:322-393; bulk1/bulk2 are Pi at the start point
+ * and the freshly updated Pi (NULL without the bulk extension). */
+void orc_visc_update_ev(const orc_params *p, real *evnew, const real *ev1, const real *pi1,
+                        const real *pi2, const real *bulk1, const real *bulk2, const real *Src,
+                        real tau, int step, int32_t *iters) {
+    int64_t n = ncells(p);
+    const real DT = RC(p->dt);
+    set_threads(p);
+    #pragma omp parallel for schedule(static)
+    for (int64_t I = 0; I < n; I++) {
+        const real *e = ev1 + 4 * I;
+        real ed = e[0], vx = e[1], vy = e[2], vz = e[3];
+        real pressure = eos_P(p, ed) + bulk_at(bulk1, I);
+        real u0 = gamma3(vx, vy, vz);
+        real umu[4] = {u0 * RC(1.0f), u0 * vx, u0 * vy, u0 * vz};
+        real old_time = tau - (step - 1) * DT;
+        real T0m[4];
+        for (int q = 0; q < 4; q++) {
+            real g0 = q == 0 ? RC(1.0f) : RC(0.0f);
+            T0m[q] = ((ed + pressure) * u0 * umu[q] - pressure * g0 + pi1[10 * I + q]) * old_time;
+        }
+        for (int q = 0; q < 4; q++)
+            T0m[q] = T0m[q] + Src[4 * I + q] * DT / step - pi2[10 * I + q] * tau;
+        real T00 = r_max(ACU, T0m[0]) / tau;
+        real T01 = ((real)fabs((double)T0m[1]) < ACU) ? RC(0.0f) : T0m[1] / tau;
+        real T02 = ((real)fabs((double)T0m[2]) < ACU) ? RC(0.0f) : T0m[2] / tau;
+        real T03 = ((real)fabs((double)T0m[3]) < ACU) ? RC(0.0f) : T0m[3] / tau;
+        real M = (real)sqrt((double)(T01 * T01 + T02 * T02 + T03 * T03));
+        real SC = RC(0.999f);
+        if (M > T00) {
+            T01 *= SC * T00 / M;
+            T02 *= SC * T00 / M;
+            T03 *= SC * T00 / M;
+            M = SC * T00;
+        }
+        real bnew = bulk_at(bulk2, I);
+        int it = 0;
+        real ed_find = solve_ed(p, T00, M, bnew, &it);
+        if (iters) iters[I] = it;
+        if (ed_find < ACU) { T01 = RC(0.0f); T02 = RC(0.0f); T03 = RC(0.0f); }
+        ed_find = r_max(RC(0.0f), ed_find);
+        real pr = eos_P(p, ed_find) + bnew;
+        real epv = r_max(ACU, T00 + pr);
+        real *o = evnew + 4 * I;
+        o[0] = ed_find; o[1] = T01 / epv; o[2] = T02 / epv; o[3] = T03 / epv;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Israel-Stewart kernels (kernel_IS.cl)                                      */
+/* ------------------------------------------------------------------------- */
+static const real GMN_DIAG[4] = {RC(1.0f), RC(-1.0f), RC(-1.0f), RC(-1.0f)};
+static int IDX(int i, int j) {
+    return (i < j) ? ((7 * i + 2 * j - i * i) / 2) : ((7 * j + 2 * i - j * j) / 2);
+}
+
+/* visc_initialize, kernel_IS.cl:10-36 */
+void orc_visc_initialize(const orc_params *p, real *pi1, real *goodcell, real *udiff,
+                         const real *ev, real tau) {
+    int64_t n = ncells(p);
+    for (int64_t I = 0; I < n; I++) {
+        real cpTs[4];
+        eos4(p, ev[4 * I], cpTs);
+        real etav = etaos(p, cpTs[2]) * cpTs[3] * HBARC;
+        real tmp = RC(2.0f) / RC(3.0f) * etav / tau;
+        pi1[10 * I + 4] = tmp;
+        pi1[10 * I + 7] = tmp;
+        pi1[10 * I + 9] = RC(-2.0f) * tmp;
+        udiff[4 * I + 0] = udiff[4 * I + 1] = udiff[4 * I + 2] = udiff[4 * I + 3] = RC(0.0f);
+        goodcell[I] = RC(1.0f);
+    }
+}
+
+/* visc_src_christoffel, kernel_IS.cl:41-67 */
+void orc_is_christoffel(const orc_params *p, real *Src, const real *pi1, const real *ev,
+                        real tau, int step) {
+    int64_t n = ncells(p);
+    set_threads(p);
+    #pragma omp parallel for schedule(static)
+    for (int64_t I = 0; I < n; I++) {
+        real *s = Src + 10 * I;
+        const real *q = pi1 + 10 * I;
+        if (step == 1) for (int m = 0; m < 10; m++) s[m] = RC(0.0f);
+        const real *e = ev + 4 * I;
+        real uz = e[3] * gamma3(e[1], e[2], e[3]);
+        s[0] -= RC(2.0f) * uz * q[3] / tau;
+        s[1] -= uz * q[6] / tau;
+        s[2] -= uz * q[8] / tau;
+        s[3] -= uz * (q[0] + q[9]) / tau;
+        s[6] -= uz * q[1] / tau;
+        s[8] -= uz * q[2] / tau;
+        s[9] -= RC(2.0f) * uz * q[3] / tau;
+    }
+}
+
+/* kt1d_real, helper.h:249-285 */
+static real kt1d_real(real Q_im2, real Q_im1, real Q_i, real Q_ip1, real Q_ip2,
+                      real v_mh, real v_ph, real lam_mh, real lam_ph) {
+    real DA0 = slope(Q_im1, Q_i, Q_ip1);
+    real DA1 = slope(Q_i, Q_ip1, Q_ip2);
+    real AL = Q_i + RC(0.5f) * DA0;
+    real AR = Q_ip1 - RC(0.5f) * DA1;
+    real Jp = AR * v_ph, Jm = AL * v_ph;
+    real src = RC(0.5f) * (Jp + Jm) - RC(0.5f) * lam_ph * (AR - AL);
+    DA1 = DA0;
+    DA0 = slope(Q_im2, Q_im1, Q_i);
+    AL = Q_im1 + RC(0.5f) * DA0;
+    AR = Q_i - RC(0.5f) * DA1;
+    Jp = AR * v_mh; Jm = AL * v_mh;
+    src -= RC(0.5f) * (Jp + Jm) - RC(0.5f) * lam_mh * (AR - AL);
+    return src;
+}
+
+/* visc_src_alongx/y/z, kernel_IS.cl:74-318.  ud receives the limited gradient of u^mu
+ * along the axis (plus the Christoffel term for eta); bulk (optional) is evolved as an
+ * 11th component with the same flux. */
+void orc_is_along(const orc_params *p, int axis, real *Src, real *ud, const real *pi1,
+                  const real *ev, real *bulk_src, const real *bulk, real tau) {
+    int64_t np_ = npencils(p, axis);
+    const real DX = RC(axis == 0 ? p->dx : (axis == 1 ? p->dy : p->dz));
+    set_threads(p);
+    #pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < np_; q++) {
+        int64_t base, stride; int len;
+        pencil(p, axis, q, &base, &stride, &len);
+        for (int t = 0; t < len; t++) {
+            int64_t J[5];
+            for (int s = 0; s < 5; s++) J[s] = ghost(base, stride, len, t + s - 2);
+            const real *e[5];
+            real u0[5];
+            for (int s = 0; s < 5; s++) { e[s] = ev + 4 * J[s]; u0[s] = gamma_ev(e[s]); }
+            real v_mh = RC(0.5f) * (e[1][1 + axis] + e[2][1 + axis]);
+            real v_ph = RC(0.5f) * (e[3][1 + axis] + e[2][1 + axis]);
+            real evm[4], evp[4];
+            for (int c = 0; c < 4; c++) {
+                evm[c] = RC(0.5f) * (e[1][c] + e[2][c]);
+                evp[c] = RC(0.5f) * (e[3][c] + e[2][c]);
+            }
+            real lam_mh = max_prop_speed(p, evm, v_mh);
+            real lam_ph = max_prop_speed(p, evp, v_ph);
+            int64_t IND = base + (int64_t)t * stride;
+            real um1[4], ui[4], up1[4];
+            umu4(e[1], um1); umu4(e[2], ui); umu4(e[3], up1);
+            for (int c = 0; c < 4; c++) {
+                real d = minmod(RC(0.5f) * (up1[c] - um1[c]),
+                                minmod(THETA * (ui[c] - um1[c]), THETA * (up1[c] - ui[c])));
+                if (axis < 2) ud[4 * IND + c] = d / DX;
+                else {
+                    real chr = (c == 0 ? e[2][3] : (c == 3 ? RC(1.0f) : RC(0.0f))) * u0[2] / tau;
+                    ud[4 * IND + c] = d / (DX * tau) + chr;
+                }
+            }
+            for (int mn = 0; mn < 10; mn++) {
+                real k = kt1d_real(u0[0] * pi1[10 * J[0] + mn], u0[1] * pi1[10 * J[1] + mn],
+                                   u0[2] * pi1[10 * J[2] + mn], u0[3] * pi1[10 * J[3] + mn],
+                                   u0[4] * pi1[10 * J[4] + mn], v_mh, v_ph, lam_mh, lam_ph);
+                if (axis < 2) Src[10 * IND + mn] = Src[10 * IND + mn] - k / DX;
+                else Src[10 * IND + mn] = Src[10 * IND + mn] - k / (tau * DX);
+            }
+            if (bulk && bulk_src) {
+                real k = kt1d_real(u0[0] * bulk[J[0]], u0[1] * bulk[J[1]], u0[2] * bulk[J[2]],
+                                   u0[3] * bulk[J[3]], u0[4] * bulk[J[4]], v_mh, v_ph, lam_mh, lam_ph);
+                if (axis < 2) bulk_src[IND] = bulk_src[IND] - k / DX;
+                else bulk_src[IND] = bulk_src[IND] - k / (tau * DX);
+            }
+        }
+    }
+}
+
+/* PiPi self coupling used by the GUBSER_VISC_TEST branch, kernel_IS.cl:322-347 */
+static real dot4(const real *a, const real *b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+}
+static real pipi(int lam, int mu, int nu, const real *pim, const real *u) {
+    real A[4] = {pim[IDX(0, lam)], -pim[IDX(1, lam)], -pim[IDX(2, lam)], -pim[IDX(3, lam)]};
+    real D[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) D[a][b] = (a == b ? GMN_DIAG[a] : RC(0.0f)) - u[a] * u[b];
+    real first = dot4(D[mu], A) * dot4(D[nu], A);
+    real temp[4] = {dot4(D[0], A), dot4(D[1], A), dot4(D[2], A), dot4(D[3], A)};
+    real second = RC(-1.0f) / RC(3.0f) * ((mu == nu ? GMN_DIAG[mu] : RC(0.0f)) - u[mu] * u[nu]) * dot4(temp, A);
+    return first + second;
+}
+
+/* update_pimn, kernel_IS.cl:501-709.  With the bulk extension, Pi is relaxed to
+ * -zeta*theta with tau_Pi = 3 (zeta/s) hbarc / T and source Pi*theta/3 - Pi*u^tau/tau
+ * (the conservative form of tau_Pi D Pi + Pi = -zeta theta - (2/3) tau_Pi Pi theta). */
+void orc_update_pimn(const orc_params *p, real *pinew, real *goodcell, const real *pi1,
+                     const real *pistep, const real *ev1, const real *ev2, const real *udiff,
+                     const real *udx, const real *udy, const real *udz, real *Src,
+                     real *bulknew, const real *bulk1, const real *bulkstep, real *bulk_src,
+                     real tau, int step) {
+    int64_t n = ncells(p);
+    const real DT = RC(p->dt);
+    set_threads(p);
+    #pragma omp parallel for schedule(static)
+    for (int64_t I = 0; I < n; I++) {
+        const real *e1 = ev1 + 4 * I, *e2 = ev2 + 4 * I;
+        real u_old[4], u_new[4], udt[4];
+        umu4(e1, u_old); umu4(e2, u_new);
+        for (int c = 0; c < 4; c++) udt[c] = (u_new[c] - u_old[c]) / DT;
+        if (step == 1) {
+            for (int c = 0; c < 4; c++) {
+                udt[c] = udiff[4 * I + c] / DT;
+                u_new[c] = u_old[c] + udiff[4 * I + c];
+            }
+        }
+        const real *ux = udx + 4 * I, *uy = udy + 4 * I, *uz = udz + 4 * I;
+        /* dalpha_u[nu] = (d_t, d_x, d_y, d_z) u^nu */
+        real da[4][4];
+        for (int c = 0; c < 4; c++) { da[c][0] = udt[c]; da[c][1] = ux[c]; da[c][2] = uy[c]; da[c][3] = uz[c]; }
+        real DU[4], u[4], ed_step;
+        const real *us = (step == 1) ? u_old : u_new;
+        for (int c = 0; c < 4; c++) DU[c] = dot4(us, da[c]);
+        for (int c = 0; c < 4; c++) u[c] = us[c];
+        ed_step = (step == 1) ? e1[0] : e2[0];
+        real theta = udt[0] + ux[1] + uy[2] + uz[3];
+        real cpTs[4];
+        eos4(p, ed_step, cpTs);
+        real temperature = cpTs[2];
+        real local_etaos = etaos(p, temperature);
+        real etav = local_etaos * cpTs[3] * HBARC;
+        real one_over_taupi = temperature / (RC(3.0f) * r_max(ACU, local_etaos) * HBARC);
+        real coef_pipi = RC(0.0f);
+        if (p->gubser_visc_test) {
+            real ETAOS = RC(p->etaos_ymin);
+            real etavH = ETAOS * RC(4.0f) / RC(3.0f);
+            real taupiH = RC(p->lam1) * RC(p->lam1) * etavH / RC(3.0f);
+            one_over_taupi = RC(1.0f) / (taupiH * (real)pow((double)ed_step, (double)RC(-0.25f)));
+            coef_pipi = RC(p->lam1) / ed_step;
+            etav = ETAOS * RC(4.0f) / RC(3.0f) * (real)pow((double)ed_step, (double)RC(0.75f));
+        }
+        real pi2[10];
+        for (int m = 0; m < 10; m++) pi2[m] = pistep[10 * I + m];
+        real max_pimn_abs = RC(0.0f);
+        real out[10];
+        for (int mu = 0; mu < 4; mu++) {
+            for (int nu = mu; nu < 4; nu++) {
+                real gmunu = (mu == nu) ? GMN_DIAG[mu] : RC(0.0f);
+                real sigma = GMN_DIAG[mu] * da[nu][mu] + GMN_DIAG[nu] * da[mu][nu]
+                             - (u[mu] * DU[nu] + u[nu] * DU[mu])
+                             - RC(2.0f) / RC(3.0f) * (gmunu - u[mu] * u[nu]) * theta;
+                int mn = IDX(mu, nu);
+                if (u[0] > RC(10000.0f)) sigma = RC(0.0f);
+                real piNS = etav * sigma;
+                real pi_old = pi1[10 * I + mn] * u_old[0];
+                real src = -pi2[mn] * theta / RC(3.0f) - pi2[mn] * u[0] / tau;
+                for (int l = 0; l < 4; l++)
+                    src -= (u[mu] * pi2[IDX(nu, l)] + u[nu] * pi2[IDX(mu, l)]) * DU[l] * GMN_DIAG[l];
+                if (p->gubser_visc_test) {
+                    src -= one_over_taupi * coef_pipi * (pipi(0, mu, nu, pi2, u) - pipi(1, mu, nu, pi2, u)
+                                                         - pipi(2, mu, nu, pi2, u) - pipi(3, mu, nu, pi2, u));
+                }
+                Src[10 * I + mn] += src;
+                pi_old = (pi_old + Src[10 * I + mn] * DT / step + DT * one_over_taupi * piNS)
+                         / (u_new[0] + DT * one_over_taupi);
+                if ((real)fabs((double)pi_old) > max_pimn_abs) max_pimn_abs = (real)fabs((double)pi_old);
+                out[mn] = pi_old;
+            }
+        }
+        real bout = RC(0.0f);
+        if (p->bulk_on && bulknew) {
+            real zs = zetaos(p, temperature);
+            real zetav = zs * cpTs[3] * HBARC;
+            real one_over_taubulk = temperature / (RC(3.0f) * r_max(ACU, zs) * HBARC);
+            real b2 = bulkstep[I];
+            real bsrc = b2 * theta / RC(3.0f) - b2 * u[0] / tau;
+            bulk_src[I] += bsrc;
+            real b_old = bulk1[I] * u_old[0];
+            bout = (b_old + bulk_src[I] * DT / step - DT * one_over_taubulk * zetav * theta)
+                   / (u_new[0] + DT * one_over_taubulk);
+            if ((real)fabs((double)bout) > max_pimn_abs) max_pimn_abs = (real)fabs((double)bout);
+        }
+        real pre = eos_P(p, e2[0]);
+        real T00 = (e2[0] + pre) * u_new[0] * u_new[0] - pre;
+        goodcell[I] = RC(1.0f);
+        if (max_pimn_abs > T00) {
+            for (int m = 0; m < 10; m++) out[m] = RC(0.0f);
+            bout = RC(0.0f);
+            goodcell[I] = RC(0.0f);
+        }
+        for (int m = 0; m < 10; m++) pinew[10 * I + m] = out[m];
+        if (p->bulk_on && bulknew) bulknew[I] = bout;
+    }
+}
+
+/* get_udiff, kernel_IS.cl:714-724 */
+void orc_get_udiff(const orc_params *p, real *udiff, const real *ev0, const real *ev1) {
+    int64_t n = ncells(p);
+    for (int64_t I = 0; I < n; I++) {
+        real a[4], b[4];
+        umu4(ev0 + 4 * I, a); umu4(ev1 + 4 * I, b);
+        for (int c = 0; c < 4; c++) udiff[4 * I + c] = b[c] - a[c];
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Driver-level sequencing (ideal.py / visc.py)                               */
+/* ------------------------------------------------------------------------- */
+typedef struct orc_state {
+    real *ev[3];      /* d_ev[0..2] */
+    real *pi[3];      /* d_pi[0..2] */
+    real *bulk[3];    /* bulk Pi [0..2] (extension; NULL when off) */
+    real *Src;        /* d_Src (ideal.d_Src) */
+    real *IS_src;     /* d_IS_src */
+    real *bulk_src;   /* IS source of Pi (extension) */
+    real *udx, *udy, *udz, *udiff, *goodcell;
+    int32_t *iters;   /* optional: root-solve iteration counts of the last update_ev */
+} orc_state;
+
+/* CLIdeal.stepUpdate(step), ideal.py:211-255 */
+void orc_ideal_stage(const orc_params *p, orc_state *s, int step, real tau) {
+    orc_ideal_christoffel(p, s->Src, s->ev[step], tau, step);
+    for (int a = 0; a < 3; a++) orc_ideal_along(p, a, s->Src, s->ev[step], tau);
+    orc_ideal_update_ev(p, s->ev[3 - step], s->ev[1], s->Src, tau, step, s->iters);
+}
+
+/* CLVisc.IS_stepUpdate(step), visc.py:287-314 */
+void orc_is_stage(const orc_params *p, orc_state *s, int step, real tau) {
+    int bulk = p->bulk_on && s->bulk[0];
+    orc_is_christoffel(p, s->IS_src, s->pi[step], s->ev[step], tau, step);
+    if (bulk && step == 1) memset(s->bulk_src, 0, sizeof(real) * ncells(p));
+    real *uds[3] = {s->udx, s->udy, s->udz};
+    for (int a = 0; a < 3; a++)
+        orc_is_along(p, a, s->IS_src, uds[a], s->pi[step], s->ev[step],
+                     bulk ? s->bulk_src : NULL, bulk ? s->bulk[step] : NULL, tau);
+    orc_update_pimn(p, s->pi[3 - step], s->goodcell, s->pi[1], s->pi[step], s->ev[1], s->ev[2],
+                    s->udiff, s->udx, s->udy, s->udz, s->IS_src,
+                    bulk ? s->bulk[3 - step] : NULL, bulk ? s->bulk[1] : NULL,
+                    bulk ? s->bulk[step] : NULL, bulk ? s->bulk_src : NULL, tau, step);
+}
+
+/* CLVisc.visc_stepUpdate(step), visc.py:232-283 */
+void orc_visc_stage(const orc_params *p, orc_state *s, int step, real tau) {
+    int bulk = p->bulk_on && s->bulk[0];
+    orc_visc_christoffel(p, s->Src, s->ev[step], s->pi[step], bulk ? s->bulk[step] : NULL, tau, step);
+    for (int a = 0; a < 3; a++)
+        orc_visc_along(p, a, s->Src, s->ev[step], s->pi[step], bulk ? s->bulk[step] : NULL, tau);
+    orc_visc_update_ev(p, s->ev[3 - step], s->ev[1], s->pi[0], s->pi[3 - step],
+                       bulk ? s->bulk[0] : NULL, bulk ? s->bulk[3 - step] : NULL,
+                       s->Src, tau, step, s->iters);
+}
+
+/* tau after update_time(loop): cfg.real(TAU0 + (loop+1)*DT), ideal.py:317-320 */
+static real tau_of(const orc_params *p, int64_t loop_plus_one) {
+    return (real)((double)p->tau0 + (double)loop_plus_one * (double)p->dt);
+}
+
+/* nsteps of CLIdeal.evolve's step body (ideal.py:341-344) starting at loop index loop0 */
+void orc_ideal_steps(const orc_params *p, orc_state *s, int64_t loop0, int64_t nsteps) {
+    for (int64_t n = loop0; n < loop0 + nsteps; n++) {
+        real tau = tau_of(p, n);
+        orc_ideal_stage(p, s, 1, tau);
+        tau = tau_of(p, n + 1);
+        orc_ideal_stage(p, s, 2, tau);
+    }
+}
+
+/* nsteps of CLVisc.evolve's step body (visc.py:527-544) starting at loop index loop0 */
+void orc_visc_steps(const orc_params *p, orc_state *s, int64_t loop0, int64_t nsteps) {
+    int64_t n4 = 4 * ncells(p), n10 = 10 * ncells(p);
+    for (int64_t n = loop0; n < loop0 + nsteps; n++) {
+        memcpy(s->pi[0], s->pi[1], sizeof(real) * n10);
+        memcpy(s->ev[0], s->ev[1], sizeof(real) * n4);
+        if (p->bulk_on && s->bulk[0]) memcpy(s->bulk[0], s->bulk[1], sizeof(real) * ncells(p));
+        real tau = tau_of(p, n);
+        orc_is_stage(p, s, 1, tau);
+        orc_visc_stage(p, s, 1, tau);
+        tau = tau_of(p, n + 1);
+        orc_is_stage(p, s, 2, tau);
+        orc_visc_stage(p, s, 2, tau);
+        orc_get_udiff(p, s->udiff, s->ev[0], s->ev[1]);
+    }
+}
+
+int orc_real_size(void) { return (int)sizeof(real); }

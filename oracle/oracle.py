@@ -1,0 +1,235 @@
+"""ctypes wrapper of the CPU oracle (oracle/clvisc_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the oracle restates the reference kernels on the CPU
+so that the HIP path can be checked against it.  Only the test suite, the smoke
+check and bench.py's CPU-baseline leg import this module.
+
+State arrays use the reference's AoS layouts: ev (N,4) = (e, vx, vy, veta),
+pi (N,10) in the order 00,01,02,03,11,12,13,22,23,33, cell I = (i*NY+j)*NZ+k.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, 'lib')
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ('nx', C.c_int32), ('ny', C.c_int32), ('nz', C.c_int32),
+        ('riemann_test', C.c_int32), ('gubser_visc_test', C.c_int32),
+        ('eos_kind', C.c_int32),
+        ('eos_nrow', C.c_int32), ('eos_ncol', C.c_int32), ('eos_num_ed', C.c_int32),
+        ('eos_n', C.c_int32),
+        ('dt', C.c_float), ('dx', C.c_float), ('dy', C.c_float), ('dz', C.c_float),
+        ('tau0', C.c_float),
+        ('etaos_xmin', C.c_float), ('etaos_ymin', C.c_float),
+        ('etaos_left_slop', C.c_float), ('etaos_right_slop', C.c_float), ('lam1', C.c_float),
+        ('eos_ed_start', C.c_float), ('eos_ed_step', C.c_float),
+        ('eos_log2_emin', C.c_float), ('eos_inv_dlog2', C.c_float),
+        ('eos_table', C.c_void_p),
+        ('bulk_on', C.c_int32),
+        ('zetaos_max', C.c_float), ('zetaos_tpeak', C.c_float), ('zetaos_width', C.c_float),
+        ('fp64_root', C.c_int32), ('fp64_root_vmin', C.c_float),
+        ('nthreads', C.c_int32),
+    ]
+
+
+class OrcState(C.Structure):
+    _fields_ = [
+        ('ev', C.c_void_p * 3), ('pi', C.c_void_p * 3), ('bulk', C.c_void_p * 3),
+        ('Src', C.c_void_p), ('IS_src', C.c_void_p), ('bulk_src', C.c_void_p),
+        ('udx', C.c_void_p), ('udy', C.c_void_p), ('udz', C.c_void_p),
+        ('udiff', C.c_void_p), ('goodcell', C.c_void_p), ('iters', C.c_void_p),
+    ]
+
+
+_LIBS = {}
+
+
+def build():
+    """Compile the oracle libraries (make in oracle/)."""
+    subprocess.run(['make', '-s', '-C', HERE], check=True)
+
+
+def lib(precision='f32'):
+    if precision not in _LIBS:
+        path = os.path.join(LIBDIR, 'liborc_%s.so' % precision)
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.orc_max_ed.restype = C.c_float if precision == 'f32' else C.c_double
+        _LIBS[precision] = L
+    return _LIBS[precision]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def make_params(nx, ny, nz, dt, dx, dy, dz, tau0, eos_table, etaos_xmin=0.18, etaos_ymin=0.08,
+                etaos_left_slop=0.0, etaos_right_slop=0.0, lam1=-10.0, riemann_test=False,
+                gubser_visc_test=False, bulk=None, fp64_root_vmin=None, nthreads=0):
+    """Build an OrcParams.  ``eos_table`` is a clvisc_amd.eos.EosTable; ``bulk`` is
+    None or (zetaos_max, zetaos_tpeak, zetaos_width)."""
+    p = OrcParams()
+    p.nx, p.ny, p.nz = nx, ny, nz
+    p.dt, p.dx, p.dy, p.dz, p.tau0 = dt, dx, dy, dz, tau0
+    p.etaos_xmin, p.etaos_ymin = etaos_xmin, etaos_ymin
+    p.etaos_left_slop, p.etaos_right_slop, p.lam1 = etaos_left_slop, etaos_right_slop, lam1
+    p.riemann_test, p.gubser_visc_test = int(bool(riemann_test)), int(bool(gubser_visc_test))
+    t = eos_table
+    p.eos_kind = t.kind_id
+    p.eos_nrow, p.eos_ncol, p.eos_num_ed, p.eos_n = t.nrow, t.ncol, t.num_ed, t.n
+    p.eos_ed_start, p.eos_ed_step = t.ed_start, t.ed_step
+    p.eos_log2_emin, p.eos_inv_dlog2 = t.log2_emin, t.inv_dlog2
+    p._table = np.ascontiguousarray(t.rgba, dtype=np.float32)   # keep alive
+    p.eos_table = p._table.ctypes.data
+    if bulk is not None:
+        p.bulk_on = 1
+        p.zetaos_max, p.zetaos_tpeak, p.zetaos_width = bulk
+    if fp64_root_vmin is not None:
+        p.fp64_root, p.fp64_root_vmin = 1, fp64_root_vmin
+    p.nthreads = nthreads
+    return p
+
+
+class Oracle(object):
+    """Reference-sequenced CPU hydro (buffers named after the reference drivers)."""
+
+    def __init__(self, params, precision='f32', viscous=True, keep_iters=False):
+        self.p = params
+        self.precision = precision
+        self.L = lib(precision)
+        self.dtype = np.float32 if precision == 'f32' else np.float64
+        self.real = self.dtype
+        n = params.nx * params.ny * params.nz
+        self.size = n
+        z = lambda *s: np.zeros(s, self.dtype)
+        self.ev = [z(n, 4) for _ in range(3)]
+        self.pi = [z(n, 10) for _ in range(3)]
+        self.bulk = [z(n) for _ in range(3)] if params.bulk_on else [None] * 3
+        self.Src = z(n, 4)
+        self.IS_src = z(n, 10)
+        self.bulk_src = z(n) if params.bulk_on else None
+        self.udx, self.udy, self.udz, self.udiff = z(n, 4), z(n, 4), z(n, 4), z(n, 4)
+        self.goodcell = np.ones(n, self.dtype)
+        self.iters = np.zeros(n, np.int32) if keep_iters else None
+        self.tau = self.dtype(params.tau0)
+        self.loop = 0
+        s = OrcState()
+        for k in range(3):
+            s.ev[k] = self.ev[k].ctypes.data
+            s.pi[k] = self.pi[k].ctypes.data
+            s.bulk[k] = self.bulk[k].ctypes.data if self.bulk[k] is not None else None
+        s.Src, s.IS_src = self.Src.ctypes.data, self.IS_src.ctypes.data
+        s.bulk_src = self.bulk_src.ctypes.data if self.bulk_src is not None else None
+        s.udx, s.udy, s.udz = self.udx.ctypes.data, self.udy.ctypes.data, self.udz.ctypes.data
+        s.udiff, s.goodcell = self.udiff.ctypes.data, self.goodcell.ctypes.data
+        s.iters = self.iters.ctypes.data if self.iters is not None else None
+        self.state = s
+        self._rt = C.c_float if precision == 'f32' else C.c_double
+        if viscous:
+            # CLVisc.__init__: IS_initialize on the zero field, then the ideal prediction
+            # of the zero field into d_ev[2] (pyvisc/visc.py:101-102, 143-149, 468-470)
+            self.L.orc_visc_initialize(C.byref(self.p), _ptr(self.pi[1]), _ptr(self.goodcell),
+                                       _ptr(self.udiff), _ptr(self.ev[1]), self._rt(self.tau))
+            self.ideal_stage(1)
+
+    # -- stage-level (reference names) -------------------------------------------------
+    def ideal_stage(self, step, tau=None):
+        tau = self.tau if tau is None else tau
+        self.L.orc_ideal_stage(C.byref(self.p), C.byref(self.state), C.c_int(step), self._rt(tau))
+
+    def is_stage(self, step, tau=None):
+        tau = self.tau if tau is None else tau
+        self.L.orc_is_stage(C.byref(self.p), C.byref(self.state), C.c_int(step), self._rt(tau))
+
+    def visc_stage(self, step, tau=None):
+        tau = self.tau if tau is None else tau
+        self.L.orc_visc_stage(C.byref(self.p), C.byref(self.state), C.c_int(step), self._rt(tau))
+
+    def update_udiff(self, a, b):
+        self.L.orc_get_udiff(C.byref(self.p), _ptr(self.udiff), _ptr(a), _ptr(b))
+
+    def update_time(self, loop):
+        self.tau = self.dtype(np.float64(self.p.tau0) + (loop + 1) * np.float64(self.p.dt))
+
+    # -- step-level ------------------------------------------------------------------
+    def ideal_steps(self, nsteps):
+        self.L.orc_ideal_steps(C.byref(self.p), C.byref(self.state), C.c_int64(self.loop),
+                               C.c_int64(nsteps))
+        self.loop += nsteps
+        self.update_time(self.loop - 1)
+
+    def visc_steps(self, nsteps):
+        self.L.orc_visc_steps(C.byref(self.p), C.byref(self.state), C.c_int64(self.loop),
+                              C.c_int64(nsteps))
+        self.loop += nsteps
+        self.update_time(self.loop - 1)
+
+    def max_ed(self, which=1):
+        return float(self.L.orc_max_ed(C.byref(self.p), _ptr(self.ev[which])))
+
+    def load_ini(self, ev, pi=None, bulk=None):
+        self.ev[1][...] = np.asarray(ev, self.dtype).reshape(self.size, 4)
+        if pi is not None:
+            self.pi[1][...] = np.asarray(pi, self.dtype).reshape(self.size, 10)
+        if bulk is not None and self.bulk[1] is not None:
+            self.bulk[1][...] = np.asarray(bulk, self.dtype).reshape(self.size)
+
+
+# -- unit-level helpers ------------------------------------------------------------
+def eos_lookup(params, eps, precision='f32'):
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    eps = np.ascontiguousarray(eps, dt)
+    out = np.zeros((eps.size, 4), dt)
+    L.orc_eos(C.byref(params), _ptr(eps), C.c_int64(eps.size), _ptr(out))
+    return out
+
+
+def eos_funcs(params, eps, precision='f32'):
+    """(P, S, T, CS2) with the above-table fallback."""
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    eps = np.ascontiguousarray(eps, dt)
+    out = np.zeros((eps.size, 4), dt)
+    L.orc_eos_funcs(C.byref(params), _ptr(eps), C.c_int64(eps.size), _ptr(out))
+    return out
+
+
+def minmod4(a, b, precision='f32'):
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    a = np.ascontiguousarray(a, dt)
+    b = np.ascontiguousarray(b, dt)
+    out = np.zeros(4, dt)
+    L.orc_minmod4(_ptr(a), _ptr(b), _ptr(out))
+    return out
+
+
+def root_solve(params, T00, M, precision='f32'):
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    T00 = np.ascontiguousarray(T00, dt)
+    M = np.ascontiguousarray(M, dt)
+    ed = np.zeros(T00.size, dt)
+    it = np.zeros(T00.size, np.int32)
+    L.orc_root(C.byref(params), _ptr(T00), _ptr(M), C.c_int64(T00.size), _ptr(ed), _ptr(it))
+    return ed, it
+
+
+def max_prop_speed(params, ev, vk, precision='f32'):
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    ev = np.ascontiguousarray(ev, dt)
+    vk = np.ascontiguousarray(vk, dt)
+    out = np.zeros(vk.size, dt)
+    L.orc_max_prop_speed(C.byref(params), _ptr(ev), _ptr(vk), C.c_int64(vk.size), _ptr(out))
+    return out
