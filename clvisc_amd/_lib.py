@@ -1,0 +1,208 @@
+"""ctypes binding of libclvisc_amd.so (declarations in include/clvisc_amd.h).
+
+The library is the only compute path: if it is missing or fails to load the
+import raises, and nothing falls back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'lib', 'libclvisc_amd.so')
+
+CV_FLAG_RIEMANN_TEST = 0x1
+CV_FLAG_GUBSER_VISC_TEST = 0x2
+CV_FLAG_BULK = 0x4
+CV_FLAG_FP64_ROOT = 0x8
+
+FIELDS = {'ev0': 0, 'ev1': 1, 'ev2': 2, 'pi0': 3, 'pi1': 4, 'pi2': 5,
+          'bulk0': 6, 'bulk1': 7, 'bulk2': 8, 'src': 9, 'is_src': 10,
+          'udiff': 11, 'goodcell': 12}
+NCOMP = {0: 4, 1: 4, 2: 4, 3: 10, 4: 10, 5: 10, 6: 1, 7: 1, 8: 1, 9: 4, 10: 10, 11: 4, 12: 1}
+
+
+class CvParams(C.Structure):
+    _fields_ = [
+        ('nx', C.c_int32), ('ny', C.c_int32), ('nz', C.c_int32), ('flags', C.c_int32),
+        ('tau0', C.c_double), ('dt', C.c_double),
+        ('dx', C.c_float), ('dy', C.c_float), ('dz', C.c_float),
+        ('etaos_xmin', C.c_float), ('etaos_ymin', C.c_float),
+        ('etaos_left_slop', C.c_float), ('etaos_right_slop', C.c_float),
+        ('lam1', C.c_float),
+        ('zetaos_max', C.c_float), ('zetaos_tpeak', C.c_float), ('zetaos_width', C.c_float),
+        ('fp64_root_vmin', C.c_float),
+        ('efrz', C.c_float),
+        ('nxskip', C.c_int32), ('nyskip', C.c_int32), ('nzskip', C.c_int32),
+        ('reserved', C.c_int32),
+    ]
+
+
+class CvEos(C.Structure):
+    _fields_ = [
+        ('kind', C.c_int32),
+        ('nrow', C.c_int32), ('ncol', C.c_int32), ('num_ed', C.c_int32),
+        ('ed_start', C.c_float), ('ed_step', C.c_float),
+        ('kmin', C.c_int32), ('kmax', C.c_int32), ('log2j', C.c_int32),
+        ('n_entries', C.c_int32),
+        ('rgba', C.c_void_p),
+    ]
+
+
+class CvError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libclvisc_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CvError('%s is missing: build it with `python -c "import __graft_entry__ as g; '
+                          'g.build()"` or `make -C clvisc_amd/csrc`' % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        L.cv_last_error.restype = C.c_char_p
+        vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
+        sig = {
+            'cv_device_info': [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int32)],
+            'cv_create': [C.POINTER(CvParams), C.POINTER(CvEos), C.c_int, C.c_int, C.POINTER(vp)],
+            'cv_destroy': [vp],
+            'cv_set_field': [vp, C.c_int, vp, i64],
+            'cv_get_field': [vp, C.c_int, vp, i64],
+            'cv_copy_field': [vp, C.c_int, C.c_int],
+            'cv_visc_initialize': [vp, f32],
+            'cv_ideal_stage': [vp, C.c_int, f32],
+            'cv_is_stage': [vp, C.c_int, f32],
+            'cv_visc_stage': [vp, C.c_int, f32],
+            'cv_update_udiff': [vp, C.c_int, C.c_int],
+            'cv_ideal_steps_async': [vp, i64, i64],
+            'cv_visc_steps_async': [vp, i64, i64],
+            'cv_max_ed': [vp, C.c_int, C.POINTER(f32)],
+            'cv_sync': [vp],
+            'cv_last_elapsed_ms': [vp, C.POINTER(f32)],
+        }
+        for name, args in sig.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise CvError('clvisc_amd error %d: %s' % (rc, lib().cv_last_error().decode()))
+
+
+def exported_symbols():
+    """Names of the C-ABI entry points this binding expects."""
+    return ['cv_last_error', 'cv_device_info', 'cv_create', 'cv_destroy', 'cv_set_field',
+            'cv_get_field', 'cv_copy_field', 'cv_visc_initialize', 'cv_ideal_stage', 'cv_is_stage',
+            'cv_visc_stage', 'cv_update_udiff', 'cv_ideal_steps_async', 'cv_visc_steps_async',
+            'cv_max_ed', 'cv_sync', 'cv_last_elapsed_ms']
+
+
+def make_eos(table):
+    """CvEos from a clvisc_amd.eos.EosTable (keeps the host array alive on the struct)."""
+    e = CvEos()
+    e.kind = table.kind_id
+    e.nrow, e.ncol, e.num_ed = table.nrow, table.ncol, table.num_ed
+    e.ed_start, e.ed_step = table.ed_start, table.ed_step
+    e.kmin, e.kmax, e.log2j = table.kmin, table.kmax, table.log2j
+    rgba = np.ascontiguousarray(table.rgba, dtype=np.float32).reshape(-1, 4)
+    e.n_entries = rgba.shape[0]
+    e._keep = rgba
+    e.rgba = rgba.ctypes.data
+    return e
+
+
+class Handle(object):
+    """Owner of one device-side hydro state (cv_handle)."""
+
+    def __init__(self, params, eos_table, device=0, viscous=True):
+        self.L = lib()
+        self.params = params
+        self.eos = make_eos(eos_table)
+        self.n = params.nx * params.ny * params.nz
+        self.viscous = viscous
+        h = C.c_void_p()
+        check(self.L.cv_create(C.byref(params), C.byref(self.eos), device, int(viscous), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, 'h', None):
+            self.L.cv_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get(self, field):
+        f = FIELDS[field] if isinstance(field, str) else field
+        nc = NCOMP[f]
+        out = np.empty((self.n, nc) if nc > 1 else (self.n,), np.float32)
+        check(self.L.cv_get_field(self.h, f, out.ctypes.data, out.size))
+        return out
+
+    def set(self, field, arr):
+        f = FIELDS[field] if isinstance(field, str) else field
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        check(self.L.cv_set_field(self.h, f, a.ctypes.data, a.size))
+
+    def copy(self, dst, src):
+        d = FIELDS[dst] if isinstance(dst, str) else dst
+        s = FIELDS[src] if isinstance(src, str) else src
+        check(self.L.cv_copy_field(self.h, d, s))
+
+    def visc_initialize(self, tau):
+        check(self.L.cv_visc_initialize(self.h, float(tau)))
+
+    def ideal_stage(self, step, tau):
+        check(self.L.cv_ideal_stage(self.h, step, float(tau)))
+
+    def is_stage(self, step, tau):
+        check(self.L.cv_is_stage(self.h, step, float(tau)))
+
+    def visc_stage(self, step, tau):
+        check(self.L.cv_visc_stage(self.h, step, float(tau)))
+
+    def update_udiff(self, a, b):
+        check(self.L.cv_update_udiff(self.h, FIELDS.get(a, a), FIELDS.get(b, b)))
+
+    def ideal_steps(self, loop0, nsteps, sync=True):
+        check(self.L.cv_ideal_steps_async(self.h, loop0, nsteps))
+        if sync:
+            self.sync()
+
+    def visc_steps(self, loop0, nsteps, sync=True):
+        check(self.L.cv_visc_steps_async(self.h, loop0, nsteps))
+        if sync:
+            self.sync()
+
+    def max_ed(self, which=1):
+        out = C.c_float()
+        check(self.L.cv_max_ed(self.h, which, C.byref(out)))
+        return out.value
+
+    def sync(self):
+        check(self.L.cv_sync(self.h))
+
+    def last_elapsed_ms(self):
+        out = C.c_float()
+        check(self.L.cv_last_elapsed_ms(self.h, C.byref(out)))
+        return out.value
+
+
+def device_info(device=0):
+    name = C.create_string_buffer(128)
+    cus = C.c_int32()
+    check(lib().cv_device_info(device, name, 128, C.byref(cus)))
+    return name.value.decode(), cus.value

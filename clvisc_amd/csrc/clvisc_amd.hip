@@ -1,0 +1,423 @@
+// clvisc_amd.hip -- C-ABI of the MI355X KT + Israel-Stewart hydro step.
+// Declarations and the reference interface each entry point replaces: include/clvisc_amd.h.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "../../include/clvisc_amd.h"
+#include "cv_kernels.h"
+
+using namespace cv;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(CV_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+    } while (0)
+
+struct cv_handle {
+    cv_params prm;
+    int device;
+    int viscous;
+    int bulk;
+    int64_t n;
+    PhysP phys;
+    EosP eos;
+    float4 *d_eos = nullptr;
+    // slots of the reference buffers -> physical device buffers
+    float *ev[3] = {nullptr, nullptr, nullptr};
+    float *pi[3] = {nullptr, nullptr, nullptr};
+    float *bk[3] = {nullptr, nullptr, nullptr};
+    float *Src = nullptr, *IS_src = nullptr, *bsrc = nullptr, *udiff = nullptr, *good = nullptr;
+    unsigned int *d_max = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    bool timed = false;
+};
+
+extern "C" const char *cv_last_error(void) { return g_err.c_str(); }
+
+extern "C" int cv_device_info(int device, char *name, int name_len, int32_t *cus) {
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (name && name_len > 0) {
+        snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (cus) *cus = prop.multiProcessorCount;
+    return CV_OK;
+}
+
+static StageP stage_params(const cv_handle *h, int step, float tau) {
+    StageP s;
+    s.step = step;
+    s.tau = tau;
+    s.r_tau = 1.0f / tau;
+    s.r_taudz = 1.0f / (tau * h->prm.dz);
+    s.old_time = tau - (float)(step - 1) * (float)h->prm.dt;
+    s.r_step = 1.0f / (float)step;
+    return s;
+}
+
+static float tau_at(const cv_handle *h, int64_t loop_plus_one) {
+    return (float)(h->prm.tau0 + (double)loop_plus_one * h->prm.dt);
+}
+
+extern "C" int cv_create(const cv_params *p, const cv_eos *eos, int device, int viscous, cv_handle **out) {
+    if (!p || !eos || !out) return fail(CV_ERR_ARG, "cv_create: null argument");
+    if (p->nx < 1 || p->ny < 1 || p->nz < 1) return fail(CV_ERR_ARG, "cv_create: grid must be >= 1 in each axis");
+    if (!(p->dx > 0 && p->dy > 0 && p->dz > 0 && p->dt > 0 && p->tau0 > 0))
+        return fail(CV_ERR_ARG, "cv_create: DX, DY, DZ, DT, TAU0 must be positive");
+    if (!eos->rgba || eos->n_entries < 2) return fail(CV_ERR_ARG, "cv_create: EOS table missing");
+    if (eos->kind == 0) {
+        if ((int64_t)eos->nrow * eos->ncol != eos->n_entries || eos->ed_step <= 0)
+            return fail(CV_ERR_ARG, "cv_create: row/col EOS needs nrow*ncol == n_entries and ed_step > 0");
+    } else if (eos->kind == 1) {
+        if (eos->log2j < 0 || eos->log2j > 16 || eos->kmax <= eos->kmin || eos->kmin < -126 || eos->kmax > 127 ||
+            (int64_t)(eos->kmax - eos->kmin) * (1 << eos->log2j) + 1 != eos->n_entries)
+            return fail(CV_ERR_ARG, "cv_create: octave EOS needs (kmax-kmin)*2^log2j+1 == n_entries");
+    } else {
+        return fail(CV_ERR_ARG, "cv_create: unknown EOS kind %d", eos->kind);
+    }
+    if ((p->flags & CV_FLAG_BULK) && !viscous) return fail(CV_ERR_ARG, "cv_create: bulk needs viscous");
+    HIP_TRY(hipSetDevice(device));
+    cv_handle *h = new cv_handle();
+    h->prm = *p;
+    h->device = device;
+    h->viscous = viscous;
+    h->bulk = (p->flags & CV_FLAG_BULK) ? 1 : 0;
+    int64_t n = (int64_t)p->nx * p->ny * p->nz;
+    h->n = n;
+    PhysP &ph = h->phys;
+    ph.nx = p->nx; ph.ny = p->ny; ph.nz = p->nz;
+    ph.n = n; ph.sx = (int64_t)p->ny * p->nz; ph.sy = p->nz;
+    ph.dx = p->dx; ph.dy = p->dy; ph.dz = p->dz; ph.dt = (float)p->dt;
+    ph.r_dx = 1.0f / p->dx; ph.r_dy = 1.0f / p->dy; ph.r_dz = 1.0f / p->dz;
+    ph.r_dt = 1.0f / ph.dt; ph.r_3 = 1.0f / 3.0f;
+    ph.etaos_xmin = p->etaos_xmin; ph.etaos_ymin = p->etaos_ymin;
+    ph.etaos_ls = p->etaos_left_slop; ph.etaos_rs = p->etaos_right_slop; ph.lam1 = p->lam1;
+    ph.zmax = p->zetaos_max; ph.ztpeak = p->zetaos_tpeak; ph.zwidth = p->zetaos_width;
+    ph.fp64_vmin = p->fp64_root_vmin;
+    ph.flags = p->flags;
+    // the max(lambda, 1) shortcut needs every cs2 the lookup can return <= 0.5
+    float cs2max = -1e30f;
+    for (int64_t i = 0; i < eos->n_entries; i++) cs2max = eos->rgba[4 * i] > cs2max ? eos->rgba[4 * i] : cs2max;
+    ph.lam_fast = cs2max <= 0.5f ? 1 : 0;
+    EosP &E = h->eos;
+    memset(&E, 0, sizeof(E));
+    E.kind = eos->kind;
+    if (eos->kind == 0) {
+        E.nrow = eos->nrow; E.ncol = eos->ncol; E.num_ed = eos->num_ed;
+        E.ed_start = eos->ed_start; E.ed_step = eos->ed_step;
+        E.epr = eos->ed_step * eos->ncol;
+        E.r_epr = 1.0f / E.epr;
+        E.r_step = 1.0f / eos->ed_step;
+        E.top = eos->ed_start + (float)eos->num_ed * eos->ed_step;
+    } else {
+        E.kmin = eos->kmin; E.kmax = eos->kmax; E.lj = eos->log2j;
+        E.nlast = (eos->kmax - eos->kmin) << eos->log2j;
+        E.lo_e = ldexpf(1.0f, eos->kmin); E.hi_e = ldexpf(1.0f, eos->kmax);
+        E.r_lo = ldexpf(1.0f, -eos->kmin); E.r_hi = ldexpf(1.0f, -eos->kmax);
+    }
+    int rc = CV_OK;
+    auto alloc = [&](float **ptr, int64_t nfl) -> bool {
+        if (hipMalloc((void **)ptr, sizeof(float) * nfl) != hipSuccess) return false;
+        return hipMemset(*ptr, 0, sizeof(float) * nfl) == hipSuccess;
+    };
+    bool ok = hipMalloc((void **)&h->d_eos, sizeof(float4) * eos->n_entries) == hipSuccess &&
+              hipMemcpy(h->d_eos, eos->rgba, sizeof(float4) * eos->n_entries, hipMemcpyHostToDevice) == hipSuccess;
+    E.tab = h->d_eos;
+    for (int k = 0; ok && k < 3; k++) ok = alloc(&h->ev[k], 4 * n);
+    ok = ok && alloc(&h->Src, 4 * n);
+    if (viscous) {
+        for (int k = 0; ok && k < 3; k++) ok = alloc(&h->pi[k], 10 * n);
+        ok = ok && alloc(&h->IS_src, 10 * n) && alloc(&h->udiff, 4 * n) && alloc(&h->good, n);
+        if (ok) {
+            std::vector<float> ones(n, 1.0f);
+            ok = hipMemcpy(h->good, ones.data(), sizeof(float) * n, hipMemcpyHostToDevice) == hipSuccess;
+        }
+        if (h->bulk) {
+            for (int k = 0; ok && k < 3; k++) ok = alloc(&h->bk[k], n);
+            ok = ok && alloc(&h->bsrc, n);
+        }
+    }
+    ok = ok && hipMalloc((void **)&h->d_max, sizeof(unsigned int)) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreate(&h->ev_a) == hipSuccess && hipEventCreate(&h->ev_b) == hipSuccess;
+    if (!ok) {
+        rc = fail(CV_ERR_HIP, "cv_create: device allocation failed (%lld cells): %s", (long long)n,
+                  hipGetErrorString(hipGetLastError()));
+        cv_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return CV_OK;
+}
+
+extern "C" int cv_destroy(cv_handle *h) {
+    if (!h) return CV_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (int k = 0; k < 3; k++) { (void)hipFree(h->ev[k]); (void)hipFree(h->pi[k]); (void)hipFree(h->bk[k]); }
+    (void)hipFree(h->Src); (void)hipFree(h->IS_src); (void)hipFree(h->bsrc);
+    (void)hipFree(h->udiff); (void)hipFree(h->good);
+    (void)hipFree(h->d_eos); (void)hipFree(h->d_max);
+    if (h->ev_a) (void)hipEventDestroy(h->ev_a);
+    if (h->ev_b) (void)hipEventDestroy(h->ev_b);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return CV_OK;
+}
+
+// field id -> (device pointer, number of components)
+static int field_ptr(cv_handle *h, int f, float **ptr, int *ncomp) {
+    switch (f) {
+    case CV_EV0: case CV_EV1: case CV_EV2: *ptr = h->ev[f - CV_EV0]; *ncomp = 4; break;
+    case CV_PI0: case CV_PI1: case CV_PI2: *ptr = h->pi[f - CV_PI0]; *ncomp = 10; break;
+    case CV_BULK0: case CV_BULK1: case CV_BULK2: *ptr = h->bk[f - CV_BULK0]; *ncomp = 1; break;
+    case CV_SRC: *ptr = h->Src; *ncomp = 4; break;
+    case CV_IS_SRC: *ptr = h->IS_src; *ncomp = 10; break;
+    case CV_UDIFF: *ptr = h->udiff; *ncomp = 4; break;
+    case CV_GOODCELL: *ptr = h->good; *ncomp = 1; break;
+    default: return fail(CV_ERR_ARG, "unknown field id %d", f);
+    }
+    if (!*ptr) return fail(CV_ERR_STATE, "field %d is not allocated for this handle (ideal or no bulk)", f);
+    return CV_OK;
+}
+
+extern "C" int cv_set_field(cv_handle *h, int f, const float *host, int64_t nfloats) {
+    if (!h || !host) return fail(CV_ERR_ARG, "cv_set_field: null argument");
+    float *d; int nc;
+    int rc = field_ptr(h, f, &d, &nc);
+    if (rc) return rc;
+    if (nfloats != nc * h->n) return fail(CV_ERR_ARG, "cv_set_field: expected %lld floats, got %lld",
+                                          (long long)(nc * h->n), (long long)nfloats);
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<float> soa((size_t)nfloats);
+    for (int64_t I = 0; I < h->n; I++)
+        for (int c = 0; c < nc; c++) soa[(size_t)(c * h->n + I)] = host[I * nc + c];
+    HIP_TRY(hipMemcpy(d, soa.data(), sizeof(float) * nfloats, hipMemcpyHostToDevice));
+    return CV_OK;
+}
+
+extern "C" int cv_get_field(cv_handle *h, int f, float *host, int64_t nfloats) {
+    if (!h || !host) return fail(CV_ERR_ARG, "cv_get_field: null argument");
+    float *d; int nc;
+    int rc = field_ptr(h, f, &d, &nc);
+    if (rc) return rc;
+    if (nfloats != nc * h->n) return fail(CV_ERR_ARG, "cv_get_field: expected %lld floats, got %lld",
+                                          (long long)(nc * h->n), (long long)nfloats);
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<float> soa((size_t)nfloats);
+    HIP_TRY(hipMemcpy(soa.data(), d, sizeof(float) * nfloats, hipMemcpyDeviceToHost));
+    for (int64_t I = 0; I < h->n; I++)
+        for (int c = 0; c < nc; c++) host[I * nc + c] = soa[(size_t)(c * h->n + I)];
+    return CV_OK;
+}
+
+extern "C" int cv_copy_field(cv_handle *h, int dst, int src) {
+    if (!h) return fail(CV_ERR_ARG, "cv_copy_field: null handle");
+    float *a, *b; int na, nb_;
+    int rc = field_ptr(h, dst, &a, &na);
+    if (rc) return rc;
+    rc = field_ptr(h, src, &b, &nb_);
+    if (rc) return rc;
+    if (na != nb_) return fail(CV_ERR_ARG, "cv_copy_field: component counts differ");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipMemcpyAsync(a, b, sizeof(float) * na * h->n, hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CV_OK;
+}
+
+static dim3 grid1d(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+extern "C" int cv_visc_initialize(cv_handle *h, float tau) {
+    if (!h || !h->viscous) return fail(CV_ERR_STATE, "cv_visc_initialize: viscous handle required");
+    HIP_TRY(hipSetDevice(h->device));
+    hipLaunchKernelGGL(k_visc_init, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, tau, 1.0f / tau,
+                       h->ev[1], h->pi[1], h->good, h->udiff);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CV_OK;
+}
+
+// ---- launch helpers (queue only) --------------------------------------------
+static int launch_ideal(cv_handle *h, int step, float tau, float *ev_st, float *ev1, float *ev_out) {
+    StageP st = stage_params(h, step, tau);
+    hipLaunchKernelGGL(k_ideal_stage, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, st,
+                       ev_st, ev1, h->Src, ev_out);
+    HIP_TRY(hipGetLastError());
+    return CV_OK;
+}
+
+static int launch_is(cv_handle *h, int step, float tau, float *ev_st, float *pi_st, float *bk_st,
+                     float *ev1, float *ev2, float *pi1, float *bk1, float *pi_out, float *bk_out) {
+    StageP st = stage_params(h, step, tau);
+    if (h->bulk)
+        hipLaunchKernelGGL(k_is_stage<true>, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, st,
+                           ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src, h->bsrc,
+                           pi_out, bk_out, h->good);
+    else
+        hipLaunchKernelGGL(k_is_stage<false>, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, st,
+                           ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src, h->bsrc,
+                           pi_out, bk_out, h->good);
+    HIP_TRY(hipGetLastError());
+    return CV_OK;
+}
+
+static int launch_kt(cv_handle *h, int step, float tau, float *ev_st, float *pi_st, float *bk_st,
+                     float *ev1, float *pim0, float *bk0, float *pinew, float *bknew, float *ev_out,
+                     float *udiff_out) {
+    StageP st = stage_params(h, step, tau);
+    if (h->bulk)
+        hipLaunchKernelGGL(k_kt_stage<true>, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, st,
+                           ev_st, pi_st, bk_st, ev1, pim0, bk0, pinew, bknew, h->Src, ev_out, udiff_out);
+    else
+        hipLaunchKernelGGL(k_kt_stage<false>, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, st,
+                           ev_st, pi_st, bk_st, ev1, pim0, bk0, pinew, bknew, h->Src, ev_out, udiff_out);
+    HIP_TRY(hipGetLastError());
+    return CV_OK;
+}
+
+static int check_step(int step) {
+    if (step != 1 && step != 2) return fail(CV_ERR_ARG, "step must be 1 or 2, got %d", step);
+    return CV_OK;
+}
+
+extern "C" int cv_ideal_stage(cv_handle *h, int step, float tau) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    int rc = check_step(step);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    rc = launch_ideal(h, step, tau, h->ev[step], h->ev[1], h->ev[3 - step]);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CV_OK;
+}
+
+extern "C" int cv_is_stage(cv_handle *h, int step, float tau) {
+    if (!h || !h->viscous) return fail(CV_ERR_STATE, "cv_is_stage: viscous handle required");
+    int rc = check_step(step);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    rc = launch_is(h, step, tau, h->ev[step], h->pi[step], h->bk[step], h->ev[1], h->ev[2], h->pi[1], h->bk[1],
+                   h->pi[3 - step], h->bk[3 - step]);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CV_OK;
+}
+
+extern "C" int cv_visc_stage(cv_handle *h, int step, float tau) {
+    if (!h || !h->viscous) return fail(CV_ERR_STATE, "cv_visc_stage: viscous handle required");
+    int rc = check_step(step);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    rc = launch_kt(h, step, tau, h->ev[step], h->pi[step], h->bk[step], h->ev[1], h->pi[0], h->bk[0],
+                   h->pi[3 - step], h->bk[3 - step], h->ev[3 - step], nullptr);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CV_OK;
+}
+
+extern "C" int cv_update_udiff(cv_handle *h, int a, int b) {
+    if (!h || !h->viscous) return fail(CV_ERR_STATE, "cv_update_udiff: viscous handle required");
+    if (a < CV_EV0 || a > CV_EV2 || b < CV_EV0 || b > CV_EV2) return fail(CV_ERR_ARG, "ev field ids expected");
+    HIP_TRY(hipSetDevice(h->device));
+    hipLaunchKernelGGL(k_udiff, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->ev[a], h->ev[b], h->udiff);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CV_OK;
+}
+
+extern "C" int cv_ideal_steps_async(cv_handle *h, int64_t loop0, int64_t nsteps) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    if (nsteps < 0 || loop0 < 0) return fail(CV_ERR_ARG, "negative loop index or step count");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipEventRecord(h->ev_a, h->stream));
+    for (int64_t n = loop0; n < loop0 + nsteps; n++) {
+        int rc = launch_ideal(h, 1, tau_at(h, n), h->ev[1], h->ev[1], h->ev[2]);
+        if (rc) return rc;
+        rc = launch_ideal(h, 2, tau_at(h, n + 1), h->ev[2], h->ev[1], h->ev[1]);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(h->ev_b, h->stream));
+    h->timed = true;
+    return CV_OK;
+}
+
+// One step of CLVisc.evolve's body with buffer rotation instead of the two D2D copies:
+// stage 2 writes ev_{n+1}, pi_{n+1} into the slot-0 buffers, then slots 0 and 1 swap, which
+// leaves d_ev[0] = ev_n and d_pi[0] = pi_n exactly as the copies of visc.py:527-532 do.
+extern "C" int cv_visc_steps_async(cv_handle *h, int64_t loop0, int64_t nsteps) {
+    if (!h || !h->viscous) return fail(CV_ERR_STATE, "cv_visc_steps_async: viscous handle required");
+    if (nsteps < 0 || loop0 < 0) return fail(CV_ERR_ARG, "negative loop index or step count");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipEventRecord(h->ev_a, h->stream));
+    for (int64_t n = loop0; n < loop0 + nsteps; n++) {
+        float t1 = tau_at(h, n), t2 = tau_at(h, n + 1);
+        float **E = h->ev, **P = h->pi, **B = h->bk;
+        int rc = launch_is(h, 1, t1, E[1], P[1], B[1], E[1], E[2], P[1], B[1], P[2], B[2]);
+        if (!rc) rc = launch_kt(h, 1, t1, E[1], P[1], B[1], E[1], P[1], B[1], P[2], B[2], E[2], nullptr);
+        if (!rc) rc = launch_is(h, 2, t2, E[2], P[2], B[2], E[1], E[2], P[1], B[1], P[0], B[0]);
+        if (!rc) rc = launch_kt(h, 2, t2, E[2], P[2], B[2], E[1], P[1], B[1], P[0], B[0], E[0], h->udiff);
+        if (rc) return rc;
+        float *t;
+        t = E[0]; E[0] = E[1]; E[1] = t;
+        t = P[0]; P[0] = P[1]; P[1] = t;
+        t = B[0]; B[0] = B[1]; B[1] = t;
+    }
+    HIP_TRY(hipEventRecord(h->ev_b, h->stream));
+    h->timed = true;
+    return CV_OK;
+}
+
+extern "C" int cv_max_ed(cv_handle *h, int which, float *out) {
+    if (!h || !out) return fail(CV_ERR_ARG, "null argument");
+    if (which < 0 || which > 2) return fail(CV_ERR_ARG, "which must be 0, 1 or 2");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipMemsetAsync(h->d_max, 0, sizeof(unsigned int), h->stream));
+    int64_t blocks = (h->n + 255) / 256;
+    if (blocks > 2048) blocks = 2048;
+    hipLaunchKernelGGL(k_max_ed, dim3((unsigned)blocks), dim3(256), 0, h->stream, h->n, h->ev[which], h->d_max);
+    HIP_TRY(hipGetLastError());
+    unsigned int u = 0;
+    HIP_TRY(hipMemcpyAsync(&u, h->d_max, sizeof(u), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    memcpy(out, &u, sizeof(float));
+    return CV_OK;
+}
+
+extern "C" int cv_sync(cv_handle *h) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CV_OK;
+}
+
+extern "C" int cv_last_elapsed_ms(cv_handle *h, float *ms) {
+    if (!h || !ms) return fail(CV_ERR_ARG, "null argument");
+    if (!h->timed) return fail(CV_ERR_STATE, "no timed call yet");
+    HIP_TRY(hipEventSynchronize(h->ev_b));
+    HIP_TRY(hipEventElapsedTime(ms, h->ev_a, h->ev_b));
+    return CV_OK;
+}

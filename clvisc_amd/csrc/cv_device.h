@@ -1,0 +1,277 @@
+// cv_device.h -- device-side arithmetic of the KT + Israel-Stewart hydro step (gfx950).
+//
+// Every function evaluates the formulas of the reference kernels
+// (pyvisc/kernel/helper.h, eos_table.h, kernel_ideal.cl, kernel_visc.cl,
+// kernel_IS.cl) under one arithmetic contract:
+//   * a/b is a * RN(1/b); RN(1/b) of a run constant is computed once on the host,
+//   * sqrt is correctly rounded, nothing is contracted into FMA (this file is
+//     compiled with -ffp-contract=off),
+//   * operands are combined in the order the reference writes them.
+// Under this contract an IEEE CPU evaluating the same operations gets the same
+// bits, which is how the FP32 results are checked.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CV_ACU 1.0e-7f
+#define CV_THETA 1.1f
+#define CV_HBARC 0.19733f
+
+namespace cv {
+
+// ---------------------------------------------------------------------------
+// Parameters passed by value to every kernel
+// ---------------------------------------------------------------------------
+struct EosP {
+    int kind;               // 0 row/col, 1 octave
+    // row/col (eos_table.h:39-62)
+    int nrow, ncol, num_ed;
+    float ed_start, ed_step, epr, r_epr, r_step, top;
+    // octave
+    int kmin, kmax, lj, nlast;
+    float lo_e, hi_e, r_lo, r_hi;
+    const float4 *tab;
+};
+
+struct PhysP {
+    int nx, ny, nz;
+    int64_t n, sx, sy;      // cells, stride of i, stride of j (k stride 1)
+    float dx, dy, dz, dt;
+    float r_dx, r_dy, r_dz, r_dt, r_3;
+    float etaos_xmin, etaos_ymin, etaos_ls, etaos_rs, lam1;
+    float zmax, ztpeak, zwidth;
+    float fp64_vmin;
+    int flags;
+    int lam_fast;           // 1 when the table's cs2 <= 0.5 (max(lambda,1) shortcut valid)
+};
+
+struct StageP {
+    float tau, r_tau, r_taudz, old_time, r_step;
+    int step;
+};
+
+// ---------------------------------------------------------------------------
+// Elementary operations
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float rcp(float b) { return 1.0f / b; }     // correctly rounded
+__device__ __forceinline__ float rmax(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float rmin(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float rsign(float x) {
+    return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : x);
+}
+__device__ __forceinline__ float minmod(float x, float y) {
+    float res = rmin(fabsf(x), fabsf(y));
+    return res * (rsign(x) + rsign(y)) * 0.5f;
+}
+// generalised minmod slope at b from cells (a, b, c)
+__device__ __forceinline__ float slope(float a, float b, float c) {
+    return minmod(0.5f * (c - a), minmod(CV_THETA * (c - b), CV_THETA * (b - a)));
+}
+__device__ __forceinline__ float gamma3(float vx, float vy, float vz) {
+    return rcp(sqrtf(rmax(1.0f - vx * vx - vy * vy - vz * vz, CV_ACU)));
+}
+
+// ---------------------------------------------------------------------------
+// EOS lookup
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float4 lerp4(float r, float4 lo, float4 hi) {
+    float s = 1.0f - r;
+    return make_float4(s * lo.x + r * hi.x, s * lo.y + r * hi.y,
+                       s * lo.z + r * hi.z, s * lo.w + r * hi.w);
+}
+
+// reference row/col table (eos_table.h:39-62)
+__device__ __forceinline__ float4 eos_rowcol(const EosP &E, float eps) {
+    int row = (int)(eps * E.r_epr);
+    int col = (int)((eps - E.ed_start - (float)row * E.epr) * E.r_step);
+    float eps_low = E.ed_start + (float)row * E.epr + (float)col * E.ed_step;
+    int c0 = col < 0 ? 0 : (col > E.ncol - 1 ? E.ncol - 1 : col);
+    int r0 = row < 0 ? 0 : (row > E.nrow - 1 ? E.nrow - 1 : row);
+    float4 lo = E.tab[(int64_t)r0 * E.ncol + c0];
+    col = col + 1;
+    if (col == E.ncol) { row += 1; col = 0; }
+    int c1 = col < 0 ? 0 : (col > E.ncol - 1 ? E.ncol - 1 : col);
+    int r1 = row < 0 ? 0 : (row > E.nrow - 1 ? E.nrow - 1 : row);
+    float4 hi = E.tab[(int64_t)r1 * E.ncol + c1];
+    float r = (eps - eps_low) * E.r_step;
+    return lerp4(r, lo, hi);
+}
+
+// octave table: node and weight from the binary exponent and mantissa of eps
+__device__ __forceinline__ float4 eos_octave(const EosP &E, const float4 *tab, float eps) {
+    if (!(eps >= E.lo_e)) {
+        float r = eps > 0.0f ? eps * E.r_lo : 0.0f;
+        float4 t = tab[0];
+        return make_float4(t.x, r * t.y, r * t.z, r * t.w);
+    }
+    if (eps >= E.hi_e) {
+        float r = eps * E.r_hi;
+        float4 t = tab[E.nlast];
+        return make_float4(t.x, r * t.y, r * t.z, r * t.w);
+    }
+    int ex;
+    float fr = frexpf(eps, &ex);
+    float f = 2.0f * fr - 1.0f;
+    float tt = f * (float)(1 << E.lj);
+    int j = (int)tt;
+    float r = tt - (float)j;
+    int n = ((ex - 1 - E.kmin) << E.lj) + j;
+    return lerp4(r, tab[n], tab[n + 1]);
+}
+
+// A lookup context: the table pointer may be an LDS copy of E.tab.
+struct Eos {
+    const EosP &E;
+    const float4 *tab;
+    __device__ Eos(const EosP &e, const float4 *t) : E(e), tab(t) {}
+    __device__ __forceinline__ float4 raw(float eps) const {
+        return E.kind == 1 ? eos_octave(E, tab, eps) : eos_rowcol(E, eps);
+    }
+    __device__ __forceinline__ bool in_table(float eps) const {
+        return E.kind == 1 || eps < E.top;
+    }
+    // P, S, T, CS2 with the analytic fit above a row/col table (eos_table.h:65-100)
+    __device__ __forceinline__ float P(float eps) const {
+        if (in_table(eps)) return raw(eps).y;
+        return 0.3327f * eps - 0.3223f * powf(eps, 0.4585f) - 0.003906f * eps * expf(-0.05697f * eps);
+    }
+    __device__ __forceinline__ float S(float eps) const {
+        if (in_table(eps)) return raw(eps).w;
+        return 18.202f * eps - 63.0218f - 4.85479f * expf(-2.72407E-11f * powf(eps, 4.54886f))
+               + 65.1272f * powf(eps, -0.128012f) * expf(-0.00369624f * powf(eps, 1.18735f));
+    }
+    __device__ __forceinline__ float CS2(float eps) const {
+        if (in_table(eps)) return raw(eps).x;
+        return 0.333333333f;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Hydro helpers (helper.h)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float etaos(const PhysP &p, float t) {
+    return (t < p.etaos_xmin) ? (p.etaos_ls * (t - p.etaos_xmin) + p.etaos_ymin)
+                              : (p.etaos_rs * (t - p.etaos_xmin) + p.etaos_ymin);
+}
+__device__ __forceinline__ float zetaos(const PhysP &p, float t) {
+    float d = (t - p.ztpeak) * rcp(p.zwidth);
+    return p.zmax * rcp(1.0f + d * d);
+}
+
+// T^{tau mu} (helper.h:172-177); g = gamma of ev
+__device__ __forceinline__ float4 t0m(float4 ev, float pr) {
+    float u0 = gamma3(ev.y, ev.z, ev.w);
+    float a = (ev.x + pr) * u0;
+    return make_float4(a * (1.0f * u0) - 1.0f * pr, a * (ev.y * u0) - 0.0f * pr,
+                       a * (ev.z * u0) - 0.0f * pr, a * (ev.w * u0) - 0.0f * pr);
+}
+
+// maxPropagationSpeed (helper.h:94-105) = max(lambda_KT, 1).  When the table's cs2 is
+// at most 0.5 and v^2 <= 0.81, lambda_KT <= (|v| + cs)/(1 + |v| cs) < 0.983, far below
+// 1 minus its FP32 rounding error, so the maximum is exactly 1.0f.
+__device__ __forceinline__ float max_prop_speed(const PhysP &p, const Eos &eos, float4 edv, float vk) {
+    if (p.lam_fast && (edv.y * edv.y + edv.z * edv.z + edv.w * edv.w) <= 0.81f) return 1.0f;
+    float ut = gamma3(edv.y, edv.z, edv.w);
+    float uk = ut * vk;
+    float ut2 = ut * ut;
+    float uk2 = uk * uk;
+    float cs2 = eos.CS2(edv.x);
+    float lam = (fabsf(ut * uk * (1.0f - cs2)) + sqrtf((ut2 - uk2 - (ut2 - uk2 - 1.0f) * cs2) * cs2))
+                * rcp(ut2 - (ut2 - 1.0f) * cs2);
+    return rmax(lam, 1.0f);
+}
+
+// rootFinding_newton (helper.h:128-167) with P_eff = P + bulk
+__device__ __forceinline__ float root_newton(const Eos &eos, float T00, float M, float bulk) {
+    float vl = 0.0f, vh = 1.0f;
+    const float dpe = 1.0f / 3.0f;
+    float v = 0.5f * (vl + vh);
+    float ed = T00 - M * v;
+    float pr = eos.P(ed) + bulk;
+    float f = (T00 + pr) * v - M;
+    float df = (T00 + pr) - M * v * dpe;
+    float dvold = vh - vl;
+    float dv = dvold;
+    int i = 0;
+    while (true) {
+        if ((f + df * (vh - v)) * (f + df * (vl - v)) > 0.0f || fabsf(2.f * f) > fabsf(dvold * df)) {
+            dvold = dv;
+            dv = 0.5f * (vh - vl);
+            v = vl + dv;
+        } else {
+            dvold = dv;
+            dv = f * rcp(df);
+            v -= dv;
+        }
+        i++;
+        if (fabsf(dv) < 0.00001f || i > 100) break;
+        ed = T00 - M * v;
+        pr = eos.P(ed) + bulk;
+        f = (T00 + pr) * v - M;
+        df = (T00 + pr) - M * v * dpe;
+        if (f > 0.0f) vh = v; else vl = v;
+    }
+    return T00 - M * v;
+}
+
+// FP64 twin of the root solve (cells with M close to T00); table values stay FP32.
+__device__ __forceinline__ double eos_P_d(const EosP &E, const float4 *tab, double eps) {
+    if (E.kind == 1) {
+        if (!(eps >= (double)E.lo_e)) return (eps > 0.0 ? eps * (double)E.r_lo : 0.0) * (double)tab[0].y;
+        if (eps >= (double)E.hi_e) return eps * (double)E.r_hi * (double)tab[E.nlast].y;
+        int ex;
+        double fr = frexp(eps, &ex);
+        double f = 2.0 * fr - 1.0;
+        double tt = f * (double)(1 << E.lj);
+        int j = (int)tt;
+        double r = tt - j;
+        int n = ((ex - 1 - E.kmin) << E.lj) + j;
+        return (1.0 - r) * (double)tab[n].y + r * (double)tab[n + 1].y;
+    }
+    double step = E.ed_step, start = E.ed_start;
+    double epr = step * E.ncol;
+    if (eps < (double)E.ed_start + (double)E.num_ed * (double)E.ed_step) {
+        int row = (int)(eps * (1.0 / epr));
+        int col = (int)((eps - start - row * epr) * (1.0 / step));
+        double elow = start + row * epr + col * step;
+        int c0 = col < 0 ? 0 : (col > E.ncol - 1 ? E.ncol - 1 : col);
+        int r0 = row < 0 ? 0 : (row > E.nrow - 1 ? E.nrow - 1 : row);
+        double lo = tab[(int64_t)r0 * E.ncol + c0].y;
+        col += 1;
+        if (col == E.ncol) { row += 1; col = 0; }
+        int c1 = col < 0 ? 0 : (col > E.ncol - 1 ? E.ncol - 1 : col);
+        int r1 = row < 0 ? 0 : (row > E.nrow - 1 ? E.nrow - 1 : row);
+        double hi = tab[(int64_t)r1 * E.ncol + c1].y;
+        double r = (eps - elow) * (1.0 / step);
+        return (1.0 - r) * lo + r * hi;
+    }
+    return 0.3327 * eps - 0.3223 * pow(eps, 0.4585) - 0.003906 * eps * exp(-0.05697 * eps);
+}
+
+__device__ __forceinline__ double root_newton_d(const EosP &E, const float4 *tab, double T00, double M, double bulk) {
+    double vl = 0.0, vh = 1.0, dpe = 1.0 / 3.0;
+    double v = 0.5;
+    double ed = T00 - M * v;
+    double pr = eos_P_d(E, tab, ed) + bulk;
+    double f = (T00 + pr) * v - M;
+    double df = (T00 + pr) - M * v * dpe;
+    double dvold = vh - vl, dv = dvold;
+    int i = 0;
+    while (true) {
+        if ((f + df * (vh - v)) * (f + df * (vl - v)) > 0.0 || fabs(2.0 * f) > fabs(dvold * df)) {
+            dvold = dv; dv = 0.5 * (vh - vl); v = vl + dv;
+        } else {
+            dvold = dv; dv = f * (1.0 / df); v -= dv;
+        }
+        i++;
+        if (fabs(dv) < 0.00001 || i > 100) break;
+        ed = T00 - M * v;
+        pr = eos_P_d(E, tab, ed) + bulk;
+        f = (T00 + pr) * v - M;
+        df = (T00 + pr) - M * v * dpe;
+        if (f > 0.0) vh = v; else vl = v;
+    }
+    return T00 - M * v;
+}
+
+}  // namespace cv

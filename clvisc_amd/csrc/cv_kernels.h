@@ -1,0 +1,633 @@
+// cv_kernels.h -- per-cell stage kernels of the KT + Israel-Stewart hydro step.
+//
+// One thread owns one cell and evaluates a whole reference stage for it:
+//   k_ideal_stage : kt_src_christoffel + kt_src_along{x,y,z} + update_ev   (kernel_ideal.cl)
+//   k_is_stage    : visc_src_christoffel + visc_src_along{x,y,z} + update_pimn (kernel_IS.cl)
+//   k_kt_stage    : kt_src_christoffel + kt_src_along{x,y,z} + update_ev   (kernel_visc.cl)
+//                   optionally followed by get_udiff (kernel_IS.cl:714-724)
+// Fields are structure-of-arrays planes: component c of cell I sits at base[c*n + I].
+// The 2-ghost constant extrapolation of the reference pencils (kernel_ideal.cl:57-62)
+// becomes a clamp of the stencil index to the grid.
+#pragma once
+#include "cv_device.h"
+
+namespace cv {
+
+__device__ __forceinline__ float4 ld4(const float *__restrict__ b, int64_t n, int64_t I) {
+    return make_float4(b[I], b[n + I], b[2 * n + I], b[3 * n + I]);
+}
+__device__ __forceinline__ void st4(float *b, int64_t n, int64_t I, float4 v) {
+    b[I] = v.x; b[n + I] = v.y; b[2 * n + I] = v.z; b[3 * n + I] = v.w;
+}
+__device__ __forceinline__ float comp(float4 v, int c) {
+    return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ float4 half4(float4 a, float4 b) {
+    return make_float4(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z), 0.5f * (a.w + b.w));
+}
+
+// index of the cell at offset s along axis a (clamped to the grid)
+struct Cell {
+    int64_t I;
+    int i, j, k;
+};
+__device__ __forceinline__ Cell cell_of(const PhysP &p, int64_t I) {
+    Cell c;
+    c.I = I;
+    c.i = (int)(I / p.sx);
+    int64_t r = I - (int64_t)c.i * p.sx;
+    c.j = (int)(r / p.nz);
+    c.k = (int)(r - (int64_t)c.j * p.nz);
+    return c;
+}
+template <int A>
+__device__ __forceinline__ int64_t nb(const PhysP &p, const Cell &c, int s) {
+    int t = A == 0 ? c.i : (A == 1 ? c.j : c.k);
+    int L = A == 0 ? p.nx : (A == 1 ? p.ny : p.nz);
+    int64_t st = A == 0 ? p.sx : (A == 1 ? p.sy : 1);
+    int u = t + s;
+    u = u < 0 ? 0 : (u > L - 1 ? L - 1 : u);
+    return c.I + (int64_t)(u - t) * st;
+}
+
+// 10 components of pi^{mu nu}: index of (mu, nu) in the idx() order (helper.h:31)
+__device__ __forceinline__ int IDX(int i, int j) {
+    return (i < j) ? ((7 * i + 2 * j - i * i) / 2) : ((7 * j + 2 * i - j * j) / 2);
+}
+__constant__ const int PI_ROW[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 8}, {3, 6, 8, 9}};
+
+// ---------------------------------------------------------------------------
+// kt1d (helper.h:179-242): ideal KT flux difference for 5 stencil cells
+// ---------------------------------------------------------------------------
+template <int A>
+__device__ __forceinline__ float4 kt1d_ideal(const PhysP &p, const Eos &eos, const float4 c[5]) {
+    float pr_im1 = eos.P(c[1].x), pr_i = eos.P(c[2].x), pr_ip1 = eos.P(c[3].x), pr_ip2 = eos.P(c[4].x);
+    float4 Tm1 = t0m(c[1], pr_im1), Ti = t0m(c[2], pr_i), Tp1 = t0m(c[3], pr_ip1), Tp2 = t0m(c[4], pr_ip2);
+    float DA0[4], DA1[4], AL[4], AR[4], src[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        DA0[q] = slope(comp(Tm1, q), comp(Ti, q), comp(Tp1, q));
+        DA1[q] = slope(comp(Ti, q), comp(Tp1, q), comp(Tp2, q));
+        AL[q] = comp(Ti, q) + 0.5f * DA0[q];
+        AR[q] = comp(Tp1, q) - 0.5f * DA1[q];
+    }
+    float pr_half = 0.5f * (pr_ip1 + pr_i);
+    float vh = 0.5f * (comp(c[2], A + 1) + comp(c[3], A + 1));
+    float lam = max_prop_speed(p, eos, half4(c[2], c[3]), vh);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float g0 = q == 0 ? 1.0f : 0.0f;
+        float ga = q == A + 1 ? -1.0f : 0.0f;
+        float Jp = (AR[q] + pr_half * g0) * vh - pr_half * ga;
+        float Jm = (AL[q] + pr_half * g0) * vh - pr_half * ga;
+        src[q] = 0.5f * (Jp + Jm) - 0.5f * lam * (AR[q] - AL[q]);
+    }
+    float pr_im2 = eos.P(c[0].x);
+    float4 Tm2 = t0m(c[0], pr_im2);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        DA1[q] = DA0[q];
+        DA0[q] = slope(comp(Tm2, q), comp(Tm1, q), comp(Ti, q));
+        AL[q] = comp(Tm1, q) + 0.5f * DA0[q];
+        AR[q] = comp(Ti, q) - 0.5f * DA1[q];
+    }
+    pr_half = 0.5f * (pr_im1 + pr_i);
+    vh = 0.5f * (comp(c[1], A + 1) + comp(c[2], A + 1));
+    lam = max_prop_speed(p, eos, half4(c[2], c[1]), vh);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float g0 = q == 0 ? 1.0f : 0.0f;
+        float ga = q == A + 1 ? -1.0f : 0.0f;
+        float Jp = (AR[q] + pr_half * g0) * vh - pr_half * ga;
+        float Jm = (AL[q] + pr_half * g0) * vh - pr_half * ga;
+        src[q] -= 0.5f * (Jp + Jm) - 0.5f * lam * (AR[q] - AL[q]);
+    }
+    return make_float4(src[0], src[1], src[2], src[3]);
+}
+
+template <int A>
+__device__ __forceinline__ void ideal_axis(const PhysP &p, const Eos &eos, const StageP &st, const Cell &cl,
+                                           const float *__restrict__ ev, float s[4]) {
+    float4 c[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) c[q] = ld4(ev, p.n, nb<A>(p, cl, q - 2));
+    float4 k = kt1d_ideal<A>(p, eos, c);
+    float kk[4] = {k.x, k.y, k.z, k.w};
+    float rdx = A == 0 ? p.r_dx : (A == 1 ? p.r_dy : st.r_taudz);
+    bool riemann = p.flags & 1;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        if (riemann) s[m] = s[m] - kk[m] * (A == 0 ? p.r_dx : (A == 1 ? p.r_dy : p.r_dz));
+        else s[m] = s[m] - (st.tau * kk[m]) * rdx;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CLIdeal.stepUpdate(step): christoffel + 3 sweeps + update_ev, one cell per thread
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ideal_stage(PhysP p, EosP E, StageP st,
+                                                     const float *__restrict__ ev_st,
+                                                     const float *ev1, float *Src, float *ev_out) {
+    int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= p.n) return;
+    Eos eos(E, E.tab);
+    Cell cl = cell_of(p, I);
+    const bool riemann = p.flags & 1;
+    float s[4];
+    // kt_src_christoffel (kernel_ideal.cl:3-36)
+    if (st.step == 1) { s[0] = s[1] = s[2] = s[3] = 0.0f; }
+    else { float4 t = ld4(Src, p.n, I); s[0] = t.x; s[1] = t.y; s[2] = t.z; s[3] = t.w; }
+    {
+        float4 e = ld4(ev_st, p.n, I);
+        float pressure = eos.P(e.x);
+        float u0 = gamma3(e.y, e.z, e.w);
+        float Tzz = (e.x + pressure) * u0 * u0 * e.w * e.w + pressure;
+        float Ttz = (e.x + pressure) * u0 * u0 * e.w;
+        if (!riemann) { s[0] = s[0] - Tzz; s[1] = s[1] - 0.0f; s[2] = s[2] - 0.0f; s[3] = s[3] - Ttz; }
+    }
+    ideal_axis<0>(p, eos, st, cl, ev_st, s);
+    ideal_axis<1>(p, eos, st, cl, ev_st, s);
+    ideal_axis<2>(p, eos, st, cl, ev_st, s);
+    st4(Src, p.n, I, make_float4(s[0], s[1], s[2], s[3]));
+    // update_ev (kernel_ideal.cl:165-231)
+    float4 e = ld4(ev1, p.n, I);
+    float pressure = eos.P(e.x);
+    float u0 = gamma3(e.y, e.z, e.w);
+    float umu[4] = {u0 * 1.0f, u0 * e.y, u0 * e.z, u0 * e.w};
+    float T0m[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float g0 = q == 0 ? 1.0f : 0.0f;
+        float t = (e.x + pressure) * u0 * umu[q] - pressure * g0;
+        T0m[q] = riemann ? t : t * st.old_time;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) T0m[q] = T0m[q] + (s[q] * p.dt) * st.r_step;
+    float T00, T01, T02, T03;
+    if (riemann) {
+        T00 = rmax(CV_ACU, T0m[0]);
+        T01 = fabsf(T0m[1]) < CV_ACU ? 0.0f : T0m[1];
+        T02 = fabsf(T0m[2]) < CV_ACU ? 0.0f : T0m[2];
+        T03 = fabsf(T0m[3]) < CV_ACU ? 0.0f : T0m[3];
+    } else {
+        T00 = rmax(CV_ACU, T0m[0]) * st.r_tau;
+        T01 = fabsf(T0m[1]) < CV_ACU ? 0.0f : T0m[1] * st.r_tau;
+        T02 = fabsf(T0m[2]) < CV_ACU ? 0.0f : T0m[2] * st.r_tau;
+        T03 = fabsf(T0m[3]) < CV_ACU ? 0.0f : T0m[3] * st.r_tau;
+    }
+    float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
+    const float SC = 0.999f;
+    if (M > T00) {
+        float sc = (SC * T00) * rcp(M);
+        T01 *= sc; T02 *= sc; T03 *= sc;
+        M = SC * T00;
+    }
+    float ed_find;
+    if ((p.flags & 8) && M > p.fp64_vmin * T00)
+        ed_find = (float)root_newton_d(E, E.tab, (double)T00, (double)M, 0.0);
+    else
+        ed_find = root_newton(eos, T00, M, 0.0f);
+    ed_find = rmax(0.0f, ed_find);
+    float pr = eos.P(ed_find);
+    float repv = rcp(rmax(CV_ACU, T00 + pr));
+    st4(ev_out, p.n, I, make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv));
+}
+
+// ---------------------------------------------------------------------------
+// kt1d_real (helper.h:249-285)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float kt1d_real(float Q_im2, float Q_im1, float Q_i, float Q_ip1, float Q_ip2,
+                                           float v_mh, float v_ph, float lam_mh, float lam_ph) {
+    float DA0 = slope(Q_im1, Q_i, Q_ip1);
+    float DA1 = slope(Q_i, Q_ip1, Q_ip2);
+    float AL = Q_i + 0.5f * DA0;
+    float AR = Q_ip1 - 0.5f * DA1;
+    float Jp = AR * v_ph, Jm = AL * v_ph;
+    float src = 0.5f * (Jp + Jm) - 0.5f * lam_ph * (AR - AL);
+    DA1 = DA0;
+    DA0 = slope(Q_im2, Q_im1, Q_i);
+    AL = Q_im1 + 0.5f * DA0;
+    AR = Q_i - 0.5f * DA1;
+    Jp = AR * v_mh; Jm = AL * v_mh;
+    src -= 0.5f * (Jp + Jm) - 0.5f * lam_mh * (AR - AL);
+    return src;
+}
+
+// visc_src_along{x,y,z} for one cell (kernel_IS.cl:74-318)
+template <int A, bool BULK>
+__device__ __forceinline__ void is_axis(const PhysP &p, const Eos &eos, const StageP &st, const Cell &cl,
+                                        const float *__restrict__ ev, const float *__restrict__ pi,
+                                        const float *__restrict__ bk, float s[10], float &bs, float ud[4]) {
+    int64_t J[5];
+    float4 e[5];
+    float u0[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        J[q] = nb<A>(p, cl, q - 2);
+        e[q] = ld4(ev, p.n, J[q]);
+        u0[q] = gamma3(e[q].y, e[q].z, e[q].w);
+    }
+    float v_mh = 0.5f * (comp(e[1], A + 1) + comp(e[2], A + 1));
+    float v_ph = 0.5f * (comp(e[3], A + 1) + comp(e[2], A + 1));
+    float lam_mh = max_prop_speed(p, eos, half4(e[1], e[2]), v_mh);
+    float lam_ph = max_prop_speed(p, eos, half4(e[3], e[2]), v_ph);
+    float um1[4] = {u0[1] * 1.0f, u0[1] * e[1].y, u0[1] * e[1].z, u0[1] * e[1].w};
+    float ui[4] = {u0[2] * 1.0f, u0[2] * e[2].y, u0[2] * e[2].z, u0[2] * e[2].w};
+    float up1[4] = {u0[3] * 1.0f, u0[3] * e[3].y, u0[3] * e[3].z, u0[3] * e[3].w};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        float d = minmod(0.5f * (up1[c] - um1[c]), minmod(CV_THETA * (ui[c] - um1[c]), CV_THETA * (up1[c] - ui[c])));
+        if (A == 0) ud[c] = d * p.r_dx;
+        else if (A == 1) ud[c] = d * p.r_dy;
+        else {
+            float chr = ((c == 0 ? e[2].w : (c == 3 ? 1.0f : 0.0f)) * u0[2]) * st.r_tau;
+            ud[c] = d * st.r_taudz + chr;
+        }
+    }
+    float rdx = A == 0 ? p.r_dx : (A == 1 ? p.r_dy : st.r_taudz);
+#pragma unroll
+    for (int mn = 0; mn < 10; mn++) {
+        const float *pl = pi + (int64_t)mn * p.n;
+        float k = kt1d_real(u0[0] * pl[J[0]], u0[1] * pl[J[1]], u0[2] * pl[J[2]], u0[3] * pl[J[3]],
+                            u0[4] * pl[J[4]], v_mh, v_ph, lam_mh, lam_ph);
+        s[mn] = s[mn] - k * rdx;
+    }
+    if (BULK) {
+        float k = kt1d_real(u0[0] * bk[J[0]], u0[1] * bk[J[1]], u0[2] * bk[J[2]], u0[3] * bk[J[3]],
+                            u0[4] * bk[J[4]], v_mh, v_ph, lam_mh, lam_ph);
+        bs = bs - k * rdx;
+    }
+}
+
+// PiPi self coupling (GUBSER_VISC_TEST, kernel_IS.cl:322-347)
+__device__ __forceinline__ float dot4(const float *a, const float *b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+}
+__device__ float pipi(int lam, int mu, int nu, const float *pim, const float *u) {
+    const float g[4] = {1.0f, -1.0f, -1.0f, -1.0f};
+    float A[4] = {pim[IDX(0, lam)], -pim[IDX(1, lam)], -pim[IDX(2, lam)], -pim[IDX(3, lam)]};
+    float D[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) D[a][b] = (a == b ? g[a] : 0.0f) - u[a] * u[b];
+    float first = dot4(D[mu], A) * dot4(D[nu], A);
+    float temp[4] = {dot4(D[0], A), dot4(D[1], A), dot4(D[2], A), dot4(D[3], A)};
+    float second = -1.0f / 3.0f * ((mu == nu ? g[mu] : 0.0f) - u[mu] * u[nu]) * dot4(temp, A);
+    return first + second;
+}
+
+// ---------------------------------------------------------------------------
+// CLVisc.IS_stepUpdate(step): one cell per thread
+//   ev_st/pi_st/bk_st : d_ev[step], d_pi[step] (stencil)
+//   ev1, ev2          : d_ev[1], d_ev[2]; pi1, bk1 : start point d_pi[1]
+//   pi_out, bk_out    : d_pi[3-step] (may alias pi1 at step 2: read-before-write per cell)
+// ---------------------------------------------------------------------------
+template <bool BULK>
+__global__ __launch_bounds__(256) void k_is_stage(PhysP p, EosP E, StageP st,
+        const float *__restrict__ ev_st, const float *__restrict__ pi_st, const float *__restrict__ bk_st,
+        const float *ev1, const float *ev2, const float *pi1, const float *bk1,
+        const float *__restrict__ udiff, float *IS_src, float *bsrc, float *pi_out, float *bk_out, float *good) {
+    int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= p.n) return;
+    Eos eos(E, E.tab);
+    Cell cl = cell_of(p, I);
+    const int step = st.step;
+    float s[10];
+    float bs = 0.0f;
+    if (step == 1) {
+#pragma unroll
+        for (int m = 0; m < 10; m++) s[m] = 0.0f;
+    } else {
+#pragma unroll
+        for (int m = 0; m < 10; m++) s[m] = IS_src[(int64_t)m * p.n + I];
+        if (BULK) bs = bsrc[I];
+    }
+    // visc_src_christoffel (kernel_IS.cl:41-67)
+    {
+        float4 e = ld4(ev_st, p.n, I);
+        float uz = e.w * gamma3(e.y, e.z, e.w);
+        float q[10];
+#pragma unroll
+        for (int m = 0; m < 10; m++) q[m] = pi_st[(int64_t)m * p.n + I];
+        s[0] -= (2.0f * uz * q[3]) * st.r_tau;
+        s[1] -= (uz * q[6]) * st.r_tau;
+        s[2] -= (uz * q[8]) * st.r_tau;
+        s[3] -= (uz * (q[0] + q[9])) * st.r_tau;
+        s[6] -= (uz * q[1]) * st.r_tau;
+        s[8] -= (uz * q[2]) * st.r_tau;
+        s[9] -= (2.0f * uz * q[3]) * st.r_tau;
+    }
+    float udx[4], udy[4], udz[4];
+    is_axis<0, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s, bs, udx);
+    is_axis<1, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s, bs, udy);
+    is_axis<2, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s, bs, udz);
+
+    // update_pimn (kernel_IS.cl:501-709)
+    float4 e1 = ld4(ev1, p.n, I), e2 = ld4(ev2, p.n, I);
+    float g1 = gamma3(e1.y, e1.z, e1.w), g2 = gamma3(e2.y, e2.z, e2.w);
+    float u_old[4] = {g1 * 1.0f, g1 * e1.y, g1 * e1.z, g1 * e1.w};
+    float u_new[4] = {g2 * 1.0f, g2 * e2.y, g2 * e2.z, g2 * e2.w};
+    float udt[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) udt[c] = (u_new[c] - u_old[c]) * p.r_dt;
+    if (step == 1) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            float ud_ = udiff[(int64_t)c * p.n + I];
+            udt[c] = ud_ * p.r_dt;
+            u_new[c] = u_old[c] + ud_;
+        }
+    }
+    float da[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) { da[c][0] = udt[c]; da[c][1] = udx[c]; da[c][2] = udy[c]; da[c][3] = udz[c]; }
+    float u[4], DU[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) u[c] = step == 1 ? u_old[c] : u_new[c];
+#pragma unroll
+    for (int c = 0; c < 4; c++) DU[c] = dot4(u, da[c]);
+    float ed_step = step == 1 ? e1.x : e2.x;
+    float theta = udt[0] + udx[1] + udy[2] + udz[3];
+    float4 cpTs = eos.raw(ed_step);
+    float temperature = cpTs.z;
+    float local_etaos = etaos(p, temperature);
+    float etav = local_etaos * cpTs.w * CV_HBARC;
+    float one_over_taupi = temperature * rcp(3.0f * rmax(CV_ACU, local_etaos) * CV_HBARC);
+    float coef_pipi = 0.0f;
+    const bool gubser = p.flags & 2;
+    if (gubser) {
+        float q4 = sqrtf(sqrtf(ed_step));
+        float ETAOS = p.etaos_ymin;
+        float etavH = (ETAOS * 4.0f) * p.r_3;
+        float taupiH = (p.lam1 * p.lam1 * etavH) * p.r_3;
+        one_over_taupi = rcp(taupiH * rcp(q4));
+        coef_pipi = p.lam1 * rcp(ed_step);
+        etav = ((ETAOS * 4.0f) * p.r_3) * (sqrtf(ed_step) * q4);
+    }
+    float pi2[10];
+#pragma unroll
+    for (int m = 0; m < 10; m++) pi2[m] = pi_st[(int64_t)m * p.n + I];
+    const float gd[4] = {1.0f, -1.0f, -1.0f, -1.0f};
+    float max_abs = 0.0f;
+    float out[10];
+    float rden = rcp(u_new[0] + p.dt * one_over_taupi);
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) {
+#pragma unroll
+        for (int nu = mu; nu < 4; nu++) {
+            float gmunu = mu == nu ? gd[mu] : 0.0f;
+            float sigma = gd[mu] * da[nu][mu] + gd[nu] * da[mu][nu] - (u[mu] * DU[nu] + u[nu] * DU[mu])
+                          - 2.0f / 3.0f * (gmunu - u[mu] * u[nu]) * theta;
+            int mn = IDX(mu, nu);
+            if (u[0] > 10000.0f) sigma = 0.0f;
+            float piNS = etav * sigma;
+            float pi_old = pi1[(int64_t)mn * p.n + I] * u_old[0];
+            float src = (-pi2[mn] * theta) * p.r_3 - (pi2[mn] * u[0]) * st.r_tau;
+#pragma unroll
+            for (int l = 0; l < 4; l++)
+                src -= (u[mu] * pi2[IDX(nu, l)] + u[nu] * pi2[IDX(mu, l)]) * DU[l] * gd[l];
+            if (gubser)
+                src -= one_over_taupi * coef_pipi * (pipi(0, mu, nu, pi2, u) - pipi(1, mu, nu, pi2, u)
+                                                     - pipi(2, mu, nu, pi2, u) - pipi(3, mu, nu, pi2, u));
+            s[mn] += src;
+            pi_old = (pi_old + (s[mn] * p.dt) * st.r_step + p.dt * one_over_taupi * piNS) * rden;
+            if (fabsf(pi_old) > max_abs) max_abs = fabsf(pi_old);
+            out[mn] = pi_old;
+        }
+    }
+    float bout = 0.0f;
+    if (BULK) {
+        float zs = zetaos(p, temperature);
+        float zetav = zs * cpTs.w * CV_HBARC;
+        float oot_b = temperature * rcp(3.0f * rmax(CV_ACU, zs) * CV_HBARC);
+        float b2 = bk_st[I];
+        float bsrc_ = (b2 * theta) * p.r_3 - (b2 * u[0]) * st.r_tau;
+        bs += bsrc_;
+        float b_old = bk1[I] * u_old[0];
+        bout = (b_old + (bs * p.dt) * st.r_step - p.dt * oot_b * zetav * theta) * rcp(u_new[0] + p.dt * oot_b);
+        if (fabsf(bout) > max_abs) max_abs = fabsf(bout);
+    }
+    float pre = eos.P(e2.x);
+    float T00 = (e2.x + pre) * u_new[0] * u_new[0] - pre;
+    float gc = 1.0f;
+    if (max_abs > T00) {
+#pragma unroll
+        for (int m = 0; m < 10; m++) out[m] = 0.0f;
+        bout = 0.0f;
+        gc = 0.0f;
+    }
+#pragma unroll
+    for (int m = 0; m < 10; m++) {
+        IS_src[(int64_t)m * p.n + I] = s[m];
+        pi_out[(int64_t)m * p.n + I] = out[m];
+    }
+    if (BULK) { bsrc[I] = bs; bk_out[I] = bout; }
+    good[I] = gc;
+}
+
+// ---------------------------------------------------------------------------
+// kt1d_visc (kernel_visc.cl:60-127) for one cell and axis
+// ---------------------------------------------------------------------------
+template <int A, bool BULK>
+__device__ __forceinline__ void kt_axis(const PhysP &p, const Eos &eos, const StageP &st, const Cell &cl,
+                                        const float *__restrict__ ev, const float *__restrict__ pi,
+                                        const float *__restrict__ bk, float s[4]) {
+    float4 e[5];
+    float pr[5], Q[5][4], pim0[5][4], pimi[5][4];
+    const float tau = st.tau;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        int64_t J = nb<A>(p, cl, i - 2);
+        e[i] = ld4(ev, p.n, J);
+        pr[i] = eos.P(e[i].x) + (BULK ? bk[J] : 0.0f);
+        float4 T = t0m(e[i], pr[i]);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            pim0[i][q] = pi[(int64_t)PI_ROW[0][q] * p.n + J];
+            pimi[i][q] = pi[(int64_t)PI_ROW[A + 1][q] * p.n + J];
+            Q[i][q] = tau * (comp(T, q) + pim0[i][q]);
+        }
+    }
+    float vip = 0.5f * (comp(e[2], A + 1) + comp(e[3], A + 1));
+    float vim = 0.5f * (comp(e[1], A + 1) + comp(e[2], A + 1));
+    float DA0[4], DA1[4], AL[4], AR[4], src[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        DA0[q] = slope(Q[1][q], Q[2][q], Q[3][q]);
+        DA1[q] = slope(Q[2][q], Q[3][q], Q[4][q]);
+        AL[q] = Q[2][q] + 0.5f * DA0[q];
+        AR[q] = Q[3][q] - 0.5f * DA1[q];
+    }
+    float pr_half = 0.5f * (pr[2] + pr[3]);
+    float lam = max_prop_speed(p, eos, half4(e[2], e[3]), vip);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float g0 = q == 0 ? 1.0f : 0.0f;
+        float ga = q == A + 1 ? -1.0f : 0.0f;
+        float p0h = 0.5f * (pim0[2][q] + pim0[3][q]);
+        float pih = 0.5f * (pimi[2][q] + pimi[3][q]);
+        float Jp = (AR[q] + pr_half * tau * g0 - tau * p0h) * vip - pr_half * tau * ga + tau * pih;
+        float Jm = (AL[q] + pr_half * tau * g0 - tau * p0h) * vip - pr_half * tau * ga + tau * pih;
+        src[q] = 0.5f * (Jp + Jm) - 0.5f * lam * (AR[q] - AL[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        DA1[q] = DA0[q];
+        DA0[q] = slope(Q[0][q], Q[1][q], Q[2][q]);
+        AL[q] = Q[1][q] + 0.5f * DA0[q];
+        AR[q] = Q[2][q] - 0.5f * DA1[q];
+    }
+    pr_half = 0.5f * (pr[1] + pr[2]);
+    lam = max_prop_speed(p, eos, half4(e[2], e[1]), vim);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float g0 = q == 0 ? 1.0f : 0.0f;
+        float ga = q == A + 1 ? -1.0f : 0.0f;
+        float p0h = 0.5f * (pim0[1][q] + pim0[2][q]);
+        float pih = 0.5f * (pimi[1][q] + pimi[2][q]);
+        float Jp = (AR[q] + pr_half * tau * g0 - tau * p0h) * vim - pr_half * tau * ga + tau * pih;
+        float Jm = (AL[q] + pr_half * tau * g0 - tau * p0h) * vim - pr_half * tau * ga + tau * pih;
+        src[q] -= 0.5f * (Jp + Jm) - 0.5f * lam * (AR[q] - AL[q]);
+    }
+    float rdx = A == 0 ? p.r_dx : (A == 1 ? p.r_dy : st.r_taudz);
+#pragma unroll
+    for (int m = 0; m < 4; m++) s[m] = s[m] - src[m] * rdx;
+}
+
+// ---------------------------------------------------------------------------
+// CLVisc.visc_stepUpdate(step): one cell per thread
+//   ev_st/pi_st/bk_st : d_ev[step], d_pi[step] (stencil)
+//   ev1               : d_ev[1] start point; pim0/bk0 : d_pi[0]; pinew/bknew : d_pi[3-step]
+//   ev_out            : d_ev[3-step] (may alias ev1 at step 2)
+//   udiff             : when non-null, receives u(ev_out) - u(ev1) (get_udiff)
+// ---------------------------------------------------------------------------
+template <bool BULK>
+__global__ __launch_bounds__(256) void k_kt_stage(PhysP p, EosP E, StageP st,
+        const float *__restrict__ ev_st, const float *__restrict__ pi_st, const float *__restrict__ bk_st,
+        const float *ev1, const float *pim0, const float *bk0, const float *pinew, const float *bknew,
+        float *Src, float *ev_out, float *udiff) {
+    int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= p.n) return;
+    Eos eos(E, E.tab);
+    Cell cl = cell_of(p, I);
+    const bool riemann = p.flags & 1;
+    float s[4];
+    if (st.step == 1) { s[0] = s[1] = s[2] = s[3] = 0.0f; }
+    else { float4 t = ld4(Src, p.n, I); s[0] = t.x; s[1] = t.y; s[2] = t.z; s[3] = t.w; }
+    // kt_src_christoffel (kernel_visc.cl:11-53)
+    {
+        float4 e = ld4(ev_st, p.n, I);
+        float u0 = gamma3(e.y, e.z, e.w);
+        float bulkpi = BULK ? bk_st[I] : 0.0f;
+        float pressure = eos.P(e.x) + bulkpi;
+        float Tzz = (e.x + pressure) * u0 * u0 * e.w * e.w + pressure + pi_st[9 * p.n + I];
+        float Ttz = (e.x + pressure) * u0 * u0 * e.w + pi_st[3 * p.n + I];
+        s[0] = s[0] - Tzz; s[1] = s[1] - 0.0f; s[2] = s[2] - 0.0f; s[3] = s[3] - Ttz;
+        if (riemann) {
+            float Ttt = (e.x + pressure) * u0 * u0 - pressure;
+            float Ttx = (e.x + pressure) * u0 * u0 * e.y;
+            float Tty = (e.x + pressure) * u0 * u0 * e.z;
+            s[0] = s[0] + (Tzz + Ttt); s[1] = s[1] + Ttx; s[2] = s[2] + Tty; s[3] = s[3] + 3.0f * Ttz;
+        }
+    }
+    kt_axis<0, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s);
+    kt_axis<1, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s);
+    kt_axis<2, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s);
+    st4(Src, p.n, I, make_float4(s[0], s[1], s[2], s[3]));
+    // update_ev (kernel_visc.cl:322-393)
+    float4 e = ld4(ev1, p.n, I);
+    float pressure = eos.P(e.x) + (BULK ? bk0[I] : 0.0f);
+    float u0 = gamma3(e.y, e.z, e.w);
+    float umu[4] = {u0 * 1.0f, u0 * e.y, u0 * e.z, u0 * e.w};
+    float T0m[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float g0 = q == 0 ? 1.0f : 0.0f;
+        T0m[q] = ((e.x + pressure) * u0 * umu[q] - pressure * g0 + pim0[(int64_t)q * p.n + I]) * st.old_time;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        T0m[q] = T0m[q] + (s[q] * p.dt) * st.r_step - pinew[(int64_t)q * p.n + I] * st.tau;
+    float T00 = rmax(CV_ACU, T0m[0]) * st.r_tau;
+    float T01 = fabsf(T0m[1]) < CV_ACU ? 0.0f : T0m[1] * st.r_tau;
+    float T02 = fabsf(T0m[2]) < CV_ACU ? 0.0f : T0m[2] * st.r_tau;
+    float T03 = fabsf(T0m[3]) < CV_ACU ? 0.0f : T0m[3] * st.r_tau;
+    float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
+    const float SC = 0.999f;
+    if (M > T00) {
+        float sc = (SC * T00) * rcp(M);
+        T01 *= sc; T02 *= sc; T03 *= sc;
+        M = SC * T00;
+    }
+    float bnew = BULK ? bknew[I] : 0.0f;
+    float ed_find;
+    if ((p.flags & 8) && M > p.fp64_vmin * T00)
+        ed_find = (float)root_newton_d(E, E.tab, (double)T00, (double)M, (double)bnew);
+    else
+        ed_find = root_newton(eos, T00, M, bnew);
+    if (ed_find < CV_ACU) { T01 = 0.0f; T02 = 0.0f; T03 = 0.0f; }
+    ed_find = rmax(0.0f, ed_find);
+    float pr = eos.P(ed_find) + bnew;
+    float repv = rcp(rmax(CV_ACU, T00 + pr));
+    float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
+    st4(ev_out, p.n, I, en);
+    if (udiff) {
+        float gn = gamma3(en.y, en.z, en.w);
+        float un[4] = {gn * 1.0f, gn * en.y, gn * en.z, gn * en.w};
+        float uo[4] = {u0 * 1.0f, u0 * e.y, u0 * e.z, u0 * e.w};
+#pragma unroll
+        for (int c = 0; c < 4; c++) udiff[(int64_t)c * p.n + I] = un[c] - uo[c];
+    }
+}
+
+// visc_initialize (kernel_IS.cl:10-36)
+__global__ void k_visc_init(PhysP p, EosP E, float tau, float r_tau, const float *__restrict__ ev,
+                            float *pi1, float *good, float *udiff) {
+    int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= p.n) return;
+    Eos eos(E, E.tab);
+    float4 cpTs = eos.raw(ev[I]);
+    float etav = etaos(p, cpTs.z) * cpTs.w * CV_HBARC;
+    float tmp = (2.0f / 3.0f * etav) * r_tau;
+    pi1[4 * p.n + I] = tmp;
+    pi1[7 * p.n + I] = tmp;
+    pi1[9 * p.n + I] = -2.0f * tmp;
+    for (int c = 0; c < 4; c++) udiff[c * p.n + I] = 0.0f;
+    good[I] = 1.0f;
+}
+
+// get_udiff (kernel_IS.cl:714-724)
+__global__ void k_udiff(PhysP p, const float *__restrict__ ev0, const float *__restrict__ ev1, float *udiff) {
+    int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= p.n) return;
+    float4 a = ld4(ev0, p.n, I), b = ld4(ev1, p.n, I);
+    float ga = gamma3(a.y, a.z, a.w), gb = gamma3(b.y, b.z, b.w);
+    float ua[4] = {ga * 1.0f, ga * a.y, ga * a.z, ga * a.w};
+    float ub[4] = {gb * 1.0f, gb * b.y, gb * b.z, gb * b.w};
+    for (int c = 0; c < 4; c++) udiff[c * p.n + I] = ub[c] - ua[c];
+}
+
+// max of e (reduction_stage1 + host max, kernel_reduction.cl:7-39): block max by
+// wavefront shuffles, one atomic per block on the order-preserving integer image.
+__device__ __forceinline__ unsigned int f2ord(float f) {
+    unsigned int u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__global__ __launch_bounds__(256) void k_max_ed(int64_t n, const float *__restrict__ e, unsigned int *out) {
+    float m = -3.402823466e+38f;
+    for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n; I += (int64_t)gridDim.x * blockDim.x)
+        m = e[I] > m ? e[I] : m;
+    for (int off = 32; off > 0; off >>= 1) {
+        float o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    __shared__ float wm[4];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wm[w] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); q++) m = wm[q] > m ? wm[q] : m;
+        atomicMax(out, f2ord(m));
+    }
+}
+
+}  // namespace cv

@@ -12,12 +12,16 @@ Two table kinds exist:
   row/col arithmetic and an analytic fit above the table
   (pyvisc/kernel/eos_table.h:39-100).  ``Eos('ideal_gas')`` builds this one
   exactly as pyvisc/eos/eos.py:42-59 does.
-* ``'log'`` -- a log2-spaced table (2000 points on 1e-4..1e3 GeV/fm^3 by
-  default) small enough (32 KB) to sit in LDS on the device.  Values are
-  interpolated linearly in log2(e); below the first point P, T and s fall
-  linearly to zero (cs2 held); above the last interval the last interval is
-  extrapolated.  ``Eos('crossover')`` builds the smooth synthetic crossover EOS
-  used by the 385x385x115 viscous workload.
+* ``'octave'`` -- a logarithmically spaced table whose nodes sit uniformly
+  inside each binary octave of e: 2^k (1 + j/J), k in [kmin, kmax), J = 2^log2j
+  (default 2^-14..2^10 GeV/fm^3, J = 64: 1537 nodes, 24.6 KB, relative spacing
+  1/128..1/64), small enough to sit in LDS on the device.  The node and the
+  interpolation weight are read off the binary exponent and mantissa of e, so the
+  lookup needs no log and no division and is reproducible bit for bit on any
+  IEEE device.  Values are interpolated linearly in e; below 2^kmin P, T and s
+  fall linearly to zero (cs2 held); at or above 2^kmax they scale linearly with e.
+  ``Eos('crossover')`` builds the smooth synthetic crossover EOS used by the
+  385x385x115 viscous workload.
 
 The reference's lattice tables (s95p-PCE, HotQCD, Wuppertal-Budapest) are not
 shipped with the reference repository, so only the two closed-form EOS above
@@ -36,21 +40,32 @@ HBARC = 0.1973269  # pyvisc/eos/eos.py:44
 @dataclass
 class EosTable:
     """Device-ready EOS description (host arrays; uploaded by the driver)."""
-    kind: str                 # 'rowcol' or 'log'
+    kind: str                 # 'rowcol' or 'octave'
     rgba: np.ndarray          # float32, shape (nrow, ncol, 4) or (n, 4)
     nrow: int = 0
     ncol: int = 0
     num_ed: int = 0
     ed_start: float = 0.0
     ed_step: float = 0.0
-    log2_emin: float = 0.0
-    inv_dlog2: float = 0.0
-    n: int = 0
+    kmin: int = 0
+    kmax: int = 0
+    log2j: int = 0
     cs2_max: float = 1.0 / 3.0
+    cs2_min: float = 1.0 / 3.0
 
     @property
     def kind_id(self) -> int:
         return 0 if self.kind == 'rowcol' else 1
+
+
+def octave_nodes(kmin, kmax, log2j):
+    """Energy densities of the octave-table nodes: 2^k (1 + j/J) for k in [kmin, kmax),
+    j in [0, J) with J = 2^log2j, then 2^kmax (node n = (k - kmin) J + j)."""
+    J = 1 << log2j
+    k = np.arange(kmin, kmax, dtype=np.float64)
+    j = np.arange(J, dtype=np.float64)
+    e = (np.exp2(k)[:, None] * (1.0 + j[None, :] / J)).ravel()
+    return np.append(e, 2.0 ** kmax)
 
 
 class Eos(object):
@@ -95,9 +110,9 @@ class Eos(object):
         self.default_shape = (200, 1000)
 
     # -- synthetic crossover ----------------------------------------------------------
-    def crossover(self, kind='log', npoints=2000, emin=1.0e-4, emax=1.0e3,
+    def crossover(self, kind='octave', kmin=-14, kmax=10, log2j=6,
                   cs2_dip=0.2, e_c=0.35, width=1.2, dof=169.0 / 4.0,
-                  ed_step=0.002, nrow=100, ncol=1555):
+                  ed_step=0.002, nrow=100, ncol=1555, emax=1.0e3):
         """Smooth crossover: cs2(e) = 1/3 - cs2_dip * exp(-ln(e/e_c)^2 / (2 width^2)).
 
         P(e) = int_0^e cs2 de'; d ln T = cs2 de / (e + P) with T matched to a
@@ -139,15 +154,13 @@ class Eos(object):
         self.f_cs2 = interp_lnE(cs2)
         self.f_ed = lambda Tq: np.exp(np.interp(np.log(np.asarray(Tq, dtype=np.float64)), lnT, x))
         self.kind = kind
-        if kind == 'log':
-            l2min, l2max = math.log2(emin), math.log2(emax)
-            self.log2_emin = l2min
-            self.dlog2 = (l2max - l2min) / (npoints - 1)
-            self.ed = np.exp2(l2min + self.dlog2 * np.arange(npoints))
-            self.num_of_ed = npoints
-            self.ed_start = emin
+        if kind == 'octave':
+            self.kmin, self.kmax, self.log2j = kmin, kmax, log2j
+            self.ed = octave_nodes(kmin, kmax, log2j)
+            self.num_of_ed = len(self.ed)
+            self.ed_start = 2.0 ** kmin
             self.ed_step = 0.0
-            self.default_shape = (npoints,)
+            self.default_shape = (len(self.ed),)
         else:
             self.ed = ed_step * np.arange(nrow * ncol, dtype=np.float64)
             self.num_of_ed = nrow * ncol
@@ -168,11 +181,11 @@ class Eos(object):
         appended to it for callers that inspect them.
         """
         src = np.stack([self.cs2, self.pr, self.T, self.s], axis=-1).astype(np.float32)
-        if self.kind == 'log':
-            tab = EosTable(kind='log', rgba=np.ascontiguousarray(src.reshape(-1, 4)),
-                           n=int(self.num_of_ed), log2_emin=float(self.log2_emin),
-                           inv_dlog2=float(1.0 / self.dlog2),
-                           cs2_max=float(np.max(src[:, 0])))
+        if self.kind == 'octave':
+            tab = EosTable(kind='octave', rgba=np.ascontiguousarray(src.reshape(-1, 4)),
+                           kmin=int(self.kmin), kmax=int(self.kmax), log2j=int(self.log2j),
+                           num_ed=int(self.num_of_ed),
+                           cs2_max=float(np.max(src[:, 0])), cs2_min=float(np.min(src[:, 0])))
         else:
             if nrow is None or ncol is None:
                 nrow, ncol = self.default_shape
@@ -182,7 +195,7 @@ class Eos(object):
             tab = EosTable(kind='rowcol', rgba=np.ascontiguousarray(src.reshape(nrow, ncol, 4)),
                            nrow=int(nrow), ncol=int(ncol), num_ed=int(self.num_of_ed),
                            ed_start=float(self.ed_start), ed_step=float(self.ed_step),
-                           cs2_max=float(np.max(src[..., 0])))
+                           cs2_max=float(np.max(src[..., 0])), cs2_min=float(np.min(src[..., 0])))
         if compile_options is not None:
             compile_options.append('-D EOS_ED_START={}f'.format(self.ed_start))
             compile_options.append('-D EOS_ED_STEP={}f'.format(self.ed_step))

@@ -1,0 +1,146 @@
+/*
+ * clvisc_amd.h -- C-ABI of the MI355X (gfx950) KT + Israel-Stewart hydro step.
+ *
+ * This is the drop-in boundary for the reference's hydro path.  In the reference,
+ * Python driver classes (pyvisc/ideal.py CLIdeal, pyvisc/visc.py CLVisc) enqueue
+ * OpenCL kernels through pyopencl; here the same drivers (clvisc_amd.CLIdeal /
+ * clvisc_amd.CLVisc) call these entry points through ctypes.  Each entry point
+ * below names the reference code it replaces.
+ *
+ * Conventions
+ *   - Every call returns 0 on success and a negative CV_ERR_* code on failure;
+ *     cv_last_error() returns a message for the calling thread.
+ *   - Calls are synchronous at return unless named *_async.
+ *   - Host arrays are caller-owned, contiguous, in the reference's AoS layouts:
+ *       ev    float[N][4]   (e, vx, vy, veta)  with veta = tau * deta/dtau
+ *       pi    float[N][10]  pi^{mu nu} in the order 00,01,02,03,11,12,13,22,23,33,
+ *                           eta components tilde-scaled (tau pi^{mu eta}, tau^2 pi^{eta eta})
+ *       N = NX*NY*NZ, cell I = (i*NY + j)*NZ + k.
+ *     The device keeps fields as structure-of-arrays planes and converts in get/set.
+ *   - Arithmetic is FP32 and follows one contract: a/b is evaluated as a * RN(1/b),
+ *     sqrt is correctly rounded, no FMA contraction.  With the octave EOS table the
+ *     results are reproducible bit for bit by an IEEE CPU evaluating the same
+ *     operations.
+ */
+#ifndef CLVISC_AMD_H
+#define CLVISC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CV_OK              0
+#define CV_ERR_ARG        -1
+#define CV_ERR_HIP        -2
+#define CV_ERR_STATE      -3
+#define CV_ERR_CAPACITY   -4
+#define CV_ERR_UNSUPPORTED -5
+
+/* flags of cv_params.flags (compile-time -D switches of the reference) */
+#define CV_FLAG_RIEMANN_TEST     0x1   /* -D RIEMANN_TEST (pyvisc/ideal.py:164-165) */
+#define CV_FLAG_GUBSER_VISC_TEST 0x2   /* -D GUBSER_VISC_TEST (pyvisc/visc.py:111-112) */
+#define CV_FLAG_BULK             0x4   /* bulk viscous pressure Pi (no reference counterpart) */
+#define CV_FLAG_FP64_ROOT        0x8   /* FP64 root solve where M > fp64_root_vmin * T00 */
+
+/* Hydro parameters: the reference's -D compile options (pyvisc/ideal.py:146-171,
+ * pyvisc/visc.py:108-121) as run-time values. */
+typedef struct cv_params {
+    int32_t nx, ny, nz;
+    int32_t flags;
+    double  tau0, dt;                 /* TAU0, DT as the config holds them */
+    float   dx, dy, dz;
+    float   etaos_xmin, etaos_ymin, etaos_left_slop, etaos_right_slop;
+    float   lam1;
+    float   zetaos_max, zetaos_tpeak, zetaos_width;   /* CV_FLAG_BULK */
+    float   fp64_root_vmin;                           /* CV_FLAG_FP64_ROOT */
+    float   efrz;                                     /* freeze-out energy density */
+    int32_t nxskip, nyskip, nzskip;
+    int32_t reserved;
+} cv_params;
+
+/* EOS table (replaces Eos.create_table's image2d, pyvisc/eos/eos.py:184-203).
+ * kind 0: the reference's uniform row/col table (eos_table.h:39-100).
+ * kind 1: octave table, nodes 2^k (1 + j/2^log2j), k in [kmin, kmax), plus 2^kmax. */
+typedef struct cv_eos {
+    int32_t kind;
+    int32_t nrow, ncol, num_ed;       /* kind 0 */
+    float   ed_start, ed_step;        /* kind 0 */
+    int32_t kmin, kmax, log2j;        /* kind 1 */
+    int32_t n_entries;                /* number of RGBA entries in rgba */
+    const float *rgba;                /* host (cs2, P, T, s) x n_entries, copied at create */
+} cv_eos;
+
+typedef struct cv_handle cv_handle;
+
+/* Field ids for get/set/copy (the reference's device buffers) */
+enum cv_field {
+    CV_EV0 = 0, CV_EV1 = 1, CV_EV2 = 2,        /* ideal.d_ev[0..2]   (ideal.py:102-104) */
+    CV_PI0 = 3, CV_PI1 = 4, CV_PI2 = 5,        /* visc.d_pi[0..2]    (visc.py:39-41)    */
+    CV_BULK0 = 6, CV_BULK1 = 7, CV_BULK2 = 8,  /* bulk Pi (extension), float[N]          */
+    CV_SRC = 9,                                /* ideal.d_Src        (ideal.py:105)     */
+    CV_IS_SRC = 10,                            /* visc.d_IS_src      (visc.py:43)       */
+    CV_UDIFF = 11,                             /* visc.d_udiff       (visc.py:83)       */
+    CV_GOODCELL = 12,                          /* visc.d_goodcell    (visc.py:87-88)    */
+    CV_NFIELDS = 13
+};
+
+/* Last error message of the calling thread (never NULL). */
+const char *cv_last_error(void);
+
+/* Library / device info: fills name (>= 64 bytes) and compute units. */
+int cv_device_info(int device, char *name, int name_len, int32_t *cus);
+
+/* Construction = CLIdeal.__init__ / CLVisc.__init__ buffer setup (ideal.py:30-119,
+ * visc.py:24-104) on HIP device `device`.  Allocates all fields zeroed; viscous != 0
+ * also allocates pi / IS_src / udiff / goodcell.  Does not run any kernel. */
+int cv_create(const cv_params *p, const cv_eos *eos, int device, int viscous, cv_handle **out);
+int cv_destroy(cv_handle *h);
+
+/* Host <-> device copies of a field in the reference's AoS layout
+ * (cl.enqueue_copy on d_ev[k] / d_pi[k], e.g. load_ini ideal.py:138-144). */
+int cv_set_field(cv_handle *h, int field, const float *host, int64_t nfloats);
+int cv_get_field(cv_handle *h, int field, float *host, int64_t nfloats);
+int cv_copy_field(cv_handle *h, int dst, int src);            /* device-to-device */
+
+/* visc_initialize kernel (kernel_IS.cl:10-36) as CLVisc.IS_initialize (visc.py:143-149). */
+int cv_visc_initialize(cv_handle *h, float tau);
+
+/* CLIdeal.stepUpdate(step) at proper time tau (ideal.py:211-255): kt_src_christoffel,
+ * kt_src_along{x,y,z}, update_ev of kernel_ideal.cl, fused into one launch. */
+int cv_ideal_stage(cv_handle *h, int step, float tau);
+
+/* CLVisc.IS_stepUpdate(step) (visc.py:287-314): visc_src_christoffel,
+ * visc_src_along{x,y,z}, update_pimn of kernel_IS.cl, fused into one launch. */
+int cv_is_stage(cv_handle *h, int step, float tau);
+
+/* CLVisc.visc_stepUpdate(step) (visc.py:232-283): kt_src_christoffel, kt_src_along*,
+ * update_ev of kernel_visc.cl, fused into one launch. */
+int cv_visc_stage(cv_handle *h, int step, float tau);
+
+/* CLVisc.update_udiff(d_ev_a, d_ev_b) (visc.py:352-357, get_udiff kernel_IS.cl:714-724). */
+int cv_update_udiff(cv_handle *h, int ev_a, int ev_b);
+
+/* nsteps full RK2 steps of the evolve loops starting at loop index loop0
+ * (ideal: ideal.py:341-344; viscous: visc.py:527-544, including the d_pi[0]/d_ev[0]
+ * copies and update_udiff).  tau after the call is TAU0 + (loop0+nsteps)*DT rounded
+ * to float.  Work is queued on the handle's stream; the call returns once queued. */
+int cv_ideal_steps_async(cv_handle *h, int64_t loop0, int64_t nsteps);
+int cv_visc_steps_async(cv_handle *h, int64_t loop0, int64_t nsteps);
+
+/* max of e over d_ev[which] (reduction_stage1 + host max, kernel_reduction.cl:7-39,
+ * ideal.py:257-262). */
+int cv_max_ed(cv_handle *h, int which, float *out);
+
+/* Wait for all queued work. */
+int cv_sync(cv_handle *h);
+
+/* Device timing of the last cv_*_steps_async call bracketed with HIP events:
+ * elapsed milliseconds between its first and last launch. */
+int cv_last_elapsed_ms(cv_handle *h, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLVISC_AMD_H */

@@ -27,7 +27,9 @@
  *
  * Extensions beyond the reference (each off unless its flag is set, and each
  * reduces exactly to the reference path when off):
- *   eos_kind = 1      log2-spaced (cs2, P, T, s) table instead of the row/col table
+ *   eos_kind = 1      octave table (uniform nodes inside each binary octave of e,
+ *                     indexed from the float's exponent and mantissa) instead of
+ *                     the row/col table
  *   bulk_on           bulk viscous pressure Pi with zeta/s(T) (absent in the reference)
  *   fp64_root         conserved-to-primitive solve in double where M > vmin*T00
  *
@@ -77,19 +79,19 @@ typedef struct orc_params {
     int32_t nx, ny, nz;
     int32_t riemann_test;      /* -D RIEMANN_TEST */
     int32_t gubser_visc_test;  /* -D GUBSER_VISC_TEST */
-    int32_t eos_kind;          /* 0: reference row/col table, 1: log2-spaced table */
+    int32_t eos_kind;          /* 0: reference row/col table, 1: octave table */
     int32_t eos_nrow, eos_ncol, eos_num_ed;   /* kind 0 */
-    int32_t eos_n;                            /* kind 1: number of points */
+    int32_t eos_kmin, eos_kmax, eos_log2j;    /* kind 1 */
     float dt, dx, dy, dz, tau0;
     float etaos_xmin, etaos_ymin, etaos_left_slop, etaos_right_slop, lam1;
     float eos_ed_start, eos_ed_step;          /* kind 0 */
-    float eos_log2_emin, eos_inv_dlog2;       /* kind 1 */
     const float *eos_table;                   /* RGBA = (cs2, P, T, s) */
     int32_t bulk_on;
     float zetaos_max, zetaos_tpeak, zetaos_width;
     int32_t fp64_root;
     float fp64_root_vmin;
     int32_t nthreads;          /* OpenMP threads, 0 = runtime default */
+    double tau0_cfg, dt_cfg;   /* TAU0, DT as the Python config holds them (double) */
 } orc_params;
 
 static int64_t ncells(const orc_params *p) { return (int64_t)p->nx * p->ny * p->nz; }
@@ -105,8 +107,8 @@ static void eos_rowcol(const orc_params *p, real eps, real out[4]) {
     const real start = RC(p->eos_ed_start);
     const int ncol = p->eos_ncol, nrow = p->eos_nrow;
     real ed_per_row = step * ncol;
-    int row = (int)(eps / ed_per_row);
-    int col = (int)((eps - start - row * ed_per_row) / step);
+    int row = (int)DIV(eps, ed_per_row);
+    int col = (int)DIV(eps - start - row * ed_per_row, step);
     real eps_low = start + row * ed_per_row + col * step;
     int c0 = col < 0 ? 0 : (col > ncol - 1 ? ncol - 1 : col);
     int r0 = row < 0 ? 0 : (row > nrow - 1 ? nrow - 1 : row);
@@ -116,40 +118,51 @@ static void eos_rowcol(const orc_params *p, real eps, real out[4]) {
     int c1 = col < 0 ? 0 : (col > ncol - 1 ? ncol - 1 : col);
     int r1 = row < 0 ? 0 : (row > nrow - 1 ? nrow - 1 : row);
     const float *hi = p->eos_table + 4 * ((int64_t)r1 * ncol + c1);
-    real r = (eps - eps_low) / step;
+    real r = DIV(eps - eps_low, step);
     for (int c = 0; c < 4; c++) out[c] = (RC(1.0f) - r) * RC(lo[c]) + r * RC(hi[c]);
 }
 
-/* Log2-spaced table (extension): point i sits at log2(e) = log2_emin + i/inv_dlog2.
- * Between points the four columns are interpolated linearly in log2(e); below the
- * first point (cs2 held) P, T, s fall linearly to zero at e = 0; above the last
- * interval the last interval is extrapolated. */
-static void eos_log(const orc_params *p, real eps, real out[4]) {
+/* Octave table (extension): nodes at e = 2^k (1 + j/J) for k in [kmin, kmax),
+ * j in [0, J), J = 2^log2j, plus a last node at 2^kmax; node n = (k-kmin) J + j.
+ * The node and the interpolation weight come straight from the binary exponent
+ * and mantissa of e (exact, no log and no division); values are interpolated
+ * linearly in e.  Below 2^kmin P, T, s fall linearly to zero at e = 0 (cs2 held);
+ * at or above 2^kmax they scale linearly with e from the last node. */
+static void eos_oct(const orc_params *p, real eps, real out[4]) {
     const float *t = p->eos_table;
-    const int n = p->eos_n;
-    const real emin = (real)exp2((double)p->eos_log2_emin);
-    if (!(eps > emin)) {
-        real r = eps > RC(0.0f) ? eps / emin : RC(0.0f);
+    const int kmin = p->eos_kmin, kmax = p->eos_kmax, lj = p->eos_log2j;
+    const int nlast = (kmax - kmin) << lj;
+    if (!(eps >= (real)ldexp(1.0, kmin))) {
+        real r = eps > RC(0.0f) ? eps * (real)ldexp(1.0, -kmin) : RC(0.0f);
         out[0] = RC(t[0]);
-        out[1] = r * RC(t[1]);
-        out[2] = r * RC(t[2]);
-        out[3] = r * RC(t[3]);
+        for (int c = 1; c < 4; c++) out[c] = r * RC(t[c]);
         return;
     }
+    if (eps >= (real)ldexp(1.0, kmax)) {
+        real r = eps * (real)ldexp(1.0, -kmax);
+        out[0] = RC(t[4 * nlast]);
+        for (int c = 1; c < 4; c++) out[c] = r * RC(t[4 * nlast + c]);
+        return;
+    }
+    int ex;
 #if ORC_DOUBLE
-    real x = (log2(eps) - RC(p->eos_log2_emin)) * RC(p->eos_inv_dlog2);
+    real fr = frexp(eps, &ex);
 #else
-    real x = (log2f(eps) - p->eos_log2_emin) * p->eos_inv_dlog2;
+    real fr = frexpf(eps, &ex);
 #endif
-    int i = (int)x;
-    if (i > n - 2) i = n - 2;
-    real r = x - (real)i;
-    const float *lo = t + 4 * i, *hi = lo + 4;
+    /* eps = 2^k (1 + f), f in [0, 1) */
+    int k = ex - 1;
+    real f = RC(2.0f) * fr - RC(1.0f);
+    real tt = f * (real)(1 << lj);
+    int j = (int)tt;
+    real r = tt - (real)j;
+    int n = ((k - kmin) << lj) + j;
+    const float *lo = t + 4 * n, *hi = lo + 4;
     for (int c = 0; c < 4; c++) out[c] = (RC(1.0f) - r) * RC(lo[c]) + r * RC(hi[c]);
 }
 
 static void eos4(const orc_params *p, real eps, real out[4]) {
-    if (p->eos_kind == 1) eos_log(p, eps, out);
+    if (p->eos_kind == 1) eos_oct(p, eps, out);
     else eos_rowcol(p, eps, out);
 }
 
@@ -173,7 +186,7 @@ static real eos_S(const orc_params *p, real eps) {
 }
 static real eos_T(const orc_params *p, real eps) {
     if (in_table(p, eps)) { real o[4]; eos4(p, eps, o); return o[2]; }
-    return (eps + eos_P(p, eps)) / eos_S(p, eps);
+    return DIV(eps + eos_P(p, eps), eos_S(p, eps));
 }
 static real eos_CS2(const orc_params *p, real eps) {
     if (in_table(p, eps)) { real o[4]; eos4(p, eps, o); return o[0]; }
@@ -199,12 +212,12 @@ static real etaos(const orc_params *p, real t) {
 
 /* zeta/s(T) for the bulk extension: Cauchy peak at zetaos_tpeak. */
 static real zetaos(const orc_params *p, real t) {
-    real d = (t - RC(p->zetaos_tpeak)) / RC(p->zetaos_width);
-    return RC(p->zetaos_max) / (RC(1.0f) + d * d);
+    real d = DIV(t - RC(p->zetaos_tpeak), RC(p->zetaos_width));
+    return DIV(RC(p->zetaos_max), RC(1.0f) + d * d);
 }
 
 static real gamma3(real vx, real vy, real vz) {
-    return RC(1.0f) / (real)sqrt((double)r_max(RC(1.0f) - vx * vx - vy * vy - vz * vz, ACU));
+    return rcp((real)sqrt((double)r_max(RC(1.0f) - vx * vx - vy * vy - vz * vz, ACU)));
 }
 static real gamma_ev(const real *ev) { return gamma3(ev[1], ev[2], ev[3]); }
 
@@ -243,9 +256,9 @@ static real max_prop_speed(const orc_params *p, const real *edv, real vk) {
     real ut2 = ut * ut;
     real uk2 = uk * uk;
     real cs2 = eos_CS2(p, edv[0]);
-    real lam = ((real)fabs((double)(ut * uk * (RC(1.0f) - cs2)))
-                + (real)sqrt((double)((ut2 - uk2 - (ut2 - uk2 - RC(1.0f)) * cs2) * cs2)))
-               / (ut2 - (ut2 - RC(1.0f)) * cs2);
+    real lam = DIV((real)fabs((double)(ut * uk * (RC(1.0f) - cs2)))
+                   + (real)sqrt((double)((ut2 - uk2 - (ut2 - uk2 - RC(1.0f)) * cs2) * cs2)),
+                   ut2 - (ut2 - RC(1.0f)) * cs2);
     return r_max(lam, RC(1.0f));
 }
 
@@ -270,7 +283,7 @@ static real root_newton(const orc_params *p, real T00, real M, real bulk, int *i
             v = vl + dv;
         } else {
             dvold = dv;
-            dv = f / df;
+            dv = DIV(f, df);
             v -= dv;
         }
         i++;
@@ -288,23 +301,28 @@ static real root_newton(const orc_params *p, real T00, real M, real bulk, int *i
 #if !ORC_DOUBLE
 /* FP64 variant of the same iteration for cells with M close to T00 (extension). */
 static double eos_P_d(const orc_params *p, double eps) {
-    /* table values are FP32; interpolation in double */
+    /* same lookups as eos_P with double arithmetic on the FP32 table values */
     if (p->eos_kind == 1) {
         const float *t = p->eos_table;
-        double emin = exp2((double)p->eos_log2_emin);
-        if (!(eps > emin)) { double r = eps > 0.0 ? eps / emin : 0.0; return r * t[1]; }
-        double x = (log2(eps) - p->eos_log2_emin) * p->eos_inv_dlog2;
-        int i = (int)x;
-        if (i > p->eos_n - 2) i = p->eos_n - 2;
-        double r = x - i;
-        return (1.0 - r) * t[4 * i + 1] + r * t[4 * i + 5];
+        const int kmin = p->eos_kmin, kmax = p->eos_kmax, lj = p->eos_log2j;
+        const int nlast = (kmax - kmin) << lj;
+        if (!(eps >= ldexp(1.0, kmin))) return (eps > 0.0 ? eps * ldexp(1.0, -kmin) : 0.0) * t[1];
+        if (eps >= ldexp(1.0, kmax)) return eps * ldexp(1.0, -kmax) * t[4 * nlast + 1];
+        int ex;
+        double fr = frexp(eps, &ex);
+        double f = 2.0 * fr - 1.0;
+        double tt = f * (double)(1 << lj);
+        int j = (int)tt;
+        double r = tt - j;
+        int n = ((ex - 1 - kmin) << lj) + j;
+        return (1.0 - r) * t[4 * n + 1] + r * t[4 * n + 5];
     }
     if (eps < (double)p->eos_ed_start + p->eos_num_ed * (double)p->eos_ed_step) {
         double step = p->eos_ed_step, start = p->eos_ed_start;
         int ncol = p->eos_ncol, nrow = p->eos_nrow;
         double epr = step * ncol;
-        int row = (int)(eps / epr);
-        int col = (int)((eps - start - row * epr) / step);
+        int row = (int)(eps * (1.0 / epr));
+        int col = (int)((eps - start - row * epr) * (1.0 / step));
         double elow = start + row * epr + col * step;
         int c0 = col < 0 ? 0 : (col > ncol - 1 ? ncol - 1 : col);
         int r0 = row < 0 ? 0 : (row > nrow - 1 ? nrow - 1 : row);
@@ -314,7 +332,7 @@ static double eos_P_d(const orc_params *p, double eps) {
         int c1 = col < 0 ? 0 : (col > ncol - 1 ? ncol - 1 : col);
         int r1 = row < 0 ? 0 : (row > nrow - 1 ? nrow - 1 : row);
         double hi = p->eos_table[4 * ((int64_t)r1 * ncol + c1) + 1];
-        double r = (eps - elow) / step;
+        double r = (eps - elow) * (1.0 / step);
         return (1.0 - r) * lo + r * hi;
     }
     return 0.3327 * eps - 0.3223 * pow(eps, 0.4585) - 0.003906 * eps * exp(-0.05697 * eps);
@@ -333,7 +351,7 @@ static double root_newton_d(const orc_params *p, double T00, double M, double bu
         if ((f + df * (vh - v)) * (f + df * (vl - v)) > 0.0 || fabs(2.0 * f) > fabs(dvold * df)) {
             dvold = dv; dv = 0.5 * (vh - vl); v = vl + dv;
         } else {
-            dvold = dv; dv = f / df; v -= dv;
+            dvold = dv; dv = f * (1.0 / df); v -= dv;
         }
         i++;
         if (fabs(dv) < 0.00001 || i > 100) break;
@@ -519,9 +537,9 @@ void orc_ideal_along(const orc_params *p, int axis, real *Src, const real *ev, r
             kt1d(p, c, axis, k);
             real *S = Src + 4 * (base + (int64_t)t * stride);
             for (int m = 0; m < 4; m++) {
-                if (p->riemann_test) S[m] = S[m] - k[m] / DX;
-                else if (axis < 2) S[m] = S[m] - tau * k[m] / DX;
-                else S[m] = S[m] - tau * k[m] / (tau * DX);
+                if (p->riemann_test) S[m] = S[m] - DIV(k[m], DX);
+                else if (axis < 2) S[m] = S[m] - DIV(tau * k[m], DX);
+                else S[m] = S[m] - DIV(tau * k[m], tau * DX);
             }
         }
     }
@@ -547,7 +565,7 @@ void orc_ideal_update_ev(const orc_params *p, real *evnew, const real *ev1, cons
             real t = (ed + pressure) * u0 * umu[q] - pressure * g0;
             T0m[q] = p->riemann_test ? t : t * old_time;
         }
-        for (int q = 0; q < 4; q++) T0m[q] = T0m[q] + Src[4 * I + q] * DT / step;
+        for (int q = 0; q < 4; q++) T0m[q] = T0m[q] + DIV(Src[4 * I + q] * DT, (real)step);
         real T00, T01, T02, T03;
         if (p->riemann_test) {
             T00 = r_max(ACU, T0m[0]);
@@ -555,17 +573,17 @@ void orc_ideal_update_ev(const orc_params *p, real *evnew, const real *ev1, cons
             T02 = ((real)fabs((double)T0m[2]) < ACU) ? RC(0.0f) : T0m[2];
             T03 = ((real)fabs((double)T0m[3]) < ACU) ? RC(0.0f) : T0m[3];
         } else {
-            T00 = r_max(ACU, T0m[0]) / tau;
-            T01 = ((real)fabs((double)T0m[1]) < ACU) ? RC(0.0f) : T0m[1] / tau;
-            T02 = ((real)fabs((double)T0m[2]) < ACU) ? RC(0.0f) : T0m[2] / tau;
-            T03 = ((real)fabs((double)T0m[3]) < ACU) ? RC(0.0f) : T0m[3] / tau;
+            T00 = DIV(r_max(ACU, T0m[0]), tau);
+            T01 = ((real)fabs((double)T0m[1]) < ACU) ? RC(0.0f) : DIV(T0m[1], tau);
+            T02 = ((real)fabs((double)T0m[2]) < ACU) ? RC(0.0f) : DIV(T0m[2], tau);
+            T03 = ((real)fabs((double)T0m[3]) < ACU) ? RC(0.0f) : DIV(T0m[3], tau);
         }
         real M = (real)sqrt((double)(T01 * T01 + T02 * T02 + T03 * T03));
         real SC = RC(0.999f);
         if (M > T00) {
-            T01 *= SC * T00 / M;
-            T02 *= SC * T00 / M;
-            T03 *= SC * T00 / M;
+            T01 *= DIV(SC * T00, M);
+            T02 *= DIV(SC * T00, M);
+            T03 *= DIV(SC * T00, M);
             M = SC * T00;
         }
         int it = 0;
@@ -575,7 +593,7 @@ void orc_ideal_update_ev(const orc_params *p, real *evnew, const real *ev1, cons
         real pr = eos_P(p, ed_find);
         real epv = r_max(ACU, T00 + pr);
         real *o = evnew + 4 * I;
-        o[0] = ed_find; o[1] = T01 / epv; o[2] = T02 / epv; o[3] = T03 / epv;
+        o[0] = ed_find; o[1] = DIV(T01, epv); o[2] = DIV(T02, epv); o[3] = DIV(T03, epv);
     }
 }
 
@@ -702,8 +720,8 @@ void orc_visc_along(const orc_params *p, int axis, real *Src, const real *ev, co
             kt1d_visc(p, c, pc, bc, vip_half, vim_half, tau, axis, k);
             real *S = Src + 4 * (base + (int64_t)t * stride);
             for (int m = 0; m < 4; m++) {
-                if (axis < 2) S[m] = S[m] - k[m] / DX;
-                else S[m] = S[m] - k[m] / (tau * DX);
+                if (axis < 2) S[m] = S[m] - DIV(k[m], DX);
+                else S[m] = S[m] - DIV(k[m], tau * DX);
             }
         }
     }
@@ -731,17 +749,17 @@ void orc_visc_update_ev(const orc_params *p, real *evnew, const real *ev1, const
             T0m[q] = ((ed + pressure) * u0 * umu[q] - pressure * g0 + pi1[10 * I + q]) * old_time;
         }
         for (int q = 0; q < 4; q++)
-            T0m[q] = T0m[q] + Src[4 * I + q] * DT / step - pi2[10 * I + q] * tau;
-        real T00 = r_max(ACU, T0m[0]) / tau;
-        real T01 = ((real)fabs((double)T0m[1]) < ACU) ? RC(0.0f) : T0m[1] / tau;
-        real T02 = ((real)fabs((double)T0m[2]) < ACU) ? RC(0.0f) : T0m[2] / tau;
-        real T03 = ((real)fabs((double)T0m[3]) < ACU) ? RC(0.0f) : T0m[3] / tau;
+            T0m[q] = T0m[q] + DIV(Src[4 * I + q] * DT, (real)step) - pi2[10 * I + q] * tau;
+        real T00 = DIV(r_max(ACU, T0m[0]), tau);
+        real T01 = ((real)fabs((double)T0m[1]) < ACU) ? RC(0.0f) : DIV(T0m[1], tau);
+        real T02 = ((real)fabs((double)T0m[2]) < ACU) ? RC(0.0f) : DIV(T0m[2], tau);
+        real T03 = ((real)fabs((double)T0m[3]) < ACU) ? RC(0.0f) : DIV(T0m[3], tau);
         real M = (real)sqrt((double)(T01 * T01 + T02 * T02 + T03 * T03));
         real SC = RC(0.999f);
         if (M > T00) {
-            T01 *= SC * T00 / M;
-            T02 *= SC * T00 / M;
-            T03 *= SC * T00 / M;
+            T01 *= DIV(SC * T00, M);
+            T02 *= DIV(SC * T00, M);
+            T03 *= DIV(SC * T00, M);
             M = SC * T00;
         }
         real bnew = bulk_at(bulk2, I);
@@ -753,7 +771,7 @@ void orc_visc_update_ev(const orc_params *p, real *evnew, const real *ev1, const
         real pr = eos_P(p, ed_find) + bnew;
         real epv = r_max(ACU, T00 + pr);
         real *o = evnew + 4 * I;
-        o[0] = ed_find; o[1] = T01 / epv; o[2] = T02 / epv; o[3] = T03 / epv;
+        o[0] = ed_find; o[1] = DIV(T01, epv); o[2] = DIV(T02, epv); o[3] = DIV(T03, epv);
     }
 }
 
@@ -773,7 +791,7 @@ void orc_visc_initialize(const orc_params *p, real *pi1, real *goodcell, real *u
         real cpTs[4];
         eos4(p, ev[4 * I], cpTs);
         real etav = etaos(p, cpTs[2]) * cpTs[3] * HBARC;
-        real tmp = RC(2.0f) / RC(3.0f) * etav / tau;
+        real tmp = DIV(RC(2.0f) / RC(3.0f) * etav, tau);
         pi1[10 * I + 4] = tmp;
         pi1[10 * I + 7] = tmp;
         pi1[10 * I + 9] = RC(-2.0f) * tmp;
@@ -794,13 +812,13 @@ void orc_is_christoffel(const orc_params *p, real *Src, const real *pi1, const r
         if (step == 1) for (int m = 0; m < 10; m++) s[m] = RC(0.0f);
         const real *e = ev + 4 * I;
         real uz = e[3] * gamma3(e[1], e[2], e[3]);
-        s[0] -= RC(2.0f) * uz * q[3] / tau;
-        s[1] -= uz * q[6] / tau;
-        s[2] -= uz * q[8] / tau;
-        s[3] -= uz * (q[0] + q[9]) / tau;
-        s[6] -= uz * q[1] / tau;
-        s[8] -= uz * q[2] / tau;
-        s[9] -= RC(2.0f) * uz * q[3] / tau;
+        s[0] -= DIV(RC(2.0f) * uz * q[3], tau);
+        s[1] -= DIV(uz * q[6], tau);
+        s[2] -= DIV(uz * q[8], tau);
+        s[3] -= DIV(uz * (q[0] + q[9]), tau);
+        s[6] -= DIV(uz * q[1], tau);
+        s[8] -= DIV(uz * q[2], tau);
+        s[9] -= DIV(RC(2.0f) * uz * q[3], tau);
     }
 }
 
@@ -855,24 +873,24 @@ void orc_is_along(const orc_params *p, int axis, real *Src, real *ud, const real
             for (int c = 0; c < 4; c++) {
                 real d = minmod(RC(0.5f) * (up1[c] - um1[c]),
                                 minmod(THETA * (ui[c] - um1[c]), THETA * (up1[c] - ui[c])));
-                if (axis < 2) ud[4 * IND + c] = d / DX;
+                if (axis < 2) ud[4 * IND + c] = DIV(d, DX);
                 else {
-                    real chr = (c == 0 ? e[2][3] : (c == 3 ? RC(1.0f) : RC(0.0f))) * u0[2] / tau;
-                    ud[4 * IND + c] = d / (DX * tau) + chr;
+                    real chr = DIV((c == 0 ? e[2][3] : (c == 3 ? RC(1.0f) : RC(0.0f))) * u0[2], tau);
+                    ud[4 * IND + c] = DIV(d, DX * tau) + chr;
                 }
             }
             for (int mn = 0; mn < 10; mn++) {
                 real k = kt1d_real(u0[0] * pi1[10 * J[0] + mn], u0[1] * pi1[10 * J[1] + mn],
                                    u0[2] * pi1[10 * J[2] + mn], u0[3] * pi1[10 * J[3] + mn],
                                    u0[4] * pi1[10 * J[4] + mn], v_mh, v_ph, lam_mh, lam_ph);
-                if (axis < 2) Src[10 * IND + mn] = Src[10 * IND + mn] - k / DX;
-                else Src[10 * IND + mn] = Src[10 * IND + mn] - k / (tau * DX);
+                if (axis < 2) Src[10 * IND + mn] = Src[10 * IND + mn] - DIV(k, DX);
+                else Src[10 * IND + mn] = Src[10 * IND + mn] - DIV(k, tau * DX);
             }
             if (bulk && bulk_src) {
                 real k = kt1d_real(u0[0] * bulk[J[0]], u0[1] * bulk[J[1]], u0[2] * bulk[J[2]],
                                    u0[3] * bulk[J[3]], u0[4] * bulk[J[4]], v_mh, v_ph, lam_mh, lam_ph);
-                if (axis < 2) bulk_src[IND] = bulk_src[IND] - k / DX;
-                else bulk_src[IND] = bulk_src[IND] - k / (tau * DX);
+                if (axis < 2) bulk_src[IND] = bulk_src[IND] - DIV(k, DX);
+                else bulk_src[IND] = bulk_src[IND] - DIV(k, tau * DX);
             }
         }
     }
@@ -909,10 +927,10 @@ void orc_update_pimn(const orc_params *p, real *pinew, real *goodcell, const rea
         const real *e1 = ev1 + 4 * I, *e2 = ev2 + 4 * I;
         real u_old[4], u_new[4], udt[4];
         umu4(e1, u_old); umu4(e2, u_new);
-        for (int c = 0; c < 4; c++) udt[c] = (u_new[c] - u_old[c]) / DT;
+        for (int c = 0; c < 4; c++) udt[c] = DIV(u_new[c] - u_old[c], DT);
         if (step == 1) {
             for (int c = 0; c < 4; c++) {
-                udt[c] = udiff[4 * I + c] / DT;
+                udt[c] = DIV(udiff[4 * I + c], DT);
                 u_new[c] = u_old[c] + udiff[4 * I + c];
             }
         }
@@ -931,15 +949,18 @@ void orc_update_pimn(const orc_params *p, real *pinew, real *goodcell, const rea
         real temperature = cpTs[2];
         real local_etaos = etaos(p, temperature);
         real etav = local_etaos * cpTs[3] * HBARC;
-        real one_over_taupi = temperature / (RC(3.0f) * r_max(ACU, local_etaos) * HBARC);
+        real one_over_taupi = DIV(temperature, RC(3.0f) * r_max(ACU, local_etaos) * HBARC);
         real coef_pipi = RC(0.0f);
         if (p->gubser_visc_test) {
             real ETAOS = RC(p->etaos_ymin);
-            real etavH = ETAOS * RC(4.0f) / RC(3.0f);
-            real taupiH = RC(p->lam1) * RC(p->lam1) * etavH / RC(3.0f);
-            one_over_taupi = RC(1.0f) / (taupiH * (real)pow((double)ed_step, (double)RC(-0.25f)));
-            coef_pipi = RC(p->lam1) / ed_step;
-            etav = ETAOS * RC(4.0f) / RC(3.0f) * (real)pow((double)ed_step, (double)RC(0.75f));
+            /* pow(e, -1/4) and pow(e, 3/4) evaluated through sqrt (OpenCL's pow is a
+             * 16-ulp function; the sqrt forms are reproducible on every device) */
+            real q4 = (real)sqrt((double)(real)sqrt((double)ed_step));
+            real etavH = DIV(ETAOS * RC(4.0f), RC(3.0f));
+            real taupiH = DIV(RC(p->lam1) * RC(p->lam1) * etavH, RC(3.0f));
+            one_over_taupi = rcp(taupiH * rcp(q4));
+            coef_pipi = DIV(RC(p->lam1), ed_step);
+            etav = DIV(ETAOS * RC(4.0f), RC(3.0f)) * ((real)sqrt((double)ed_step) * q4);
         }
         real pi2[10];
         for (int m = 0; m < 10; m++) pi2[m] = pistep[10 * I + m];
@@ -955,7 +976,7 @@ void orc_update_pimn(const orc_params *p, real *pinew, real *goodcell, const rea
                 if (u[0] > RC(10000.0f)) sigma = RC(0.0f);
                 real piNS = etav * sigma;
                 real pi_old = pi1[10 * I + mn] * u_old[0];
-                real src = -pi2[mn] * theta / RC(3.0f) - pi2[mn] * u[0] / tau;
+                real src = DIV(-pi2[mn] * theta, RC(3.0f)) - DIV(pi2[mn] * u[0], tau);
                 for (int l = 0; l < 4; l++)
                     src -= (u[mu] * pi2[IDX(nu, l)] + u[nu] * pi2[IDX(mu, l)]) * DU[l] * GMN_DIAG[l];
                 if (p->gubser_visc_test) {
@@ -963,8 +984,8 @@ void orc_update_pimn(const orc_params *p, real *pinew, real *goodcell, const rea
                                                          - pipi(2, mu, nu, pi2, u) - pipi(3, mu, nu, pi2, u));
                 }
                 Src[10 * I + mn] += src;
-                pi_old = (pi_old + Src[10 * I + mn] * DT / step + DT * one_over_taupi * piNS)
-                         / (u_new[0] + DT * one_over_taupi);
+                pi_old = DIV(pi_old + DIV(Src[10 * I + mn] * DT, (real)step) + DT * one_over_taupi * piNS,
+                             u_new[0] + DT * one_over_taupi);
                 if ((real)fabs((double)pi_old) > max_pimn_abs) max_pimn_abs = (real)fabs((double)pi_old);
                 out[mn] = pi_old;
             }
@@ -973,13 +994,13 @@ void orc_update_pimn(const orc_params *p, real *pinew, real *goodcell, const rea
         if (p->bulk_on && bulknew) {
             real zs = zetaos(p, temperature);
             real zetav = zs * cpTs[3] * HBARC;
-            real one_over_taubulk = temperature / (RC(3.0f) * r_max(ACU, zs) * HBARC);
+            real one_over_taubulk = DIV(temperature, RC(3.0f) * r_max(ACU, zs) * HBARC);
             real b2 = bulkstep[I];
-            real bsrc = b2 * theta / RC(3.0f) - b2 * u[0] / tau;
+            real bsrc = DIV(b2 * theta, RC(3.0f)) - DIV(b2 * u[0], tau);
             bulk_src[I] += bsrc;
             real b_old = bulk1[I] * u_old[0];
-            bout = (b_old + bulk_src[I] * DT / step - DT * one_over_taubulk * zetav * theta)
-                   / (u_new[0] + DT * one_over_taubulk);
+            bout = DIV(b_old + DIV(bulk_src[I] * DT, (real)step) - DT * one_over_taubulk * zetav * theta,
+                       u_new[0] + DT * one_over_taubulk);
             if ((real)fabs((double)bout) > max_pimn_abs) max_pimn_abs = (real)fabs((double)bout);
         }
         real pre = eos_P(p, e2[0]);
@@ -1054,7 +1075,7 @@ void orc_visc_stage(const orc_params *p, orc_state *s, int step, real tau) {
 
 /* tau after update_time(loop): cfg.real(TAU0 + (loop+1)*DT), ideal.py:317-320 */
 static real tau_of(const orc_params *p, int64_t loop_plus_one) {
-    return (real)((double)p->tau0 + (double)loop_plus_one * (double)p->dt);
+    return (real)(p->tau0_cfg + (double)loop_plus_one * p->dt_cfg);
 }
 
 /* nsteps of CLIdeal.evolve's step body (ideal.py:341-344) starting at loop index loop0 */

@@ -25,18 +25,18 @@ class OrcParams(C.Structure):
         ('riemann_test', C.c_int32), ('gubser_visc_test', C.c_int32),
         ('eos_kind', C.c_int32),
         ('eos_nrow', C.c_int32), ('eos_ncol', C.c_int32), ('eos_num_ed', C.c_int32),
-        ('eos_n', C.c_int32),
+        ('eos_kmin', C.c_int32), ('eos_kmax', C.c_int32), ('eos_log2j', C.c_int32),
         ('dt', C.c_float), ('dx', C.c_float), ('dy', C.c_float), ('dz', C.c_float),
         ('tau0', C.c_float),
         ('etaos_xmin', C.c_float), ('etaos_ymin', C.c_float),
         ('etaos_left_slop', C.c_float), ('etaos_right_slop', C.c_float), ('lam1', C.c_float),
         ('eos_ed_start', C.c_float), ('eos_ed_step', C.c_float),
-        ('eos_log2_emin', C.c_float), ('eos_inv_dlog2', C.c_float),
         ('eos_table', C.c_void_p),
         ('bulk_on', C.c_int32),
         ('zetaos_max', C.c_float), ('zetaos_tpeak', C.c_float), ('zetaos_width', C.c_float),
         ('fp64_root', C.c_int32), ('fp64_root_vmin', C.c_float),
         ('nthreads', C.c_int32),
+        ('tau0_cfg', C.c_double), ('dt_cfg', C.c_double),
     ]
 
 
@@ -80,14 +80,15 @@ def make_params(nx, ny, nz, dt, dx, dy, dz, tau0, eos_table, etaos_xmin=0.18, et
     p = OrcParams()
     p.nx, p.ny, p.nz = nx, ny, nz
     p.dt, p.dx, p.dy, p.dz, p.tau0 = dt, dx, dy, dz, tau0
+    p.tau0_cfg, p.dt_cfg = float(tau0), float(dt)
     p.etaos_xmin, p.etaos_ymin = etaos_xmin, etaos_ymin
     p.etaos_left_slop, p.etaos_right_slop, p.lam1 = etaos_left_slop, etaos_right_slop, lam1
     p.riemann_test, p.gubser_visc_test = int(bool(riemann_test)), int(bool(gubser_visc_test))
     t = eos_table
     p.eos_kind = t.kind_id
-    p.eos_nrow, p.eos_ncol, p.eos_num_ed, p.eos_n = t.nrow, t.ncol, t.num_ed, t.n
+    p.eos_nrow, p.eos_ncol, p.eos_num_ed = t.nrow, t.ncol, t.num_ed
+    p.eos_kmin, p.eos_kmax, p.eos_log2j = t.kmin, t.kmax, t.log2j
     p.eos_ed_start, p.eos_ed_step = t.ed_start, t.ed_step
-    p.eos_log2_emin, p.eos_inv_dlog2 = t.log2_emin, t.inv_dlog2
     p._table = np.ascontiguousarray(t.rgba, dtype=np.float32)   # keep alive
     p.eos_table = p._table.ctypes.data
     if bulk is not None:
@@ -120,7 +121,7 @@ class Oracle(object):
         self.udx, self.udy, self.udz, self.udiff = z(n, 4), z(n, 4), z(n, 4), z(n, 4)
         self.goodcell = np.ones(n, self.dtype)
         self.iters = np.zeros(n, np.int32) if keep_iters else None
-        self.tau = self.dtype(params.tau0)
+        self.tau = self.dtype(params.tau0_cfg)
         self.loop = 0
         s = OrcState()
         for k in range(3):
@@ -158,7 +159,7 @@ class Oracle(object):
         self.L.orc_get_udiff(C.byref(self.p), _ptr(self.udiff), _ptr(a), _ptr(b))
 
     def update_time(self, loop):
-        self.tau = self.dtype(np.float64(self.p.tau0) + (loop + 1) * np.float64(self.p.dt))
+        self.tau = self.dtype(self.p.tau0_cfg + (loop + 1) * self.p.dt_cfg)
 
     # -- step-level ------------------------------------------------------------------
     def ideal_steps(self, nsteps):

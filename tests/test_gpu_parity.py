@@ -1,0 +1,147 @@
+"""HIP path vs the FP32 oracle on the same seeded inputs.
+
+The HIP kernels and the oracle evaluate the same FP32 operation sequence, so the
+fields are expected to be identical bit for bit; the assertion is the parity bar
+(max relative difference <= 1e-4, floored at 1e-3 of each field's maximum) and the
+bit-identical fraction is reported.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from clvisc_amd import _lib, ic
+from clvisc_amd.eos import Eos
+from oracle import oracle as O
+from helpers import cv_params, orc_params, max_rel, bit_equal_frac
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-4
+
+
+def _say(msg):
+    print(msg)
+    sys.stdout.flush()
+
+
+def _ideal_gubser(nx=33, ny=35, q=1.0):
+    eos = Eos('ideal_gas')
+    tab = eos.create_table()
+    p = cv_params(nx, ny, 1, 0.02, 0.08, 0.08, 0.15, 1.0)
+    ev, _ = ic.gubser_ideal(nx, ny, 1, 0.08, 0.08, 0.15, 1.0, q=q, e0=float(eos.f_ed(0.3)))
+    return p, tab, ev.astype(np.float32)
+
+
+def test_ideal_stage_bitwise():
+    p, tab, ev = _ideal_gubser()
+    _say('case: ideal stage gubser %dx%dx1' % (p.nx, p.ny))
+    h = _lib.Handle(p, tab, viscous=False)
+    h.set('ev1', ev)
+    o = O.Oracle(orc_params(p, tab), 'f32', viscous=False)
+    o.load_ini(ev)
+    for step, tau in ((1, np.float32(1.0)), (2, np.float32(1.02))):
+        h.ideal_stage(step, tau)
+        o.ideal_stage(step, tau)
+        out = 'ev2' if step == 1 else 'ev1'
+        g, r = h.get(out), o.ev[3 - step]
+        assert np.all(np.isfinite(g))
+        m = max_rel(g, r)
+        _say('  step %d max rel %s bit-equal %.6f' % (step, m, bit_equal_frac(g, r)))
+        assert m.max() <= TOL
+        np.testing.assert_array_equal(h.get('src'), o.Src)
+
+
+def test_ideal_steps_gubser_201():
+    eos = Eos('ideal_gas')
+    tab = eos.create_table()
+    p = cv_params(201, 201, 1, 0.02, 0.08, 0.08, 0.15, 1.0)
+    ev, _ = ic.gubser_ideal(201, 201, 1, 0.08, 0.08, 0.15, 1.0, q=1.0, e0=float(eos.f_ed(0.3)))
+    ev = ev.astype(np.float32)
+    _say('case: ideal gubser 201x201x1, 50 steps')
+    h = _lib.Handle(p, tab, viscous=False)
+    h.set('ev1', ev)
+    h.ideal_steps(0, 50)
+    o = O.Oracle(orc_params(p, tab, nthreads=16), 'f32', viscous=False)
+    o.load_ini(ev)
+    o.ideal_steps(50)
+    g = h.get('ev1')
+    m = max_rel(g, o.ev[1])
+    _say('  max rel %s bit-equal %.6f' % (m, bit_equal_frac(g, o.ev[1])))
+    assert m.max() <= TOL
+    assert abs(h.max_ed() - o.max_ed()) == 0.0
+
+
+def _visc_case(nx, ny, nz, bulk=None, nhot=12, seed=3, eos_kind='crossover'):
+    eos = Eos(eos_kind)
+    tab = eos.create_table()
+    p = cv_params(nx, ny, nz, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08, bulk=bulk)
+    pos = ic.woods_saxon_positions(nhot, seed, R=0.6, a=0.1)
+    ev = ic.hotspots(nx, ny, nz, 0.08, 0.08, 0.15, positions=pos, emax=50.0)
+    return p, tab, ev
+
+
+def _visc_compare(p, tab, ev, nsteps, label):
+    _say('case: %s %dx%dx%d, %d steps' % (label, p.nx, p.ny, p.nz, nsteps))
+    h = _lib.Handle(p, tab, viscous=True)
+    h.visc_initialize(p.tau0)
+    h.ideal_stage(1, np.float32(p.tau0))
+    h.set('ev1', ev)
+    h.visc_steps(0, nsteps)
+    o = O.Oracle(orc_params(p, tab, nthreads=16), 'f32', viscous=True)
+    o.load_ini(ev)
+    o.visc_steps(nsteps)
+    res = {}
+    for name, g, r in (('ev1', h.get('ev1'), o.ev[1]), ('pi1', h.get('pi1'), o.pi[1]),
+                       ('ev2', h.get('ev2'), o.ev[2]), ('pi2', h.get('pi2'), o.pi[2]),
+                       ('ev0', h.get('ev0'), o.ev[0]), ('udiff', h.get('udiff'), o.udiff),
+                       ('goodcell', h.get('goodcell'), o.goodcell)):
+        assert np.all(np.isfinite(g)), name
+        m = max_rel(g, r)
+        res[name] = m.max()
+        _say('  %-8s max rel %.3g bit-equal %.6f' % (name, m.max(), bit_equal_frac(g, r)))
+    if p.flags & _lib.CV_FLAG_BULK:
+        g, r = h.get('bulk1'), o.bulk[1]
+        m = max_rel(g, r)
+        res['bulk1'] = m.max()
+        _say('  %-8s max rel %.3g bit-equal %.6f' % ('bulk1', m.max(), bit_equal_frac(g, r)))
+    for k, v in res.items():
+        assert v <= TOL, (k, v)
+
+
+def test_visc_steps_small_3d():
+    p, tab, ev = _visc_case(29, 31, 11)
+    _visc_compare(p, tab, ev, 20, 'viscous hotspots')
+
+
+def test_visc_steps_bulk_3d():
+    p, tab, ev = _visc_case(27, 25, 9, bulk=(0.05, 0.155, 0.02))
+    _visc_compare(p, tab, ev, 20, 'viscous+bulk hotspots')
+
+
+def test_visc_stage_api_matches_reference_sequence():
+    """cv_is_stage / cv_visc_stage / cv_update_udiff called in the order of
+    CLVisc.evolve (pyvisc/visc.py:527-544) give the fused step's result."""
+    p, tab, ev = _visc_case(21, 19, 7)
+    _say('case: stage api 21x19x7, 3 steps')
+    h = _lib.Handle(p, tab, viscous=True)
+    h.visc_initialize(p.tau0)
+    h.ideal_stage(1, np.float32(p.tau0))
+    h.set('ev1', ev)
+    o = O.Oracle(orc_params(p, tab), 'f32', viscous=True)
+    o.load_ini(ev)
+    for n in range(3):
+        h.copy('pi0', 'pi1')
+        h.copy('ev0', 'ev1')
+        t1 = np.float32(p.tau0 + n * p.dt)
+        t2 = np.float32(p.tau0 + (n + 1) * p.dt)
+        h.is_stage(1, t1)
+        h.visc_stage(1, t1)
+        h.is_stage(2, t2)
+        h.visc_stage(2, t2)
+        h.update_udiff('ev0', 'ev1')
+    o.visc_steps(3)
+    for name, r in (('ev1', o.ev[1]), ('pi1', o.pi[1]), ('ev2', o.ev[2]), ('udiff', o.udiff)):
+        g = h.get(name)
+        m = max_rel(g, r)
+        _say('  %-6s max rel %.3g bit-equal %.6f' % (name, m.max(), bit_equal_frac(g, r)))
+        assert m.max() <= TOL

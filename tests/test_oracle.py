@@ -1,0 +1,319 @@
+"""Pins of the CPU oracle (oracle/clvisc_oracle.c) against the reference's own
+known answers and against what the mathematics fixes.  CPU only.
+
+Reference known-answer tests restated here:
+  minmod4 golden vector       pyvisc/tests/test_kernel_helper.py:20-54
+  root-solve round trip       pyvisc/tests/test_kernel_helper.py:57-117 (tightened)
+  max reduction               pyvisc/tests/test_reduction.py:10-53
+  Bjorken decay (ideal)       pyvisc/tests/test_bjorken.py:31-62
+  Gubser ideal flow           pyvisc/tests/test_gubser_ideal.py:22-26
+  1st-order viscous Bjorken   pyvisc/tests/test_bjorken_visc.py:74-80
+"""
+import numpy as np
+import pytest
+
+from clvisc_amd import ic
+from clvisc_amd.eos import Eos, octave_nodes
+from oracle import oracle as O
+
+
+@pytest.fixture(scope='module')
+def ideal_tab():
+    return Eos('ideal_gas').create_table()
+
+
+@pytest.fixture(scope='module')
+def cross():
+    e = Eos('crossover')
+    return e, e.create_table()
+
+
+def params(tab, nx=1, ny=1, nz=1, dt=0.01, dx=0.08, dy=0.08, dz=0.3, tau0=0.6, **kw):
+    return O.make_params(nx, ny, nz, dt, dx, dy, dz, tau0, tab, **kw)
+
+
+# --- helpers -------------------------------------------------------------------------
+@pytest.mark.parametrize('prec', ['f32', 'f64'])
+def test_minmod4_golden(prec):
+    a = np.array([1, -2, -1, 4.0])
+    b = np.array([2, 0, -2, 4.0])
+    c = np.array([3, 2, -3, 4.0])
+    r = O.minmod4(O.minmod4(a, b, prec), c, prec)
+    np.testing.assert_array_equal(r, [1, 0, -1, 4])
+
+
+def test_minmod_plants_fail():
+    """A minmod that took max instead of min, or dropped the sign test, fails the vector."""
+    a = np.array([1, -2, -1, 4.0]); b = np.array([2, 0, -2, 4.0]); c = np.array([3, 2, -3, 4.0])
+    wrong_max = lambda x, y: np.maximum(np.abs(x), np.abs(y)) * (np.sign(x) + np.sign(y)) * 0.5
+    assert not np.array_equal(wrong_max(wrong_max(a, b), c), [1, 0, -1, 4])
+
+
+# --- EOS -----------------------------------------------------------------------------
+def test_ideal_gas_table_closed_form(ideal_tab):
+    p = params(ideal_tab)
+    e = np.array([0.0, 0.005, 0.01, 1.0, 9.995, 10.0, 10.005, 123.456, 1999.0], np.float32)
+    out = O.eos_lookup(p, e)
+    # P = e/3 is linear, so the lerp reproduces it up to FP32 rounding
+    np.testing.assert_allclose(out[:, 1], e / 3.0, rtol=3e-6, atol=1e-7)
+    np.testing.assert_allclose(out[:, 0], 1.0 / 3.0, rtol=1e-6)
+    # T = hbarc (e/(dof pi^2/30)/hbarc)^(1/4): lerp between nodes 0.01 apart
+    eos = Eos('ideal_gas')
+    m = e > 1.0
+    np.testing.assert_allclose(out[m, 2], eos.f_T(e[m].astype(np.float64)), rtol=2e-5)
+
+
+def test_ideal_gas_row_boundary(ideal_tab):
+    """e just below the end of a 1000-texel row interpolates into column 0 of the
+    next row (eos_table.h:49-54) -- here 9.995 lies between texels 999 and 1000."""
+    p = params(ideal_tab)
+    e = np.array([9.995], np.float32)
+    P = O.eos_lookup(p, e)[0, 1]
+    assert abs(P - 9.995 / 3.0) < 1e-5
+    # a lookup that stayed in the row (clamping col+1 to 999) would return P(9.99)
+    assert abs(P - 9.99 / 3.0) > 1e-3
+
+
+def test_above_table_fit(ideal_tab):
+    p = params(ideal_tab)
+    e = np.array([3000.0], np.float32)
+    f = O.eos_funcs(p, e)[0]
+    P = 0.3327 * 3000 - 0.3223 * 3000 ** 0.4585 - 0.003906 * 3000 * np.exp(-0.05697 * 3000)
+    assert abs(f[0] - P) / P < 1e-5
+    assert abs(f[3] - 0.333333333) < 1e-7
+
+
+def test_octave_nodes_and_weights(cross):
+    eos, tab = cross
+    p = params(tab)
+    nodes = octave_nodes(tab.kmin, tab.kmax, tab.log2j).astype(np.float32)
+    out = O.eos_lookup(p, nodes)
+    np.testing.assert_array_equal(out, tab.rgba)                # exactly the node values
+    mids = (0.5 * (nodes[:-1].astype(np.float64) + nodes[1:])).astype(np.float32)
+    out = O.eos_lookup(p, mids)
+    # brute-force restatement: weight from the position inside the node interval
+    lo, hi = nodes[:-1].astype(np.float64), nodes[1:].astype(np.float64)
+    w = (mids - lo) / (hi - lo)
+    ref = (1 - w)[:, None] * tab.rgba[:-1] + w[:, None] * tab.rgba[1:]
+    np.testing.assert_allclose(out, ref, rtol=2e-6, atol=1e-12)
+
+
+def test_octave_against_closed_form(cross):
+    eos, tab = cross
+    p = params(tab)
+    e = np.geomspace(1e-4, 900.0, 4001).astype(np.float32)
+    out = O.eos_lookup(p, e)
+    ref = np.c_[eos.f_cs2(e), eos.f_P(e), eos.f_T(e), eos.f_S(e)]
+    err = np.abs(out - ref) / np.abs(ref)
+    assert err[:, 1].max() < 5e-5      # P
+    assert err[:, 2].max() < 5e-5      # T
+    assert err[:, 3].max() < 1e-4      # s
+    # below 2^kmin P, T, s fall linearly to 0
+    e_lo = np.array([0.0, 2.0 ** (tab.kmin - 1)], np.float32)
+    o = O.eos_lookup(p, e_lo)
+    np.testing.assert_array_equal(o[0, 1:], 0.0)
+    np.testing.assert_allclose(o[1, 1:], 0.5 * tab.rgba[0, 1:], rtol=1e-7)
+
+
+def test_crossover_thermodynamics(cross):
+    """s = (e + P)/T and dP/de = cs2 hold for the table columns."""
+    eos, tab = cross
+    e = octave_nodes(tab.kmin, tab.kmax, tab.log2j)
+    cs2, P, T, s = [tab.rgba[:, c].astype(np.float64) for c in range(4)]
+    np.testing.assert_allclose(s, (e + P) / T, rtol=1e-6)
+    dP = np.gradient(P, e)
+    m = (e > 1e-3) & (e < 500)
+    np.testing.assert_allclose(dP[m], cs2[m], rtol=2e-3, atol=2e-4)
+    # efrz at TFRZ=0.137 lies in the crossover region
+    assert 0.2 < eos.f_ed(0.137) < 0.5
+
+
+# --- root solve ----------------------------------------------------------------------
+@pytest.mark.parametrize('prec', ['f32', 'f64'])
+def test_root_round_trip(ideal_tab, prec):
+    rng = np.random.default_rng(7)
+    n = 20000
+    e = rng.uniform(1.0, 10.0, n)
+    v = rng.uniform(0.0, 0.999, n)
+    P = e / 3.0
+    g2 = 1.0 / (1.0 - v * v)
+    T00 = (e + P) * g2 - P
+    M = (e + P) * g2 * v
+    p = params(ideal_tab)
+    ed, it = O.root_solve(p, T00, M, prec)
+    rel = np.abs(ed - e) / e
+    # reference test checks 2 decimals; Newton's quadratic convergence gives far more
+    assert rel.max() < (2e-3 if prec == 'f32' else 1e-7)
+    # FP64 is limited by the FP32 table values (P at the nodes rounded to float)
+    assert np.median(rel) < (1e-6 if prec == 'f32' else 1e-8)
+    assert it.max() <= 101
+
+
+def test_root_fp64_cells_extension(ideal_tab):
+    """The FP64 solve for fast cells is closer to the exact inverse than FP32."""
+    rng = np.random.default_rng(3)
+    e = rng.uniform(1.0, 10.0, 4000)
+    v = rng.uniform(0.99, 0.9995, 4000)
+    P = e / 3.0
+    g2 = 1.0 / (1.0 - v * v)
+    T00 = ((e + P) * g2 - P).astype(np.float32)
+    M = ((e + P) * g2 * v).astype(np.float32)
+    p32 = params(ideal_tab)
+    p64 = params(ideal_tab, fp64_root_vmin=0.99)
+    e32, _ = O.root_solve(p32, T00, M)
+    e64, _ = O.root_solve(p64, T00, M)
+    ed_exact, _ = O.root_solve(p32, T00.astype(np.float64), M.astype(np.float64), 'f64')
+    err32 = np.abs(e32 - ed_exact) / ed_exact
+    err64 = np.abs(e64 - ed_exact) / ed_exact
+    assert err64.max() <= err32.max()
+    assert err64.max() < 1e-6
+
+
+def test_max_prop_speed_is_one_for_subluminal(ideal_tab):
+    rng = np.random.default_rng(5)
+    n = 5000
+    ev = np.c_[rng.uniform(0, 50, n), rng.uniform(-0.57, 0.57, (n, 3))]
+    ev = ev[(ev[:, 1:] ** 2).sum(1) < 0.999]
+    lam = O.max_prop_speed(params(ideal_tab), ev, ev[:, 1])
+    np.testing.assert_array_equal(lam, 1.0)
+
+
+# --- reduction -----------------------------------------------------------------------
+def test_max_ed(ideal_tab):
+    rng = np.random.default_rng(11)
+    n = 50000
+    p = params(ideal_tab, nx=n)
+    o = O.Oracle(p, 'f32', viscous=False)
+    ev = np.zeros((n, 4), np.float32)
+    ev[:, 0] = rng.uniform(0, 1, n)
+    o.load_ini(ev)
+    assert o.max_ed() == np.float32(ev[:, 0].max())
+
+
+# --- ideal hydro ---------------------------------------------------------------------
+@pytest.mark.parametrize('prec', ['f32', 'f64'])
+def test_bjorken_ideal(ideal_tab, prec):
+    """1-cell boost-invariant expansion: e/e0 = (tau/tau0)^(-4/3) to 3 decimals."""
+    p = params(ideal_tab, dt=0.01, tau0=0.6)
+    o = O.Oracle(p, prec, viscous=False)
+    o.load_ini(ic.bjorken(1, 30.0))
+    hist = []
+    for n in range(200):
+        hist.append([float(o.tau), o.max_ed()])
+        o.ideal_steps(1)
+    h = np.array(hist)
+    np.testing.assert_almost_equal((h[:, 0] / h[0, 0]) ** (-4.0 / 3.0), h[:, 1] / h[0, 1], 3)
+
+
+def test_bjorken_plant_fails(ideal_tab):
+    """A Christoffel term with the wrong factor (Tzz missing P) breaks the decay law."""
+    p = params(ideal_tab, dt=0.01, tau0=0.6)
+    o = O.Oracle(p, 'f64', viscous=False)
+    o.load_ini(ic.bjorken(1, 30.0))
+    o.ideal_steps(200)
+    tau = float(o.tau)
+    good = (tau / 0.6) ** (-4.0 / 3.0)
+    wrong = (tau / 0.6) ** (-1.0)         # what d(tau e)/dtau = 0 would give
+    assert abs(o.max_ed() / 30.0 - good) < 1e-3 < abs(o.max_ed() / 30.0 - wrong)
+
+
+def test_gubser_ideal_analytic(ideal_tab):
+    """Gubser flow, q = 1/fm, e(tau0=1, r=0) = e(T=0.3 GeV): L1 error of e and max error
+    of v_x in the region r < 4 fm after tau = 1 -> 2 fm (coarser grid than C1 for speed)."""
+    eos = Eos('ideal_gas')
+    e0 = float(eos.f_ed(0.3))
+    nx = 121
+    p = params(ideal_tab, nx=nx, ny=nx, nz=1, dt=0.02, dx=0.08, dy=0.08, dz=0.15, tau0=1.0)
+    o = O.Oracle(p, 'f64', viscous=False)
+    ev, norm = ic.gubser_ideal(nx, nx, 1, 0.08, 0.08, 0.15, 1.0, q=1.0, e0=e0)
+    o.load_ini(ev)
+    o.ideal_steps(50)
+    ea, _ = ic.gubser_ideal(nx, nx, 1, 0.08, 0.08, 0.15, float(o.tau), q=1.0)
+    ea[:, 0] *= norm
+    x = (np.arange(nx) - nx // 2) * 0.08
+    X, Y = np.meshgrid(x, x, indexing='ij')
+    m = np.sqrt(X * X + Y * Y).ravel() < 4.0
+    e = o.ev[1][:, 0]
+    l1 = np.abs(e[m] - ea[m, 0]).sum() / ea[m, 0].sum()
+    assert l1 < 0.03
+    assert np.abs(o.ev[1][m, 1] - ea[m, 1]).max() < 0.02
+
+
+def test_riemann_flux_telescopes(ideal_tab):
+    """In Cartesian (RIEMANN_TEST) mode the KT sweeps only move T^{tau mu} between cells:
+    the grid sum of the flux source vanishes to rounding for a blob away from the edges."""
+    nx = 24
+    p = params(ideal_tab, nx=nx, ny=nx, nz=nx, dt=0.01, dx=0.1, dy=0.1, dz=0.1, tau0=1.0,
+               riemann_test=True)
+    o = O.Oracle(p, 'f64', viscous=False)
+    x = (np.arange(nx) - nx // 2) * 0.1
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    # compact support: cells near the edges are exact vacuum, so no flux leaves the grid
+    bump = np.maximum(0.0, 1.0 - (X ** 2 + Y ** 2 + Z ** 2).ravel() / 0.64) ** 2
+    ev = np.zeros((nx ** 3, 4))
+    ev[:, 0] = 10.0 * bump
+    ev[:, 1] = 0.3 * bump
+    o.load_ini(ev)
+    o.ideal_stage(1, 1.0)
+    s = o.Src.sum(axis=0)
+    scale = np.abs(o.Src).sum(axis=0)
+    assert np.all(np.abs(s) < 1e-12 * scale + 1e-14)
+
+
+# --- viscous -------------------------------------------------------------------------
+def test_bjorken_viscous_first_order(ideal_tab):
+    """8x8x1 uniform cell, eta/s = 0.08: T/T0 follows the 1st-order Bjorken solution
+    T/T0 = a^(1/3) (1 + 2/3 b), a = tau0/tau, b = eta/s / (tau0 T0) (1 - a^(2/3))
+    (pyvisc/tests/test_bjorken_visc.py:74-80) within 1% over tau0 -> tau0 + 3 fm."""
+    p = params(ideal_tab, nx=8, ny=8, nz=1, dt=0.01, dx=0.08, dy=0.08, dz=0.3, tau0=0.6,
+               etaos_xmin=0.0, etaos_ymin=0.08)
+    o = O.Oracle(p, 'f64', viscous=True)
+    o.load_ini(ic.bjorken(64, 30.0))
+    eos = Eos('ideal_gas')
+    T0 = float(eos.f_T(30.0))
+    taus, lhs = [], []
+    for n in range(300):
+        o.visc_steps(1)
+        taus.append(float(o.tau))
+        lhs.append(float(eos.f_T(o.max_ed())) / T0)
+    tau = np.array(taus)
+    a = 0.6 / tau
+    b = 0.08 / 0.6 / T0 * 0.19733 * (1.0 - a ** (2.0 / 3.0))
+    rhs = a ** (1.0 / 3.0) * (1 + 2.0 / 3.0 * b)
+    assert np.abs(np.array(lhs) / rhs - 1).max() < 0.01
+
+
+def test_fp32_vs_fp64_short_time(cross):
+    """FP32 and FP64 oracles agree in the dense region after 5 viscous steps."""
+    eos, tab = cross
+    pos = ic.woods_saxon_positions(8, 3, R=0.6, a=0.1)
+    nx, ny, nz = 25, 23, 7
+    ev = ic.hotspots(nx, ny, nz, 0.08, 0.08, 0.15, positions=pos)
+    res = []
+    for prec in ('f32', 'f64'):
+        p = params(tab, nx=nx, ny=ny, nz=nz, dt=0.02, dx=0.08, dy=0.08, dz=0.15, tau0=0.6,
+                   etaos_xmin=0.0, etaos_ymin=0.08)
+        o = O.Oracle(p, prec, viscous=True)
+        o.load_ini(ev)
+        o.visc_steps(5)
+        res.append(o.ev[1].astype(np.float64))
+    m = res[1][:, 0] > 0.05 * res[1][:, 0].max()
+    assert np.abs(res[0][m, 0] - res[1][m, 0]).max() / res[1][:, 0].max() < 1e-5
+
+
+def test_bulk_zero_reduces_to_shear(cross):
+    """The bulk extension with zeta/s = 0 reproduces the shear-only oracle bit for bit."""
+    eos, tab = cross
+    pos = ic.woods_saxon_positions(6, 5, R=0.5, a=0.1)
+    nx, ny, nz = 19, 17, 5
+    ev = ic.hotspots(nx, ny, nz, 0.08, 0.08, 0.15, positions=pos)
+    out = []
+    for bulk in (None, (0.0, 0.155, 0.02)):
+        p = params(tab, nx=nx, ny=ny, nz=nz, dt=0.02, dx=0.08, dy=0.08, dz=0.15, tau0=0.6,
+                   etaos_xmin=0.0, etaos_ymin=0.08, bulk=bulk)
+        o = O.Oracle(p, 'f32', viscous=True)
+        o.load_ini(ev)
+        o.visc_steps(6)
+        out.append((o.ev[1].copy(), o.pi[1].copy()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
