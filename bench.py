@@ -1,0 +1,173 @@
+#!/usr/bin/env python
+"""Throughput of the viscous KT + Israel-Stewart (+ bulk) hydro step on one MI355X.
+
+Workload: 385 x 385 x 115 cells, DX = DY = 0.08 fm, DZ = 0.15, DTAU = 0.02 fm,
+TAU0 = 0.6 fm, eta/s = 0.08, bulk zeta/s(T) peaked at T = 0.155 GeV; initial energy
+density = 400 Gaussian hotspots (width 0.5 fm) at Woods-Saxon nucleon positions
+(R = 6.62, a = 0.546) times an eta plateau of half-width 1.3 with Gaussian tails
+(sigma 1.5), peak 50 GeV/fm^3; smooth crossover EOS in the octave table.
+A step is one full RK2 step over all cells; the unit of work is a cell-update.
+
+Prints one JSON line (stdout).  Diagnostics go to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+NX, NY, NZ = 385, 385, 115
+DX, DY, DZ, DT, TAU0 = 0.08, 0.08, 0.15, 0.02, 0.6
+BULK = (0.05, 0.155, 0.02)          # zeta/s peak, T_peak [GeV], width [GeV]
+HBM_PEAK_GBS = 8000.0
+REF_CELL_UPDATES = 1.69e7           # reference README: 1.01 s/step at 385x385x115 (BSZ=64)
+
+# Compulsory HBM bytes per cell of each launch of one step (FP32; cell = 1 cell-update).
+# stage 1 IS : read ev_n 16, pi_n 40, udiff 16, stale e 4; write pi* 40, IS_src 40, goodcell 4
+# stage 1 KT : read ev_n 16, pi_n 40, pi*^{0mu} 16;          write ev* 16, Src 16
+# stage 2 IS : read ev* 16, pi* 40, ev_n 16, pi_n 40, IS_src 40; write pi_{n+1} 40, goodcell 4
+# stage 2 KT : read ev* 16, pi* 40, ev_n 16, pi_n^{0mu} 16, pi_{n+1}^{0mu} 16, Src 16;
+#              write ev_{n+1} 16, udiff 16
+# bulk adds Pi reads/writes of 4 B each: +12, +8, +16, +12.
+ALGO_BYTES = {False: [160, 104, 196, 152], True: [172, 112, 212, 164]}
+LAUNCH_NAMES = ['is_stage1', 'kt_stage1', 'is_stage2', 'kt_stage2']
+
+
+def log(*a):
+    print(*a, file=sys.stderr)
+    sys.stderr.flush()
+
+
+def make_workload(bulk=True):
+    from clvisc_amd import _lib, ic
+    from clvisc_amd.eos import Eos
+    eos = Eos('crossover')
+    tab = eos.create_table()
+    p = _lib.CvParams()
+    p.nx, p.ny, p.nz = NX, NY, NZ
+    p.tau0, p.dt = TAU0, DT
+    p.dx, p.dy, p.dz = DX, DY, DZ
+    p.etaos_xmin, p.etaos_ymin, p.etaos_left_slop, p.etaos_right_slop, p.lam1 = 0.0, 0.08, 0.0, 0.0, -10.0
+    p.flags = 0
+    if bulk:
+        p.flags |= _lib.CV_FLAG_BULK
+        p.zetaos_max, p.zetaos_tpeak, p.zetaos_width = BULK
+    p.efrz = float(eos.f_ed(0.137))
+    p.nxskip = p.nyskip = p.nzskip = 1
+    t0 = time.time()
+    pos = ic.woods_saxon_positions(400, 20240601)
+    ev = ic.hotspots(NX, NY, NZ, DX, DY, DZ, positions=pos, emax=50.0)
+    log('IC built in %.1fs' % (time.time() - t0))
+    return p, tab, ev
+
+
+def cpu_baseline(p, tab, ev, nthreads):
+    """FP32 oracle on the host cores: the central 29 eta slices of the same initial
+    condition, 2 RK2 steps, reference-ordered unfused kernels."""
+    from oracle import oracle as O
+    from clvisc_amd import _lib
+    nz = 29
+    k0 = NZ // 2 - nz // 2
+    sub = ev.reshape(NX, NY, NZ, 4)[:, :, k0:k0 + nz, :].reshape(-1, 4)
+    bulk = (p.zetaos_max, p.zetaos_tpeak, p.zetaos_width) if p.flags & _lib.CV_FLAG_BULK else None
+    op = O.make_params(NX, NY, nz, DT, DX, DY, DZ, TAU0, tab, etaos_xmin=0.0, etaos_ymin=0.08,
+                       bulk=bulk, nthreads=nthreads)
+    o = O.Oracle(op, 'f32', viscous=True)
+    o.load_ini(sub)
+    steps = 2
+    t0 = time.perf_counter()
+    o.visc_steps(steps)
+    dt = time.perf_counter() - t0
+    cells = NX * NY * nz
+    return {'value': cells * steps / dt, 'unit': 'cell-updates/s', 'cores': nthreads,
+            'kind': 'port',
+            'sample': '%dx%dx%d central eta slab of the same initial condition, %d RK2 steps, '
+                      'FP32 oracle (unfused reference-ordered kernels, OpenMP), %.1f s'
+                      % (NX, NY, nz, steps, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=100)
+    ap.add_argument('--warmup', type=int, default=5)
+    ap.add_argument('--no-bulk', action='store_true', help='shear viscosity only')
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-threads', type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py measures one GPU only: the hydro step of one event runs on one MI355X '
+              '(multi-GPU slab decomposition is described in DESIGN.md, not built)', file=sys.stderr)
+        sys.exit(2)
+
+    from clvisc_amd import _lib
+    bulk = not args.no_bulk
+    p, tab, ev = make_workload(bulk)
+    name, cus = _lib.device_info(0)
+    log('device: %s, %d CUs' % (name, cus))
+    h = _lib.Handle(p, tab, viscous=True)
+    h.visc_initialize(TAU0)
+    h.ideal_stage(1, np.float32(TAU0))
+    h.set('ev1', ev)
+    loop = 0
+    h.visc_steps(loop, args.warmup)
+    loop += args.warmup
+    # per-launch device time of one step (HIP events on the handle's stream)
+    prof = h.profile_visc_step(loop)
+    loop += 1
+    h.sync()
+    t0 = time.perf_counter()
+    h.visc_steps(loop, args.steps, sync=False)
+    h.sync()
+    wall = time.perf_counter() - t0
+    dev_ms = h.last_elapsed_ms()
+    loop += args.steps
+    ncell = NX * NY * NZ
+    ms_per_step = 1000.0 * wall / args.steps
+    value = ncell * args.steps / wall
+    edmax = h.max_ed(1)
+    ev1 = None
+    ok = np.isfinite(edmax)
+    log('steps %d wall %.3fs device %.3f ms -> %.3f ms/step, %.4g cell-updates/s, edmax %.4g'
+        % (args.steps, wall, dev_ms, ms_per_step, value, edmax))
+    abytes = ALGO_BYTES[bulk]
+    per = [(abytes[i] * ncell / (prof[i] * 1e-3) / 1e9) for i in range(len(prof))]
+    for i, nm in enumerate(LAUNCH_NAMES):
+        log('  %-10s %8.3f ms  %6.1f GB/s algorithmic (%d B/cell)' % (nm, prof[i], per[i], abytes[i]))
+    dom = int(np.argmax(prof))
+    roof = {'bound': 'hbm', 'kernel': LAUNCH_NAMES[dom], 'achieved': round(per[dom], 2),
+            'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(per[dom] / HBM_PEAK_GBS, 5),
+            'traffic': None,
+            'step_algorithmic_bytes_per_cell': sum(abytes),
+            'step_frac': round(sum(abytes) * value / 1e9 / HBM_PEAK_GBS, 5)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        nth = args.cpu_threads or min(16, os.cpu_count() or 1)
+        t0 = time.time()
+        cpu = cpu_baseline(p, tab, ev, nth)
+        log('cpu baseline: %.4g cell-updates/s on %d threads (%.1fs)' % (cpu['value'], nth, time.time() - t0))
+    out = {
+        'metric': 'cell-updates/s, viscous KT+IS 385x385x115',
+        'value': value, 'unit': 'cell-updates/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': ms_per_step,
+        'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': value / REF_CELL_UPDATES,
+        'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': 'viscous KT + Israel-Stewart shear (eta/s=0.08)%s, 385x385x115, '
+                               'dtau=0.02 fm, 400 Woods-Saxon hotspots, crossover EOS (octave table)'
+                               % (' + bulk (zeta/s peak 0.05 at 0.155 GeV)' if bulk else ''),
+                   'grid': [NX, NY, NZ], 'device': name, 'finite': bool(ok)},
+        'roofline': roof,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(out))
+    h.close()
+
+
+if __name__ == '__main__':
+    main()

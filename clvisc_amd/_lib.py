@@ -85,6 +85,7 @@ def lib():
             'cv_max_ed': [vp, C.c_int, C.POINTER(f32)],
             'cv_sync': [vp],
             'cv_last_elapsed_ms': [vp, C.POINTER(f32)],
+            'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -104,7 +105,7 @@ def exported_symbols():
     return ['cv_last_error', 'cv_device_info', 'cv_create', 'cv_destroy', 'cv_set_field',
             'cv_get_field', 'cv_copy_field', 'cv_visc_initialize', 'cv_ideal_stage', 'cv_is_stage',
             'cv_visc_stage', 'cv_update_udiff', 'cv_ideal_steps_async', 'cv_visc_steps_async',
-            'cv_max_ed', 'cv_sync', 'cv_last_elapsed_ms']
+            'cv_max_ed', 'cv_sync', 'cv_last_elapsed_ms', 'cv_profile_visc_step']
 
 
 def make_eos(table):
@@ -194,6 +195,12 @@ class Handle(object):
 
     def sync(self):
         check(self.L.cv_sync(self.h))
+
+    def profile_visc_step(self, loop0):
+        ms = (C.c_float * 16)()
+        nl = C.c_int32()
+        check(self.L.cv_profile_visc_step(self.h, loop0, ms, 16, C.byref(nl)))
+        return [ms[i] for i in range(nl.value)]
 
     def last_elapsed_ms(self):
         out = C.c_float()

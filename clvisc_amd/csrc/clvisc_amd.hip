@@ -390,6 +390,39 @@ extern "C" int cv_visc_steps_async(cv_handle *h, int64_t loop0, int64_t nsteps) 
     return CV_OK;
 }
 
+extern "C" int cv_profile_visc_step(cv_handle *h, int64_t loop0, float *ms, int nms, int32_t *nlaunch) {
+    if (!h || !h->viscous) return fail(CV_ERR_STATE, "cv_profile_visc_step: viscous handle required");
+    if (!ms || !nlaunch || loop0 < 0) return fail(CV_ERR_ARG, "cv_profile_visc_step: bad argument");
+    HIP_TRY(hipSetDevice(h->device));
+    const int NL = 4;
+    hipEvent_t evs[NL + 1];
+    for (int i = 0; i <= NL; i++) HIP_TRY(hipEventCreate(&evs[i]));
+    float t1 = tau_at(h, loop0), t2 = tau_at(h, loop0 + 1);
+    float **E = h->ev, **P = h->pi, **B = h->bk;
+    int rc = CV_OK;
+    HIP_TRY(hipEventRecord(evs[0], h->stream));
+    rc = launch_is(h, 1, t1, E[1], P[1], B[1], E[1], E[2], P[1], B[1], P[2], B[2]);
+    if (!rc) rc = hipEventRecord(evs[1], h->stream) == hipSuccess ? CV_OK : CV_ERR_HIP;
+    if (!rc) rc = launch_kt(h, 1, t1, E[1], P[1], B[1], E[1], P[1], B[1], P[2], B[2], E[2], nullptr);
+    if (!rc) rc = hipEventRecord(evs[2], h->stream) == hipSuccess ? CV_OK : CV_ERR_HIP;
+    if (!rc) rc = launch_is(h, 2, t2, E[2], P[2], B[2], E[1], E[2], P[1], B[1], P[0], B[0]);
+    if (!rc) rc = hipEventRecord(evs[3], h->stream) == hipSuccess ? CV_OK : CV_ERR_HIP;
+    if (!rc) rc = launch_kt(h, 2, t2, E[2], P[2], B[2], E[1], P[1], B[1], P[0], B[0], E[0], h->udiff);
+    if (!rc) rc = hipEventRecord(evs[4], h->stream) == hipSuccess ? CV_OK : CV_ERR_HIP;
+    if (!rc) {
+        HIP_TRY(hipEventSynchronize(evs[NL]));
+        for (int i = 0; i < NL && i < nms; i++) HIP_TRY(hipEventElapsedTime(&ms[i], evs[i], evs[i + 1]));
+        *nlaunch = NL;
+        float *t;
+        t = E[0]; E[0] = E[1]; E[1] = t;
+        t = P[0]; P[0] = P[1]; P[1] = t;
+        t = B[0]; B[0] = B[1]; B[1] = t;
+    }
+    for (int i = 0; i <= NL; i++) (void)hipEventDestroy(evs[i]);
+    if (rc == CV_ERR_HIP) return fail(CV_ERR_HIP, "cv_profile_visc_step: event record failed");
+    return rc;
+}
+
 extern "C" int cv_max_ed(cv_handle *h, int which, float *out) {
     if (!h || !out) return fail(CV_ERR_ARG, "null argument");
     if (which < 0 || which > 2) return fail(CV_ERR_ARG, "which must be 0, 1 or 2");

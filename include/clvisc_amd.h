@@ -133,6 +133,12 @@ int cv_visc_steps_async(cv_handle *h, int64_t loop0, int64_t nsteps);
  * ideal.py:257-262). */
 int cv_max_ed(cv_handle *h, int which, float *out);
 
+/* Profiling: runs one viscous step (as cv_visc_steps_async with nsteps = 1) with HIP
+ * events around each of its launches, waits, and writes the launch durations in
+ * milliseconds to ms[0..nms-1] in launch order; returns the number of launches in
+ * *nlaunch.  Same results as the unprofiled step. */
+int cv_profile_visc_step(cv_handle *h, int64_t loop0, float *ms, int nms, int32_t *nlaunch);
+
 /* Wait for all queued work. */
 int cv_sync(cv_handle *h);
 
