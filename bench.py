@@ -28,14 +28,13 @@ HBM_PEAK_GBS = 8000.0
 REF_CELL_UPDATES = 1.69e7           # reference README: 1.01 s/step at 385x385x115 (BSZ=64)
 
 # Compulsory HBM bytes per cell of each launch of one step (FP32; cell = 1 cell-update).
-# stage 1 IS : read ev_n 16, pi_n 40, udiff 16, stale e 4; write pi* 40, IS_src 40, goodcell 4
-# stage 1 KT : read ev_n 16, pi_n 40, pi*^{0mu} 16;          write ev* 16, Src 16
-# stage 2 IS : read ev* 16, pi* 40, ev_n 16, pi_n 40, IS_src 40; write pi_{n+1} 40, goodcell 4
-# stage 2 KT : read ev* 16, pi* 40, ev_n 16, pi_n^{0mu} 16, pi_{n+1}^{0mu} 16, Src 16;
-#              write ev_{n+1} 16, udiff 16
-# bulk adds Pi reads/writes of 4 B each: +12, +8, +16, +12.
-ALGO_BYTES = {False: [160, 104, 196, 152], True: [172, 112, 212, 164]}
-LAUNCH_NAMES = ['is_stage1', 'kt_stage1', 'is_stage2', 'kt_stage2']
+# stage 1: read ev_n 16, pi_n 40, udiff 16, stale e 4;
+#          write pi* 40, ev* 16, stage-1 sources IS_src 40 + Src 16, goodcell 4      -> 192
+# stage 2: read ev* 16, pi* 40, ev_n 16, pi_n 40, stage-1 sources 56;
+#          write pi_{n+1} 40, ev_{n+1} 16, udiff 16, goodcell 4                      -> 244
+# bulk Pi adds 4 B per read/write: stage 1 +12 (Pi_n; Pi*, its source), stage 2 +16.
+ALGO_BYTES = {False: [192, 244], True: [204, 260]}
+LAUNCH_NAMES = ['visc_stage1', 'visc_stage2']
 
 
 def log(*a):

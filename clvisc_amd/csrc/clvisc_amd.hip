@@ -298,6 +298,43 @@ static int launch_kt(cv_handle *h, int step, float tau, float *ev_st, float *pi_
     return CV_OK;
 }
 
+// fused IS + KT stage of the step path (k_visc_stage)
+static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *pi_st, float *bk_st,
+                        float *ev1, float *ev2, float *pi1, float *bk1, float *pi_out, float *bk_out,
+                        float *ev_out, float *udiff_out) {
+    StageP st = stage_params(h, step, tau);
+    dim3 g = grid1d(h->n), b(256);
+#define CV_STAGE_ARGS h->phys, h->eos, st, ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src, \
+                      h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out
+    if (h->bulk) {
+        if (step == 1) hipLaunchKernelGGL((k_visc_stage<true, 1>), g, b, 0, h->stream, CV_STAGE_ARGS);
+        else hipLaunchKernelGGL((k_visc_stage<true, 2>), g, b, 0, h->stream, CV_STAGE_ARGS);
+    } else {
+        if (step == 1) hipLaunchKernelGGL((k_visc_stage<false, 1>), g, b, 0, h->stream, CV_STAGE_ARGS);
+        else hipLaunchKernelGGL((k_visc_stage<false, 2>), g, b, 0, h->stream, CV_STAGE_ARGS);
+    }
+#undef CV_STAGE_ARGS
+    HIP_TRY(hipGetLastError());
+    return CV_OK;
+}
+
+// one RK2 step of the evolve loop; stage 2 writes ev_{n+1}, pi_{n+1} into the slot-0
+// buffers, then slots 0 and 1 swap (see cv_visc_steps_async)
+static int launch_visc_step(cv_handle *h, int64_t n, hipEvent_t mid = nullptr) {
+    float t1 = tau_at(h, n), t2 = tau_at(h, n + 1);
+    float **E = h->ev, **P = h->pi, **B = h->bk;
+    int rc = launch_stage(h, 1, t1, E[1], P[1], B[1], E[1], E[2], P[1], B[1], P[2], B[2], E[2], nullptr);
+    if (rc) return rc;
+    if (mid) HIP_TRY(hipEventRecord(mid, h->stream));
+    rc = launch_stage(h, 2, t2, E[2], P[2], B[2], E[1], E[2], P[1], B[1], P[0], B[0], E[0], h->udiff);
+    if (rc) return rc;
+    float *t;
+    t = E[0]; E[0] = E[1]; E[1] = t;
+    t = P[0]; P[0] = P[1]; P[1] = t;
+    t = B[0]; B[0] = B[1]; B[1] = t;
+    return CV_OK;
+}
+
 static int check_step(int step) {
     if (step != 1 && step != 2) return fail(CV_ERR_ARG, "step must be 1 or 2, got %d", step);
     return CV_OK;
@@ -373,17 +410,8 @@ extern "C" int cv_visc_steps_async(cv_handle *h, int64_t loop0, int64_t nsteps) 
     HIP_TRY(hipSetDevice(h->device));
     HIP_TRY(hipEventRecord(h->ev_a, h->stream));
     for (int64_t n = loop0; n < loop0 + nsteps; n++) {
-        float t1 = tau_at(h, n), t2 = tau_at(h, n + 1);
-        float **E = h->ev, **P = h->pi, **B = h->bk;
-        int rc = launch_is(h, 1, t1, E[1], P[1], B[1], E[1], E[2], P[1], B[1], P[2], B[2]);
-        if (!rc) rc = launch_kt(h, 1, t1, E[1], P[1], B[1], E[1], P[1], B[1], P[2], B[2], E[2], nullptr);
-        if (!rc) rc = launch_is(h, 2, t2, E[2], P[2], B[2], E[1], E[2], P[1], B[1], P[0], B[0]);
-        if (!rc) rc = launch_kt(h, 2, t2, E[2], P[2], B[2], E[1], P[1], B[1], P[0], B[0], E[0], h->udiff);
+        int rc = launch_visc_step(h, n);
         if (rc) return rc;
-        float *t;
-        t = E[0]; E[0] = E[1]; E[1] = t;
-        t = P[0]; P[0] = P[1]; P[1] = t;
-        t = B[0]; B[0] = B[1]; B[1] = t;
     }
     HIP_TRY(hipEventRecord(h->ev_b, h->stream));
     h->timed = true;
@@ -394,32 +422,23 @@ extern "C" int cv_profile_visc_step(cv_handle *h, int64_t loop0, float *ms, int 
     if (!h || !h->viscous) return fail(CV_ERR_STATE, "cv_profile_visc_step: viscous handle required");
     if (!ms || !nlaunch || loop0 < 0) return fail(CV_ERR_ARG, "cv_profile_visc_step: bad argument");
     HIP_TRY(hipSetDevice(h->device));
-    const int NL = 4;
-    hipEvent_t evs[NL + 1];
-    for (int i = 0; i <= NL; i++) HIP_TRY(hipEventCreate(&evs[i]));
-    float t1 = tau_at(h, loop0), t2 = tau_at(h, loop0 + 1);
-    float **E = h->ev, **P = h->pi, **B = h->bk;
-    int rc = CV_OK;
-    HIP_TRY(hipEventRecord(evs[0], h->stream));
-    rc = launch_is(h, 1, t1, E[1], P[1], B[1], E[1], E[2], P[1], B[1], P[2], B[2]);
-    if (!rc) rc = hipEventRecord(evs[1], h->stream) == hipSuccess ? CV_OK : CV_ERR_HIP;
-    if (!rc) rc = launch_kt(h, 1, t1, E[1], P[1], B[1], E[1], P[1], B[1], P[2], B[2], E[2], nullptr);
-    if (!rc) rc = hipEventRecord(evs[2], h->stream) == hipSuccess ? CV_OK : CV_ERR_HIP;
-    if (!rc) rc = launch_is(h, 2, t2, E[2], P[2], B[2], E[1], E[2], P[1], B[1], P[0], B[0]);
-    if (!rc) rc = hipEventRecord(evs[3], h->stream) == hipSuccess ? CV_OK : CV_ERR_HIP;
-    if (!rc) rc = launch_kt(h, 2, t2, E[2], P[2], B[2], E[1], P[1], B[1], P[0], B[0], E[0], h->udiff);
-    if (!rc) rc = hipEventRecord(evs[4], h->stream) == hipSuccess ? CV_OK : CV_ERR_HIP;
+    hipEvent_t e0, e1, e2;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventCreate(&e2));
+    HIP_TRY(hipEventRecord(e0, h->stream));
+    int rc = launch_visc_step(h, loop0, e1);
     if (!rc) {
-        HIP_TRY(hipEventSynchronize(evs[NL]));
-        for (int i = 0; i < NL && i < nms; i++) HIP_TRY(hipEventElapsedTime(&ms[i], evs[i], evs[i + 1]));
-        *nlaunch = NL;
-        float *t;
-        t = E[0]; E[0] = E[1]; E[1] = t;
-        t = P[0]; P[0] = P[1]; P[1] = t;
-        t = B[0]; B[0] = B[1]; B[1] = t;
+        HIP_TRY(hipEventRecord(e2, h->stream));
+        HIP_TRY(hipEventSynchronize(e2));
+        float a = 0.0f, b = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&a, e0, e1));
+        HIP_TRY(hipEventElapsedTime(&b, e1, e2));
+        if (nms > 0) ms[0] = a;
+        if (nms > 1) ms[1] = b;
+        *nlaunch = 2;
     }
-    for (int i = 0; i <= NL; i++) (void)hipEventDestroy(evs[i]);
-    if (rc == CV_ERR_HIP) return fail(CV_ERR_HIP, "cv_profile_visc_step: event record failed");
+    (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
     return rc;
 }
 

@@ -11,6 +11,10 @@
 #pragma once
 #include "cv_device.h"
 
+#ifndef CV_PHASE_FENCE
+#define CV_PHASE_FENCE() __builtin_amdgcn_sched_barrier(0)
+#endif
+
 namespace cv {
 
 __device__ __forceinline__ float4 ld4(const float *__restrict__ b, int64_t n, int64_t I) {
@@ -18,6 +22,15 @@ __device__ __forceinline__ float4 ld4(const float *__restrict__ b, int64_t n, in
 }
 __device__ __forceinline__ void st4(float *b, int64_t n, int64_t I, float4 v) {
     b[I] = v.x; b[n + I] = v.y; b[2 * n + I] = v.z; b[3 * n + I] = v.w;
+}
+// Loads through a uniform plane base and a 32-bit byte offset (cells < 2^30 per field):
+// lets the compiler use the scalar-base + 32-bit vector-offset form of global loads, so
+// one offset register serves every plane of a field.
+__device__ __forceinline__ float ldo(const float *__restrict__ base, uint32_t boff) {
+    return *(const float *)((const char *)base + boff);
+}
+__device__ __forceinline__ float4 ld4o(const float *__restrict__ b, int64_t n, uint32_t boff) {
+    return make_float4(ldo(b, boff), ldo(b + n, boff), ldo(b + 2 * n, boff), ldo(b + 3 * n, boff));
 }
 __device__ __forceinline__ float comp(float4 v, int c) {
     return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
@@ -48,6 +61,16 @@ __device__ __forceinline__ int64_t nb(const PhysP &p, const Cell &c, int s) {
     int u = t + s;
     u = u < 0 ? 0 : (u > L - 1 ? L - 1 : u);
     return c.I + (int64_t)(u - t) * st;
+}
+
+template <int A>
+__device__ __forceinline__ uint32_t nbo(const PhysP &p, const Cell &c, int s) {
+    int t = A == 0 ? c.i : (A == 1 ? c.j : c.k);
+    int L = A == 0 ? p.nx : (A == 1 ? p.ny : p.nz);
+    uint32_t st = A == 0 ? (uint32_t)p.sx : (A == 1 ? (uint32_t)p.sy : 1u);
+    int u = t + s;
+    u = u < 0 ? 0 : (u > L - 1 ? L - 1 : u);
+    return ((uint32_t)c.I + (uint32_t)(u - t) * st) * 4u;
 }
 
 // 10 components of pi^{mu nu}: index of (mu, nu) in the idx() order (helper.h:31)
@@ -236,7 +259,7 @@ __device__ __forceinline__ void is_axis(const PhysP &p, const Eos &eos, const St
     float up1[4] = {u0[3] * 1.0f, u0[3] * e[3].y, u0[3] * e[3].z, u0[3] * e[3].w};
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-        float d = minmod(0.5f * (up1[c] - um1[c]), minmod(CV_THETA * (ui[c] - um1[c]), CV_THETA * (up1[c] - ui[c])));
+        float d = slope(um1[c], ui[c], up1[c]);
         if (A == 0) ud[c] = d * p.r_dx;
         else if (A == 1) ud[c] = d * p.r_dy;
         else {
@@ -576,6 +599,327 @@ __global__ __launch_bounds__(256) void k_kt_stage(PhysP p, EosP E, StageP st,
         float uo[4] = {u0 * 1.0f, u0 * e.y, u0 * e.z, u0 * e.w};
 #pragma unroll
         for (int c = 0; c < 4; c++) udiff[(int64_t)c * p.n + I] = un[c] - uo[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused stage kernel of the step path: IS_stepUpdate(step) and visc_stepUpdate(step) of
+// one cell in one thread.  The stencil of each axis is loaded once and serves the
+// Israel-Stewart flux, the KT flux and the velocity gradient; lambda of each face is
+// evaluated once for both (the reference evaluates the same expression twice).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float4 t0m_u(float4 ev, float pr, float u0) {
+    float a = (ev.x + pr) * u0;
+    return make_float4(a * (1.0f * u0) - 1.0f * pr, a * (ev.y * u0) - 0.0f * pr,
+                       a * (ev.z * u0) - 0.0f * pr, a * (ev.w * u0) - 0.0f * pr);
+}
+
+template <int A, bool BULK>
+__device__ __forceinline__ void visc_axis(const PhysP &p, const Eos &eos, const StageP &st, const Cell &cl,
+                                          const float *__restrict__ ev, const float *__restrict__ pi,
+                                          const float *__restrict__ bk, float s_is[10], float &bs,
+                                          float s_kt[4], float ud[4]) {
+    uint32_t J[5];
+    float4 e[5];
+    float u0[5], pr[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        J[q] = nbo<A>(p, cl, q - 2);
+        e[q] = ld4o(ev, p.n, J[q]);
+        u0[q] = gamma3(e[q].y, e[q].z, e[q].w);
+        pr[q] = eos.P(e[q].x) + (BULK ? ldo(bk, J[q]) : 0.0f);
+    }
+    const float v_mh = 0.5f * (comp(e[1], A + 1) + comp(e[2], A + 1));
+    const float v_ph = 0.5f * (comp(e[3], A + 1) + comp(e[2], A + 1));
+    const float lam_mh = max_prop_speed(p, eos, half4(e[1], e[2]), v_mh);
+    const float lam_ph = max_prop_speed(p, eos, half4(e[3], e[2]), v_ph);
+    const float rdx = A == 0 ? p.r_dx : (A == 1 ? p.r_dy : st.r_taudz);
+    // velocity gradient (kernel_IS.cl:134-137, 301-304)
+    {
+        float um1[4] = {u0[1] * 1.0f, u0[1] * e[1].y, u0[1] * e[1].z, u0[1] * e[1].w};
+        float ui[4] = {u0[2] * 1.0f, u0[2] * e[2].y, u0[2] * e[2].z, u0[2] * e[2].w};
+        float up1[4] = {u0[3] * 1.0f, u0[3] * e[3].y, u0[3] * e[3].z, u0[3] * e[3].w};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            float d = slope(um1[c], ui[c], up1[c]);
+            if (A == 0) ud[c] = d * p.r_dx;
+            else if (A == 1) ud[c] = d * p.r_dy;
+            else {
+                float chr = ((c == 0 ? e[2].w : (c == 3 ? 1.0f : 0.0f)) * u0[2]) * st.r_tau;
+                ud[c] = d * st.r_taudz + chr;
+            }
+        }
+    }
+    // Israel-Stewart flux of u^tau pi^{mu nu} (and u^tau Pi)
+#pragma unroll
+    for (int mn = 0; mn < 10; mn++) {
+        const float *pl = pi + (int64_t)mn * p.n;
+        float k = kt1d_real(u0[0] * ldo(pl, J[0]), u0[1] * ldo(pl, J[1]), u0[2] * ldo(pl, J[2]),
+                            u0[3] * ldo(pl, J[3]), u0[4] * ldo(pl, J[4]), v_mh, v_ph, lam_mh, lam_ph);
+        s_is[mn] = s_is[mn] - k * rdx;
+    }
+    if (BULK) {
+        float k = kt1d_real(u0[0] * ldo(bk, J[0]), u0[1] * ldo(bk, J[1]), u0[2] * ldo(bk, J[2]),
+                            u0[3] * ldo(bk, J[3]), u0[4] * ldo(bk, J[4]), v_mh, v_ph, lam_mh, lam_ph);
+        bs = bs - k * rdx;
+    }
+    // KT flux of tau (T0^{tau mu} + pi^{tau mu}) (kernel_visc.cl:60-127)
+    const float tau = st.tau;
+    float src[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float Q[5], p0[5], pa[5];
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            float4 T = t0m_u(e[i], pr[i], u0[i]);
+            p0[i] = ldo(pi + (int64_t)PI_ROW[0][q] * p.n, J[i]);
+            pa[i] = ldo(pi + (int64_t)PI_ROW[A + 1][q] * p.n, J[i]);
+            Q[i] = tau * (comp(T, q) + p0[i]);
+        }
+        const float g0 = q == 0 ? 1.0f : 0.0f;
+        const float ga = q == A + 1 ? -1.0f : 0.0f;
+        float DA0 = slope(Q[1], Q[2], Q[3]);
+        float DA1 = slope(Q[2], Q[3], Q[4]);
+        float AL = Q[2] + 0.5f * DA0;
+        float AR = Q[3] - 0.5f * DA1;
+        float pr_half = 0.5f * (pr[2] + pr[3]);
+        float p0h = 0.5f * (p0[2] + p0[3]);
+        float pih = 0.5f * (pa[2] + pa[3]);
+        float Jp = (AR + pr_half * tau * g0 - tau * p0h) * v_ph - pr_half * tau * ga + tau * pih;
+        float Jm = (AL + pr_half * tau * g0 - tau * p0h) * v_ph - pr_half * tau * ga + tau * pih;
+        float sq = 0.5f * (Jp + Jm) - 0.5f * lam_ph * (AR - AL);
+        DA1 = DA0;
+        DA0 = slope(Q[0], Q[1], Q[2]);
+        AL = Q[1] + 0.5f * DA0;
+        AR = Q[2] - 0.5f * DA1;
+        pr_half = 0.5f * (pr[1] + pr[2]);
+        p0h = 0.5f * (p0[1] + p0[2]);
+        pih = 0.5f * (pa[1] + pa[2]);
+        Jp = (AR + pr_half * tau * g0 - tau * p0h) * v_mh - pr_half * tau * ga + tau * pih;
+        Jm = (AL + pr_half * tau * g0 - tau * p0h) * v_mh - pr_half * tau * ga + tau * pih;
+        sq -= 0.5f * (Jp + Jm) - 0.5f * lam_mh * (AR - AL);
+        src[q] = sq;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++) s_kt[m] = s_kt[m] - src[m] * rdx;
+}
+
+// STEP 1: ev_st = ev1 = d_ev[1] (ev_n), pi_st = pi1 = pi_n, ev2 = stale d_ev[2];
+//         writes pi*, Pi*, ev*, IS_src and Src (stage-1 sources), goodcell.
+// STEP 2: ev_st = ev2 = ev*, pi_st = pi*, ev1 = ev_n, pi1 = pi_n;
+//         writes pi_{n+1}, Pi_{n+1}, ev_{n+1}, udiff, goodcell (the step path does not
+//         write the stage-2 sums of the sources back).
+#ifndef CV_STAGE_WAVES
+#define CV_STAGE_WAVES 1
+#endif
+template <bool BULK, int STEP>
+__global__ __launch_bounds__(256, CV_STAGE_WAVES) void k_visc_stage(PhysP p, EosP E, StageP st,
+        const float *__restrict__ ev_st, const float *__restrict__ pi_st, const float *__restrict__ bk_st,
+        const float *ev1, const float *ev2, const float *pi1, const float *bk1,
+        const float *__restrict__ udiff_in, float *IS_src, float *bsrc, float *Src,
+        float *pi_out, float *bk_out, float *ev_out, float *good, float *udiff_out) {
+    int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= p.n) return;
+    Eos eos(E, E.tab);
+    Cell cl = cell_of(p, I);
+    const int64_t n = p.n;
+    float s[10], bs = 0.0f, t[4];
+    if (STEP == 1) {
+#pragma unroll
+        for (int m = 0; m < 10; m++) s[m] = 0.0f;
+        t[0] = t[1] = t[2] = t[3] = 0.0f;
+    } else {
+#pragma unroll
+        for (int m = 0; m < 10; m++) s[m] = IS_src[(int64_t)m * n + I];
+        if (BULK) bs = bsrc[I];
+        float4 q = ld4(Src, n, I);
+        t[0] = q.x; t[1] = q.y; t[2] = q.z; t[3] = q.w;
+    }
+    const float4 ec = ld4(ev_st, n, I);
+    float q[10];
+#pragma unroll
+    for (int m = 0; m < 10; m++) q[m] = pi_st[(int64_t)m * n + I];
+    // visc_src_christoffel (kernel_IS.cl:41-67)
+    {
+        float uz = ec.w * gamma3(ec.y, ec.z, ec.w);
+        s[0] -= (2.0f * uz * q[3]) * st.r_tau;
+        s[1] -= (uz * q[6]) * st.r_tau;
+        s[2] -= (uz * q[8]) * st.r_tau;
+        s[3] -= (uz * (q[0] + q[9])) * st.r_tau;
+        s[6] -= (uz * q[1]) * st.r_tau;
+        s[8] -= (uz * q[2]) * st.r_tau;
+        s[9] -= (2.0f * uz * q[3]) * st.r_tau;
+    }
+    // kt_src_christoffel (kernel_visc.cl:11-53)
+    {
+        float u0 = gamma3(ec.y, ec.z, ec.w);
+        float bulkpi = BULK ? bk_st[I] : 0.0f;
+        float pressure = eos.P(ec.x) + bulkpi;
+        float Tzz = (ec.x + pressure) * u0 * u0 * ec.w * ec.w + pressure + q[9];
+        float Ttz = (ec.x + pressure) * u0 * u0 * ec.w + q[3];
+        t[0] = t[0] - Tzz; t[1] = t[1] - 0.0f; t[2] = t[2] - 0.0f; t[3] = t[3] - Ttz;
+        if (p.flags & 1) {
+            float Ttt = (ec.x + pressure) * u0 * u0 - pressure;
+            float Ttx = (ec.x + pressure) * u0 * u0 * ec.y;
+            float Tty = (ec.x + pressure) * u0 * u0 * ec.z;
+            t[0] = t[0] + (Tzz + Ttt); t[1] = t[1] + Ttx; t[2] = t[2] + Tty; t[3] = t[3] + 3.0f * Ttz;
+        }
+    }
+    float udx[4], udy[4], udz[4];
+    // scheduling fences keep each axis' stencil loads within its own phase (register pressure)
+    CV_PHASE_FENCE();
+    visc_axis<0, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s, bs, t, udx);
+    CV_PHASE_FENCE();
+    visc_axis<1, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s, bs, t, udy);
+    CV_PHASE_FENCE();
+    visc_axis<2, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s, bs, t, udz);
+    CV_PHASE_FENCE();
+
+    // ---- update_pimn (kernel_IS.cl:501-709) ----
+    const float4 e1 = STEP == 1 ? ec : ld4(ev1, n, I);
+    const float4 e2 = STEP == 2 ? ec : ld4(ev2, n, I);
+    float g1 = gamma3(e1.y, e1.z, e1.w);
+    float u_old[4] = {g1 * 1.0f, g1 * e1.y, g1 * e1.z, g1 * e1.w};
+    float u_new[4], udt[4];
+    if (STEP == 1) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            float ud_ = udiff_in[(int64_t)c * n + I];
+            udt[c] = ud_ * p.r_dt;
+            u_new[c] = u_old[c] + ud_;
+        }
+    } else {
+        float g2 = gamma3(e2.y, e2.z, e2.w);
+        u_new[0] = g2 * 1.0f; u_new[1] = g2 * e2.y; u_new[2] = g2 * e2.z; u_new[3] = g2 * e2.w;
+#pragma unroll
+        for (int c = 0; c < 4; c++) udt[c] = (u_new[c] - u_old[c]) * p.r_dt;
+    }
+    float da[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) { da[c][0] = udt[c]; da[c][1] = udx[c]; da[c][2] = udy[c]; da[c][3] = udz[c]; }
+    float u[4], DU[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) u[c] = STEP == 1 ? u_old[c] : u_new[c];
+#pragma unroll
+    for (int c = 0; c < 4; c++) DU[c] = dot4(u, da[c]);
+    const float ed_step = STEP == 1 ? e1.x : e2.x;
+    const float theta = udt[0] + udx[1] + udy[2] + udz[3];
+    const float4 cpTs = eos.raw(ed_step);
+    const float temperature = cpTs.z;
+    const float local_etaos = etaos(p, temperature);
+    float etav = local_etaos * cpTs.w * CV_HBARC;
+    float one_over_taupi = temperature * rcp(3.0f * rmax(CV_ACU, local_etaos) * CV_HBARC);
+    float coef_pipi = 0.0f;
+    const bool gubser = p.flags & 2;
+    if (gubser) {
+        float q4 = sqrtf(sqrtf(ed_step));
+        float ETAOS = p.etaos_ymin;
+        float etavH = (ETAOS * 4.0f) * p.r_3;
+        float taupiH = (p.lam1 * p.lam1 * etavH) * p.r_3;
+        one_over_taupi = rcp(taupiH * rcp(q4));
+        coef_pipi = p.lam1 * rcp(ed_step);
+        etav = ((ETAOS * 4.0f) * p.r_3) * (sqrtf(ed_step) * q4);
+    }
+    const float gd[4] = {1.0f, -1.0f, -1.0f, -1.0f};
+    float max_abs = 0.0f;
+    float out[10];
+    const float rden = rcp(u_new[0] + p.dt * one_over_taupi);
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) {
+#pragma unroll
+        for (int nu = mu; nu < 4; nu++) {
+            float gmunu = mu == nu ? gd[mu] : 0.0f;
+            float sigma = gd[mu] * da[nu][mu] + gd[nu] * da[mu][nu] - (u[mu] * DU[nu] + u[nu] * DU[mu])
+                          - 2.0f / 3.0f * (gmunu - u[mu] * u[nu]) * theta;
+            int mn = IDX(mu, nu);
+            if (u[0] > 10000.0f) sigma = 0.0f;
+            float piNS = etav * sigma;
+            float pi_old = (STEP == 1 ? q[mn] : pi1[(int64_t)mn * n + I]) * u_old[0];
+            float src = (-q[mn] * theta) * p.r_3 - (q[mn] * u[0]) * st.r_tau;
+#pragma unroll
+            for (int l = 0; l < 4; l++)
+                src -= (u[mu] * q[IDX(nu, l)] + u[nu] * q[IDX(mu, l)]) * DU[l] * gd[l];
+            if (gubser)
+                src -= one_over_taupi * coef_pipi * (pipi(0, mu, nu, q, u) - pipi(1, mu, nu, q, u)
+                                                     - pipi(2, mu, nu, q, u) - pipi(3, mu, nu, q, u));
+            s[mn] += src;
+            pi_old = (pi_old + (s[mn] * p.dt) * st.r_step + p.dt * one_over_taupi * piNS) * rden;
+            if (fabsf(pi_old) > max_abs) max_abs = fabsf(pi_old);
+            out[mn] = pi_old;
+        }
+    }
+    float bout = 0.0f, b_start = 0.0f;
+    if (BULK) {
+        float zs = zetaos(p, temperature);
+        float zetav = zs * cpTs.w * CV_HBARC;
+        float oot_b = temperature * rcp(3.0f * rmax(CV_ACU, zs) * CV_HBARC);
+        float b2 = bk_st[I];
+        float bsrc_ = (b2 * theta) * p.r_3 - (b2 * u[0]) * st.r_tau;
+        bs += bsrc_;
+        b_start = STEP == 1 ? b2 : bk1[I];
+        float b_old = b_start * u_old[0];
+        bout = (b_old + (bs * p.dt) * st.r_step - p.dt * oot_b * zetav * theta) * rcp(u_new[0] + p.dt * oot_b);
+        if (fabsf(bout) > max_abs) max_abs = fabsf(bout);
+    }
+    {
+        float pre = eos.P(e2.x);
+        float T00 = (e2.x + pre) * u_new[0] * u_new[0] - pre;
+        float gc = 1.0f;
+        if (max_abs > T00) {
+#pragma unroll
+            for (int m = 0; m < 10; m++) out[m] = 0.0f;
+            bout = 0.0f;
+            gc = 0.0f;
+        }
+        good[I] = gc;
+    }
+#pragma unroll
+    for (int m = 0; m < 10; m++) pi_out[(int64_t)m * n + I] = out[m];
+    if (BULK) bk_out[I] = bout;
+    if (STEP == 1) {
+#pragma unroll
+        for (int m = 0; m < 10; m++) IS_src[(int64_t)m * n + I] = s[m];
+        if (BULK) bsrc[I] = bs;
+        st4(Src, n, I, make_float4(t[0], t[1], t[2], t[3]));
+    }
+
+    // ---- update_ev (kernel_visc.cl:322-393): start point ev_n, pi_n; new pi in registers ----
+    float pressure = eos.P(e1.x) + (BULK ? b_start : 0.0f);
+    float T0m[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        float g0 = c == 0 ? 1.0f : 0.0f;
+        float pim0 = STEP == 1 ? q[c] : pi1[(int64_t)c * n + I];
+        T0m[c] = ((e1.x + pressure) * g1 * u_old[c] - pressure * g0 + pim0) * st.old_time;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) T0m[c] = T0m[c] + (t[c] * p.dt) * st.r_step - out[c] * st.tau;
+    float T00 = rmax(CV_ACU, T0m[0]) * st.r_tau;
+    float T01 = fabsf(T0m[1]) < CV_ACU ? 0.0f : T0m[1] * st.r_tau;
+    float T02 = fabsf(T0m[2]) < CV_ACU ? 0.0f : T0m[2] * st.r_tau;
+    float T03 = fabsf(T0m[3]) < CV_ACU ? 0.0f : T0m[3] * st.r_tau;
+    float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
+    if (M > T00) {
+        float sc = (0.999f * T00) * rcp(M);
+        T01 *= sc; T02 *= sc; T03 *= sc;
+        M = 0.999f * T00;
+    }
+    float ed_find;
+    if ((p.flags & 8) && M > p.fp64_vmin * T00)
+        ed_find = (float)root_newton_d(E, E.tab, (double)T00, (double)M, (double)bout);
+    else
+        ed_find = root_newton(eos, T00, M, bout);
+    if (ed_find < CV_ACU) { T01 = 0.0f; T02 = 0.0f; T03 = 0.0f; }
+    ed_find = rmax(0.0f, ed_find);
+    float pr = eos.P(ed_find) + bout;
+    float repv = rcp(rmax(CV_ACU, T00 + pr));
+    float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
+    st4(ev_out, n, I, en);
+    if (STEP == 2) {
+        float gn = gamma3(en.y, en.z, en.w);
+        float un[4] = {gn * 1.0f, gn * en.y, gn * en.z, gn * en.w};
+#pragma unroll
+        for (int c = 0; c < 4; c++) udiff_out[(int64_t)c * n + I] = un[c] - u_old[c];
     }
 }
 

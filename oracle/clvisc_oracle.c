@@ -232,9 +232,19 @@ static real minmod(real x, real y) {
 }
 
 /* generalised minmod slope at b from (a, b, c) = cells (i-1, i, i+1), as written in
- * kt1d / kt1d_visc / kt1d_real (helper.h:213-217, kernel_visc.cl:80-84). */
+ * kt1d / kt1d_visc / kt1d_real (helper.h:213-217, kernel_visc.cl:80-84):
+ *   minmod(0.5 (c-a), minmod(THETA (c-b), THETA (b-a))).
+ * Evaluated in the equivalent closed form: when c-b and b-a have the same strict sign
+ * (and then c-a has it too) the value is that sign times the smallest of
+ * 0.5|c-a|, THETA|c-b|, THETA|b-a|, otherwise 0.  Same value bit for bit (every product
+ * with 0.5, 2 or a sign is exact); only the sign of an exact zero may differ. */
 static real slope(real a, real b, real c) {
-    return minmod(RC(0.5f) * (c - a), minmod(THETA * (c - b), THETA * (b - a)));
+    real d2 = c - b, d3 = b - a;
+    if (!((d2 > RC(0.0f) && d3 > RC(0.0f)) || (d2 < RC(0.0f) && d3 < RC(0.0f)))) return RC(0.0f);
+    real m = RC(0.5f) * (real)fabs((double)(c - a));
+    real t2 = THETA * (real)fabs((double)d2), t3 = THETA * (real)fabs((double)d3);
+    m = r_min(m, r_min(t2, t3));
+    return d2 > RC(0.0f) ? m : -m;
 }
 
 /* T^{tau mu} = (e+P) u^tau u^mu - P g^{tau mu}  (helper.h:172-177) */
@@ -871,8 +881,7 @@ void orc_is_along(const orc_params *p, int axis, real *Src, real *ud, const real
             real um1[4], ui[4], up1[4];
             umu4(e[1], um1); umu4(e[2], ui); umu4(e[3], up1);
             for (int c = 0; c < 4; c++) {
-                real d = minmod(RC(0.5f) * (up1[c] - um1[c]),
-                                minmod(THETA * (ui[c] - um1[c]), THETA * (up1[c] - ui[c])));
+                real d = slope(um1[c], ui[c], up1[c]);
                 if (axis < 2) ud[4 * IND + c] = DIV(d, DX);
                 else {
                     real chr = DIV((c == 0 ? e[2][3] : (c == 3 ? RC(1.0f) : RC(0.0f))) * u0[2], tau);
