@@ -86,6 +86,8 @@ def lib():
             'cv_sync': [vp],
             'cv_last_elapsed_ms': [vp, C.POINTER(f32)],
             'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
+            'cv_set_stage_kernel': [vp, C.c_int, C.c_int, C.c_int],
+            'cv_get_stage_kernel': [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -105,7 +107,8 @@ def exported_symbols():
     return ['cv_last_error', 'cv_device_info', 'cv_create', 'cv_destroy', 'cv_set_field',
             'cv_get_field', 'cv_copy_field', 'cv_visc_initialize', 'cv_ideal_stage', 'cv_is_stage',
             'cv_visc_stage', 'cv_update_udiff', 'cv_ideal_steps_async', 'cv_visc_steps_async',
-            'cv_max_ed', 'cv_sync', 'cv_last_elapsed_ms', 'cv_profile_visc_step']
+            'cv_max_ed', 'cv_sync', 'cv_last_elapsed_ms', 'cv_profile_visc_step',
+            'cv_set_stage_kernel', 'cv_get_stage_kernel']
 
 
 def make_eos(table):
@@ -201,6 +204,14 @@ class Handle(object):
         nl = C.c_int32()
         check(self.L.cv_profile_visc_step(self.h, loop0, ms, 16, C.byref(nl)))
         return [ms[i] for i in range(nl.value)]
+
+    def set_stage_kernel(self, kind, ty=0, tz=0):
+        check(self.L.cv_set_stage_kernel(self.h, kind, ty, tz))
+
+    def stage_kernel(self):
+        k, ty, tz = C.c_int32(), C.c_int32(), C.c_int32()
+        check(self.L.cv_get_stage_kernel(self.h, C.byref(k), C.byref(ty), C.byref(tz)))
+        return k.value, ty.value, tz.value
 
     def last_elapsed_ms(self):
         out = C.c_float()

@@ -11,6 +11,7 @@
 
 #include "../../include/clvisc_amd.h"
 #include "cv_kernels.h"
+#include "cv_tiled.h"
 
 using namespace cv;
 
@@ -51,7 +52,55 @@ struct cv_handle {
     hipStream_t stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     bool timed = false;
+    int cus = 0;
+    int kernel = 1;           // step path: 0 per-cell fused kernel, 1 tiled kernel
+    TileP tile;
+    int tile_grid = 0;
 };
+
+// Tile of the tiled stage kernel: ty x tz cells of one x-plane with a 2-cell halo in y and
+// z (none along an axis of length 1).  Checks the LDS capacities of cv_tiled.h.
+static int set_tile(cv_handle *h, int ty, int tz) {
+    const cv_params &p = h->prm;
+    if (ty < 1 || tz < 1 || ty * tz > CV_TILE_THREADS)
+        return fail(CV_ERR_ARG, "tile %dx%d: need 1 <= ty*tz <= %d", ty, tz, CV_TILE_THREADS);
+    TileP t;
+    t.ty = ty; t.tz = tz;
+    t.hy = p.ny > 1 ? 2 : 0;
+    t.hz = p.nz > 1 ? 2 : 0;
+    t.HY = ty + 2 * t.hy; t.HZ = tz + 2 * t.hz;
+    if (t.HY * t.HZ > CV_TILE_MAXC) return fail(CV_ERR_ARG, "tile %dx%d: %d halo cells exceed %d", ty, tz, t.HY * t.HZ, CV_TILE_MAXC);
+    if ((t.hy > 0 && (ty + 1) * tz > CV_TILE_MAXF) || (t.hz > 0 && ty * (tz + 1) > CV_TILE_MAXF))
+        return fail(CV_ERR_ARG, "tile %dx%d: too many faces for the LDS face buffer", ty, tz);
+    t.ntj = (p.ny + ty - 1) / ty;
+    t.ntk = (p.nz + tz - 1) / tz;
+    t.ntiles = (int64_t)p.nx * t.ntj * t.ntk;
+    h->tile = t;
+    int64_t g = (int64_t)(h->cus > 0 ? h->cus : 256) * 2 * 4;
+    h->tile_grid = (int)(t.ntiles < g ? t.ntiles : g);
+    return CV_OK;
+}
+
+// default tile: least padding over (ny, nz), then most threads
+static int default_tile(cv_handle *h) {
+    const cv_params &p = h->prm;
+    if (p.nz == 1) return set_tile(h, p.ny >= 256 ? 256 : p.ny, 1);
+    int best_ty = 0, best_tz = 0;
+    double best = 1e30;
+    for (int tz = 4; tz <= 64; tz++) {
+        for (int ty = 1; ty * tz <= CV_TILE_THREADS; ty++) {
+            int hy = p.ny > 1 ? 2 : 0;
+            if ((ty + 2 * hy) * (tz + 4) > CV_TILE_MAXC) continue;
+            if ((hy > 0 && (ty + 1) * tz > CV_TILE_MAXF) || ty * (tz + 1) > CV_TILE_MAXF) continue;
+            if (ty * tz < 192) continue;
+            double pad = (double)((p.ny + ty - 1) / ty * ty) * ((p.nz + tz - 1) / tz * tz) / ((double)p.ny * p.nz);
+            double score = pad * (256.0 / (ty * tz)) * (1.0 + 0.02 * ((double)(ty + 4) * (tz + 4) / (ty * tz)));
+            if (score < best) { best = score; best_ty = ty; best_tz = tz; }
+        }
+    }
+    if (!best_ty) return set_tile(h, 8, 32);
+    return set_tile(h, best_ty, best_tz);
+}
 
 extern "C" const char *cv_last_error(void) { return g_err.c_str(); }
 
@@ -97,6 +146,9 @@ extern "C" int cv_create(const cv_params *p, const cv_eos *eos, int device, int 
         return fail(CV_ERR_ARG, "cv_create: unknown EOS kind %d", eos->kind);
     }
     if ((p->flags & CV_FLAG_BULK) && !viscous) return fail(CV_ERR_ARG, "cv_create: bulk needs viscous");
+    if ((int64_t)p->nx * p->ny * p->nz >= ((int64_t)1 << 30))
+        return fail(CV_ERR_ARG, "cv_create: %lld cells exceed the 2^30 cells of one field plane",
+                    (long long)p->nx * p->ny * p->nz);
     HIP_TRY(hipSetDevice(device));
     cv_handle *h = new cv_handle();
     h->prm = *p;
@@ -157,6 +209,14 @@ extern "C" int cv_create(const cv_params *p, const cv_eos *eos, int device, int 
             for (int k = 0; ok && k < 3; k++) ok = alloc(&h->bk[k], n);
             ok = ok && alloc(&h->bsrc, n);
         }
+    }
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->cus = prop.multiProcessorCount;
+    }
+    if (ok && (rc = default_tile(h)) != CV_OK) {
+        cv_destroy(h);
+        return rc;
     }
     ok = ok && hipMalloc((void **)&h->d_max, sizeof(unsigned int)) == hipSuccess;
     ok = ok && hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) == hipSuccess;
@@ -303,6 +363,21 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
                         float *ev1, float *ev2, float *pi1, float *bk1, float *pi_out, float *bk_out,
                         float *ev_out, float *udiff_out) {
     StageP st = stage_params(h, step, tau);
+    if (h->kernel == 1) {
+        dim3 g(h->tile_grid), b(CV_TILE_THREADS);
+#define CV_TILED_ARGS h->phys, h->eos, st, h->tile, ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src, \
+                      h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out
+        if (h->bulk) {
+            if (step == 1) hipLaunchKernelGGL((k_visc_tiled<true, 1>), g, b, 0, h->stream, CV_TILED_ARGS);
+            else hipLaunchKernelGGL((k_visc_tiled<true, 2>), g, b, 0, h->stream, CV_TILED_ARGS);
+        } else {
+            if (step == 1) hipLaunchKernelGGL((k_visc_tiled<false, 1>), g, b, 0, h->stream, CV_TILED_ARGS);
+            else hipLaunchKernelGGL((k_visc_tiled<false, 2>), g, b, 0, h->stream, CV_TILED_ARGS);
+        }
+#undef CV_TILED_ARGS
+        HIP_TRY(hipGetLastError());
+        return CV_OK;
+    }
     dim3 g = grid1d(h->n), b(256);
 #define CV_STAGE_ARGS h->phys, h->eos, st, ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src, \
                       h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out
@@ -440,6 +515,24 @@ extern "C" int cv_profile_visc_step(cv_handle *h, int64_t loop0, float *ms, int 
     }
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
     return rc;
+}
+
+extern "C" int cv_set_stage_kernel(cv_handle *h, int kind, int ty, int tz) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    if (kind != 0 && kind != 1) return fail(CV_ERR_ARG, "kernel kind must be 0 (per-cell) or 1 (tiled)");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (kind == 1) {
+        int rc = (ty > 0 && tz > 0) ? set_tile(h, ty, tz) : default_tile(h);
+        if (rc) return rc;
+    }
+    h->kernel = kind;
+    return CV_OK;
+}
+
+extern "C" int cv_get_stage_kernel(cv_handle *h, int32_t *kind, int32_t *ty, int32_t *tz) {
+    if (!h || !kind || !ty || !tz) return fail(CV_ERR_ARG, "null argument");
+    *kind = h->kernel; *ty = h->tile.ty; *tz = h->tile.tz;
+    return CV_OK;
 }
 
 extern "C" int cv_max_ed(cv_handle *h, int which, float *out) {

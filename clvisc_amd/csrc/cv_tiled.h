@@ -1,0 +1,474 @@
+// cv_tiled.h -- tiled fused stage kernel of the viscous step (gfx950).
+//
+// One workgroup owns a tile of TY x TZ cells of one x-plane and loops (persistent) over
+// tiles.  Per tile:
+//   1. per-cell quantities (e, v, u^tau, P+Pi, Q = tau (T0^{tau mu} + pi^{tau mu}), pi, Pi)
+//      of the tile and its 2-cell y/z halo are computed once into LDS;
+//   2. x: each thread reads its own x-stencil (cells i-2..i+2) from global memory and
+//      evaluates the two x faces of its cell;
+//   3. y, z: every face of the tile is evaluated once from LDS, stored in LDS and
+//      differenced by the two cells it separates;
+//   4. update_pimn and update_ev of the cell (as in the per-cell kernel).
+// A face value F(i+1/2) is the expression the reference evaluates as the first half of
+// kt1d_visc / kt1d_real at cell i and again as the second half at cell i+1
+// (kernel_visc.cl:86-124, helper.h:275-302): both halves are the same function of the
+// cells i-1, i, i+1, i+2, so evaluating it once gives the reference's value.
+// Boundary ghosts (kernel_ideal.cl:57-62) are the clamped cell indices.
+// The octave EOS table is copied into LDS once per workgroup.
+#pragma once
+#include "cv_kernels.h"
+
+namespace cv {
+
+// per-cell fields held in LDS (structure of arrays)
+enum { CF_E = 0, CF_VX, CF_VY, CF_VZ, CF_U0, CF_P, CF_Q0, CF_Q1, CF_Q2, CF_Q3, CF_PI0, CF_BK = CF_PI0 + 10, CF_N };
+
+#define CV_TILE_THREADS 256
+#define CV_TILE_MAXC 432       // cells of tile + halo held in LDS (2 workgroups per CU)
+#define CV_TILE_MAXF 288       // faces of one axis held in LDS
+#define CV_EOS_LDS 1600        // octave-table entries held in LDS
+
+struct TileP {
+    int ty, tz;                // tile interior
+    int hy, hz;                // halo widths (2, or 0 for an axis of length 1)
+    int HY, HZ;                // ty + 2 hy, tz + 2 hz
+    int ntj, ntk;              // tiles along j, k
+    int64_t ntiles;            // nx * ntj * ntk
+};
+
+// per-cell quantities of one cell from its raw fields
+__device__ __forceinline__ void cell_prims(const PhysP &p, const Eos &eos, float tau, float4 e, const float pi[10],
+                                           float bk, float out[CF_N]) {
+    float u0 = gamma3(e.y, e.z, e.w);
+    float pr = eos.P(e.x) + bk;
+    float4 T = t0m_u(e, pr, u0);
+    out[CF_E] = e.x; out[CF_VX] = e.y; out[CF_VY] = e.z; out[CF_VZ] = e.w;
+    out[CF_U0] = u0; out[CF_P] = pr;
+    out[CF_Q0] = tau * (T.x + pi[0]);
+    out[CF_Q1] = tau * (T.y + pi[1]);
+    out[CF_Q2] = tau * (T.z + pi[2]);
+    out[CF_Q3] = tau * (T.w + pi[3]);
+#pragma unroll
+    for (int m = 0; m < 10; m++) out[CF_PI0 + m] = pi[m];
+    out[CF_BK] = bk;
+}
+
+// Face value between cells L and R along axis A, from the cells Lm = L-1, L, R, Rp = R+1.
+// G(field, cell) returns a per-cell quantity.  out: 4 KT components, 10 IS components, Pi.
+template <int A, bool BULK, typename G>
+__device__ __forceinline__ void face_flux(const PhysP &p, const Eos &eos, float tau, const G &g, int Lm, int L,
+                                          int R, int Rp, float out[15]) {
+    const float4 eL = make_float4(g(CF_E, L), g(CF_VX, L), g(CF_VY, L), g(CF_VZ, L));
+    const float4 eR = make_float4(g(CF_E, R), g(CF_VX, R), g(CF_VY, R), g(CF_VZ, R));
+    const float vh = 0.5f * (comp(eL, A + 1) + comp(eR, A + 1));
+    const float lam = max_prop_speed(p, eos, half4(eL, eR), vh);
+    // KT flux of Q (kernel_visc.cl:86-101)
+    const float pr_half = 0.5f * (g(CF_P, L) + g(CF_P, R));
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        float DAL = slope(g(CF_Q0 + q, Lm), g(CF_Q0 + q, L), g(CF_Q0 + q, R));
+        float DAR = slope(g(CF_Q0 + q, L), g(CF_Q0 + q, R), g(CF_Q0 + q, Rp));
+        float AL = g(CF_Q0 + q, L) + 0.5f * DAL;
+        float AR = g(CF_Q0 + q, R) - 0.5f * DAR;
+        const float g0 = q == 0 ? 1.0f : 0.0f;
+        const float ga = q == A + 1 ? -1.0f : 0.0f;
+        float p0h = 0.5f * (g(CF_PI0 + PI_ROW[0][q], L) + g(CF_PI0 + PI_ROW[0][q], R));
+        float pih = 0.5f * (g(CF_PI0 + PI_ROW[A + 1][q], L) + g(CF_PI0 + PI_ROW[A + 1][q], R));
+        float Jp = (AR + pr_half * tau * g0 - tau * p0h) * vh - pr_half * tau * ga + tau * pih;
+        float Jm = (AL + pr_half * tau * g0 - tau * p0h) * vh - pr_half * tau * ga + tau * pih;
+        out[q] = 0.5f * (Jp + Jm) - 0.5f * lam * (AR - AL);
+    }
+    // IS flux of u^tau pi^{mu nu} (helper.h:275-289)
+    const float u0Lm = g(CF_U0, Lm), u0L = g(CF_U0, L), u0R = g(CF_U0, R), u0Rp = g(CF_U0, Rp);
+#pragma unroll
+    for (int mn = 0; mn < 11; mn++) {
+        if (mn == 10 && !BULK) { out[14] = 0.0f; break; }
+        const int f = mn < 10 ? CF_PI0 + mn : CF_BK;
+        float QLm = u0Lm * g(f, Lm), QL = u0L * g(f, L), QR = u0R * g(f, R), QRp = u0Rp * g(f, Rp);
+        float DAL = slope(QLm, QL, QR);
+        float DAR = slope(QL, QR, QRp);
+        float AL = QL + 0.5f * DAL;
+        float AR = QR - 0.5f * DAR;
+        float Jp = AR * vh, Jm = AL * vh;
+        out[4 + mn] = 0.5f * (Jp + Jm) - 0.5f * lam * (AR - AL);
+    }
+}
+
+template <bool BULK, int STEP>
+__global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E, StageP st, TileP tp,
+        const float *__restrict__ ev_st, const float *__restrict__ pi_st, const float *__restrict__ bk_st,
+        const float *ev1, const float *ev2, const float *pi1, const float *bk1,
+        const float *__restrict__ udiff_in, float *IS_src, float *bsrc, float *Src,
+        float *pi_out, float *bk_out, float *ev_out, float *good, float *udiff_out) {
+    __shared__ float s_c[CF_N][CV_TILE_MAXC];
+    __shared__ float s_f[15][CV_TILE_MAXF];
+    __shared__ float4 s_eos[CV_EOS_LDS];
+    const int tid = threadIdx.x;
+    const int64_t n = p.n;
+    const float tau = st.tau;
+
+    // EOS table into LDS (octave table small enough), else global
+    const float4 *tab = E.tab;
+    if (E.kind == 1 && E.nlast + 1 <= CV_EOS_LDS) {
+        for (int q = tid; q <= E.nlast; q += CV_TILE_THREADS) s_eos[q] = E.tab[q];
+        tab = s_eos;
+    }
+    __syncthreads();
+    Eos eos(E, tab);
+
+    const int ty = tp.ty, tz = tp.tz, HY = tp.HY, HZ = tp.HZ, hy = tp.hy, hz = tp.hz;
+    const int my = tid / tz, mz = tid - (tid / tz) * tz;       // own cell in the tile
+    const bool lane_in_tile = tid < ty * tz;
+
+    for (int64_t tile = blockIdx.x; tile < tp.ntiles; tile += gridDim.x) {
+        const int kt = (int)(tile % tp.ntk);
+        const int64_t r = tile / tp.ntk;
+        const int jt = (int)(r % tp.ntj);
+        const int i = (int)(r / tp.ntj);
+        const int j0 = jt * ty, k0 = kt * tz;
+        const uint32_t plane = (uint32_t)((int64_t)i * p.sx);
+
+        // ---- 1. per-cell quantities of tile + y/z halo into LDS ----
+        for (int c = tid; c < HY * HZ; c += CV_TILE_THREADS) {
+            const int ly = c / HZ, lz = c - (c / HZ) * HZ;
+            const bool hy_out = ly < hy || ly >= hy + ty, hz_out = lz < hz || lz >= hz + tz;
+            if (hy_out && hz_out) continue;                       // corner: not needed
+            int j = j0 + ly - hy, k = k0 + lz - hz;
+            j = j < 0 ? 0 : (j > p.ny - 1 ? p.ny - 1 : j);
+            k = k < 0 ? 0 : (k > p.nz - 1 ? p.nz - 1 : k);
+            const uint32_t bo = (plane + (uint32_t)j * (uint32_t)p.sy + (uint32_t)k) * 4u;
+            float4 e = ld4o(ev_st, n, bo);
+            float pv[10];
+#pragma unroll
+            for (int m = 0; m < 10; m++) pv[m] = ldo(pi_st + (int64_t)m * n, bo);
+            float bkv = BULK ? ldo(bk_st, bo) : 0.0f;
+            float out[CF_N];
+            cell_prims(p, eos, tau, e, pv, bkv, out);
+#pragma unroll
+            for (int f = 0; f < CF_N; f++) s_c[f][c] = out[f];
+        }
+        __syncthreads();
+
+        const int j = j0 + my, k = k0 + mz;
+        const bool active = lane_in_tile && j < p.ny && k < p.nz;
+        const int cc = (my + hy) * HZ + (mz + hz);                  // own LDS cell
+        const uint32_t I = plane + (uint32_t)j * (uint32_t)p.sy + (uint32_t)k;
+        const uint32_t bo = I * 4u;
+        auto gl = [&](int f, int c) { return s_c[f][c]; };
+
+        float s[10], bs = 0.0f, t[4], udx[4], udy[4], udz[4];
+        float q[10];
+        float4 ec = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (active) {
+            if (STEP == 1) {
+#pragma unroll
+                for (int m = 0; m < 10; m++) s[m] = 0.0f;
+                t[0] = t[1] = t[2] = t[3] = 0.0f;
+            } else {
+#pragma unroll
+                for (int m = 0; m < 10; m++) s[m] = ldo(IS_src + (int64_t)m * n, bo);
+                if (BULK) bs = ldo(bsrc, bo);
+                float4 qq = ld4o(Src, n, bo);
+                t[0] = qq.x; t[1] = qq.y; t[2] = qq.z; t[3] = qq.w;
+            }
+            ec = make_float4(s_c[CF_E][cc], s_c[CF_VX][cc], s_c[CF_VY][cc], s_c[CF_VZ][cc]);
+#pragma unroll
+            for (int m = 0; m < 10; m++) q[m] = s_c[CF_PI0 + m][cc];
+            // visc_src_christoffel (kernel_IS.cl:41-67)
+            {
+                float uz = ec.w * s_c[CF_U0][cc];
+                s[0] -= (2.0f * uz * q[3]) * st.r_tau;
+                s[1] -= (uz * q[6]) * st.r_tau;
+                s[2] -= (uz * q[8]) * st.r_tau;
+                s[3] -= (uz * (q[0] + q[9])) * st.r_tau;
+                s[6] -= (uz * q[1]) * st.r_tau;
+                s[8] -= (uz * q[2]) * st.r_tau;
+                s[9] -= (2.0f * uz * q[3]) * st.r_tau;
+            }
+            // kt_src_christoffel (kernel_visc.cl:11-53)
+            {
+                float u0 = s_c[CF_U0][cc];
+                float pressure = s_c[CF_P][cc];
+                float Tzz = (ec.x + pressure) * u0 * u0 * ec.w * ec.w + pressure + q[9];
+                float Ttz = (ec.x + pressure) * u0 * u0 * ec.w + q[3];
+                t[0] = t[0] - Tzz; t[1] = t[1] - 0.0f; t[2] = t[2] - 0.0f; t[3] = t[3] - Ttz;
+                if (p.flags & 1) {
+                    float Ttt = (ec.x + pressure) * u0 * u0 - pressure;
+                    float Ttx = (ec.x + pressure) * u0 * u0 * ec.y;
+                    float Tty = (ec.x + pressure) * u0 * u0 * ec.z;
+                    t[0] = t[0] + (Tzz + Ttt); t[1] = t[1] + Ttx; t[2] = t[2] + Tty; t[3] = t[3] + 3.0f * Ttz;
+                }
+            }
+            // ---- 2. x: own stencil i-2..i+2 from global memory ----
+            {
+                float xc[5][CF_N];
+#pragma unroll
+                for (int w = 0; w < 5; w++) {
+                    if (w == 2) {
+#pragma unroll
+                        for (int f = 0; f < CF_N; f++) xc[2][f] = s_c[f][cc];
+                    } else {
+                        int ii = i + w - 2;
+                        ii = ii < 0 ? 0 : (ii > p.nx - 1 ? p.nx - 1 : ii);
+                        const uint32_t b2 = (I + (uint32_t)(ii - i) * (uint32_t)p.sx) * 4u;
+                        float4 e = ld4o(ev_st, n, b2);
+                        float pv[10];
+#pragma unroll
+                        for (int m = 0; m < 10; m++) pv[m] = ldo(pi_st + (int64_t)m * n, b2);
+                        float bkv = BULK ? ldo(bk_st, b2) : 0.0f;
+                        cell_prims(p, eos, tau, e, pv, bkv, xc[w]);
+                    }
+                }
+                auto gx = [&](int f, int c) { return xc[c][f]; };
+                float Fp[15], Fm[15];
+                face_flux<0, BULK>(p, eos, tau, gx, 1, 2, 3, 4, Fp);
+                face_flux<0, BULK>(p, eos, tau, gx, 0, 1, 2, 3, Fm);
+#pragma unroll
+                for (int m = 0; m < 4; m++) t[m] = t[m] - (Fp[m] - Fm[m]) * p.r_dx;
+#pragma unroll
+                for (int m = 0; m < 10; m++) s[m] = s[m] - (Fp[4 + m] - Fm[4 + m]) * p.r_dx;
+                if (BULK) bs = bs - (Fp[14] - Fm[14]) * p.r_dx;
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    float um1 = c == 0 ? xc[1][CF_U0] * 1.0f : xc[1][CF_U0] * xc[1][CF_E + c];
+                    float ui = c == 0 ? xc[2][CF_U0] * 1.0f : xc[2][CF_U0] * xc[2][CF_E + c];
+                    float up1 = c == 0 ? xc[3][CF_U0] * 1.0f : xc[3][CF_U0] * xc[3][CF_E + c];
+                    udx[c] = slope(um1, ui, up1) * p.r_dx;
+                }
+            }
+        }
+
+        // ---- 3. y faces: face f in [0, ty] lies between tile rows f-1 and f ----
+        if (tp.hy > 0) {
+            for (int fc = tid; fc < (ty + 1) * tz; fc += CV_TILE_THREADS) {
+                const int f = fc / tz, z = fc - (fc / tz) * tz;
+                const int Lc = (f - 1 + hy) * HZ + (z + hz);
+                float F[15];
+                face_flux<1, BULK>(p, eos, tau, gl, Lc - HZ, Lc, Lc + HZ, Lc + 2 * HZ, F);
+#pragma unroll
+                for (int m = 0; m < 15; m++) s_f[m][fc] = F[m];
+            }
+        }
+        __syncthreads();
+        if (active) {
+            if (tp.hy > 0) {
+                const int fp_ = (my + 1) * tz + mz, fm_ = my * tz + mz;
+#pragma unroll
+                for (int m = 0; m < 4; m++) t[m] = t[m] - (s_f[m][fp_] - s_f[m][fm_]) * p.r_dy;
+#pragma unroll
+                for (int m = 0; m < 10; m++) s[m] = s[m] - (s_f[4 + m][fp_] - s_f[4 + m][fm_]) * p.r_dy;
+                if (BULK) bs = bs - (s_f[14][fp_] - s_f[14][fm_]) * p.r_dy;
+            } else {
+                // ny == 1: both faces are the same expression of the same cells -> 0
+#pragma unroll
+                for (int m = 0; m < 4; m++) t[m] = t[m] - 0.0f * p.r_dy;
+#pragma unroll
+                for (int m = 0; m < 10; m++) s[m] = s[m] - 0.0f * p.r_dy;
+                if (BULK) bs = bs - 0.0f * p.r_dy;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                float um1, ui, up1;
+                const int cm = tp.hy > 0 ? cc - HZ : cc, cp = tp.hy > 0 ? cc + HZ : cc;
+                um1 = c == 0 ? s_c[CF_U0][cm] * 1.0f : s_c[CF_U0][cm] * s_c[CF_E + c][cm];
+                ui = c == 0 ? s_c[CF_U0][cc] * 1.0f : s_c[CF_U0][cc] * s_c[CF_E + c][cc];
+                up1 = c == 0 ? s_c[CF_U0][cp] * 1.0f : s_c[CF_U0][cp] * s_c[CF_E + c][cp];
+                udy[c] = slope(um1, ui, up1) * p.r_dy;
+            }
+        }
+        __syncthreads();
+        // ---- 4. z faces: face f in [0, tz] lies between tile columns f-1 and f ----
+        if (tp.hz > 0) {
+            for (int fc = tid; fc < ty * (tz + 1); fc += CV_TILE_THREADS) {
+                const int y = fc / (tz + 1), f = fc - (fc / (tz + 1)) * (tz + 1);
+                const int Lc = (y + hy) * HZ + (f - 1 + hz);
+                float F[15];
+                face_flux<2, BULK>(p, eos, tau, gl, Lc - 1, Lc, Lc + 1, Lc + 2, F);
+#pragma unroll
+                for (int m = 0; m < 15; m++) s_f[m][fc] = F[m];
+            }
+        }
+        __syncthreads();
+        if (active) {
+            if (tp.hz > 0) {
+                const int fp_ = my * (tz + 1) + mz + 1, fm_ = my * (tz + 1) + mz;
+#pragma unroll
+                for (int m = 0; m < 4; m++) t[m] = t[m] - (s_f[m][fp_] - s_f[m][fm_]) * st.r_taudz;
+#pragma unroll
+                for (int m = 0; m < 10; m++) s[m] = s[m] - (s_f[4 + m][fp_] - s_f[4 + m][fm_]) * st.r_taudz;
+                if (BULK) bs = bs - (s_f[14][fp_] - s_f[14][fm_]) * st.r_taudz;
+            } else {
+#pragma unroll
+                for (int m = 0; m < 4; m++) t[m] = t[m] - 0.0f * st.r_taudz;
+#pragma unroll
+                for (int m = 0; m < 10; m++) s[m] = s[m] - 0.0f * st.r_taudz;
+                if (BULK) bs = bs - 0.0f * st.r_taudz;
+            }
+            const float u0c = s_c[CF_U0][cc];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int cm = tp.hz > 0 ? cc - 1 : cc, cp = tp.hz > 0 ? cc + 1 : cc;
+                float um1 = c == 0 ? s_c[CF_U0][cm] * 1.0f : s_c[CF_U0][cm] * s_c[CF_E + c][cm];
+                float ui = c == 0 ? u0c * 1.0f : u0c * s_c[CF_E + c][cc];
+                float up1 = c == 0 ? s_c[CF_U0][cp] * 1.0f : s_c[CF_U0][cp] * s_c[CF_E + c][cp];
+                float chr = ((c == 0 ? ec.w : (c == 3 ? 1.0f : 0.0f)) * u0c) * st.r_tau;
+                udz[c] = slope(um1, ui, up1) * st.r_taudz + chr;
+            }
+        }
+
+        // ---- 5. update_pimn (kernel_IS.cl:501-709) and update_ev (kernel_visc.cl:322-393) ----
+        if (active) {
+            const float4 e1 = STEP == 1 ? ec : ld4o(ev1, n, bo);
+            const float4 e2 = STEP == 2 ? ec : ld4o(ev2, n, bo);
+            float g1 = STEP == 1 ? s_c[CF_U0][cc] : gamma3(e1.y, e1.z, e1.w);
+            float u_old[4] = {g1 * 1.0f, g1 * e1.y, g1 * e1.z, g1 * e1.w};
+            float u_new[4], udt[4];
+            if (STEP == 1) {
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    float ud_ = ldo(udiff_in + (int64_t)c * n, bo);
+                    udt[c] = ud_ * p.r_dt;
+                    u_new[c] = u_old[c] + ud_;
+                }
+            } else {
+                float g2 = s_c[CF_U0][cc];
+                u_new[0] = g2 * 1.0f; u_new[1] = g2 * e2.y; u_new[2] = g2 * e2.z; u_new[3] = g2 * e2.w;
+#pragma unroll
+                for (int c = 0; c < 4; c++) udt[c] = (u_new[c] - u_old[c]) * p.r_dt;
+            }
+            float da[4][4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) { da[c][0] = udt[c]; da[c][1] = udx[c]; da[c][2] = udy[c]; da[c][3] = udz[c]; }
+            float u[4], DU[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) u[c] = STEP == 1 ? u_old[c] : u_new[c];
+#pragma unroll
+            for (int c = 0; c < 4; c++) DU[c] = dot4(u, da[c]);
+            const float ed_step = STEP == 1 ? e1.x : e2.x;
+            const float theta = udt[0] + udx[1] + udy[2] + udz[3];
+            const float4 cpTs = eos.raw(ed_step);
+            const float temperature = cpTs.z;
+            const float local_etaos = etaos(p, temperature);
+            float etav = local_etaos * cpTs.w * CV_HBARC;
+            float one_over_taupi = temperature * rcp(3.0f * rmax(CV_ACU, local_etaos) * CV_HBARC);
+            float coef_pipi = 0.0f;
+            const bool gubser = p.flags & 2;
+            if (gubser) {
+                float q4 = sqrtf(sqrtf(ed_step));
+                float ETAOS = p.etaos_ymin;
+                float etavH = (ETAOS * 4.0f) * p.r_3;
+                float taupiH = (p.lam1 * p.lam1 * etavH) * p.r_3;
+                one_over_taupi = rcp(taupiH * rcp(q4));
+                coef_pipi = p.lam1 * rcp(ed_step);
+                etav = ((ETAOS * 4.0f) * p.r_3) * (sqrtf(ed_step) * q4);
+            }
+            const float gd[4] = {1.0f, -1.0f, -1.0f, -1.0f};
+            float max_abs = 0.0f;
+            float out[10];
+            const float rden = rcp(u_new[0] + p.dt * one_over_taupi);
+#pragma unroll
+            for (int mu = 0; mu < 4; mu++) {
+#pragma unroll
+                for (int nu = mu; nu < 4; nu++) {
+                    float gmunu = mu == nu ? gd[mu] : 0.0f;
+                    float sigma = gd[mu] * da[nu][mu] + gd[nu] * da[mu][nu] - (u[mu] * DU[nu] + u[nu] * DU[mu])
+                                  - 2.0f / 3.0f * (gmunu - u[mu] * u[nu]) * theta;
+                    int mn = IDX(mu, nu);
+                    if (u[0] > 10000.0f) sigma = 0.0f;
+                    float piNS = etav * sigma;
+                    float pi_old = (STEP == 1 ? q[mn] : ldo(pi1 + (int64_t)mn * n, bo)) * u_old[0];
+                    float src = (-q[mn] * theta) * p.r_3 - (q[mn] * u[0]) * st.r_tau;
+#pragma unroll
+                    for (int l = 0; l < 4; l++)
+                        src -= (u[mu] * q[IDX(nu, l)] + u[nu] * q[IDX(mu, l)]) * DU[l] * gd[l];
+                    if (gubser)
+                        src -= one_over_taupi * coef_pipi * (pipi(0, mu, nu, q, u) - pipi(1, mu, nu, q, u)
+                                                             - pipi(2, mu, nu, q, u) - pipi(3, mu, nu, q, u));
+                    s[mn] += src;
+                    pi_old = (pi_old + (s[mn] * p.dt) * st.r_step + p.dt * one_over_taupi * piNS) * rden;
+                    if (fabsf(pi_old) > max_abs) max_abs = fabsf(pi_old);
+                    out[mn] = pi_old;
+                }
+            }
+            float bout = 0.0f, b_start = 0.0f;
+            if (BULK) {
+                float zs = zetaos(p, temperature);
+                float zetav = zs * cpTs.w * CV_HBARC;
+                float oot_b = temperature * rcp(3.0f * rmax(CV_ACU, zs) * CV_HBARC);
+                float b2 = s_c[CF_BK][cc];
+                float bsrc_ = (b2 * theta) * p.r_3 - (b2 * u[0]) * st.r_tau;
+                bs += bsrc_;
+                b_start = STEP == 1 ? b2 : ldo(bk1, bo);
+                float b_old = b_start * u_old[0];
+                bout = (b_old + (bs * p.dt) * st.r_step - p.dt * oot_b * zetav * theta) * rcp(u_new[0] + p.dt * oot_b);
+                if (fabsf(bout) > max_abs) max_abs = fabsf(bout);
+            }
+            {
+                float pre = eos.P(e2.x);
+                float T00 = (e2.x + pre) * u_new[0] * u_new[0] - pre;
+                float gc = 1.0f;
+                if (max_abs > T00) {
+#pragma unroll
+                    for (int m = 0; m < 10; m++) out[m] = 0.0f;
+                    bout = 0.0f;
+                    gc = 0.0f;
+                }
+                *(float *)((char *)good + bo) = gc;
+            }
+#pragma unroll
+            for (int m = 0; m < 10; m++) *(float *)((char *)(pi_out + (int64_t)m * n) + bo) = out[m];
+            if (BULK) *(float *)((char *)bk_out + bo) = bout;
+            if (STEP == 1) {
+#pragma unroll
+                for (int m = 0; m < 10; m++) *(float *)((char *)(IS_src + (int64_t)m * n) + bo) = s[m];
+                if (BULK) *(float *)((char *)bsrc + bo) = bs;
+#pragma unroll
+                for (int m = 0; m < 4; m++) *(float *)((char *)(Src + (int64_t)m * n) + bo) = t[m];
+            }
+            // update_ev
+            float pressure = eos.P(e1.x) + (BULK ? b_start : 0.0f);
+            float T0m[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                float g0 = c == 0 ? 1.0f : 0.0f;
+                float pim0 = STEP == 1 ? q[c] : ldo(pi1 + (int64_t)c * n, bo);
+                T0m[c] = ((e1.x + pressure) * g1 * u_old[c] - pressure * g0 + pim0) * st.old_time;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) T0m[c] = T0m[c] + (t[c] * p.dt) * st.r_step - out[c] * st.tau;
+            float T00 = rmax(CV_ACU, T0m[0]) * st.r_tau;
+            float T01 = fabsf(T0m[1]) < CV_ACU ? 0.0f : T0m[1] * st.r_tau;
+            float T02 = fabsf(T0m[2]) < CV_ACU ? 0.0f : T0m[2] * st.r_tau;
+            float T03 = fabsf(T0m[3]) < CV_ACU ? 0.0f : T0m[3] * st.r_tau;
+            float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
+            if (M > T00) {
+                float sc = (0.999f * T00) * rcp(M);
+                T01 *= sc; T02 *= sc; T03 *= sc;
+                M = 0.999f * T00;
+            }
+            float ed_find;
+            if ((p.flags & 8) && M > p.fp64_vmin * T00)
+                ed_find = (float)root_newton_d(E, tab, (double)T00, (double)M, (double)bout);
+            else
+                ed_find = root_newton(eos, T00, M, bout);
+            if (ed_find < CV_ACU) { T01 = 0.0f; T02 = 0.0f; T03 = 0.0f; }
+            ed_find = rmax(0.0f, ed_find);
+            float pr = eos.P(ed_find) + bout;
+            float repv = rcp(rmax(CV_ACU, T00 + pr));
+            float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
+            *(float *)((char *)ev_out + bo) = en.x;
+            *(float *)((char *)(ev_out + n) + bo) = en.y;
+            *(float *)((char *)(ev_out + 2 * n) + bo) = en.z;
+            *(float *)((char *)(ev_out + 3 * n) + bo) = en.w;
+            if (STEP == 2) {
+                float gn = gamma3(en.y, en.z, en.w);
+                float un[4] = {gn * 1.0f, gn * en.y, gn * en.z, gn * en.w};
+#pragma unroll
+                for (int c = 0; c < 4; c++) *(float *)((char *)(udiff_out + (int64_t)c * n) + bo) = un[c] - u_old[c];
+            }
+        }
+        __syncthreads();   // LDS reused by the next tile
+    }
+}
+
+}  // namespace cv

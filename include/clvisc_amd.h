@@ -139,6 +139,13 @@ int cv_max_ed(cv_handle *h, int which, float *out);
  * *nlaunch.  Same results as the unprofiled step. */
 int cv_profile_visc_step(cv_handle *h, int64_t loop0, float *ms, int nms, int32_t *nlaunch);
 
+/* Kernel of the step path (cv_visc_steps_async): kind 0 = one thread per cell, every
+ * stencil quantity re-evaluated per cell; kind 1 (default) = tiled kernel, one workgroup
+ * per ty x tz tile of an x-plane, per-cell quantities and y/z faces evaluated once in LDS
+ * (ty, tz <= 0 picks the default tile for the grid).  Both give the same FP32 results. */
+int cv_set_stage_kernel(cv_handle *h, int kind, int ty, int tz);
+int cv_get_stage_kernel(cv_handle *h, int32_t *kind, int32_t *ty, int32_t *tz);
+
 /* Wait for all queued work. */
 int cv_sync(cv_handle *h);
 

@@ -80,9 +80,12 @@ def _visc_case(nx, ny, nz, bulk=None, nhot=12, seed=3, eos_kind='crossover'):
     return p, tab, ev
 
 
-def _visc_compare(p, tab, ev, nsteps, label):
-    _say('case: %s %dx%dx%d, %d steps' % (label, p.nx, p.ny, p.nz, nsteps))
+def _visc_compare(p, tab, ev, nsteps, label, kernel=None):
+    _say('case: %s %dx%dx%d, %d steps, kernel %s' % (label, p.nx, p.ny, p.nz, nsteps, kernel))
     h = _lib.Handle(p, tab, viscous=True)
+    if kernel is not None:
+        h.set_stage_kernel(*kernel)
+    _say('  stage kernel %s' % (h.stage_kernel(),))
     h.visc_initialize(p.tau0)
     h.ideal_stage(1, np.float32(p.tau0))
     h.set('ev1', ev)
@@ -108,9 +111,16 @@ def _visc_compare(p, tab, ev, nsteps, label):
         assert v <= TOL, (k, v)
 
 
-def test_visc_steps_small_3d():
+@pytest.mark.parametrize('kernel', [(0, 0, 0), (1, 0, 0), (1, 8, 32), (1, 11, 23), (1, 4, 16), (1, 16, 16)])
+def test_visc_steps_small_3d(kernel):
     p, tab, ev = _visc_case(29, 31, 11)
-    _visc_compare(p, tab, ev, 20, 'viscous hotspots')
+    _visc_compare(p, tab, ev, 20, 'viscous hotspots', kernel)
+
+
+@pytest.mark.parametrize('kernel', [(0, 0, 0), (1, 0, 0), (1, 64, 1)])
+def test_visc_steps_2d_boost_invariant(kernel):
+    p, tab, ev = _visc_case(45, 41, 1, bulk=(0.05, 0.155, 0.02))
+    _visc_compare(p, tab, ev, 15, 'viscous+bulk 2d', kernel)
 
 
 def test_visc_steps_bulk_3d():
