@@ -17,6 +17,7 @@ CV_FLAG_RIEMANN_TEST = 0x1
 CV_FLAG_GUBSER_VISC_TEST = 0x2
 CV_FLAG_BULK = 0x4
 CV_FLAG_FP64_ROOT = 0x8
+CV_FLAG_SF_FP64 = 0x10
 
 FIELDS = {'ev0': 0, 'ev1': 1, 'ev2': 2, 'pi0': 3, 'pi1': 4, 'pi2': 5,
           'bulk0': 6, 'bulk1': 7, 'bulk2': 8, 'src': 9, 'is_src': 10,
@@ -75,6 +76,7 @@ def lib():
             'cv_set_field': [vp, C.c_int, vp, i64],
             'cv_get_field': [vp, C.c_int, vp, i64],
             'cv_copy_field': [vp, C.c_int, C.c_int],
+            'cv_gather': [vp, C.c_int, vp, i64, vp],
             'cv_visc_initialize': [vp, f32],
             'cv_ideal_stage': [vp, C.c_int, f32],
             'cv_is_stage': [vp, C.c_int, f32],
@@ -88,6 +90,12 @@ def lib():
             'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
             'cv_set_stage_kernel': [vp, C.c_int, C.c_int, C.c_int],
             'cv_get_stage_kernel': [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+            'cv_hypersf_reserve': [vp, i64],
+            'cv_hypersf_snapshot': [vp, f32],
+            'cv_hypersf_find': [vp, f32],
+            'cv_hypersf_count': [vp, C.POINTER(i64), C.POINTER(i64)],
+            'cv_hypersf_get': [vp, vp, vp, vp, vp, i64],
+            'cv_hypersf_reset': [vp],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -105,10 +113,11 @@ def check(rc):
 def exported_symbols():
     """Names of the C-ABI entry points this binding expects."""
     return ['cv_last_error', 'cv_device_info', 'cv_create', 'cv_destroy', 'cv_set_field',
-            'cv_get_field', 'cv_copy_field', 'cv_visc_initialize', 'cv_ideal_stage', 'cv_is_stage',
+            'cv_get_field', 'cv_copy_field', 'cv_gather', 'cv_visc_initialize', 'cv_ideal_stage', 'cv_is_stage',
             'cv_visc_stage', 'cv_update_udiff', 'cv_ideal_steps_async', 'cv_visc_steps_async',
             'cv_max_ed', 'cv_sync', 'cv_last_elapsed_ms', 'cv_profile_visc_step',
-            'cv_set_stage_kernel', 'cv_get_stage_kernel']
+            'cv_set_stage_kernel', 'cv_get_stage_kernel', 'cv_hypersf_reserve', 'cv_hypersf_snapshot',
+            'cv_hypersf_find', 'cv_hypersf_count', 'cv_hypersf_get', 'cv_hypersf_reset']
 
 
 def make_eos(table):
@@ -160,6 +169,15 @@ class Handle(object):
         f = FIELDS[field] if isinstance(field, str) else field
         a = np.ascontiguousarray(arr, dtype=np.float32)
         check(self.L.cv_set_field(self.h, f, a.ctypes.data, a.size))
+
+    def gather(self, field, cells):
+        """Rows of a field at the given flat cell indices (AoS, float32)."""
+        f = FIELDS[field] if isinstance(field, str) else field
+        nc = NCOMP[f]
+        idx = np.ascontiguousarray(cells, dtype=np.int64).reshape(-1)
+        out = np.empty((idx.size, nc) if nc > 1 else (idx.size,), np.float32)
+        check(self.L.cv_gather(self.h, f, idx.ctypes.data, idx.size, out.ctypes.data))
+        return out
 
     def copy(self, dst, src):
         d = FIELDS[dst] if isinstance(dst, str) else dst
@@ -217,6 +235,36 @@ class Handle(object):
         out = C.c_float()
         check(self.L.cv_last_elapsed_ms(self.h, C.byref(out)))
         return out.value
+
+    # -- freeze-out hypersurface
+    def hypersf_reserve(self, capacity):
+        check(self.L.cv_hypersf_reserve(self.h, int(capacity)))
+
+    def hypersf_snapshot(self, tau):
+        check(self.L.cv_hypersf_snapshot(self.h, float(tau)))
+
+    def hypersf_find(self, tau_new):
+        check(self.L.cv_hypersf_find(self.h, float(tau_new)))
+
+    def hypersf_count(self):
+        """(elements found, capacity)."""
+        c, cap = C.c_int64(), C.c_int64()
+        check(self.L.cv_hypersf_count(self.h, C.byref(c), C.byref(cap)))
+        return c.value, cap.value
+
+    def hypersf_get(self):
+        """(sf (n,8), txyz (n,4), pi (n,10) or None, bulk (n,) or None) of the stored elements."""
+        n, _ = self.hypersf_count()
+        sf = np.empty((n, 8), np.float32)
+        tx = np.empty((n, 4), np.float32)
+        ps = np.empty((n, 10), np.float32) if self.viscous else None
+        bk = np.empty(n, np.float32) if (self.viscous and self.params.flags & CV_FLAG_BULK) else None
+        ptr = lambda a: None if a is None else a.ctypes.data
+        check(self.L.cv_hypersf_get(self.h, ptr(sf), ptr(tx), ptr(ps), ptr(bk), n))
+        return sf, tx, ps, bk
+
+    def hypersf_reset(self):
+        check(self.L.cv_hypersf_reset(self.h))
 
 
 def device_info(device=0):

@@ -1,0 +1,158 @@
+"""BulkInfo: slices and bulk observables along the evolution, with the attributes and
+output files of the reference's pyvisc/bulkinfo.py:26-340.
+
+Slices come from the device field proxies (``DeviceField.get()``); the observables
+(eccentricities, <v_r>, total entropy) are the reference's NumPy formulas
+(pyvisc/bulkinfo.py:203-266).
+"""
+from __future__ import annotations
+
+import os
+from math import floor
+
+import numpy as np
+
+
+class BulkInfo(object):
+    """ed(x), ed(y), ed(eta), T, v slices, eccentricities vs rapidity (bulkinfo.py:26)."""
+
+    def __init__(self, cfg, eos):
+        self.cfg = cfg
+        self.eos = eos
+        NX, NY, NZ = cfg.NX, cfg.NY, cfg.NZ
+        self.ex, self.ey, self.ez = [], [], []
+        self.vx, self.vy, self.vz = [], [], []
+        self.exy, self.vx_xy, self.vy_xy, self.vz_xy = [], [], [], []
+        self.pixx_xy, self.piyy_xy, self.pitx_xy = [], [], []
+        self.exz, self.vx_xz, self.vy_xz, self.vz_xz = [], [], [], []
+        self.ecc2_vs_rapidity = []
+        self.ecc1_vs_rapidity = []
+        self.time = []
+        self.edmax = []
+        self.x = np.linspace(-floor(NX / 2) * cfg.DX, floor(NX / 2) * cfg.DX, NX, endpoint=True)
+        self.y = np.linspace(-floor(NY / 2) * cfg.DY, floor(NY / 2) * cfg.DY, NY, endpoint=True)
+        self.z = np.linspace(-floor(NZ / 2) * cfg.DZ, floor(NZ / 2) * cfg.DZ, NZ, endpoint=True)
+
+    def get(self, tau, d_ev1, edmax, d_pi=None):
+        """Record the slices of d_ev1 (and pi^xx, pi^yy, pi^tx in the mid-eta plane)
+        at tau (bulkinfo.py:129-200)."""
+        self.time.append(tau)
+        self.edmax.append(edmax)
+        NX, NY, NZ = self.cfg.NX, self.cfg.NY, self.cfg.NZ
+        bulk = d_ev1.get().reshape(NX, NY, NZ, 4)
+        self._ecc_vs_rapidity(bulk)
+        i0, j0, k0 = NX // 2, NY // 2, NZ // 2
+        evx0, evy0, evz0 = bulk[:, j0, k0], bulk[i0, :, k0], bulk[i0, j0, :]
+        self.ex.append(evx0[:, 0].copy())
+        self.ey.append(evy0[:, 0].copy())
+        self.ez.append(evz0[:, 0].copy())
+        self.vx.append(evx0[:, 1].copy())
+        self.vy.append(evy0[:, 2].copy())
+        self.vz.append(evz0[:, 3].copy())
+        evxy, evxz = bulk[:, :, k0], bulk[:, j0, :]
+        self.exy.append(evxy[..., 0].copy())
+        self.vx_xy.append(evxy[..., 1].copy())
+        self.vy_xy.append(evxy[..., 2].copy())
+        self.exz.append(evxz[..., 0].copy())
+        self.vx_xz.append(evxz[..., 1].copy())
+        self.vy_xz.append(evxz[..., 2].copy())
+        self.vz_xz.append(evxz[..., 3].copy())
+        if d_pi is not None:
+            # the reference reads cell gid_x*NY*(NZ/2) + gid_y*(NZ/2) + NZ/2 (bulkinfo.py:112),
+            # which is the mid-eta plane only for NZ == 1; kept as written
+            gx, gy = np.meshgrid(np.arange(NX), np.arange(NY), indexing='ij')
+            oid = (gx * NY * (NZ // 2) + gy * (NZ // 2) + NZ // 2).reshape(-1)
+            pi = d_pi.gather(oid)
+            self.pixx_xy.append(pi[:, 4].reshape(NX, NY))
+            self.piyy_xy.append(pi[:, 7].reshape(NX, NY))
+            self.pitx_xy.append(pi[:, 1].reshape(NX, NY))
+
+    def eccp(self, ed, vx, vy, vz=0.0, pixx=None, piyy=None, pitx=None):
+        """(<T^0x>/<Txx+Tyy>, <Txx-Tyy>/<Txx+Tyy>) (bulkinfo.py:203-229)."""
+        ed = np.array(ed, copy=True)
+        ed[ed < 1.0E-10] = 1.0E-10
+        pre = self.eos.f_P(ed)
+        vr2 = vx * vx + vy * vy + vz * vz
+        vr2 = np.where(vr2 > 1.0, 0.999999, vr2)
+        u0 = 1.0 / np.sqrt(1.0 - vr2)
+        Tyy = (ed + pre) * u0 * u0 * vy * vy + pre
+        Txx = (ed + pre) * u0 * u0 * vx * vx + pre
+        T0x = (ed + pre) * u0 * u0 * vx
+        if pixx is not None:
+            v2 = ((Txx - Tyy).sum() + (pixx - piyy).sum()) / ((Txx + Tyy).sum() + (pixx + piyy).sum())
+        else:
+            v2 = (Txx - Tyy).sum() / (Txx + Tyy).sum()
+        v1 = T0x.sum() / (Txx + Tyy).sum()
+        return v1, v2
+
+    def mean_vr(self, ed, vx, vy, vz=0.0):
+        """<gamma e |v_T|>/<gamma e> over the plane (bulkinfo.py:231-240)."""
+        ed = np.array(ed, copy=True)
+        ed[ed < 1.0E-10] = 1.0E-10
+        vr2 = vx * vx + vy * vy + vz * vz
+        vr2 = np.where(vr2 > 1.0, 0.999999, vr2)
+        u0 = 1.0 / np.sqrt(1.0 - vr2)
+        return (u0 * ed * np.sqrt(vx * vx + vy * vy)).sum() / (u0 * ed).sum()
+
+    def total_entropy(self, tau, ed, vx, vy, vz=0.0):
+        """tau DX DY sum(u0 s) over the plane (bulkinfo.py:242-249)."""
+        ed = np.array(ed, copy=True)
+        ed[ed < 1.0E-10] = 1.0E-10
+        vr2 = vx * vx + vy * vy + vz * vz
+        vr2 = np.where(vr2 > 1.0, 0.999999, vr2)
+        u0 = 1.0 / np.sqrt(1.0 - vr2)
+        return (u0 * self.eos.f_S(ed)).sum() * tau * self.cfg.DX * self.cfg.DY
+
+    def _ecc_vs_rapidity(self, bulk):
+        NZ = self.cfg.NZ
+        ecc1 = np.empty(NZ)
+        ecc2 = np.empty(NZ)
+        for k in range(NZ):
+            ecc1[k], ecc2[k] = self.eccp(bulk[:, :, k, 0], bulk[:, :, k, 1], bulk[:, :, k, 2], bulk[:, :, k, 3])
+        self.ecc1_vs_rapidity.append(ecc1)
+        self.ecc2_vs_rapidity.append(ecc2)
+
+    def save(self, viscous_on=False):
+        """The reference's text outputs (bulkinfo.py:268-340)."""
+        path_out = os.path.abspath(self.cfg.fPathOut)
+        f_T = self.eos.f_T
+        np.savetxt(path_out + '/ex.dat', np.array(self.ex).T)
+        np.savetxt(path_out + '/ey.dat', np.array(self.ey).T)
+        np.savetxt(path_out + '/ez.dat', np.array(self.ez).T)
+        np.savetxt(path_out + '/Tx.dat', f_T(np.array(self.ex)).T)
+        np.savetxt(path_out + '/Ty.dat', f_T(np.array(self.ey)).T)
+        np.savetxt(path_out + '/Tz.dat', f_T(np.array(self.ez)).T)
+        np.savetxt(path_out + '/vx.dat', np.array(self.vx).T)
+        np.savetxt(path_out + '/vy.dat', np.array(self.vy).T)
+        np.savetxt(path_out + '/vz.dat', np.array(self.vz).T)
+        if len(self.ecc2_vs_rapidity) != 0:
+            np.savetxt(path_out + '/ecc2.dat', np.array(self.ecc2_vs_rapidity).T)
+            np.savetxt(path_out + '/ecc1.dat', np.array(self.ecc1_vs_rapidity).T)
+        entropy, vr, ecc2, ecc2_visc = [], [], [], []
+        for idx, exy in enumerate(self.exy):
+            vx, vy = self.vx_xy[idx], self.vy_xy[idx]
+            np.savetxt(path_out + '/edxy%d.dat' % idx, exy)
+            np.savetxt(path_out + '/Txy%d.dat' % idx, f_T(exy))
+            np.savetxt(path_out + '/vx_xy%d.dat' % idx, vx)
+            np.savetxt(path_out + '/vy_xy%d.dat' % idx, vy)
+            ecc2.append(self.eccp(exy, vx, vy)[1])
+            vr.append(self.mean_vr(exy, vx, vy))
+            entropy.append(self.total_entropy(self.time[idx], exy, vx, vy))
+            if viscous_on and idx < len(self.pixx_xy):
+                ecc2_visc.append(self.eccp(exy, vx, vy, pixx=self.pixx_xy[idx], piyy=self.piyy_xy[idx],
+                                           pitx=self.pitx_xy[idx])[1])
+        for idx, exz in enumerate(self.exz):
+            np.savetxt(path_out + '/ed_xz%d.dat' % idx, exz)
+            np.savetxt(path_out + '/vx_xz%d.dat' % idx, self.vx_xz[idx])
+            np.savetxt(path_out + '/vy_xz%d.dat' % idx, self.vy_xz[idx])
+            np.savetxt(path_out + '/vz_xz%d.dat' % idx, self.vz_xz[idx])
+            np.savetxt(path_out + '/T_xz%d.dat' % idx, f_T(exz))
+        np.savetxt(path_out + '/eccp.dat', np.array(list(zip(self.time, ecc2))), header='tau  eccp')
+        if viscous_on:
+            np.savetxt(path_out + '/eccp_visc.dat', np.array(list(zip(self.time, ecc2_visc))),
+                       header='tau  eccp_visc')
+        np.savetxt(path_out + '/Tmax.dat', np.array(list(zip(self.time, f_T(np.array(self.edmax))))),
+                   header='tau, Tmax')
+        np.savetxt(path_out + '/edmax.dat', np.array(list(zip(self.time, self.edmax))), header='tau, edmax')
+        np.savetxt(path_out + '/vr.dat', np.array(list(zip(self.time, vr))), header='tau <vr>')
+        np.savetxt(path_out + '/entropy.dat', np.array(list(zip(self.time, entropy))), header='tau  entropy')

@@ -12,6 +12,7 @@
 #include "../../include/clvisc_amd.h"
 #include "cv_kernels.h"
 #include "cv_tiled.h"
+#include "cv_hypersf.h"
 
 using namespace cv;
 
@@ -56,6 +57,13 @@ struct cv_handle {
     int kernel = 1;           // step path: 0 per-cell fused kernel, 1 tiled kernel
     TileP tile;
     int tile_grid = 0;
+    // freeze-out surface: fields at the previous surface time, appended elements
+    float *sf_ev = nullptr, *sf_pi = nullptr, *sf_bk = nullptr;
+    float sf_tau_old = 0.0f;
+    bool sf_have_old = false;
+    float *sf_out = nullptr, *sf_txyz = nullptr, *sf_pisf = nullptr, *sf_bksf = nullptr;
+    unsigned long long *sf_count = nullptr;
+    int64_t sf_cap = 0;
 };
 
 // Tile of the tiled stage kernel: ty x tz cells of one x-plane with a 2-cell halo in y and
@@ -239,6 +247,9 @@ extern "C" int cv_destroy(cv_handle *h) {
     (void)hipFree(h->Src); (void)hipFree(h->IS_src); (void)hipFree(h->bsrc);
     (void)hipFree(h->udiff); (void)hipFree(h->good);
     (void)hipFree(h->d_eos); (void)hipFree(h->d_max);
+    (void)hipFree(h->sf_ev); (void)hipFree(h->sf_pi); (void)hipFree(h->sf_bk);
+    (void)hipFree(h->sf_out); (void)hipFree(h->sf_txyz); (void)hipFree(h->sf_pisf); (void)hipFree(h->sf_bksf);
+    (void)hipFree(h->sf_count);
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
     if (h->ev_b) (void)hipEventDestroy(h->ev_b);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -309,6 +320,35 @@ extern "C" int cv_copy_field(cv_handle *h, int dst, int src) {
 }
 
 static dim3 grid1d(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+extern "C" int cv_gather(cv_handle *h, int f, const int64_t *cells, int64_t ncells, float *host) {
+    if (!h || (ncells > 0 && (!cells || !host))) return fail(CV_ERR_ARG, "cv_gather: null argument");
+    if (ncells < 0 || ncells > ((int64_t)1 << 31)) return fail(CV_ERR_ARG, "cv_gather: bad cell count");
+    float *d; int nc;
+    int rc = field_ptr(h, f, &d, &nc);
+    if (rc) return rc;
+    for (int64_t q = 0; q < ncells; q++)
+        if (cells[q] < 0 || cells[q] >= h->n)
+            return fail(CV_ERR_ARG, "cv_gather: cell %lld out of [0, %lld)", (long long)cells[q], (long long)h->n);
+    if (ncells == 0) return CV_OK;
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    int64_t *d_idx = nullptr;
+    float *d_out = nullptr;
+    hipError_t e = hipMalloc((void **)&d_idx, sizeof(int64_t) * ncells);
+    if (e == hipSuccess) e = hipMalloc((void **)&d_out, sizeof(float) * nc * ncells);
+    if (e == hipSuccess) e = hipMemcpy(d_idx, cells, sizeof(int64_t) * ncells, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_gather, grid1d(ncells), dim3(256), 0, h->stream, h->n, nc, d, d_idx, ncells, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(host, d_out, sizeof(float) * nc * ncells, hipMemcpyDeviceToHost);
+    (void)hipFree(d_out);
+    (void)hipFree(d_idx);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_gather: %s", hipGetErrorString(e));
+    return CV_OK;
+}
 
 extern "C" int cv_visc_initialize(cv_handle *h, float tau) {
     if (!h || !h->viscous) return fail(CV_ERR_STATE, "cv_visc_initialize: viscous handle required");
@@ -564,5 +604,130 @@ extern "C" int cv_last_elapsed_ms(cv_handle *h, float *ms) {
     if (!h->timed) return fail(CV_ERR_STATE, "no timed call yet");
     HIP_TRY(hipEventSynchronize(h->ev_b));
     HIP_TRY(hipEventElapsedTime(ms, h->ev_a, h->ev_b));
+    return CV_OK;
+}
+
+// ---- freeze-out hypersurface ------------------------------------------------
+static void sf_free_out(cv_handle *h) {
+    (void)hipFree(h->sf_out); (void)hipFree(h->sf_txyz); (void)hipFree(h->sf_pisf); (void)hipFree(h->sf_bksf);
+    h->sf_out = h->sf_txyz = h->sf_pisf = h->sf_bksf = nullptr;
+    h->sf_cap = 0;
+}
+
+extern "C" int cv_hypersf_reserve(cv_handle *h, int64_t capacity) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    if (capacity < 1 || capacity > ((int64_t)1 << 31))
+        return fail(CV_ERR_ARG, "cv_hypersf_reserve: capacity %lld out of [1, 2^31]", (long long)capacity);
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    sf_free_out(h);
+    bool ok = hipMalloc((void **)&h->sf_out, sizeof(float) * 8 * capacity) == hipSuccess &&
+              hipMalloc((void **)&h->sf_txyz, sizeof(float) * 4 * capacity) == hipSuccess;
+    if (ok && h->viscous) ok = hipMalloc((void **)&h->sf_pisf, sizeof(float) * 10 * capacity) == hipSuccess;
+    if (ok && h->bulk) ok = hipMalloc((void **)&h->sf_bksf, sizeof(float) * capacity) == hipSuccess;
+    if (ok && !h->sf_count) {
+        ok = hipMalloc((void **)&h->sf_count, sizeof(unsigned long long)) == hipSuccess &&
+             hipMemset(h->sf_count, 0, sizeof(unsigned long long)) == hipSuccess;
+    }
+    if (!ok) {
+        sf_free_out(h);
+        return fail(CV_ERR_HIP, "cv_hypersf_reserve: allocation of %lld elements failed", (long long)capacity);
+    }
+    h->sf_cap = capacity;
+    return CV_OK;
+}
+
+extern "C" int cv_hypersf_reset(cv_handle *h) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    HIP_TRY(hipSetDevice(h->device));
+    if (h->sf_count) {
+        HIP_TRY(hipMemsetAsync(h->sf_count, 0, sizeof(unsigned long long), h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    h->sf_have_old = false;
+    return CV_OK;
+}
+
+extern "C" int cv_hypersf_snapshot(cv_handle *h, float tau) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    HIP_TRY(hipSetDevice(h->device));
+    if (!h->sf_cap) {
+        int rc = cv_hypersf_reserve(h, 1500000);
+        if (rc) return rc;
+    }
+    if (!h->sf_ev) {
+        bool ok = hipMalloc((void **)&h->sf_ev, sizeof(float) * 4 * h->n) == hipSuccess;
+        if (ok && h->viscous) ok = hipMalloc((void **)&h->sf_pi, sizeof(float) * 10 * h->n) == hipSuccess;
+        if (ok && h->bulk) ok = hipMalloc((void **)&h->sf_bk, sizeof(float) * h->n) == hipSuccess;
+        if (!ok) return fail(CV_ERR_HIP, "cv_hypersf_snapshot: allocation of the previous-time fields failed");
+    }
+    HIP_TRY(hipMemcpyAsync(h->sf_ev, h->ev[1], sizeof(float) * 4 * h->n, hipMemcpyDeviceToDevice, h->stream));
+    if (h->viscous)
+        HIP_TRY(hipMemcpyAsync(h->sf_pi, h->pi[1], sizeof(float) * 10 * h->n, hipMemcpyDeviceToDevice, h->stream));
+    if (h->bulk)
+        HIP_TRY(hipMemcpyAsync(h->sf_bk, h->bk[1], sizeof(float) * h->n, hipMemcpyDeviceToDevice, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->sf_tau_old = tau;
+    h->sf_have_old = true;
+    return CV_OK;
+}
+
+extern "C" int cv_hypersf_find(cv_handle *h, float tau_new) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    if (!h->sf_have_old) return fail(CV_ERR_STATE, "cv_hypersf_find: no snapshot (call cv_hypersf_snapshot first)");
+    const cv_params &p = h->prm;
+    if (p.nxskip < 1 || p.nyskip < 1 || p.nzskip < 1) return fail(CV_ERR_ARG, "cv_hypersf_find: skips must be >= 1");
+    HIP_TRY(hipSetDevice(h->device));
+    SfP s;
+    s.nx = p.nx; s.ny = p.ny; s.nz = p.nz;
+    s.nxskip = p.nxskip; s.nyskip = p.nyskip; s.nzskip = p.nzskip;
+    s.gx = (p.nx - 1) / p.nxskip + 1; s.gy = (p.ny - 1) / p.nyskip + 1; s.gz = (p.nz - 1) / p.nzskip + 1;
+    s.n = h->n;
+    s.dx = p.dx; s.dy = p.dy; s.dz = p.dz; s.efrz = p.efrz;
+    s.cap = h->sf_cap;
+    s.viscous = h->viscous;
+    int64_t total = (int64_t)s.gx * s.gy * s.gz;
+    auto kern = (p.flags & CV_FLAG_SF_FP64) ? k_hypersf<double> : k_hypersf<float>;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, h->stream, s, h->sf_ev,
+                       h->ev[1], h->sf_pi, h->pi[1], h->bulk ? h->sf_bk : nullptr, h->bulk ? h->bk[1] : nullptr,
+                       h->sf_tau_old, tau_new, h->sf_count, h->sf_out, h->sf_txyz, h->sf_pisf, h->sf_bksf);
+    HIP_TRY(hipGetLastError());
+    return cv_hypersf_snapshot(h, tau_new);
+}
+
+extern "C" int cv_hypersf_count(cv_handle *h, int64_t *count, int64_t *capacity) {
+    if (!h || !count) return fail(CV_ERR_ARG, "null argument");
+    unsigned long long c = 0;
+    if (h->sf_count) {
+        HIP_TRY(hipSetDevice(h->device));
+        HIP_TRY(hipMemcpyAsync(&c, h->sf_count, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY(hipStreamSynchronize(h->stream));
+    }
+    *count = (int64_t)c;
+    if (capacity) *capacity = h->sf_cap;
+    return CV_OK;
+}
+
+extern "C" int cv_hypersf_get(cv_handle *h, float *sf, float *txyz, float *pisf, float *bksf, int64_t nelem) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    int64_t c = 0;
+    int rc = cv_hypersf_count(h, &c, nullptr);
+    if (rc) return rc;
+    if (c > h->sf_cap)
+        return fail(CV_ERR_CAPACITY, "cv_hypersf_get: %lld elements found, capacity %lld (raise it with "
+                    "cv_hypersf_reserve)", (long long)c, (long long)h->sf_cap);
+    if (nelem != c) return fail(CV_ERR_ARG, "cv_hypersf_get: %lld elements stored, buffers for %lld",
+                                (long long)c, (long long)nelem);
+    if (c == 0) return CV_OK;
+    if (sf) HIP_TRY(hipMemcpy(sf, h->sf_out, sizeof(float) * 8 * c, hipMemcpyDeviceToHost));
+    if (txyz) HIP_TRY(hipMemcpy(txyz, h->sf_txyz, sizeof(float) * 4 * c, hipMemcpyDeviceToHost));
+    if (pisf) {
+        if (!h->sf_pisf) return fail(CV_ERR_STATE, "cv_hypersf_get: pi on the surface needs a viscous handle");
+        HIP_TRY(hipMemcpy(pisf, h->sf_pisf, sizeof(float) * 10 * c, hipMemcpyDeviceToHost));
+    }
+    if (bksf) {
+        if (!h->sf_bksf) return fail(CV_ERR_STATE, "cv_hypersf_get: bulk on the surface needs CV_FLAG_BULK");
+        HIP_TRY(hipMemcpy(bksf, h->sf_bksf, sizeof(float) * c, hipMemcpyDeviceToHost));
+    }
     return CV_OK;
 }

@@ -974,4 +974,13 @@ __global__ __launch_bounds__(256) void k_max_ed(int64_t n, const float *__restri
     }
 }
 
+// out[q][c] = field plane c at cell idx[q] (SoA planes -> AoS rows); idx checked on the host
+__global__ __launch_bounds__(256) void k_gather(int64_t n, int ncomp, const float *__restrict__ f,
+                                                const int64_t *__restrict__ idx, int64_t m, float *out) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m) return;
+    int64_t I = idx[q];
+    for (int c = 0; c < ncomp; c++) out[q * ncomp + c] = f[(int64_t)c * n + I];
+}
+
 }  // namespace cv

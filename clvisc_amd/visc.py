@@ -1,0 +1,176 @@
+"""CLVisc: (3+1)D viscous hydro driver (KT + Israel-Stewart, optional bulk pressure)
+with the interface of the reference's pyvisc/visc.py:22-560, running on the HIP
+library through the C-ABI of include/clvisc_amd.h.
+
+The stage-level methods (IS_stepUpdate, visc_stepUpdate, update_udiff) call one
+stage entry point each, in the reference's order.  ``evolve`` runs the same loop as
+pyvisc/visc.py:473-560 but hands every run of steps between two host events (max e,
+surface, bulk info: all on loops divisible by ntskip) to the fused step path
+(cv_visc_steps_async), which gives the same fields as the stage-level sequence.
+"""
+from __future__ import annotations
+
+import os
+from time import time
+
+import numpy as np
+
+from .ideal import CLIdeal, DeviceField
+
+
+class CLVisc(object):
+    """Viscous hydro on the GPU (pyvisc/visc.py:22)."""
+
+    def __init__(self, configs, handcrafted_eos=None, gpu_id=0):
+        self.ideal = CLIdeal(configs, handcrafted_eos, gpu_id, _viscous=True)
+        self.cfg = configs
+        self.handle = self.ideal.handle
+        self.compile_options = list(self.ideal.compile_options)
+        if getattr(self.cfg, 'gubser_visc_test', False):
+            self.compile_options.append('-D GUBSER_VISC_TEST')
+        self.eos_table = self.ideal.eos_table
+        self.size = self.ideal.size
+        self.d_pi = [DeviceField(self.handle, 'pi%d' % k) for k in range(3)]
+        self.d_IS_src = DeviceField(self.handle, 'is_src')
+        self.d_udiff = DeviceField(self.handle, 'udiff')
+        self.d_goodcell = DeviceField(self.handle, 'goodcell')
+        self.bulk_on = bool(self.handle.params.flags & 0x4)
+        self.d_bulk = [DeviceField(self.handle, 'bulk%d' % k) for k in range(3)] if self.bulk_on else None
+        self.h_goodcell = np.ones(self.size, self.cfg.real)
+        # pi(tau0) and the predicted u for the first step are set before any IC is
+        # loaded (pyvisc/visc.py:101-102)
+        self.IS_initialize()
+        self.ideal_predict_for_first_step()
+
+    # -- stage-level operations (reference order) --------------------------------
+    def IS_initialize(self):
+        """visc_initialize on d_ev[1] at tau (pyvisc/visc.py:143-149)."""
+        self.handle.visc_initialize(self.ideal.tau)
+
+    def ideal_predict_for_first_step(self):
+        """Ideal stage 1 for the first u prediction (pyvisc/visc.py:468-470)."""
+        self.ideal.stepUpdate(step=1)
+
+    def IS_stepUpdate(self, step):
+        """IS source, KT flux of u pi, update of pi^{mu nu} into d_pi[3-step]
+        (pyvisc/visc.py:232-273)."""
+        self.handle.is_stage(step, self.cfg.real(self.ideal.tau))
+
+    def visc_stepUpdate(self, step, jet_eloss_src=None):
+        """KT source with pi and the primitive update into d_ev[3-step]
+        (pyvisc/visc.py:276-314)."""
+        if jet_eloss_src is not None and jet_eloss_src.get('switch_on', False):
+            raise NotImplementedError('the jet energy-loss source is not part of this hydro path')
+        self.handle.visc_stage(step, self.cfg.real(self.ideal.tau))
+
+    def update_udiff(self, d_ev0, d_ev1):
+        """udiff = u(d_ev1) - u(d_ev0) (pyvisc/visc.py:352-357)."""
+        self.handle.update_udiff(d_ev0.name, d_ev1.name)
+
+    def update_time(self, loop):
+        self.ideal.update_time(loop)
+
+    # -- freeze-out surface --------------------------------------------------------
+    def get_hypersf(self, n, ntskip, is_finished):
+        """visc_hypersf between the previous surface time and now (pyvisc/visc.py:359-397)."""
+        if n == 0:
+            self.ideal.hypersf.snapshot(self.cfg.TAU0)
+        elif (n % ntskip == 0) or is_finished:
+            self.ideal.hypersf.find(self.ideal.tau)
+
+    def save_pimn_sf(self, set_to_zero=False):
+        """pimnsf.dat (pyvisc/visc.py:400-419)."""
+        ed = self.ideal.efrz
+        pr = float(self.ideal.eos.f_P(ed))
+        print('pimn on frzsf is saved to ', os.path.join(self.cfg.fPathOut, 'pimnsf.dat'))
+        self.ideal.hypersf.save_pimn(self.cfg.fPathOut, self.cfg.TFRZ, pr, set_to_zero=set_to_zero)
+
+    def save(self, save_hypersf=True, save_bulk=False, save_pi=False, save_vorticity=False):
+        """pyvisc/visc.py:423-462 without the vorticity files."""
+        if save_vorticity:
+            raise NotImplementedError('vorticity on the surface is not part of this hydro path')
+        self.ideal.save(save_hypersf, save_bulk, viscous_on=True)
+        if save_hypersf:
+            self.save_pimn_sf()
+        if save_pi and hasattr(self, 'pimn_info'):
+            self.pimn_info.save()
+
+    def check_bad_cell(self):
+        """Mid-rapidity masks of e > efrz and good cells (pyvisc/visc.py:282-300)."""
+        NX, NY, NZ = self.cfg.NX, self.cfg.NY, self.cfg.NZ
+        self.h_goodcell = self.d_goodcell.get()
+        efrz = self.ideal.efrz
+        self.ideal.ev_to_host()
+        ed_mid = self.ideal.h_ev1[:, 0].reshape(NX, NY, NZ)[:, :, NZ // 2]
+        frz_sf = (ed_mid > efrz).astype(float) * 4.0
+        good_cells = 3 * (self.h_goodcell.reshape(NX, NY, NZ)[:, :, NZ // 2] - 1)
+        comments = '''4 for e>efrz and good cell;
+                    1 for e > efrz and bad cell;
+                    0 for e < efrz and good cell;
+                    -3 for e<efrz and bad cell'''
+        outpath = os.path.join(self.cfg.fPathOut, 'bad_cell_masks_tau%s.txt' % self.ideal.tau)
+        np.savetxt(outpath, frz_sf + good_cells, header=comments)
+
+    # -- initial conditions outside this hydro path --------------------------------
+    def optical_glauber_ini(self, *args, **kwargs):
+        raise NotImplementedError('optical Glauber IC is not part of this hydro path; load_ini the IC')
+
+    def create_ini_from_partons(self, *args, **kwargs):
+        raise NotImplementedError('parton smearing IC is not part of this hydro path; load_ini the IC')
+
+    def smear_from_p4x4(self, *args, **kwargs):
+        raise NotImplementedError('parton smearing IC is not part of this hydro path; load_ini the IC')
+
+    def get_vorticity(self, save_data=False):
+        raise NotImplementedError('vorticity is not part of this hydro path')
+
+    # -- main loop -----------------------------------------------------------------
+    def evolve(self, max_loops=1000, save_hypersf=True, save_bulk=False, plot_bulk=False, save_pi=True,
+               force_run_to_maxloop=False, save_vorticity=False, jet_eloss_src=None, debug=False):
+        """The loop of pyvisc/visc.py:473-560.  Without force_run_to_maxloop it runs
+        until max e < e_frz, as the reference does (max_loops is not consulted)."""
+        if save_vorticity:
+            raise NotImplementedError('vorticity is not part of this hydro path')
+        if jet_eloss_src is not None and jet_eloss_src.get('switch_on', False):
+            raise NotImplementedError('the jet energy-loss source is not part of this hydro path')
+        cfg = self.cfg
+        if cfg.ETAOS_YMIN < 1.0E-6 and cfg.ETAOS_LEFT_SLOP < 1.0E-6 and cfg.ETAOS_RIGHT_SLOP < 1.0E-6 \
+                and not self.bulk_on:
+            self.ideal.evolve(max_loops, save_hypersf, save_bulk, plot_bulk, force_run_to_maxloop)
+            self.save_pimn_sf(set_to_zero=True)
+            return
+        if save_pi:
+            from .pimninfo import PimnInfo
+            self.pimn_info = PimnInfo(cfg)
+        ntskip = int(cfg.ntskip)
+        loop = 0
+        while True:
+            t0 = time()
+            if force_run_to_maxloop and loop == max_loops:
+                break
+            if loop % ntskip == 0:
+                self.ideal.edmax = self.ideal.max_energy_density()
+                self.ideal.history.append([self.ideal.tau, self.ideal.edmax])
+                print('tau=', self.ideal.tau, ' EdMax= ', self.ideal.edmax)
+            is_finished = self.ideal.edmax < self.ideal.efrz
+            if save_hypersf:
+                self.get_hypersf(loop, ntskip, is_finished)
+            if (plot_bulk or save_bulk) and loop % ntskip == 0:
+                self.ideal.bulkinfo.get(self.ideal.tau, self.ideal.d_ev[1], self.ideal.edmax, self.d_pi[1])
+            if save_pi and loop % ntskip == 0:
+                self.pimn_info.get(self.ideal.tau, self.d_pi[1])
+            if is_finished and not force_run_to_maxloop:
+                break
+            # every loop up to the next multiple of ntskip has no host event (edmax is
+            # unchanged, so is is_finished); after is_finished the surface is taken
+            # every loop
+            nrun = 1 if (is_finished or (debug and loop % ntskip == 0)) else ntskip - loop % ntskip
+            if force_run_to_maxloop:
+                nrun = min(nrun, max_loops - loop)
+            self.handle.visc_steps(loop, nrun)
+            self.update_time(loop + nrun - 1)
+            if debug and loop % ntskip == 0:
+                self.check_bad_cell()
+            loop += nrun
+            print('one step: {dtime}'.format(dtime=(time() - t0) / nrun))
+        self.save(save_hypersf=save_hypersf, save_bulk=save_bulk, save_pi=save_pi)

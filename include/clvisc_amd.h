@@ -43,6 +43,8 @@ extern "C" {
 #define CV_FLAG_GUBSER_VISC_TEST 0x2   /* -D GUBSER_VISC_TEST (pyvisc/visc.py:111-112) */
 #define CV_FLAG_BULK             0x4   /* bulk viscous pressure Pi (no reference counterpart) */
 #define CV_FLAG_FP64_ROOT        0x8   /* FP64 root solve where M > fp64_root_vmin * T00 */
+#define CV_FLAG_SF_FP64          0x10  /* freeze-out surface geometry in FP64 (exact where FP32
+                                          loses the tie-breaking shift of coplanar intersections) */
 
 /* Hydro parameters: the reference's -D compile options (pyvisc/ideal.py:146-171,
  * pyvisc/visc.py:108-121) as run-time values. */
@@ -104,6 +106,11 @@ int cv_set_field(cv_handle *h, int field, const float *host, int64_t nfloats);
 int cv_get_field(cv_handle *h, int field, float *host, int64_t nfloats);
 int cv_copy_field(cv_handle *h, int dst, int src);            /* device-to-device */
 
+/* Rows of field at the given cell indices, AoS [ncells][ncomp] into host: the slice
+ * kernels of BulkInfo / PimnInfo (pyvisc/bulkinfo.py:66-126 get_ed / get_pimn,
+ * pyvisc/pimninfo.py:41-67) without copying the whole field. */
+int cv_gather(cv_handle *h, int field, const int64_t *cells, int64_t ncells, float *host);
+
 /* visc_initialize kernel (kernel_IS.cl:10-36) as CLVisc.IS_initialize (visc.py:143-149). */
 int cv_visc_initialize(cv_handle *h, float tau);
 
@@ -152,6 +159,43 @@ int cv_sync(cv_handle *h);
 /* Device timing of the last cv_*_steps_async call bracketed with HIP events:
  * elapsed milliseconds between its first and last launch. */
 int cv_last_elapsed_ms(cv_handle *h, float *ms);
+
+/* ---- freeze-out hypersurface (pyvisc/kernel/kernel_hypersf.cl:405-660) ----------
+ * Cubes of the grid thinned by (nxskip, nyskip, nzskip) x (tau_old, tau_new) where
+ * e crosses cv_params.efrz.  Elements are appended in an order that is not defined
+ * (atomic append, as the reference's atomic_inc): per element
+ *   sf    float[8]  dSigma_0..3, vx, vy, veta, eta_s   (hypersf.dat rows)
+ *   txyz  float[4]  t, x, y, z                          (sf_txyz.dat rows)
+ *   pisf  float[10] pi^{mu nu} at the centroid          (viscous; pimnsf.dat rows)
+ *   bksf  float     bulk pressure at the centroid       (CV_FLAG_BULK only)
+ * Elements beyond the capacity are counted but not stored; cv_hypersf_get then fails
+ * with CV_ERR_CAPACITY (the reference writes past its 1.5M-element buffers). */
+
+/* (Re)allocate the element buffers for capacity elements (default 1,500,000 at the
+ * first snapshot, the size of the reference's d_hypersf, pyvisc/ideal.py:113-117).
+ * Keeps the count. */
+int cv_hypersf_reserve(cv_handle *h, int64_t capacity);
+
+/* d_ev_old <- d_ev[1] (viscous: d_pi_old <- d_pi[1]; bulk: its previous field too),
+ * tau_old <- tau: the n == 0 branch of get_hypersf (pyvisc/ideal.py:275-278,
+ * pyvisc/visc.py:367-371). */
+int cv_hypersf_snapshot(cv_handle *h, float tau);
+
+/* get_hypersf / visc_hypersf between the snapshot and d_ev[1] at tau_new, then the
+ * snapshot update (pyvisc/ideal.py:279-291, pyvisc/visc.py:372-397). */
+int cv_hypersf_find(cv_handle *h, float tau_new);
+
+/* Elements found since creation or the last reset (may exceed *capacity), the
+ * reference's d_num_of_sf read back in save (pyvisc/ideal.py:295-300). */
+int cv_hypersf_count(cv_handle *h, int64_t *count, int64_t *capacity);
+
+/* Copy the nelem == count stored elements to host arrays (any may be NULL), as
+ * save / save_pimn_sf read d_hypersf, d_sf_txyz, d_pi_sf (pyvisc/ideal.py:301-313,
+ * pyvisc/visc.py:401-419). */
+int cv_hypersf_get(cv_handle *h, float *sf, float *txyz, float *pisf, float *bksf, int64_t nelem);
+
+/* count <- 0 and drop the snapshot. */
+int cv_hypersf_reset(cv_handle *h);
 
 #ifdef __cplusplus
 }

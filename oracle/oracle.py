@@ -234,3 +234,47 @@ def max_prop_speed(params, ev, vk, precision='f32'):
     out = np.zeros(vk.size, dt)
     L.orc_max_prop_speed(C.byref(params), _ptr(ev), _ptr(vk), C.c_int64(vk.size), _ptr(out))
     return out
+
+
+# -- freeze-out hypersurface ----------------------------------------------------------
+class OrcSfParams(C.Structure):
+    _fields_ = [('nx', C.c_int32), ('ny', C.c_int32), ('nz', C.c_int32),
+                ('nxskip', C.c_int32), ('nyskip', C.c_int32), ('nzskip', C.c_int32),
+                ('dx', C.c_float), ('dy', C.c_float), ('dz', C.c_float), ('efrz', C.c_float)]
+
+
+def test_cube(efrz, precision='f32'):
+    """dSigma of the reference's test cube (old corners e = 3, new corners e = 2)."""
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    out = np.zeros(4, dt)
+    rt = C.c_float if precision == 'f32' else C.c_double
+    L.orc_test_hypersf(rt(efrz), _ptr(out))
+    return out
+
+
+def hypersf(nx, ny, nz, dx, dy, dz, nxskip, nyskip, nzskip, efrz, ev_old, ev_new, tau_old, tau_new,
+            pi_old=None, pi_new=None, bk_old=None, bk_new=None, cap=200000, precision='f32'):
+    """Surface elements between two AoS ev fields (hypersf_oracle.c: orc_hypersf).
+
+    Returns (count, sf (n,8), txyz (n,4), pi (n,10) or None, bulk (n,) or None) with
+    n = min(count, cap); count is the number of elements found."""
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    rt = C.c_float if precision == 'f32' else C.c_double
+    p = OrcSfParams(nx, ny, nz, nxskip, nyskip, nzskip, dx, dy, dz, efrz)
+    viscous = pi_old is not None
+    with_bulk = viscous and bk_old is not None
+    sf = np.zeros((cap, 8), dt)
+    tx = np.zeros((cap, 4), dt)
+    ps = np.zeros((cap, 10), dt) if viscous else None
+    bs = np.zeros(cap, dt) if with_bulk else None
+    cnt = C.c_int64(0)
+    a = lambda x: None if x is None else np.ascontiguousarray(x, dt)
+    evo, evn, pio, pin, bko, bkn = a(ev_old), a(ev_new), a(pi_old), a(pi_new), a(bk_old), a(bk_new)
+    L.orc_hypersf.restype = C.c_int64
+    L.orc_hypersf(C.byref(p), _ptr(evo), _ptr(evn), _ptr(pio), _ptr(pin), _ptr(bko), _ptr(bkn),
+                  rt(tau_old), rt(tau_new), _ptr(sf), _ptr(tx), _ptr(ps), _ptr(bs), C.byref(cnt),
+                  C.c_int64(cap), C.c_int(int(viscous)))
+    n = min(cnt.value, cap)
+    return (cnt.value, sf[:n], tx[:n], (ps[:n] if viscous else None), (bs[:n] if with_bulk else None))
