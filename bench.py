@@ -151,7 +151,7 @@ def main():
         cpu = cpu_baseline(p, tab, ev, nth)
         log('cpu baseline: %.4g cell-updates/s on %d threads (%.1fs)' % (cpu['value'], nth, time.time() - t0))
     out = {
-        'metric': 'cell-updates/s, viscous KT+IS 385x385x115',
+        'metric': 'cell-updates/s, viscous KT+IS 385x385x115; learned vs plain at matched error',
         'value': value, 'unit': 'cell-updates/s', 'n_gpus': 1,
         'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': ms_per_step,
         'higher_is_better': True, 'scaling': 'weak',

@@ -155,3 +155,57 @@ def test_visc_stage_api_matches_reference_sequence():
         m = max_rel(g, r)
         _say('  %-6s max rel %.3g bit-equal %.6f' % (name, m.max(), bit_equal_frac(g, r)))
         assert m.max() <= TOL
+
+
+def _gubser_visc_handle(nx, dx, dt, tau0, L, lam1):
+    tab = Eos('ideal_gas').create_table()
+    p = cv_params(nx, nx, 1, dt, dx, dx, 0.3, tau0, etaos_xmin=0.0, etaos_ymin=0.2, lam1=lam1,
+                  flags=_lib.CV_FLAG_GUBSER_VISC_TEST)
+    h = _lib.Handle(p, tab, viscous=True)
+    h.visc_initialize(tau0)
+    h.ideal_stage(1, np.float32(tau0))
+    ev1, pi1 = ic.gubser_visc(nx, nx, 1, dx, dx, 0.3, tau0, L, lam1)
+    ev2, pi2 = ic.gubser_visc(nx, nx, 1, dx, dx, 0.3, tau0 + dt, L, lam1)
+    for name, a in (('ev1', ev1), ('pi1', pi1), ('ev2', ev2), ('pi2', pi2)):
+        h.set(name, a.astype(np.float32))
+    h.update_udiff('ev1', 'ev2')
+    return p, tab, h, (ev1, pi1, ev2, pi2)
+
+
+def test_gubser_visc_bitwise_151():
+    """GUBSER_VISC_TEST branch on the device vs the oracle, same start as
+    pyvisc/tests/test_gubser_visc.py (analytic fields at tau0 and tau0 + DT)."""
+    nx, dx, dt, tau0, L, lam1 = 151, 0.04, 0.005, 1.0, 2.0, -10.0
+    _say('case: viscous gubser %dx%dx1, 100 steps vs oracle' % (nx, nx))
+    p, tab, h, (ev1, pi1, ev2, pi2) = _gubser_visc_handle(nx, dx, dt, tau0, L, lam1)
+    h.visc_steps(0, 100)
+    o = O.Oracle(orc_params(p, tab, nthreads=16), 'f32', viscous=True)
+    o.load_ini(ev1, pi1)
+    o.ev[2][...] = ev2
+    o.pi[2][...] = pi2
+    o.update_udiff(o.ev[1], o.ev[2])
+    o.visc_steps(100)
+    for name, r in (('ev1', o.ev[1]), ('pi1', o.pi[1])):
+        g = h.get(name)
+        m = max_rel(g, r)
+        _say('  %-4s max rel %.3g bit-equal %.6f' % (name, m.max(), bit_equal_frac(g, r)))
+        assert m.max() <= TOL
+
+
+def test_gubser_visc_analytic_501():
+    """The reference's viscous Gubser setup (501^2, DX 0.04, DT 0.005, L 2, lambda_1 -10,
+    pyvisc/tests/test_gubser_visc.py:53-80) to tau = 2 fm on the device: e, v_x, pi^zz,
+    pi^xx within r < 2 fm against the analytic solution."""
+    nx, dx, dt, tau0, L, lam1 = 501, 0.04, 0.005, 1.0, 2.0, -10.0
+    _say('case: viscous gubser %dx%dx1, 200 steps vs analytic' % (nx, nx))
+    p, tab, h, _ = _gubser_visc_handle(nx, dx, dt, tau0, L, lam1)
+    h.visc_steps(0, 200)
+    ea, pa = ic.gubser_visc(nx, nx, 1, dx, dx, 0.3, tau0 + 200 * dt, L, lam1)
+    ev, pi = h.get('ev1'), h.get('pi1')
+    x = (np.arange(nx) - nx // 2) * dx
+    X, Y = np.meshgrid(x, x, indexing='ij')
+    m = np.sqrt(X * X + Y * Y).ravel() < 2.0
+    err = (np.abs(ev[m, 0] / ea[m, 0] - 1).max(), np.abs(ev[m, 1] - ea[m, 1]).max(),
+           np.abs(pi[m, 9] / pa[m, 9] - 1).max(), np.abs(pi[m, 4] - pa[m, 4]).max() / np.abs(pa[m, 4]).max())
+    _say('  r<2: e max rel %.3g, vx max abs %.3g, pizz max rel %.3g, pixx max abs/scale %.3g' % err)
+    assert err[0] < 0.01 and err[1] < 0.003 and err[2] < 0.04 and err[3] < 0.03

@@ -8,6 +8,7 @@ Reference known-answer tests restated here:
   Bjorken decay (ideal)       pyvisc/tests/test_bjorken.py:31-62
   Gubser ideal flow           pyvisc/tests/test_gubser_ideal.py:22-26
   1st-order viscous Bjorken   pyvisc/tests/test_bjorken_visc.py:74-80
+  2nd-order viscous Gubser    pyvisc/tests/test_gubser_visc.py:11-27 (GUBSER_VISC_TEST)
 """
 import numpy as np
 import pytest
@@ -317,3 +318,42 @@ def test_bulk_zero_reduces_to_shear(cross):
         out.append((o.ev[1].copy(), o.pi[1].copy()))
     np.testing.assert_array_equal(out[0][0], out[1][0])
     np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+def test_gubser_viscous_analytic(ideal_tab):
+    """Second-order viscous Gubser flow (L = 2 fm, lambda_1 = -10, eta/s = 0.2) with the
+    GUBSER_VISC_TEST relaxation branch, started from the analytic e, u, pi at tau0 = 1
+    and tau0 + DT (udiff from the two), as pyvisc/tests/test_gubser_visc.py sets it up;
+    after tau = 1 -> 1.5 fm the fields agree with the analytic solution within r < 2 fm
+    (grid 151^2, DX = 0.04, DT = 0.005; the reference's own test uses 501^2)."""
+    nx, dx, dt, tau0, L, lam1 = 151, 0.04, 0.005, 1.0, 2.0, -10.0
+    p = O.make_params(nx, nx, 1, dt, dx, dx, 0.3, tau0, ideal_tab, etaos_xmin=0.0, etaos_ymin=0.2,
+                      lam1=lam1, gubser_visc_test=True, nthreads=8)
+    o = O.Oracle(p, 'f32', viscous=True)
+    ev1, pi1 = ic.gubser_visc(nx, nx, 1, dx, dx, 0.3, tau0, L, lam1)
+    ev2, pi2 = ic.gubser_visc(nx, nx, 1, dx, dx, 0.3, tau0 + dt, L, lam1)
+    o.load_ini(ev1, pi1)
+    o.ev[2][...] = ev2
+    o.pi[2][...] = pi2
+    o.update_udiff(o.ev[1], o.ev[2])
+    o.visc_steps(100)
+    ea, pa = ic.gubser_visc(nx, nx, 1, dx, dx, 0.3, tau0 + 100 * dt, L, lam1)
+    x = (np.arange(nx) - nx // 2) * dx
+    X, Y = np.meshgrid(x, x, indexing='ij')
+    m = np.sqrt(X * X + Y * Y).ravel() < 2.0
+    assert np.abs(o.ev[1][m, 0] / ea[m, 0] - 1).max() < 0.01
+    assert np.abs(o.ev[1][m, 1] - ea[m, 1]).max() < 0.002
+    assert np.abs(o.pi[1][m, 9] / pa[m, 9] - 1).max() < 0.03
+    assert np.abs(o.pi[1][m, 4] - pa[m, 4]).max() < 0.02 * np.abs(pa[m, 4]).max()
+    # a plausible mistake -- the second-order coefficient lambda_1 off by a factor of 2 in
+    # the evolution -- fails the same bounds (this special solution does not depend on eta/s)
+    pw = O.make_params(nx, nx, 1, dt, dx, dx, 0.3, tau0, ideal_tab, etaos_xmin=0.0, etaos_ymin=0.2,
+                       lam1=lam1 / 2, gubser_visc_test=True, nthreads=8)
+    w = O.Oracle(pw, 'f32', viscous=True)
+    w.load_ini(ev1, pi1)
+    w.ev[2][...] = ev2
+    w.pi[2][...] = pi2
+    w.update_udiff(w.ev[1], w.ev[2])
+    w.visc_steps(100)
+    assert np.abs(w.pi[1][m, 9] / pa[m, 9] - 1).max() > 0.03 or \
+        np.abs(w.ev[1][m, 0] / ea[m, 0] - 1).max() > 0.01
