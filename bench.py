@@ -130,8 +130,11 @@ def main():
     ms_per_step = 1000.0 * wall / args.steps
     value = ncell * args.steps / wall
     edmax = h.max_ed(1)
-    ev1 = None
-    ok = np.isfinite(edmax)
+    # the hotspot field only decays (peak 50 GeV/fm^3 at tau0): a larger or non-finite max
+    # means the evolution went unstable and the timing is not of the physical workload
+    ok = bool(np.isfinite(edmax) and 0.0 < edmax <= 50.0)
+    if not ok:
+        log('WARNING: max e after the run is %r: the evolution is not physical' % edmax)
     log('steps %d wall %.3fs device %.3f ms -> %.3f ms/step, %.4g cell-updates/s, edmax %.4g'
         % (args.steps, wall, dev_ms, ms_per_step, value, edmax))
     abytes = ALGO_BYTES[bulk]
@@ -160,7 +163,8 @@ def main():
         'config': {'workload': 'viscous KT + Israel-Stewart shear (eta/s=0.08)%s, 385x385x115, '
                                'dtau=0.02 fm, 400 Woods-Saxon hotspots, crossover EOS (octave table)'
                                % (' + bulk (zeta/s peak 0.05 at 0.155 GeV)' if bulk else ''),
-                   'grid': [NX, NY, NZ], 'device': name, 'finite': bool(ok)},
+                   'grid': [NX, NY, NZ], 'device': name, 'physical': ok,
+                   'edmax_end': float(edmax)},
         'roofline': roof,
         'cpu_baseline': cpu,
     }

@@ -191,12 +191,15 @@ __device__ __forceinline__ float max_prop_speed(const PhysP &p, const Eos &eos, 
 }
 
 // rootFinding_newton (helper.h:128-167) with P_eff = P + bulk
-__device__ __forceinline__ float root_newton(const Eos &eos, float T00, float M, float bulk) {
+// bulk: Pi added to P(e); clamp0: effective pressure P + Pi held >= 0 (bulk runs only:
+// a negative effective pressure would allow |v| > 1 in v = M / (T00 + P + Pi))
+__device__ __forceinline__ float root_newton(const Eos &eos, float T00, float M, float bulk, bool clamp0 = false) {
     float vl = 0.0f, vh = 1.0f;
     const float dpe = 1.0f / 3.0f;
     float v = 0.5f * (vl + vh);
     float ed = T00 - M * v;
     float pr = eos.P(ed) + bulk;
+    if (clamp0) pr = rmax(pr, 0.0f);
     float f = (T00 + pr) * v - M;
     float df = (T00 + pr) - M * v * dpe;
     float dvold = vh - vl;
@@ -216,6 +219,7 @@ __device__ __forceinline__ float root_newton(const Eos &eos, float T00, float M,
         if (fabsf(dv) < 0.00001f || i > 100) break;
         ed = T00 - M * v;
         pr = eos.P(ed) + bulk;
+        if (clamp0) pr = rmax(pr, 0.0f);
         f = (T00 + pr) * v - M;
         df = (T00 + pr) - M * v * dpe;
         if (f > 0.0f) vh = v; else vl = v;
@@ -257,11 +261,13 @@ __device__ __forceinline__ double eos_P_d(const EosP &E, const float4 *tab, doub
     return 0.3327 * eps - 0.3223 * pow(eps, 0.4585) - 0.003906 * eps * exp(-0.05697 * eps);
 }
 
-__device__ __forceinline__ double root_newton_d(const EosP &E, const float4 *tab, double T00, double M, double bulk) {
+__device__ __forceinline__ double root_newton_d(const EosP &E, const float4 *tab, double T00, double M, double bulk,
+                                                bool clamp0 = false) {
     double vl = 0.0, vh = 1.0, dpe = 1.0 / 3.0;
     double v = 0.5;
     double ed = T00 - M * v;
     double pr = eos_P_d(E, tab, ed) + bulk;
+    if (clamp0) pr = pr > 0.0 ? pr : 0.0;
     double f = (T00 + pr) * v - M;
     double df = (T00 + pr) - M * v * dpe;
     double dvold = vh - vl, dv = dvold;
@@ -276,6 +282,7 @@ __device__ __forceinline__ double root_newton_d(const EosP &E, const float4 *tab
         if (fabs(dv) < 0.00001 || i > 100) break;
         ed = T00 - M * v;
         pr = eos_P_d(E, tab, ed) + bulk;
+        if (clamp0) pr = pr > 0.0 ? pr : 0.0;
         f = (T00 + pr) * v - M;
         df = (T00 + pr) - M * v * dpe;
         if (f > 0.0) vh = v; else vl = v;

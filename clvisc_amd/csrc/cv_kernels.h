@@ -427,6 +427,7 @@ __global__ __launch_bounds__(256) void k_is_stage(PhysP p, EosP E, StageP st,
         bs += bsrc_;
         float b_old = bk1[I] * u_old[0];
         bout = (b_old + (bs * p.dt) * st.r_step - p.dt * oot_b * zetav * theta) * rcp(u_new[0] + p.dt * oot_b);
+        bout = rmax(bout, -eos.P(e2.x));   // no negative effective pressure
         if (fabsf(bout) > max_abs) max_abs = fabsf(bout);
     }
     float pre = eos.P(e2.x);
@@ -584,12 +585,13 @@ __global__ __launch_bounds__(256) void k_kt_stage(PhysP p, EosP E, StageP st,
     float bnew = BULK ? bknew[I] : 0.0f;
     float ed_find;
     if ((p.flags & 8) && M > p.fp64_vmin * T00)
-        ed_find = (float)root_newton_d(E, E.tab, (double)T00, (double)M, (double)bnew);
+        ed_find = (float)root_newton_d(E, E.tab, (double)T00, (double)M, (double)bnew, BULK);
     else
-        ed_find = root_newton(eos, T00, M, bnew);
+        ed_find = root_newton(eos, T00, M, bnew, BULK);
     if (ed_find < CV_ACU) { T01 = 0.0f; T02 = 0.0f; T03 = 0.0f; }
     ed_find = rmax(0.0f, ed_find);
     float pr = eos.P(ed_find) + bnew;
+    if (BULK) pr = rmax(pr, 0.0f);
     float repv = rcp(rmax(CV_ACU, T00 + pr));
     float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
     st4(ev_out, p.n, I, en);
@@ -859,6 +861,7 @@ __global__ __launch_bounds__(256, CV_STAGE_WAVES) void k_visc_stage(PhysP p, Eos
         b_start = STEP == 1 ? b2 : bk1[I];
         float b_old = b_start * u_old[0];
         bout = (b_old + (bs * p.dt) * st.r_step - p.dt * oot_b * zetav * theta) * rcp(u_new[0] + p.dt * oot_b);
+        bout = rmax(bout, -eos.P(e2.x));   // no negative effective pressure
         if (fabsf(bout) > max_abs) max_abs = fabsf(bout);
     }
     {
@@ -906,12 +909,13 @@ __global__ __launch_bounds__(256, CV_STAGE_WAVES) void k_visc_stage(PhysP p, Eos
     }
     float ed_find;
     if ((p.flags & 8) && M > p.fp64_vmin * T00)
-        ed_find = (float)root_newton_d(E, E.tab, (double)T00, (double)M, (double)bout);
+        ed_find = (float)root_newton_d(E, E.tab, (double)T00, (double)M, (double)bout, BULK);
     else
-        ed_find = root_newton(eos, T00, M, bout);
+        ed_find = root_newton(eos, T00, M, bout, BULK);
     if (ed_find < CV_ACU) { T01 = 0.0f; T02 = 0.0f; T03 = 0.0f; }
     ed_find = rmax(0.0f, ed_find);
     float pr = eos.P(ed_find) + bout;
+    if (BULK) pr = rmax(pr, 0.0f);
     float repv = rcp(rmax(CV_ACU, T00 + pr));
     float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
     st4(ev_out, n, I, en);

@@ -401,6 +401,7 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
                 b_start = STEP == 1 ? b2 : ldo(bk1, bo);
                 float b_old = b_start * u_old[0];
                 bout = (b_old + (bs * p.dt) * st.r_step - p.dt * oot_b * zetav * theta) * rcp(u_new[0] + p.dt * oot_b);
+                bout = rmax(bout, -eos.P(e2.x));   // no negative effective pressure
                 if (fabsf(bout) > max_abs) max_abs = fabsf(bout);
             }
             {
@@ -448,12 +449,13 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
             }
             float ed_find;
             if ((p.flags & 8) && M > p.fp64_vmin * T00)
-                ed_find = (float)root_newton_d(E, tab, (double)T00, (double)M, (double)bout);
+                ed_find = (float)root_newton_d(E, tab, (double)T00, (double)M, (double)bout, BULK);
             else
-                ed_find = root_newton(eos, T00, M, bout);
+                ed_find = root_newton(eos, T00, M, bout, BULK);
             if (ed_find < CV_ACU) { T01 = 0.0f; T02 = 0.0f; T03 = 0.0f; }
             ed_find = rmax(0.0f, ed_find);
             float pr = eos.P(ed_find) + bout;
+            if (BULK) pr = rmax(pr, 0.0f);
             float repv = rcp(rmax(CV_ACU, T00 + pr));
             float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
             *(float *)((char *)ev_out + bo) = en.x;

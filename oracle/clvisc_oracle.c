@@ -272,14 +272,15 @@ static real max_prop_speed(const orc_params *p, const real *edv, real vk) {
     return r_max(lam, RC(1.0f));
 }
 
-/* rootFinding_newton (helper.h:128-167) with P_eff(e) = P(e) + bulk.  The
- * optional seed replaces the midpoint start (surrogate-seeded variant). */
-static real root_newton(const orc_params *p, real T00, real M, real bulk, int *iters) {
+/* rootFinding_newton (helper.h:128-167) with P_eff(e) = P(e) + bulk; clamp0 (bulk
+ * runs) holds P_eff >= 0 so that v = M / (T00 + P_eff) stays below 1. */
+static real root_newton(const orc_params *p, real T00, real M, real bulk, int clamp0, int *iters) {
     real vl = RC(0.0f), vh = RC(1.0f);
     real dpe = RC(1.0f) / RC(3.0f);
     real v = RC(0.5f) * (vl + vh);
     real ed = T00 - M * v;
     real pr = eos_P(p, ed) + bulk;
+    if (clamp0) pr = r_max(pr, RC(0.0f));
     real f = (T00 + pr) * v - M;
     real df = (T00 + pr) - M * v * dpe;
     real dvold = vh - vl;
@@ -300,6 +301,7 @@ static real root_newton(const orc_params *p, real T00, real M, real bulk, int *i
         if ((real)fabs((double)dv) < RC(0.00001f) || i > 100) break;
         ed = T00 - M * v;
         pr = eos_P(p, ed) + bulk;
+        if (clamp0) pr = r_max(pr, RC(0.0f));
         f = (T00 + pr) * v - M;
         df = (T00 + pr) - M * v * dpe;
         if (f > RC(0.0f)) vh = v; else vl = v;
@@ -348,11 +350,12 @@ static double eos_P_d(const orc_params *p, double eps) {
     return 0.3327 * eps - 0.3223 * pow(eps, 0.4585) - 0.003906 * eps * exp(-0.05697 * eps);
 }
 
-static double root_newton_d(const orc_params *p, double T00, double M, double bulk, int *iters) {
+static double root_newton_d(const orc_params *p, double T00, double M, double bulk, int clamp0, int *iters) {
     double vl = 0.0, vh = 1.0, dpe = 1.0 / 3.0;
     double v = 0.5;
     double ed = T00 - M * v;
     double pr = eos_P_d(p, ed) + bulk;
+    if (clamp0) pr = pr > 0.0 ? pr : 0.0;
     double f = (T00 + pr) * v - M;
     double df = (T00 + pr) - M * v * dpe;
     double dvold = vh - vl, dv = dvold;
@@ -367,6 +370,7 @@ static double root_newton_d(const orc_params *p, double T00, double M, double bu
         if (fabs(dv) < 0.00001 || i > 100) break;
         ed = T00 - M * v;
         pr = eos_P_d(p, ed) + bulk;
+        if (clamp0) pr = pr > 0.0 ? pr : 0.0;
         f = (T00 + pr) * v - M;
         df = (T00 + pr) - M * v * dpe;
         if (f > 0.0) vh = v; else vl = v;
@@ -377,12 +381,12 @@ static double root_newton_d(const orc_params *p, double T00, double M, double bu
 #endif
 
 /* solve dispatch: FP64 for fast cells when the extension is on */
-static real solve_ed(const orc_params *p, real T00, real M, real bulk, int *iters) {
+static real solve_ed(const orc_params *p, real T00, real M, real bulk, int clamp0, int *iters) {
 #if !ORC_DOUBLE
     if (p->fp64_root && M > RC(p->fp64_root_vmin) * T00)
-        return (real)root_newton_d(p, (double)T00, (double)M, (double)bulk, iters);
+        return (real)root_newton_d(p, (double)T00, (double)M, (double)bulk, clamp0, iters);
 #endif
-    return root_newton(p, T00, M, bulk, iters);
+    return root_newton(p, T00, M, bulk, clamp0, iters);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -405,7 +409,7 @@ void orc_minmod4(const real *a, const real *b, real *out) {
 void orc_root(const orc_params *p, const real *T00, const real *M, int64_t n, real *ed, int32_t *iters) {
     for (int64_t i = 0; i < n; i++) {
         int it = 0;
-        ed[i] = solve_ed(p, T00[i], M[i], RC(0.0f), &it);
+        ed[i] = solve_ed(p, T00[i], M[i], RC(0.0f), 0, &it);
         if (iters) iters[i] = it;
     }
 }
@@ -597,7 +601,7 @@ void orc_ideal_update_ev(const orc_params *p, real *evnew, const real *ev1, cons
             M = SC * T00;
         }
         int it = 0;
-        real ed_find = solve_ed(p, T00, M, RC(0.0f), &it);
+        real ed_find = solve_ed(p, T00, M, RC(0.0f), 0, &it);
         if (iters) iters[I] = it;
         ed_find = r_max(RC(0.0f), ed_find);
         real pr = eos_P(p, ed_find);
@@ -774,11 +778,12 @@ void orc_visc_update_ev(const orc_params *p, real *evnew, const real *ev1, const
         }
         real bnew = bulk_at(bulk2, I);
         int it = 0;
-        real ed_find = solve_ed(p, T00, M, bnew, &it);
+        real ed_find = solve_ed(p, T00, M, bnew, p->bulk_on, &it);
         if (iters) iters[I] = it;
         if (ed_find < ACU) { T01 = RC(0.0f); T02 = RC(0.0f); T03 = RC(0.0f); }
         ed_find = r_max(RC(0.0f), ed_find);
         real pr = eos_P(p, ed_find) + bnew;
+        if (p->bulk_on) pr = r_max(pr, RC(0.0f));
         real epv = r_max(ACU, T00 + pr);
         real *o = evnew + 4 * I;
         o[0] = ed_find; o[1] = DIV(T01, epv); o[2] = DIV(T02, epv); o[3] = DIV(T03, epv);
@@ -1010,6 +1015,7 @@ void orc_update_pimn(const orc_params *p, real *pinew, real *goodcell, const rea
             real b_old = bulk1[I] * u_old[0];
             bout = DIV(b_old + DIV(bulk_src[I] * DT, (real)step) - DT * one_over_taubulk * zetav * theta,
                        u_new[0] + DT * one_over_taubulk);
+            bout = r_max(bout, -eos_P(p, e2[0]));   /* no negative effective pressure */
             if ((real)fabs((double)bout) > max_pimn_abs) max_pimn_abs = (real)fabs((double)bout);
         }
         real pre = eos_P(p, e2[0]);

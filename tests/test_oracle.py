@@ -357,3 +357,23 @@ def test_gubser_viscous_analytic(ideal_tab):
     w.visc_steps(100)
     assert np.abs(w.pi[1][m, 9] / pa[m, 9] - 1).max() > 0.03 or \
         np.abs(w.ev[1][m, 0] / ea[m, 0] - 1).max() > 0.01
+
+
+def test_bulk_dilute_edge_stays_subluminal(cross):
+    """Regression: the benchmark's hotspot IC (boost-invariant slice, 385^2, bulk on)
+    once drove P + Pi < 0 in the dilute periphery, so v = M/(T00 + P + Pi) exceeded 1
+    by step ~66 and e blew up to 1e9.  With Pi >= -P(e) and P + Pi >= 0 in the root
+    solve every cell stays below the 0.999 clamp and max e decays."""
+    eos, tab = cross
+    nx = 385
+    pos = ic.woods_saxon_positions(400, 20240601)
+    ev = ic.hotspots(nx, nx, 1, 0.08, 0.08, 0.15, positions=pos, emax=50.0)
+    p = params(tab, nx=nx, ny=nx, nz=1, dt=0.02, dx=0.08, dy=0.08, dz=0.15, tau0=0.6,
+               etaos_xmin=0.0, etaos_ymin=0.08, bulk=(0.05, 0.155, 0.02), nthreads=8)
+    o = O.Oracle(p, 'f32', viscous=True)
+    o.load_ini(ev)
+    for _ in range(15):
+        o.visc_steps(5)
+        v = np.sqrt((o.ev[1][:, 1:4] ** 2).sum(1)).max()
+        assert v <= 0.9991, v
+    assert o.ev[1][:, 0].max() < 10.0
