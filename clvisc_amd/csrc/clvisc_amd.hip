@@ -176,6 +176,8 @@ extern "C" int cv_create(const cv_params *p, const cv_eos *eos, int device, int 
     ph.zmax = p->zetaos_max; ph.ztpeak = p->zetaos_tpeak; ph.zwidth = p->zetaos_width;
     ph.fp64_vmin = p->fp64_root_vmin;
     ph.flags = p->flags;
+    // 32-bit plane offsets of the tiled kernel: 10 planes of 4 n bytes below 2^32
+    ph.nb = (40 * n < ((int64_t)1 << 32)) ? (uint32_t)(4 * n) : 0u;
     // the max(lambda, 1) shortcut needs every cs2 the lookup can return <= 0.5
     float cs2max = -1e30f;
     for (int64_t i = 0; i < eos->n_entries; i++) cs2max = eos->rgba[4 * i] > cs2max ? eos->rgba[4 * i] : cs2max;
@@ -222,6 +224,7 @@ extern "C" int cv_create(const cv_params *p, const cv_eos *eos, int device, int 
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->cus = prop.multiProcessorCount;
     }
+    if (!ph.nb) h->kernel = 0;     // the tiled kernel needs 32-bit plane offsets
     if (ok && (rc = default_tile(h)) != CV_OK) {
         cv_destroy(h);
         return rc;
@@ -407,13 +410,19 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
         dim3 g(h->tile_grid), b(CV_TILE_THREADS);
 #define CV_TILED_ARGS h->phys, h->eos, st, h->tile, ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src, \
                       h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out
+        const bool spec = (h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST | CV_FLAG_FP64_ROOT)) != 0;
+        const int ek = h->eos.kind;
+#define CV_TILED_LAUNCH(B, S, K, X) hipLaunchKernelGGL((k_visc_tiled<B, S, K, X>), g, b, 0, h->stream, CV_TILED_ARGS)
+#define CV_TILED_SEL(B, S)                                      \
+        if (ek == 1) { if (spec) CV_TILED_LAUNCH(B, S, 1, true); else CV_TILED_LAUNCH(B, S, 1, false); } \
+        else { if (spec) CV_TILED_LAUNCH(B, S, 0, true); else CV_TILED_LAUNCH(B, S, 0, false); }
         if (h->bulk) {
-            if (step == 1) hipLaunchKernelGGL((k_visc_tiled<true, 1>), g, b, 0, h->stream, CV_TILED_ARGS);
-            else hipLaunchKernelGGL((k_visc_tiled<true, 2>), g, b, 0, h->stream, CV_TILED_ARGS);
+            if (step == 1) { CV_TILED_SEL(true, 1) } else { CV_TILED_SEL(true, 2) }
         } else {
-            if (step == 1) hipLaunchKernelGGL((k_visc_tiled<false, 1>), g, b, 0, h->stream, CV_TILED_ARGS);
-            else hipLaunchKernelGGL((k_visc_tiled<false, 2>), g, b, 0, h->stream, CV_TILED_ARGS);
+            if (step == 1) { CV_TILED_SEL(false, 1) } else { CV_TILED_SEL(false, 2) }
         }
+#undef CV_TILED_SEL
+#undef CV_TILED_LAUNCH
 #undef CV_TILED_ARGS
         HIP_TRY(hipGetLastError());
         return CV_OK;
@@ -560,6 +569,8 @@ extern "C" int cv_profile_visc_step(cv_handle *h, int64_t loop0, float *ms, int 
 extern "C" int cv_set_stage_kernel(cv_handle *h, int kind, int ty, int tz) {
     if (!h) return fail(CV_ERR_ARG, "null handle");
     if (kind != 0 && kind != 1) return fail(CV_ERR_ARG, "kernel kind must be 0 (per-cell) or 1 (tiled)");
+    if (kind == 1 && !h->phys.nb)
+        return fail(CV_ERR_UNSUPPORTED, "tiled kernel needs 40 * cells < 2^32 (%lld cells)", (long long)h->n);
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (kind == 1) {
         int rc = (ty > 0 && tz > 0) ? set_tile(h, ty, tz) : default_tile(h);

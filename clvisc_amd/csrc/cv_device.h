@@ -43,6 +43,7 @@ struct PhysP {
     float fp64_vmin;
     int flags;
     int lam_fast;           // 1 when the table's cs2 <= 0.5 (max(lambda,1) shortcut valid)
+    uint32_t nb;            // bytes of one field plane (4 n) when 10 planes fit 32-bit offsets, else 0
 };
 
 struct StageP {
@@ -65,16 +66,16 @@ __device__ __forceinline__ float minmod(float x, float y) {
 }
 // generalised minmod slope at b from cells (a, b, c):
 //   minmod(0.5 (c-a), minmod(THETA (c-b), THETA (b-a)))   (helper.h:213-217)
-// in its closed form: sign(c-b) * min(0.5|c-a|, THETA|c-b|, THETA|b-a|) when c-b and
-// b-a share a strict sign, else 0 -- the same value bit for bit (products with 0.5, 2
-// and signs are exact), in about a third of the instructions.
+// in closed form: 0 when c-b and b-a carry different sign bits, else
+// copysign(min(0.5|c-a|, THETA|c-b|, THETA|b-a|), c-b).  For finite inputs this is the
+// reference's value bit for bit except that a zero may carry the sign of c-b (a zero
+// difference makes the min 0); products with 0.5 and signs are exact.  The operands of
+// the min are >= +0, so IEEE minNum (v_min) equals the reference's min().
 __device__ __forceinline__ float slope(float a, float b, float c) {
     float d2 = c - b, d3 = b - a;
-    bool same = (d2 > 0.0f && d3 > 0.0f) || (d2 < 0.0f && d3 < 0.0f);
-    float m = 0.5f * fabsf(c - a);
-    float t2 = CV_THETA * fabsf(d2), t3 = CV_THETA * fabsf(d3);
-    m = rmin(m, rmin(t2, t3));
-    return same ? (d2 > 0.0f ? m : -m) : 0.0f;
+    float m = fminf(0.5f * fabsf(c - a), fminf(CV_THETA * fabsf(d2), CV_THETA * fabsf(d3)));
+    uint32_t sx = __float_as_uint(d2) ^ __float_as_uint(d3);
+    return (sx & 0x80000000u) ? 0.0f : copysignf(m, d2);
 }
 __device__ __forceinline__ float gamma3(float vx, float vy, float vz) {
     return rcp(sqrtf(rmax(1.0f - vx * vx - vy * vy - vz * vz, CV_ACU)));
@@ -129,15 +130,18 @@ __device__ __forceinline__ float4 eos_octave(const EosP &E, const float4 *tab, f
 }
 
 // A lookup context: the table pointer may be an LDS copy of E.tab.
-struct Eos {
+// K = 1: octave table, K = 0: row/col table, K = -1: chosen at run time from E.kind
+// (kernels specialised on the table kind carry only that kind's parameters)
+template <int K> struct EosT {
     const EosP &E;
     const float4 *tab;
-    __device__ Eos(const EosP &e, const float4 *t) : E(e), tab(t) {}
+    __device__ EosT(const EosP &e, const float4 *t) : E(e), tab(t) {}
+    __device__ __forceinline__ bool octave() const { return K == 1 || (K < 0 && E.kind == 1); }
     __device__ __forceinline__ float4 raw(float eps) const {
-        return E.kind == 1 ? eos_octave(E, tab, eps) : eos_rowcol(E, eps);
+        return octave() ? eos_octave(E, tab, eps) : eos_rowcol(E, eps);
     }
     __device__ __forceinline__ bool in_table(float eps) const {
-        return E.kind == 1 || eps < E.top;
+        return octave() || eps < E.top;
     }
     // P, S, T, CS2 with the analytic fit above a row/col table (eos_table.h:65-100)
     __device__ __forceinline__ float P(float eps) const {
@@ -154,6 +158,7 @@ struct Eos {
         return 0.333333333f;
     }
 };
+using Eos = EosT<-1>;
 
 // ---------------------------------------------------------------------------
 // Hydro helpers (helper.h)
@@ -178,7 +183,8 @@ __device__ __forceinline__ float4 t0m(float4 ev, float pr) {
 // maxPropagationSpeed (helper.h:94-105) = max(lambda_KT, 1).  When the table's cs2 is
 // at most 0.5 and v^2 <= 0.81, lambda_KT <= (|v| + cs)/(1 + |v| cs) < 0.983, far below
 // 1 minus its FP32 rounding error, so the maximum is exactly 1.0f.
-__device__ __forceinline__ float max_prop_speed(const PhysP &p, const Eos &eos, float4 edv, float vk) {
+template <class EOS>
+__device__ __forceinline__ float max_prop_speed(const PhysP &p, const EOS &eos, float4 edv, float vk) {
     if (p.lam_fast && (edv.y * edv.y + edv.z * edv.z + edv.w * edv.w) <= 0.81f) return 1.0f;
     float ut = gamma3(edv.y, edv.z, edv.w);
     float uk = ut * vk;
@@ -193,7 +199,8 @@ __device__ __forceinline__ float max_prop_speed(const PhysP &p, const Eos &eos, 
 // rootFinding_newton (helper.h:128-167) with P_eff = P + bulk
 // bulk: Pi added to P(e); clamp0: effective pressure P + Pi held >= 0 (bulk runs only:
 // a negative effective pressure would allow |v| > 1 in v = M / (T00 + P + Pi))
-__device__ __forceinline__ float root_newton(const Eos &eos, float T00, float M, float bulk, bool clamp0 = false) {
+template <class EOS>
+__device__ __forceinline__ float root_newton(const EOS &eos, float T00, float M, float bulk, bool clamp0 = false) {
     float vl = 0.0f, vh = 1.0f;
     const float dpe = 1.0f / 3.0f;
     float v = 0.5f * (vl + vh);

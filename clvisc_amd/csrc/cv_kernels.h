@@ -32,6 +32,18 @@ __device__ __forceinline__ float ldo(const float *__restrict__ base, uint32_t bo
 __device__ __forceinline__ float4 ld4o(const float *__restrict__ b, int64_t n, uint32_t boff) {
     return make_float4(ldo(b, boff), ldo(b + n, boff), ldo(b + 2 * n, boff), ldo(b + 3 * n, boff));
 }
+// Plane m of a field at byte offset boff of plane 0, as one 32-bit offset from the field's
+// base (nb = bytes per plane): every field shares the uniform plane offsets m * nb, so the
+// kernel keeps one base address per field instead of one per plane.
+__device__ __forceinline__ float ldp(const float *__restrict__ base, uint32_t boff, uint32_t nb, int m) {
+    return ldo(base, boff + (uint32_t)m * nb);
+}
+__device__ __forceinline__ float4 ld4p(const float *__restrict__ b, uint32_t boff, uint32_t nb) {
+    return make_float4(ldp(b, boff, nb, 0), ldp(b, boff, nb, 1), ldp(b, boff, nb, 2), ldp(b, boff, nb, 3));
+}
+__device__ __forceinline__ void stp(float *base, uint32_t boff, uint32_t nb, int m, float v) {
+    *(float *)((char *)base + (boff + (uint32_t)m * nb)) = v;
+}
 __device__ __forceinline__ float comp(float4 v, int c) {
     return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w));
 }

@@ -27,6 +27,12 @@ enum { CF_E = 0, CF_VX, CF_VY, CF_VZ, CF_U0, CF_P, CF_Q0, CF_Q1, CF_Q2, CF_Q3, C
 #define CV_TILE_MAXC 432       // cells of tile + halo held in LDS (2 workgroups per CU)
 #define CV_TILE_MAXF 288       // faces of one axis held in LDS
 #define CV_EOS_LDS 1600        // octave-table entries held in LDS
+#ifndef CV_TILE_EOS_IN_LDS
+#define CV_TILE_EOS_IN_LDS 1   // 0: the octave table is read through the caches instead
+#endif
+#ifndef CV_TILE_MINBLOCKS
+#define CV_TILE_MINBLOCKS 1    // workgroups per CU the register allocation must allow
+#endif
 
 struct TileP {
     int ty, tz;                // tile interior
@@ -37,7 +43,8 @@ struct TileP {
 };
 
 // per-cell quantities of one cell from its raw fields
-__device__ __forceinline__ void cell_prims(const PhysP &p, const Eos &eos, float tau, float4 e, const float pi[10],
+template <class EOS>
+__device__ __forceinline__ void cell_prims(const PhysP &p, const EOS &eos, float tau, float4 e, const float pi[10],
                                            float bk, float out[CF_N]) {
     float u0 = gamma3(e.y, e.z, e.w);
     float pr = eos.P(e.x) + bk;
@@ -55,8 +62,8 @@ __device__ __forceinline__ void cell_prims(const PhysP &p, const Eos &eos, float
 
 // Face value between cells L and R along axis A, from the cells Lm = L-1, L, R, Rp = R+1.
 // G(field, cell) returns a per-cell quantity.  out: 4 KT components, 10 IS components, Pi.
-template <int A, bool BULK, typename G>
-__device__ __forceinline__ void face_flux(const PhysP &p, const Eos &eos, float tau, const G &g, int Lm, int L,
+template <int A, bool BULK, class EOS, typename G>
+__device__ __forceinline__ void face_flux(const PhysP &p, const EOS &eos, float tau, const G &g, int Lm, int L,
                                           int R, int Rp, float out[15]) {
     const float4 eL = make_float4(g(CF_E, L), g(CF_VX, L), g(CF_VY, L), g(CF_VZ, L));
     const float4 eR = make_float4(g(CF_E, R), g(CF_VX, R), g(CF_VY, R), g(CF_VZ, R));
@@ -94,8 +101,10 @@ __device__ __forceinline__ void face_flux(const PhysP &p, const Eos &eos, float 
     }
 }
 
-template <bool BULK, int STEP>
-__global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E, StageP st, TileP tp,
+// EK: EOS table kind (0 row/col, 1 octave); SPEC: the GUBSER_VISC_TEST / RIEMANN_TEST /
+// FP64-root branches are compiled in (flags are still tested at run time)
+template <bool BULK, int STEP, int EK, bool SPEC>
+__global__ __launch_bounds__(CV_TILE_THREADS, CV_TILE_MINBLOCKS) void k_visc_tiled(PhysP p, EosP E, StageP st, TileP tp,
         const float *__restrict__ ev_st, const float *__restrict__ pi_st, const float *__restrict__ bk_st,
         const float *ev1, const float *ev2, const float *pi1, const float *bk1,
         const float *__restrict__ udiff_in, float *IS_src, float *bsrc, float *Src,
@@ -104,17 +113,17 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
     __shared__ float s_f[15][CV_TILE_MAXF];
     __shared__ float4 s_eos[CV_EOS_LDS];
     const int tid = threadIdx.x;
-    const int64_t n = p.n;
+    const uint32_t nb = p.nb;
     const float tau = st.tau;
 
     // EOS table into LDS (octave table small enough), else global
     const float4 *tab = E.tab;
-    if (E.kind == 1 && E.nlast + 1 <= CV_EOS_LDS) {
+    if (CV_TILE_EOS_IN_LDS && EK == 1 && E.nlast + 1 <= CV_EOS_LDS) {
         for (int q = tid; q <= E.nlast; q += CV_TILE_THREADS) s_eos[q] = E.tab[q];
         tab = s_eos;
     }
     __syncthreads();
-    Eos eos(E, tab);
+    EosT<EK> eos(E, tab);
 
     const int ty = tp.ty, tz = tp.tz, HY = tp.HY, HZ = tp.HZ, hy = tp.hy, hz = tp.hz;
     const int my = tid / tz, mz = tid - (tid / tz) * tz;       // own cell in the tile
@@ -137,10 +146,10 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
             j = j < 0 ? 0 : (j > p.ny - 1 ? p.ny - 1 : j);
             k = k < 0 ? 0 : (k > p.nz - 1 ? p.nz - 1 : k);
             const uint32_t bo = (plane + (uint32_t)j * (uint32_t)p.sy + (uint32_t)k) * 4u;
-            float4 e = ld4o(ev_st, n, bo);
+            float4 e = ld4p(ev_st, bo, nb);
             float pv[10];
 #pragma unroll
-            for (int m = 0; m < 10; m++) pv[m] = ldo(pi_st + (int64_t)m * n, bo);
+            for (int m = 0; m < 10; m++) pv[m] = ldp(pi_st, bo, nb, m);
             float bkv = BULK ? ldo(bk_st, bo) : 0.0f;
             float out[CF_N];
             cell_prims(p, eos, tau, e, pv, bkv, out);
@@ -166,9 +175,9 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
                 t[0] = t[1] = t[2] = t[3] = 0.0f;
             } else {
 #pragma unroll
-                for (int m = 0; m < 10; m++) s[m] = ldo(IS_src + (int64_t)m * n, bo);
+                for (int m = 0; m < 10; m++) s[m] = ldp(IS_src, bo, nb, m);
                 if (BULK) bs = ldo(bsrc, bo);
-                float4 qq = ld4o(Src, n, bo);
+                float4 qq = ld4p(Src, bo, nb);
                 t[0] = qq.x; t[1] = qq.y; t[2] = qq.z; t[3] = qq.w;
             }
             ec = make_float4(s_c[CF_E][cc], s_c[CF_VX][cc], s_c[CF_VY][cc], s_c[CF_VZ][cc]);
@@ -192,7 +201,7 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
                 float Tzz = (ec.x + pressure) * u0 * u0 * ec.w * ec.w + pressure + q[9];
                 float Ttz = (ec.x + pressure) * u0 * u0 * ec.w + q[3];
                 t[0] = t[0] - Tzz; t[1] = t[1] - 0.0f; t[2] = t[2] - 0.0f; t[3] = t[3] - Ttz;
-                if (p.flags & 1) {
+                if (SPEC && (p.flags & 1)) {
                     float Ttt = (ec.x + pressure) * u0 * u0 - pressure;
                     float Ttx = (ec.x + pressure) * u0 * u0 * ec.y;
                     float Tty = (ec.x + pressure) * u0 * u0 * ec.z;
@@ -211,10 +220,10 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
                         int ii = i + w - 2;
                         ii = ii < 0 ? 0 : (ii > p.nx - 1 ? p.nx - 1 : ii);
                         const uint32_t b2 = (I + (uint32_t)(ii - i) * (uint32_t)p.sx) * 4u;
-                        float4 e = ld4o(ev_st, n, b2);
+                        float4 e = ld4p(ev_st, b2, nb);
                         float pv[10];
 #pragma unroll
-                        for (int m = 0; m < 10; m++) pv[m] = ldo(pi_st + (int64_t)m * n, b2);
+                        for (int m = 0; m < 10; m++) pv[m] = ldp(pi_st, b2, nb, m);
                         float bkv = BULK ? ldo(bk_st, b2) : 0.0f;
                         cell_prims(p, eos, tau, e, pv, bkv, xc[w]);
                     }
@@ -318,15 +327,15 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
 
         // ---- 5. update_pimn (kernel_IS.cl:501-709) and update_ev (kernel_visc.cl:322-393) ----
         if (active) {
-            const float4 e1 = STEP == 1 ? ec : ld4o(ev1, n, bo);
-            const float4 e2 = STEP == 2 ? ec : ld4o(ev2, n, bo);
+            const float4 e1 = STEP == 1 ? ec : ld4p(ev1, bo, nb);
+            const float4 e2 = STEP == 2 ? ec : ld4p(ev2, bo, nb);
             float g1 = STEP == 1 ? s_c[CF_U0][cc] : gamma3(e1.y, e1.z, e1.w);
             float u_old[4] = {g1 * 1.0f, g1 * e1.y, g1 * e1.z, g1 * e1.w};
             float u_new[4], udt[4];
             if (STEP == 1) {
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
-                    float ud_ = ldo(udiff_in + (int64_t)c * n, bo);
+                    float ud_ = ldp(udiff_in, bo, nb, c);
                     udt[c] = ud_ * p.r_dt;
                     u_new[c] = u_old[c] + ud_;
                 }
@@ -352,7 +361,7 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
             float etav = local_etaos * cpTs.w * CV_HBARC;
             float one_over_taupi = temperature * rcp(3.0f * rmax(CV_ACU, local_etaos) * CV_HBARC);
             float coef_pipi = 0.0f;
-            const bool gubser = p.flags & 2;
+            const bool gubser = SPEC && (p.flags & 2);
             if (gubser) {
                 float q4 = sqrtf(sqrtf(ed_step));
                 float ETAOS = p.etaos_ymin;
@@ -376,7 +385,7 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
                     int mn = IDX(mu, nu);
                     if (u[0] > 10000.0f) sigma = 0.0f;
                     float piNS = etav * sigma;
-                    float pi_old = (STEP == 1 ? q[mn] : ldo(pi1 + (int64_t)mn * n, bo)) * u_old[0];
+                    float pi_old = (STEP == 1 ? q[mn] : ldp(pi1, bo, nb, mn)) * u_old[0];
                     float src = (-q[mn] * theta) * p.r_3 - (q[mn] * u[0]) * st.r_tau;
 #pragma unroll
                     for (int l = 0; l < 4; l++)
@@ -417,14 +426,14 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
                 *(float *)((char *)good + bo) = gc;
             }
 #pragma unroll
-            for (int m = 0; m < 10; m++) *(float *)((char *)(pi_out + (int64_t)m * n) + bo) = out[m];
+            for (int m = 0; m < 10; m++) stp(pi_out, bo, nb, m, out[m]);
             if (BULK) *(float *)((char *)bk_out + bo) = bout;
             if (STEP == 1) {
 #pragma unroll
-                for (int m = 0; m < 10; m++) *(float *)((char *)(IS_src + (int64_t)m * n) + bo) = s[m];
+                for (int m = 0; m < 10; m++) stp(IS_src, bo, nb, m, s[m]);
                 if (BULK) *(float *)((char *)bsrc + bo) = bs;
 #pragma unroll
-                for (int m = 0; m < 4; m++) *(float *)((char *)(Src + (int64_t)m * n) + bo) = t[m];
+                for (int m = 0; m < 4; m++) stp(Src, bo, nb, m, t[m]);
             }
             // update_ev
             float pressure = eos.P(e1.x) + (BULK ? b_start : 0.0f);
@@ -432,7 +441,7 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 float g0 = c == 0 ? 1.0f : 0.0f;
-                float pim0 = STEP == 1 ? q[c] : ldo(pi1 + (int64_t)c * n, bo);
+                float pim0 = STEP == 1 ? q[c] : ldp(pi1, bo, nb, c);
                 T0m[c] = ((e1.x + pressure) * g1 * u_old[c] - pressure * g0 + pim0) * st.old_time;
             }
 #pragma unroll
@@ -448,7 +457,7 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
                 M = 0.999f * T00;
             }
             float ed_find;
-            if ((p.flags & 8) && M > p.fp64_vmin * T00)
+            if (SPEC && (p.flags & 8) && M > p.fp64_vmin * T00)
                 ed_find = (float)root_newton_d(E, tab, (double)T00, (double)M, (double)bout, BULK);
             else
                 ed_find = root_newton(eos, T00, M, bout, BULK);
@@ -458,15 +467,15 @@ __global__ __launch_bounds__(CV_TILE_THREADS) void k_visc_tiled(PhysP p, EosP E,
             if (BULK) pr = rmax(pr, 0.0f);
             float repv = rcp(rmax(CV_ACU, T00 + pr));
             float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
-            *(float *)((char *)ev_out + bo) = en.x;
-            *(float *)((char *)(ev_out + n) + bo) = en.y;
-            *(float *)((char *)(ev_out + 2 * n) + bo) = en.z;
-            *(float *)((char *)(ev_out + 3 * n) + bo) = en.w;
+            stp(ev_out, bo, nb, 0, en.x);
+            stp(ev_out, bo, nb, 1, en.y);
+            stp(ev_out, bo, nb, 2, en.z);
+            stp(ev_out, bo, nb, 3, en.w);
             if (STEP == 2) {
                 float gn = gamma3(en.y, en.z, en.w);
                 float un[4] = {gn * 1.0f, gn * en.y, gn * en.z, gn * en.w};
 #pragma unroll
-                for (int c = 0; c < 4; c++) *(float *)((char *)(udiff_out + (int64_t)c * n) + bo) = un[c] - u_old[c];
+                for (int c = 0; c < 4; c++) stp(udiff_out, bo, nb, c, un[c] - u_old[c]);
             }
         }
         __syncthreads();   // LDS reused by the next tile

@@ -70,6 +70,13 @@ static real rcp(real b) { return RC(1.0f) / b; }
 
 /* THETA and hbarc are float literals in the reference (helper.h:7-8). */
 #define THETA RC(1.1f)
+#if ORC_DOUBLE
+#define R_FMIN fmin
+#define R_COPYSIGN copysign
+#else
+#define R_FMIN fminf
+#define R_COPYSIGN copysignf
+#endif
 #define HBARC RC(0.19733f)
 
 /* ------------------------------------------------------------------------- */
@@ -234,17 +241,16 @@ static real minmod(real x, real y) {
 /* generalised minmod slope at b from (a, b, c) = cells (i-1, i, i+1), as written in
  * kt1d / kt1d_visc / kt1d_real (helper.h:213-217, kernel_visc.cl:80-84):
  *   minmod(0.5 (c-a), minmod(THETA (c-b), THETA (b-a))).
- * Evaluated in the equivalent closed form: when c-b and b-a have the same strict sign
- * (and then c-a has it too) the value is that sign times the smallest of
- * 0.5|c-a|, THETA|c-b|, THETA|b-a|, otherwise 0.  Same value bit for bit (every product
- * with 0.5, 2 or a sign is exact); only the sign of an exact zero may differ. */
+ * Evaluated in closed form: 0 when c-b and b-a carry different sign bits, otherwise
+ * copysign(min(0.5|c-a|, THETA|c-b|, THETA|b-a|), c-b) (a zero difference makes the
+ * min 0).  Same value bit for bit for finite inputs (every product with 0.5 or a sign
+ * is exact); only the sign of an exact zero may differ. */
 static real slope(real a, real b, real c) {
     real d2 = c - b, d3 = b - a;
-    if (!((d2 > RC(0.0f) && d3 > RC(0.0f)) || (d2 < RC(0.0f) && d3 < RC(0.0f)))) return RC(0.0f);
-    real m = RC(0.5f) * (real)fabs((double)(c - a));
-    real t2 = THETA * (real)fabs((double)d2), t3 = THETA * (real)fabs((double)d3);
-    m = r_min(m, r_min(t2, t3));
-    return d2 > RC(0.0f) ? m : -m;
+    real m = R_FMIN(RC(0.5f) * (real)fabs((double)(c - a)),
+                    R_FMIN(THETA * (real)fabs((double)d2), THETA * (real)fabs((double)d3)));
+    if (signbit(d2) != signbit(d3)) return RC(0.0f);
+    return R_COPYSIGN(m, d2);
 }
 
 /* T^{tau mu} = (e+P) u^tau u^mu - P g^{tau mu}  (helper.h:172-177) */
@@ -392,6 +398,10 @@ static real solve_ed(const orc_params *p, real T00, real M, real bulk, int clamp
 /* ------------------------------------------------------------------------- */
 /* Small exported helpers for unit tests                                      */
 /* ------------------------------------------------------------------------- */
+void orc_slope(const real *a, const real *b, const real *c, int64_t n, real *out) {
+    for (int64_t i = 0; i < n; i++) out[i] = slope(a[i], b[i], c[i]);
+}
+
 void orc_eos(const orc_params *p, const real *eps, int64_t n, real *out /* n x 4 */) {
     for (int64_t i = 0; i < n; i++) eos4(p, eps[i], out + 4 * i);
 }

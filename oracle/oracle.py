@@ -236,6 +236,16 @@ def max_prop_speed(params, ev, vk, precision='f32'):
     return out
 
 
+def slope(a, b, c, precision='f32'):
+    """The generalised-minmod slope of clvisc_oracle.c at b from (a, b, c)."""
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    a, b, c = (np.ascontiguousarray(x, dt).reshape(-1) for x in (a, b, c))
+    out = np.zeros(a.size, dt)
+    L.orc_slope(_ptr(a), _ptr(b), _ptr(c), C.c_int64(a.size), _ptr(out))
+    return out
+
+
 # -- freeze-out hypersurface ----------------------------------------------------------
 class OrcSfParams(C.Structure):
     _fields_ = [('nx', C.c_int32), ('ny', C.c_int32), ('nz', C.c_int32),

@@ -377,3 +377,32 @@ def test_bulk_dilute_edge_stays_subluminal(cross):
         v = np.sqrt((o.ev[1][:, 1:4] ** 2).sum(1)).max()
         assert v <= 0.9991, v
     assert o.ev[1][:, 0].max() < 10.0
+
+
+@pytest.mark.parametrize('prec', ['f32', 'f64'])
+def test_slope_closed_form_equals_nested_minmod(prec):
+    """The closed-form slope equals the reference's nested expression
+    minmod(0.5 (c-a), minmod(THETA (c-b), THETA (b-a))) (helper.h:94-99, 213-217) in
+    value on random, tied, zero and tiny inputs (only the sign of a zero may differ)."""
+    dt = np.float32 if prec == 'f32' else np.float64
+    rng = np.random.default_rng(7)
+    n = 200000
+    base = rng.standard_normal((3, n)).astype(dt)
+    base[:, : n // 4] = np.round(base[:, : n // 4])            # ties and exact zeros
+    base[:, n // 4: n // 2] *= dt(1e-38) if prec == 'f32' else 1e-300   # tiny / subnormal
+    a, b, c = base
+    th = dt(np.float32(1.1))        # THETA is the float literal 1.1f (helper.h:7)
+    half = dt(0.5)
+
+    def minmod(x, y):
+        res = np.minimum(np.abs(x), np.abs(y))
+        return res * (np.sign(x) + np.sign(y)) * half
+
+    ref = minmod(half * (c - a), minmod(th * (c - b), th * (b - a)))
+    got = O.slope(a, b, c, prec)
+    np.testing.assert_array_equal(got, ref)     # +0 == -0 for assert_array_equal
+    # a mistake of the kind the closed form invites -- dropping the 0.5 on |c-a| -- fails
+    wrong = np.where(np.sign(c - b) == np.sign(b - a),
+                     np.sign(c - b) * np.minimum(np.abs(c - a), np.minimum(th * np.abs(c - b), th * np.abs(b - a))),
+                     0)
+    assert not np.array_equal(wrong, ref)
