@@ -90,6 +90,8 @@ def lib():
             'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
             'cv_set_stage_kernel': [vp, C.c_int, C.c_int, C.c_int],
             'cv_get_stage_kernel': [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
+            'cv_set_march_chunk': [vp, C.c_int],
+            'cv_get_march_chunk': [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
             'cv_hypersf_reserve': [vp, i64],
             'cv_hypersf_snapshot': [vp, f32],
             'cv_hypersf_find': [vp, f32],
@@ -116,7 +118,8 @@ def exported_symbols():
             'cv_get_field', 'cv_copy_field', 'cv_gather', 'cv_visc_initialize', 'cv_ideal_stage', 'cv_is_stage',
             'cv_visc_stage', 'cv_update_udiff', 'cv_ideal_steps_async', 'cv_visc_steps_async',
             'cv_max_ed', 'cv_sync', 'cv_last_elapsed_ms', 'cv_profile_visc_step',
-            'cv_set_stage_kernel', 'cv_get_stage_kernel', 'cv_hypersf_reserve', 'cv_hypersf_snapshot',
+            'cv_set_stage_kernel', 'cv_get_stage_kernel', 'cv_set_march_chunk', 'cv_get_march_chunk',
+            'cv_hypersf_reserve', 'cv_hypersf_snapshot',
             'cv_hypersf_find', 'cv_hypersf_count', 'cv_hypersf_get', 'cv_hypersf_reset']
 
 
@@ -230,6 +233,15 @@ class Handle(object):
         k, ty, tz = C.c_int32(), C.c_int32(), C.c_int32()
         check(self.L.cv_get_stage_kernel(self.h, C.byref(k), C.byref(ty), C.byref(tz)))
         return k.value, ty.value, tz.value
+
+    def set_march_chunk(self, xc):
+        check(self.L.cv_set_march_chunk(self.h, int(xc)))
+
+    def march_chunk(self):
+        """(x planes per chunk, chunks) of the marching kernel."""
+        xc, nch = C.c_int32(), C.c_int32()
+        check(self.L.cv_get_march_chunk(self.h, C.byref(xc), C.byref(nch)))
+        return xc.value, nch.value
 
     def last_elapsed_ms(self):
         out = C.c_float()

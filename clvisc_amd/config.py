@@ -138,7 +138,8 @@ def read_config(path=None, argv=None):
         parser.add_argument('--' + flag, nargs='?', const=1, type=bool, default=False, help=help_)
     parser.add_argument('--use_float32', nargs='?', const=1, type=bool, default=True,
                         help='must be True: the device path is FP32')
-    parser.add_argument('--stage_kernel', type=str, default='tiled', help="'tiled' or 'percell'")
+    parser.add_argument('--stage_kernel', type=str, default='march', choices=['march', 'tiled', 'percell'],
+                        help="step kernel: 'march' (tiled, x-marching), 'tiled' or 'percell'")
     parser.add_argument('--tile_ty', type=int, default=0, help='tile rows (0 = automatic)')
     parser.add_argument('--tile_tz', type=int, default=0, help='tile columns (0 = automatic)')
     args, unknown = parser.parse_known_args(argv)

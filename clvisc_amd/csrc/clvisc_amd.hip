@@ -54,9 +54,10 @@ struct cv_handle {
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     bool timed = false;
     int cus = 0;
-    int kernel = 1;           // step path: 0 per-cell fused kernel, 1 tiled kernel
+    int kernel = 2;           // step path: 0 per-cell fused kernel, 1 tiled kernel, 2 tiled x-march
     TileP tile;
     int tile_grid = 0;
+    int march_xc = 0;         // x planes per march chunk (0: automatic)
     // freeze-out surface: fields at the previous surface time, appended elements
     float *sf_ev = nullptr, *sf_pi = nullptr, *sf_bk = nullptr;
     float sf_tau_old = 0.0f;
@@ -83,6 +84,17 @@ static int set_tile(cv_handle *h, int ty, int tz) {
     t.ntj = (p.ny + ty - 1) / ty;
     t.ntk = (p.nz + tz - 1) / tz;
     t.ntiles = (int64_t)p.nx * t.ntj * t.ntk;
+    // march: chunks of xc planes so that about 32 work items per CU exist (3 run at once;
+    // short chunks balance the tail, each chunk start re-evaluates one x face)
+    const int64_t cols = (int64_t)t.ntj * t.ntk;
+    const int64_t target = (int64_t)(h->cus > 0 ? h->cus : 256) * 32;
+    int64_t nch = (target + cols - 1) / cols;
+    if (nch < 1) nch = 1;
+    if (nch > p.nx) nch = p.nx;
+    t.xc = (int)((p.nx + nch - 1) / nch);
+    if (h->march_xc > 0) t.xc = h->march_xc < p.nx ? h->march_xc : p.nx;
+    t.nchunks = (p.nx + t.xc - 1) / t.xc;
+    t.nwork = cols * t.nchunks;
     h->tile = t;
     int64_t g = (int64_t)(h->cus > 0 ? h->cus : 256) * 2 * 4;
     h->tile_grid = (int)(t.ntiles < g ? t.ntiles : g);
@@ -406,13 +418,16 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
                         float *ev1, float *ev2, float *pi1, float *bk1, float *pi_out, float *bk_out,
                         float *ev_out, float *udiff_out) {
     StageP st = stage_params(h, step, tau);
-    if (h->kernel == 1) {
-        dim3 g(h->tile_grid), b(CV_TILE_THREADS);
+    if (h->kernel == 1 || h->kernel == 2) {
+        const bool march = h->kernel == 2;
+        dim3 g(march ? (unsigned)h->tile.nwork : (unsigned)h->tile_grid), b(CV_TILE_THREADS);
 #define CV_TILED_ARGS h->phys, h->eos, st, h->tile, ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src, \
                       h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out
         const bool spec = (h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST | CV_FLAG_FP64_ROOT)) != 0;
         const int ek = h->eos.kind;
-#define CV_TILED_LAUNCH(B, S, K, X) hipLaunchKernelGGL((k_visc_tiled<B, S, K, X>), g, b, 0, h->stream, CV_TILED_ARGS)
+#define CV_TILED_LAUNCH2(B, S, K, X, M) hipLaunchKernelGGL((k_visc_tiled<B, S, K, X, M>), g, b, 0, h->stream, CV_TILED_ARGS)
+#define CV_TILED_LAUNCH(B, S, K, X) \
+        do { if (march) CV_TILED_LAUNCH2(B, S, K, X, true); else CV_TILED_LAUNCH2(B, S, K, X, false); } while (0)
 #define CV_TILED_SEL(B, S)                                      \
         if (ek == 1) { if (spec) CV_TILED_LAUNCH(B, S, 1, true); else CV_TILED_LAUNCH(B, S, 1, false); } \
         else { if (spec) CV_TILED_LAUNCH(B, S, 0, true); else CV_TILED_LAUNCH(B, S, 0, false); }
@@ -423,6 +438,7 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
         }
 #undef CV_TILED_SEL
 #undef CV_TILED_LAUNCH
+#undef CV_TILED_LAUNCH2
 #undef CV_TILED_ARGS
         HIP_TRY(hipGetLastError());
         return CV_OK;
@@ -568,15 +584,29 @@ extern "C" int cv_profile_visc_step(cv_handle *h, int64_t loop0, float *ms, int 
 
 extern "C" int cv_set_stage_kernel(cv_handle *h, int kind, int ty, int tz) {
     if (!h) return fail(CV_ERR_ARG, "null handle");
-    if (kind != 0 && kind != 1) return fail(CV_ERR_ARG, "kernel kind must be 0 (per-cell) or 1 (tiled)");
-    if (kind == 1 && !h->phys.nb)
+    if (kind < 0 || kind > 2) return fail(CV_ERR_ARG, "kernel kind must be 0 (per-cell), 1 (tiled) or 2 (tiled x-march)");
+    if (kind >= 1 && !h->phys.nb)
         return fail(CV_ERR_UNSUPPORTED, "tiled kernel needs 40 * cells < 2^32 (%lld cells)", (long long)h->n);
     HIP_TRY(hipStreamSynchronize(h->stream));
-    if (kind == 1) {
+    if (kind >= 1) {
         int rc = (ty > 0 && tz > 0) ? set_tile(h, ty, tz) : default_tile(h);
         if (rc) return rc;
     }
     h->kernel = kind;
+    return CV_OK;
+}
+
+extern "C" int cv_set_march_chunk(cv_handle *h, int xc) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    if (xc < 0) return fail(CV_ERR_ARG, "march chunk must be >= 0 (0: automatic)");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->march_xc = xc;
+    return set_tile(h, h->tile.ty, h->tile.tz);
+}
+
+extern "C" int cv_get_march_chunk(cv_handle *h, int32_t *xc, int32_t *nchunks) {
+    if (!h || !xc || !nchunks) return fail(CV_ERR_ARG, "null argument");
+    *xc = h->tile.xc; *nchunks = h->tile.nchunks;
     return CV_OK;
 }
 

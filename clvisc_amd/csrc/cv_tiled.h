@@ -24,14 +24,13 @@ namespace cv {
 enum { CF_E = 0, CF_VX, CF_VY, CF_VZ, CF_U0, CF_P, CF_Q0, CF_Q1, CF_Q2, CF_Q3, CF_PI0, CF_BK = CF_PI0 + 10, CF_N };
 
 #define CV_TILE_THREADS 256
-#define CV_TILE_MAXC 432       // cells of tile + halo held in LDS (2 workgroups per CU)
+#define CV_TILE_MAXC 405       // cells of tile + halo held in LDS (3 marching workgroups per CU)
 #define CV_TILE_MAXF 288       // faces of one axis held in LDS
 #define CV_EOS_LDS 1600        // octave-table entries held in LDS
+// The octave table is copied to LDS by the plane-tiled kernel; the marching kernel reads it
+// through the caches and spends the LDS on a third workgroup per CU instead (3 x 51 KB).
 #ifndef CV_TILE_EOS_IN_LDS
-#define CV_TILE_EOS_IN_LDS 1   // 0: the octave table is read through the caches instead
-#endif
-#ifndef CV_TILE_MINBLOCKS
-#define CV_TILE_MINBLOCKS 1    // workgroups per CU the register allocation must allow
+#define CV_TILE_EOS_IN_LDS 1
 #endif
 
 struct TileP {
@@ -40,6 +39,8 @@ struct TileP {
     int HY, HZ;                // ty + 2 hy, tz + 2 hz
     int ntj, ntk;              // tiles along j, k
     int64_t ntiles;            // nx * ntj * ntk
+    int xc, nchunks;           // march: x planes per chunk, chunks along x
+    int64_t nwork;             // march: ntj * ntk * nchunks
 };
 
 // per-cell quantities of one cell from its raw fields
@@ -62,9 +63,14 @@ __device__ __forceinline__ void cell_prims(const PhysP &p, const EOS &eos, float
 
 // Face value between cells L and R along axis A, from the cells Lm = L-1, L, R, Rp = R+1.
 // G(field, cell) returns a per-cell quantity.  out: 4 KT components, 10 IS components, Pi.
-template <int A, bool BULK, class EOS, typename G>
+// HAVE_DAL: the 15 slopes of cell L are given in dal (Lm is not read); WANT_DAR: the 15
+// slopes of cell R are returned in dar.  A slope is a function of three consecutive
+// cells only, so a slope handed from one face to the next is the value the next face
+// would compute.
+template <int A, bool BULK, bool HAVE_DAL = false, bool WANT_DAR = false, class EOS, typename G>
 __device__ __forceinline__ void face_flux(const PhysP &p, const EOS &eos, float tau, const G &g, int Lm, int L,
-                                          int R, int Rp, float out[15]) {
+                                          int R, int Rp, float out[15], const float *dal = nullptr,
+                                          float *dar = nullptr) {
     const float4 eL = make_float4(g(CF_E, L), g(CF_VX, L), g(CF_VY, L), g(CF_VZ, L));
     const float4 eR = make_float4(g(CF_E, R), g(CF_VX, R), g(CF_VY, R), g(CF_VZ, R));
     const float vh = 0.5f * (comp(eL, A + 1) + comp(eR, A + 1));
@@ -73,8 +79,9 @@ __device__ __forceinline__ void face_flux(const PhysP &p, const EOS &eos, float 
     const float pr_half = 0.5f * (g(CF_P, L) + g(CF_P, R));
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        float DAL = slope(g(CF_Q0 + q, Lm), g(CF_Q0 + q, L), g(CF_Q0 + q, R));
+        float DAL = HAVE_DAL ? dal[q] : slope(g(CF_Q0 + q, Lm), g(CF_Q0 + q, L), g(CF_Q0 + q, R));
         float DAR = slope(g(CF_Q0 + q, L), g(CF_Q0 + q, R), g(CF_Q0 + q, Rp));
+        if (WANT_DAR) dar[q] = DAR;
         float AL = g(CF_Q0 + q, L) + 0.5f * DAL;
         float AR = g(CF_Q0 + q, R) - 0.5f * DAR;
         const float g0 = q == 0 ? 1.0f : 0.0f;
@@ -86,14 +93,15 @@ __device__ __forceinline__ void face_flux(const PhysP &p, const EOS &eos, float 
         out[q] = 0.5f * (Jp + Jm) - 0.5f * lam * (AR - AL);
     }
     // IS flux of u^tau pi^{mu nu} (helper.h:275-289)
-    const float u0Lm = g(CF_U0, Lm), u0L = g(CF_U0, L), u0R = g(CF_U0, R), u0Rp = g(CF_U0, Rp);
+    const float u0L = g(CF_U0, L), u0R = g(CF_U0, R), u0Rp = g(CF_U0, Rp);
 #pragma unroll
     for (int mn = 0; mn < 11; mn++) {
-        if (mn == 10 && !BULK) { out[14] = 0.0f; break; }
+        if (mn == 10 && !BULK) { out[14] = 0.0f; if (WANT_DAR) dar[14] = 0.0f; break; }
         const int f = mn < 10 ? CF_PI0 + mn : CF_BK;
-        float QLm = u0Lm * g(f, Lm), QL = u0L * g(f, L), QR = u0R * g(f, R), QRp = u0Rp * g(f, Rp);
-        float DAL = slope(QLm, QL, QR);
+        float QL = u0L * g(f, L), QR = u0R * g(f, R), QRp = u0Rp * g(f, Rp);
+        float DAL = HAVE_DAL ? dal[4 + mn] : slope(g(CF_U0, Lm) * g(f, Lm), QL, QR);
         float DAR = slope(QL, QR, QRp);
+        if (WANT_DAR) dar[4 + mn] = DAR;
         float AL = QL + 0.5f * DAL;
         float AR = QR - 0.5f * DAR;
         float Jp = AR * vh, Jm = AL * vh;
@@ -102,9 +110,15 @@ __device__ __forceinline__ void face_flux(const PhysP &p, const EOS &eos, float 
 }
 
 // EK: EOS table kind (0 row/col, 1 octave); SPEC: the GUBSER_VISC_TEST / RIEMANN_TEST /
-// FP64-root branches are compiled in (flags are still tested at run time)
-template <bool BULK, int STEP, int EK, bool SPEC>
-__global__ __launch_bounds__(CV_TILE_THREADS, CV_TILE_MINBLOCKS) void k_visc_tiled(PhysP p, EosP E, StageP st, TileP tp,
+// FP64-root branches are compiled in (flags are still tested at run time).
+// MARCH = false: a work item is one tile of one x-plane; every thread evaluates its two x
+// faces from its own x-stencil (cells i-2..i+2, read from global memory).
+// MARCH = true: a work item is one tile column over a chunk of xc consecutive x-planes; the
+// workgroup walks the chunk in increasing x and each thread carries the x face F(i-1/2),
+// the slopes of its cell i and u^mu of cell i-1 from one plane to the next, so every x face
+// and every x slope is evaluated once (the chunk's first plane starts from the full stencil).
+template <bool BULK, int STEP, int EK, bool SPEC, bool MARCH>
+__global__ __launch_bounds__(CV_TILE_THREADS, MARCH ? 3 : 2) void k_visc_tiled(PhysP p, EosP E, StageP st, TileP tp,
         const float *__restrict__ ev_st, const float *__restrict__ pi_st, const float *__restrict__ bk_st,
         const float *ev1, const float *ev2, const float *pi1, const float *bk1,
         const float *__restrict__ udiff_in, float *IS_src, float *bsrc, float *Src,
@@ -118,7 +132,7 @@ __global__ __launch_bounds__(CV_TILE_THREADS, CV_TILE_MINBLOCKS) void k_visc_til
 
     // EOS table into LDS (octave table small enough), else global
     const float4 *tab = E.tab;
-    if (CV_TILE_EOS_IN_LDS && EK == 1 && E.nlast + 1 <= CV_EOS_LDS) {
+    if (CV_TILE_EOS_IN_LDS && !MARCH && EK == 1 && E.nlast + 1 <= CV_EOS_LDS) {
         for (int q = tid; q <= E.nlast; q += CV_TILE_THREADS) s_eos[q] = E.tab[q];
         tab = s_eos;
     }
@@ -129,12 +143,38 @@ __global__ __launch_bounds__(CV_TILE_THREADS, CV_TILE_MINBLOCKS) void k_visc_til
     const int my = tid / tz, mz = tid - (tid / tz) * tz;       // own cell in the tile
     const bool lane_in_tile = tid < ty * tz;
 
-    for (int64_t tile = blockIdx.x; tile < tp.ntiles; tile += gridDim.x) {
-        const int kt = (int)(tile % tp.ntk);
-        const int64_t r = tile / tp.ntk;
+    for (int64_t work = blockIdx.x; work < (MARCH ? tp.nwork : tp.ntiles); work += gridDim.x) {
+        const int kt = (int)(work % tp.ntk);
+        const int64_t r = work / tp.ntk;
         const int jt = (int)(r % tp.ntj);
-        const int i = (int)(r / tp.ntj);
+        const int xr = (int)(r / tp.ntj);            // MARCH: chunk, else the x-plane
+        const int ibeg = MARCH ? xr * tp.xc : xr;
+        const int iend = MARCH ? (ibeg + tp.xc < p.nx ? ibeg + tp.xc : p.nx) : xr + 1;
         const int j0 = jt * ty, k0 = kt * tz;
+        // state carried along x by MARCH: F(i-1/2), slopes of cell i, u^mu of cell i-1
+        float xFm[15], xDAL[15], xum1[4];
+        if (MARCH && lane_in_tile && j0 + my < p.ny && k0 + mz < p.nz) {
+            const uint32_t I0 = (uint32_t)((int64_t)ibeg * p.sx) + (uint32_t)(j0 + my) * (uint32_t)p.sy +
+                                (uint32_t)(k0 + mz);
+            float xc[4][CF_N];                                      // cells ibeg-2 .. ibeg+1
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                int ii = ibeg + w - 2;
+                ii = ii < 0 ? 0 : (ii > p.nx - 1 ? p.nx - 1 : ii);
+                const uint32_t b2 = (I0 + (uint32_t)(ii - ibeg) * (uint32_t)p.sx) * 4u;
+                float4 e = ld4p(ev_st, b2, nb);
+                float pv[10];
+#pragma unroll
+                for (int m = 0; m < 10; m++) pv[m] = ldp(pi_st, b2, nb, m);
+                float bkv = BULK ? ldo(bk_st, b2) : 0.0f;
+                cell_prims(p, eos, tau, e, pv, bkv, xc[w]);
+            }
+            auto gx = [&](int f, int c) { return xc[c][f]; };
+            face_flux<0, BULK, false, true>(p, eos, tau, gx, 0, 1, 2, 3, xFm, nullptr, xDAL);
+#pragma unroll
+            for (int c = 0; c < 4; c++) xum1[c] = c == 0 ? xc[1][CF_U0] * 1.0f : xc[1][CF_U0] * xc[1][CF_E + c];
+        }
+      for (int i = ibeg; i < iend; i++) {
         const uint32_t plane = (uint32_t)((int64_t)i * p.sx);
 
         // ---- 1. per-cell quantities of tile + y/z halo into LDS ----
@@ -208,8 +248,40 @@ __global__ __launch_bounds__(CV_TILE_THREADS, CV_TILE_MINBLOCKS) void k_visc_til
                     t[0] = t[0] + (Tzz + Ttt); t[1] = t[1] + Ttx; t[2] = t[2] + Tty; t[3] = t[3] + 3.0f * Ttz;
                 }
             }
-            // ---- 2. x: own stencil i-2..i+2 from global memory ----
-            {
+            // ---- 2. x ----
+            if (MARCH) {
+                // cells i+1 and i+2 of the own column; cell i is in LDS
+                float xr1[CF_N], xr2[CF_N];
+#pragma unroll
+                for (int w = 1; w <= 2; w++) {
+                    int ii = i + w;
+                    ii = ii > p.nx - 1 ? p.nx - 1 : ii;
+                    const uint32_t b2 = (I + (uint32_t)(ii - i) * (uint32_t)p.sx) * 4u;
+                    float4 e = ld4p(ev_st, b2, nb);
+                    float pv[10];
+#pragma unroll
+                    for (int m = 0; m < 10; m++) pv[m] = ldp(pi_st, b2, nb, m);
+                    float bkv = BULK ? ldo(bk_st, b2) : 0.0f;
+                    cell_prims(p, eos, tau, e, pv, bkv, w == 1 ? xr1 : xr2);
+                }
+                auto gx = [&](int f, int c) { return c == 1 ? s_c[f][cc] : (c == 2 ? xr1[f] : xr2[f]); };
+                float Fp[15], dar[15];
+                face_flux<0, BULK, true, true>(p, eos, tau, gx, 0, 1, 2, 3, Fp, xDAL, dar);
+#pragma unroll
+                for (int m = 0; m < 4; m++) t[m] = t[m] - (Fp[m] - xFm[m]) * p.r_dx;
+#pragma unroll
+                for (int m = 0; m < 10; m++) s[m] = s[m] - (Fp[4 + m] - xFm[4 + m]) * p.r_dx;
+                if (BULK) bs = bs - (Fp[14] - xFm[14]) * p.r_dx;
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    float ui = c == 0 ? s_c[CF_U0][cc] * 1.0f : s_c[CF_U0][cc] * s_c[CF_E + c][cc];
+                    float up1 = c == 0 ? xr1[CF_U0] * 1.0f : xr1[CF_U0] * xr1[CF_E + c];
+                    udx[c] = slope(xum1[c], ui, up1) * p.r_dx;
+                    xum1[c] = ui;
+                }
+#pragma unroll
+                for (int m = 0; m < 15; m++) { xFm[m] = Fp[m]; xDAL[m] = dar[m]; }
+            } else {
                 float xc[5][CF_N];
 #pragma unroll
                 for (int w = 0; w < 5; w++) {
@@ -478,7 +550,8 @@ __global__ __launch_bounds__(CV_TILE_THREADS, CV_TILE_MINBLOCKS) void k_visc_til
                 for (int c = 0; c < 4; c++) stp(udiff_out, bo, nb, c, un[c] - u_old[c]);
             }
         }
-        __syncthreads();   // LDS reused by the next tile
+        __syncthreads();   // LDS reused by the next plane / tile
+      }
     }
 }
 

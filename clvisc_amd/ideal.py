@@ -96,9 +96,8 @@ class CLIdeal(object):
         self.bulkinfo = BulkInfo(self.cfg, self.eos)
         params = _params_from_cfg(self.cfg, self.efrz, _viscous)
         self.handle = _lib.Handle(params, self.eos_table, device=gpu_id, viscous=_viscous)
-        kind = getattr(self.cfg, 'stage_kernel', 'tiled')
-        self.handle.set_stage_kernel(0 if kind == 'percell' else 1, int(getattr(self.cfg, 'tile_ty', 0)),
-                                     int(getattr(self.cfg, 'tile_tz', 0)))
+        kind = {'percell': 0, 'tiled': 1, 'march': 2}[getattr(self.cfg, 'stage_kernel', 'march')]
+        self.handle.set_stage_kernel(kind, int(getattr(self.cfg, 'tile_ty', 0)), int(getattr(self.cfg, 'tile_tz', 0)))
         self.d_ev = [DeviceField(self.handle, 'ev%d' % k) for k in range(3)]
         self.d_Src = DeviceField(self.handle, 'src')
         self.h_ev1 = np.zeros((self.size, 4), self.cfg.real)

@@ -147,11 +147,17 @@ int cv_max_ed(cv_handle *h, int which, float *out);
 int cv_profile_visc_step(cv_handle *h, int64_t loop0, float *ms, int nms, int32_t *nlaunch);
 
 /* Kernel of the step path (cv_visc_steps_async): kind 0 = one thread per cell, every
- * stencil quantity re-evaluated per cell; kind 1 (default) = tiled kernel, one workgroup
- * per ty x tz tile of an x-plane, per-cell quantities and y/z faces evaluated once in LDS
- * (ty, tz <= 0 picks the default tile for the grid).  Both give the same FP32 results. */
+ * stencil quantity re-evaluated per cell; kind 1 = tiled kernel, one workgroup per
+ * ty x tz tile of an x-plane, per-cell quantities and y/z faces evaluated once in LDS;
+ * kind 2 (default) = the tiled kernel marching along x over chunks of planes, every x face
+ * and slope evaluated once (ty, tz <= 0 picks the default tile for the grid).  All give
+ * the same FP32 results.  No reference counterpart (the reference's launch geometry is
+ * BSZ, pyvisc/hydro.info). */
 int cv_set_stage_kernel(cv_handle *h, int kind, int ty, int tz);
 int cv_get_stage_kernel(cv_handle *h, int32_t *kind, int32_t *ty, int32_t *tz);
+/* x planes per march chunk of kind 2 (0: automatic, about 32 work items per CU). */
+int cv_set_march_chunk(cv_handle *h, int xc);
+int cv_get_march_chunk(cv_handle *h, int32_t *xc, int32_t *nchunks);
 
 /* Wait for all queued work. */
 int cv_sync(cv_handle *h);

@@ -80,12 +80,14 @@ def _visc_case(nx, ny, nz, bulk=None, nhot=12, seed=3, eos_kind='crossover'):
     return p, tab, ev
 
 
-def _visc_compare(p, tab, ev, nsteps, label, kernel=None):
-    _say('case: %s %dx%dx%d, %d steps, kernel %s' % (label, p.nx, p.ny, p.nz, nsteps, kernel))
+def _visc_compare(p, tab, ev, nsteps, label, kernel=None, chunk=None):
+    _say('case: %s %dx%dx%d, %d steps, kernel %s chunk %s' % (label, p.nx, p.ny, p.nz, nsteps, kernel, chunk))
     h = _lib.Handle(p, tab, viscous=True)
     if kernel is not None:
         h.set_stage_kernel(*kernel)
-    _say('  stage kernel %s' % (h.stage_kernel(),))
+    if chunk is not None:
+        h.set_march_chunk(chunk)
+    _say('  stage kernel %s march chunk %s' % (h.stage_kernel(), h.march_chunk()))
     h.visc_initialize(p.tau0)
     h.ideal_stage(1, np.float32(p.tau0))
     h.set('ev1', ev)
@@ -111,16 +113,20 @@ def _visc_compare(p, tab, ev, nsteps, label, kernel=None):
         assert v <= TOL, (k, v)
 
 
-@pytest.mark.parametrize('kernel', [(0, 0, 0), (1, 0, 0), (1, 8, 32), (1, 11, 23), (1, 4, 16), (1, 16, 16)])
-def test_visc_steps_small_3d(kernel):
+@pytest.mark.parametrize('kernel,chunk', [((0, 0, 0), None), ((1, 0, 0), None), ((1, 8, 27), None),
+                                          ((1, 11, 23), None), ((1, 4, 16), None), ((1, 16, 16), None),
+                                          ((2, 0, 0), None), ((2, 7, 32), 1), ((2, 11, 23), 3),
+                                          ((2, 4, 16), 29), ((2, 16, 16), 7)])
+def test_visc_steps_small_3d(kernel, chunk):
     p, tab, ev = _visc_case(29, 31, 11)
-    _visc_compare(p, tab, ev, 20, 'viscous hotspots', kernel)
+    _visc_compare(p, tab, ev, 20, 'viscous hotspots', kernel, chunk)
 
 
-@pytest.mark.parametrize('kernel', [(0, 0, 0), (1, 0, 0), (1, 64, 1)])
-def test_visc_steps_2d_boost_invariant(kernel):
+@pytest.mark.parametrize('kernel,chunk', [((0, 0, 0), None), ((1, 0, 0), None), ((1, 64, 1), None),
+                                          ((2, 0, 0), None), ((2, 64, 1), 4)])
+def test_visc_steps_2d_boost_invariant(kernel, chunk):
     p, tab, ev = _visc_case(45, 41, 1, bulk=(0.05, 0.155, 0.02))
-    _visc_compare(p, tab, ev, 15, 'viscous+bulk 2d', kernel)
+    _visc_compare(p, tab, ev, 15, 'viscous+bulk 2d', kernel, chunk)
 
 
 def test_visc_steps_bulk_3d():
