@@ -98,6 +98,15 @@ def lib():
             'cv_hypersf_count': [vp, C.POINTER(i64), C.POINTER(i64)],
             'cv_hypersf_get': [vp, vp, vp, vp, vp, i64],
             'cv_hypersf_reset': [vp],
+            'cv_batch_create': [C.POINTER(CvParams), C.POINTER(CvEos), C.c_int, vp, C.c_int, C.POINTER(vp)],
+            'cv_batch_destroy': [vp],
+            'cv_batch_handle': [vp, C.c_int, C.POINTER(vp)],
+            'cv_batch_set_active': [vp, vp],
+            'cv_batch_steps_async': [vp, i64],
+            'cv_batch_set_loop': [vp, i64],
+            'cv_batch_sync': [vp],
+            'cv_batch_last_elapsed_ms': [vp, C.POINTER(f32)],
+            'cv_batch_info': [vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
@@ -120,7 +129,10 @@ def exported_symbols():
             'cv_max_ed', 'cv_sync', 'cv_last_elapsed_ms', 'cv_profile_visc_step',
             'cv_set_stage_kernel', 'cv_get_stage_kernel', 'cv_set_march_chunk', 'cv_get_march_chunk',
             'cv_hypersf_reserve', 'cv_hypersf_snapshot',
-            'cv_hypersf_find', 'cv_hypersf_count', 'cv_hypersf_get', 'cv_hypersf_reset']
+            'cv_hypersf_find', 'cv_hypersf_count', 'cv_hypersf_get', 'cv_hypersf_reset',
+            'cv_batch_create', 'cv_batch_destroy', 'cv_batch_handle', 'cv_batch_set_active',
+            'cv_batch_steps_async', 'cv_batch_set_loop', 'cv_batch_sync', 'cv_batch_last_elapsed_ms',
+            'cv_batch_info']
 
 
 def make_eos(table):
@@ -140,20 +152,24 @@ def make_eos(table):
 class Handle(object):
     """Owner of one device-side hydro state (cv_handle)."""
 
-    def __init__(self, params, eos_table, device=0, viscous=True):
+    def __init__(self, params, eos_table, device=0, viscous=True, _borrow=None):
         self.L = lib()
         self.params = params
-        self.eos = make_eos(eos_table)
         self.n = params.nx * params.ny * params.nz
         self.viscous = viscous
+        if _borrow is not None:          # a handle owned by a Batch
+            self.h, self._owner = _borrow
+            return
+        self._owner = None
+        self.eos = make_eos(eos_table)
         h = C.c_void_p()
         check(self.L.cv_create(C.byref(params), C.byref(self.eos), device, int(viscous), C.byref(h)))
         self.h = h
 
     def close(self):
-        if getattr(self, 'h', None):
+        if getattr(self, 'h', None) and getattr(self, '_owner', None) is None:
             self.L.cv_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -277,6 +293,73 @@ class Handle(object):
 
     def hypersf_reset(self):
         check(self.L.cv_hypersf_reset(self.h))
+
+
+class Batch(object):
+    """Owner of an event batch (cv_batch): events with their own grids stepped together."""
+
+    def __init__(self, params, eos_table, dims, device=0):
+        self.L = lib()
+        self.params = params
+        self.eos = make_eos(eos_table)
+        self.dims = [tuple(int(x) for x in d) for d in dims]
+        d = np.ascontiguousarray(np.array(self.dims, dtype=np.int32).reshape(-1))
+        b = C.c_void_p()
+        check(self.L.cv_batch_create(C.byref(params), C.byref(self.eos), len(self.dims), d.ctypes.data, device,
+                                     C.byref(b)))
+        self.b = b
+        self._handles = {}
+
+    def handle(self, e):
+        """Handle of event e (borrowed: valid while the batch lives)."""
+        h = C.c_void_p()
+        check(self.L.cv_batch_handle(self.b, int(e), C.byref(h)))
+        if e not in self._handles:
+            p = CvParams.from_buffer_copy(self.params)
+            p.nx, p.ny, p.nz = self.dims[e]
+            self._handles[e] = Handle(p, None, viscous=True, _borrow=(h, self))
+        return self._handles[e]
+
+    def set_active(self, mask):
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.int32).reshape(-1))
+        if m.size != len(self.dims):
+            raise ValueError('mask needs %d entries' % len(self.dims))
+        check(self.L.cv_batch_set_active(self.b, m.ctypes.data))
+
+    def steps(self, nsteps, sync=True):
+        check(self.L.cv_batch_steps_async(self.b, int(nsteps)))
+        if sync:
+            self.sync()
+
+    def set_loop(self, loop):
+        check(self.L.cv_batch_set_loop(self.b, int(loop)))
+
+    def sync(self):
+        check(self.L.cv_batch_sync(self.b))
+
+    def last_elapsed_ms(self):
+        out = C.c_float()
+        check(self.L.cv_batch_last_elapsed_ms(self.b, C.byref(out)))
+        return out.value
+
+    def info(self):
+        """(events, loop index of the next step, work items per stage)."""
+        n, loop, w = C.c_int32(), C.c_int64(), C.c_int64()
+        check(self.L.cv_batch_info(self.b, C.byref(n), C.byref(loop), C.byref(w)))
+        return n.value, loop.value, w.value
+
+    def close(self):
+        if getattr(self, 'b', None):
+            for h in self._handles.values():
+                h.h = None
+            self.L.cv_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def device_info(device=0):

@@ -8,6 +8,7 @@
 #include <string.h>
 #include <string>
 #include <vector>
+#include <algorithm>
 
 #include "../../include/clvisc_amd.h"
 #include "cv_kernels.h"
@@ -421,8 +422,9 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
     if (h->kernel == 1 || h->kernel == 2) {
         const bool march = h->kernel == 2;
         dim3 g(march ? (unsigned)h->tile.nwork : (unsigned)h->tile_grid), b(CV_TILE_THREADS);
-#define CV_TILED_ARGS h->phys, h->eos, st, h->tile, ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src, \
-                      h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out
+        StagePtrs sp = {ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src,
+                        h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out};
+#define CV_TILED_ARGS h->phys, h->eos, st, h->tile, sp
         const bool spec = (h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST | CV_FLAG_FP64_ROOT)) != 0;
         const int ek = h->eos.kind;
 #define CV_TILED_LAUNCH2(B, S, K, X, M) hipLaunchKernelGGL((k_visc_tiled<B, S, K, X, M>), g, b, 0, h->stream, CV_TILED_ARGS)
@@ -769,6 +771,255 @@ extern "C" int cv_hypersf_get(cv_handle *h, float *sf, float *txyz, float *pisf,
     if (bksf) {
         if (!h->sf_bksf) return fail(CV_ERR_STATE, "cv_hypersf_get: bulk on the surface needs CV_FLAG_BULK");
         HIP_TRY(hipMemcpy(bksf, h->sf_bksf, sizeof(float) * c, hipMemcpyDeviceToHost));
+    }
+    return CV_OK;
+}
+
+// ---- event batches -----------------------------------------------------------
+#include "cv_batch.h"
+
+struct cv_batch {
+    int device = 0;
+    int bulk = 0;
+    int ek = 1;
+    std::vector<cv_handle *> ev;           // one single-event state per event
+    std::vector<int32_t> active;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    bool timed = false;
+    BatchState *d_state = nullptr;
+    BatchEvent *d_events = nullptr;
+    unsigned long long *d_ctr = nullptr;
+    int64_t loop = 0;                      // loop index of the next step
+    bool dirty = true;                     // event table / graph must be rebuilt
+    int nact = 0;
+    int64_t nwork = 0;
+    int grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void batch_drop_graph(cv_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+extern "C" int cv_batch_create(const cv_params *base, const cv_eos *eos, int nevents, const int32_t *dims,
+                               int device, cv_batch **out) {
+    if (!base || !eos || !dims || !out || nevents < 1) return fail(CV_ERR_ARG, "cv_batch_create: bad argument");
+    cv_batch *b = new cv_batch();
+    b->device = device;
+    b->bulk = (base->flags & CV_FLAG_BULK) ? 1 : 0;
+    if (base->flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST | CV_FLAG_FP64_ROOT)) {
+        delete b;
+        return fail(CV_ERR_UNSUPPORTED, "cv_batch_create: test flags and the FP64 root are single-event only");
+    }
+    for (int e = 0; e < nevents; e++) {
+        cv_params p = *base;
+        p.nx = dims[3 * e]; p.ny = dims[3 * e + 1]; p.nz = dims[3 * e + 2];
+        cv_handle *h = nullptr;
+        int rc = cv_create(&p, eos, device, 1, &h);
+        if (rc == CV_OK && h->kernel != 2) rc = fail(CV_ERR_UNSUPPORTED, "event %d: grid too large for the batch", e);
+        if (rc != CV_OK) {
+            if (h) cv_destroy(h);
+            for (cv_handle *g : b->ev) cv_destroy(g);
+            delete b;
+            return rc;
+        }
+        b->ev.push_back(h);
+    }
+    b->ek = b->ev[0]->eos.kind;
+    b->active.assign(nevents, 1);
+    HIP_TRY(hipSetDevice(device));
+    bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreate(&b->ev_a) == hipSuccess && hipEventCreate(&b->ev_b) == hipSuccess &&
+              hipMalloc((void **)&b->d_state, sizeof(BatchState)) == hipSuccess &&
+              hipMalloc((void **)&b->d_events, sizeof(BatchEvent) * nevents) == hipSuccess &&
+              hipMalloc((void **)&b->d_ctr, sizeof(unsigned long long)) == hipSuccess;
+    if (!ok) {
+        int rc = fail(CV_ERR_HIP, "cv_batch_create: allocation failed");
+        cv_batch_destroy(b);
+        return rc;
+    }
+    int cus = b->ev[0]->cus > 0 ? b->ev[0]->cus : 256;
+    b->grid = cus * 3;
+    *out = b;
+    return CV_OK;
+}
+
+extern "C" int cv_batch_destroy(cv_batch *b) {
+    if (!b) return CV_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    for (cv_handle *h : b->ev) cv_destroy(h);
+    (void)hipFree(b->d_state); (void)hipFree(b->d_events); (void)hipFree(b->d_ctr);
+    if (b->ev_a) (void)hipEventDestroy(b->ev_a);
+    if (b->ev_b) (void)hipEventDestroy(b->ev_b);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return CV_OK;
+}
+
+static int batch_event(cv_batch *b, int e, cv_handle **h) {
+    if (!b) return fail(CV_ERR_ARG, "null batch");
+    if (e < 0 || e >= (int)b->ev.size()) return fail(CV_ERR_ARG, "event %d out of [0, %d)", e, (int)b->ev.size());
+    *h = b->ev[e];
+    return CV_OK;
+}
+
+// the device descriptor must match the host-side slot pointers: bring the handles up to
+// date (rotation of the steps run by the graph) before any per-event access
+static int batch_settle(cv_batch *b) {
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return CV_OK;
+}
+
+extern "C" int cv_batch_handle(cv_batch *b, int event, cv_handle **out) {
+    cv_handle *h;
+    int rc = batch_event(b, event, &h);
+    if (rc) return rc;
+    rc = batch_settle(b);
+    if (rc) return rc;
+    if (!out) return fail(CV_ERR_ARG, "null argument");
+    *out = h;
+    b->dirty = true;               // the caller may change the tile: rebuild
+    return CV_OK;
+}
+
+extern "C" int cv_batch_set_active(cv_batch *b, const int32_t *active) {
+    if (!b || !active) return fail(CV_ERR_ARG, "null argument");
+    int rc = batch_settle(b);
+    if (rc) return rc;
+    for (size_t e = 0; e < b->ev.size(); e++) b->active[e] = active[e] ? 1 : 0;
+    b->dirty = true;
+    return CV_OK;
+}
+
+// event table of the active events (largest work first) and the graph of one step
+static int batch_build(cv_batch *b) {
+    std::vector<int> order;
+    for (size_t e = 0; e < b->ev.size(); e++)
+        if (b->active[e]) order.push_back((int)e);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+        return b->ev[a]->tile.nwork * b->ev[a]->tile.xc > b->ev[c]->tile.nwork * b->ev[c]->tile.xc;
+    });
+    std::vector<BatchEvent> tab(order.size());
+    int64_t w = 0;
+    for (size_t q = 0; q < order.size(); q++) {
+        cv_handle *h = b->ev[order[q]];
+        BatchEvent &t = tab[q];
+        t.p = h->phys;
+        t.tp = h->tile;
+        t.work0 = w;
+        for (int k = 0; k < 3; k++) { t.ev[k] = h->ev[k]; t.pi[k] = h->pi[k]; t.bk[k] = h->bk[k]; }
+        t.Src = h->Src; t.IS_src = h->IS_src; t.bsrc = h->bsrc; t.udiff = h->udiff; t.good = h->good;
+        w += h->tile.nwork;
+    }
+    if (w >= ((int64_t)1 << 31)) return fail(CV_ERR_CAPACITY, "batch: %lld work items exceed 2^31", (long long)w);
+    BatchState st;
+    st.tau0 = b->ev[0]->prm.tau0;
+    st.dt = b->ev[0]->prm.dt;
+    st.loop0 = b->loop;
+    st.step = 0;
+    st.nwork = w;
+    st.nevents = (int)order.size();
+    HIP_TRY(hipSetDevice(b->device));
+    if (!tab.empty())
+        HIP_TRY(hipMemcpyAsync(b->d_events, tab.data(), sizeof(BatchEvent) * tab.size(), hipMemcpyHostToDevice,
+                               b->stream));
+    HIP_TRY(hipMemcpyAsync(b->d_state, &st, sizeof(st), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->nact = (int)order.size();
+    b->nwork = w;
+    b->dirty = false;
+    // one RK2 step: counter reset, stage 1, counter reset, stage 2, step + 1
+    batch_drop_graph(b);
+    if (b->nact == 0) return CV_OK;
+    HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    dim3 g(b->grid), blk(CV_TILE_THREADS);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_ctr);
+#define CV_BATCH_LAUNCH(B, S)                                                                                  \
+    do {                                                                                                       \
+        if (b->ek == 1) hipLaunchKernelGGL((k_batch_stage<B, S, 1>), g, blk, 0, b->stream, b->d_state,         \
+                                           b->d_events, b->ev[0]->eos, b->d_ctr);                              \
+        else hipLaunchKernelGGL((k_batch_stage<B, S, 0>), g, blk, 0, b->stream, b->d_state, b->d_events,       \
+                                b->ev[0]->eos, b->d_ctr);                                                      \
+    } while (0)
+    if (b->bulk) CV_BATCH_LAUNCH(true, 1); else CV_BATCH_LAUNCH(false, 1);
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_ctr);
+    if (b->bulk) CV_BATCH_LAUNCH(true, 2); else CV_BATCH_LAUNCH(false, 2);
+#undef CV_BATCH_LAUNCH
+    hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, b->stream, b->d_state);
+    hipError_t e1 = hipStreamEndCapture(b->stream, &b->graph);
+    if (e1 != hipSuccess) return fail(CV_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e1));
+    HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return CV_OK;
+}
+
+extern "C" int cv_batch_steps_async(cv_batch *b, int64_t nsteps) {
+    if (!b) return fail(CV_ERR_ARG, "null batch");
+    if (nsteps < 0) return fail(CV_ERR_ARG, "negative step count");
+    HIP_TRY(hipSetDevice(b->device));
+    for (cv_handle *h : b->ev) HIP_TRY(hipStreamSynchronize(h->stream));
+    if (b->dirty) {
+        int rc = batch_build(b);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipEventRecord(b->ev_a, b->stream));
+    if (b->nact > 0)
+        for (int64_t s = 0; s < nsteps; s++) HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    HIP_TRY(hipEventRecord(b->ev_b, b->stream));
+    b->timed = true;
+    // host view of the rotation: odd step counts swap slots 0 and 1 of active events
+    if (nsteps & 1) {
+        for (size_t e = 0; e < b->ev.size(); e++) {
+            if (!b->active[e]) continue;
+            cv_handle *h = b->ev[e];
+            std::swap(h->ev[0], h->ev[1]); std::swap(h->pi[0], h->pi[1]); std::swap(h->bk[0], h->bk[1]);
+        }
+    }
+    // the device step counter advanced with the host loop, and its buffer parity with the
+    // host slot swaps above: the same graph serves the next call
+    b->loop += nsteps;
+    return CV_OK;
+}
+
+extern "C" int cv_batch_set_loop(cv_batch *b, int64_t loop) {
+    if (!b) return fail(CV_ERR_ARG, "null batch");
+    if (loop < 0) return fail(CV_ERR_ARG, "negative loop index");
+    int rc = batch_settle(b);
+    if (rc) return rc;
+    b->loop = loop;
+    b->dirty = true;
+    return CV_OK;
+}
+
+extern "C" int cv_batch_sync(cv_batch *b) {
+    if (!b) return fail(CV_ERR_ARG, "null batch");
+    return batch_settle(b);
+}
+
+extern "C" int cv_batch_last_elapsed_ms(cv_batch *b, float *ms) {
+    if (!b || !ms) return fail(CV_ERR_ARG, "null argument");
+    if (!b->timed) return fail(CV_ERR_STATE, "no timed call yet");
+    HIP_TRY(hipEventSynchronize(b->ev_b));
+    HIP_TRY(hipEventElapsedTime(ms, b->ev_a, b->ev_b));
+    return CV_OK;
+}
+
+extern "C" int cv_batch_info(cv_batch *b, int32_t *nevents, int64_t *loop, int64_t *nwork) {
+    if (!b) return fail(CV_ERR_ARG, "null batch");
+    if (nevents) *nevents = (int32_t)b->ev.size();
+    if (loop) *loop = b->loop;
+    if (nwork) {
+        int64_t w = 0;
+        for (size_t e = 0; e < b->ev.size(); e++)
+            if (b->active[e]) w += b->ev[e]->tile.nwork;
+        *nwork = w;
     }
     return CV_OK;
 }

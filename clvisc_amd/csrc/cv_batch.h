@@ -1,0 +1,90 @@
+// cv_batch.h -- one persistent launch per RK stage for an ensemble of events with their
+// own grids (event-by-event hydro, the reference's pyvisc/ebe.py running one CLVisc per
+// event).  Every event keeps its own fields, tile and march chunk; the work items of all
+// events (tile column x x-chunk) form one list that the workgroups of a persistent grid
+// take from an atomic counter, largest events first.  The step index lives in device
+// memory so that a captured graph of one step can be replayed: tau, the stage constants
+// and the d_ev[0] / d_ev[1] buffer rotation follow from it, with the same FP32 operations
+// as the single-event path.
+#pragma once
+#include "cv_tiled.h"
+
+namespace cv {
+
+struct BatchEvent {
+    PhysP p;
+    TileP tp;
+    int64_t work0;                     // first work item of this event in the list
+    float *ev[3], *pi[3], *bk[3];      // physical buffers of slots 0..2 at step parity 0
+    float *Src, *IS_src, *bsrc, *udiff, *good;
+};
+
+struct BatchState {
+    double tau0, dt;                   // TAU0, DT as the configuration holds them
+    int64_t loop0;                     // loop index of step 0 of this descriptor
+    int64_t step;                      // steps taken since the descriptor was built
+    int64_t nwork;                     // work items of all active events
+    int nevents;                       // active events (entries of the event table)
+};
+
+__global__ void k_batch_reset(unsigned long long *ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *ctr = 0ull;
+}
+
+__global__ void k_batch_advance(BatchState *bs) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) bs->step = bs->step + 1;
+}
+
+template <bool BULK, int STEP, int EK>
+__global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchState *__restrict__ bs,
+                                                                     const BatchEvent *__restrict__ evt, EosP E,
+                                                                     unsigned long long *ctr) {
+    __shared__ float s_c[CF_N][CV_TILE_MAXC];
+    __shared__ float s_f[15][CV_TILE_MAXF];
+    __shared__ int s_work;
+    EosT<EK> eos(E, E.tab);
+    const int64_t n = bs->step;
+    const int par = (int)(n & 1);
+    // tau of this stage as cv_visc_steps_async computes it on the host (tau_at)
+    const float tau = (float)(bs->tau0 + (double)(bs->loop0 + n + (STEP - 1)) * bs->dt);
+    const int64_t nwork = bs->nwork;
+    const int nev = bs->nevents;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            unsigned long long t = atomicAdd(ctr, 1ull);
+            s_work = t < (unsigned long long)nwork ? (int)t : (int)nwork;
+        }
+        __syncthreads();
+        const int w = __builtin_amdgcn_readfirstlane(s_work);     // uniform: event table via scalar loads
+        __syncthreads();
+        if (w >= nwork) break;
+        int lo = 0, hi = nev - 1;                       // last event with work0 <= w
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (evt[mid].work0 <= w) lo = mid; else hi = mid - 1;
+        }
+        lo = __builtin_amdgcn_readfirstlane(lo);
+        const BatchEvent &ev = evt[lo];
+        StageP st;
+        st.step = STEP;
+        st.tau = tau;
+        st.r_tau = 1.0f / tau;
+        st.r_taudz = 1.0f / (tau * ev.p.dz);
+        st.old_time = tau - (float)(STEP - 1) * ev.p.dt;
+        st.r_step = 1.0f / (float)STEP;
+        float *E0 = ev.ev[par], *E1 = ev.ev[1 - par], *E2 = ev.ev[2];
+        float *P0 = ev.pi[par], *P1 = ev.pi[1 - par], *P2 = ev.pi[2];
+        float *B0 = ev.bk[par], *B1 = ev.bk[1 - par], *B2 = ev.bk[2];
+        StagePtrs q;
+        if (STEP == 1) {
+            q = StagePtrs{E1, P1, B1, E1, E2, P1, B1, ev.udiff, ev.IS_src, ev.bsrc, ev.Src, P2, B2, E2, ev.good,
+                          nullptr};
+        } else {
+            q = StagePtrs{E2, P2, B2, E1, E2, P1, B1, ev.udiff, ev.IS_src, ev.bsrc, ev.Src, P0, B0, E0, ev.good,
+                          ev.udiff};
+        }
+        tile_work<BULK, STEP, EK, false, true>(ev.p, eos, st, ev.tp, q, w - ev.work0, s_c, s_f);
+    }
+}
+
+}  // namespace cv

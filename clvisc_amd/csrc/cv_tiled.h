@@ -109,6 +109,15 @@ __device__ __forceinline__ void face_flux(const PhysP &p, const EOS &eos, float 
     }
 }
 
+// Field pointers of one RK stage (the buffers of CLVisc.IS_stepUpdate / visc_stepUpdate).
+struct StagePtrs {
+    const float *ev_st, *pi_st, *bk_st;        // d_ev[step], d_pi[step], Pi[step]: the stencil
+    const float *ev1, *ev2, *pi1, *bk1;        // d_ev[1], d_ev[2], d_pi[1], Pi[1]
+    const float *udiff_in;
+    float *IS_src, *bsrc, *Src, *pi_out, *bk_out, *ev_out, *good, *udiff_out;
+};
+
+// One work item of the stage.  All threads of the workgroup call it (it synchronises).
 // EK: EOS table kind (0 row/col, 1 octave); SPEC: the GUBSER_VISC_TEST / RIEMANN_TEST /
 // FP64-root branches are compiled in (flags are still tested at run time).
 // MARCH = false: a work item is one tile of one x-plane; every thread evaluates its two x
@@ -118,32 +127,25 @@ __device__ __forceinline__ void face_flux(const PhysP &p, const EOS &eos, float 
 // the slopes of its cell i and u^mu of cell i-1 from one plane to the next, so every x face
 // and every x slope is evaluated once (the chunk's first plane starts from the full stencil).
 template <bool BULK, int STEP, int EK, bool SPEC, bool MARCH>
-__global__ __launch_bounds__(CV_TILE_THREADS, MARCH ? 3 : 2) void k_visc_tiled(PhysP p, EosP E, StageP st, TileP tp,
-        const float *__restrict__ ev_st, const float *__restrict__ pi_st, const float *__restrict__ bk_st,
-        const float *ev1, const float *ev2, const float *pi1, const float *bk1,
-        const float *__restrict__ udiff_in, float *IS_src, float *bsrc, float *Src,
-        float *pi_out, float *bk_out, float *ev_out, float *good, float *udiff_out) {
-    __shared__ float s_c[CF_N][CV_TILE_MAXC];
-    __shared__ float s_f[15][CV_TILE_MAXF];
-    __shared__ float4 s_eos[CV_EOS_LDS];
+__device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, const StageP &st, const TileP &tp,
+                                          const StagePtrs &q_, int64_t work, float (*s_c)[CV_TILE_MAXC],
+                                          float (*s_f)[CV_TILE_MAXF]) {
+    const float *__restrict__ ev_st = q_.ev_st;
+    const float *__restrict__ pi_st = q_.pi_st;
+    const float *__restrict__ bk_st = q_.bk_st;
+    const float *ev1 = q_.ev1, *ev2 = q_.ev2, *pi1 = q_.pi1, *bk1 = q_.bk1;
+    const float *__restrict__ udiff_in = q_.udiff_in;
+    float *IS_src = q_.IS_src, *bsrc = q_.bsrc, *Src = q_.Src, *pi_out = q_.pi_out, *bk_out = q_.bk_out;
+    float *ev_out = q_.ev_out, *good = q_.good, *udiff_out = q_.udiff_out;
+    const EosP &E = eos.E;
+    const float4 *tab = eos.tab;
     const int tid = threadIdx.x;
     const uint32_t nb = p.nb;
     const float tau = st.tau;
-
-    // EOS table into LDS (octave table small enough), else global
-    const float4 *tab = E.tab;
-    if (CV_TILE_EOS_IN_LDS && !MARCH && EK == 1 && E.nlast + 1 <= CV_EOS_LDS) {
-        for (int q = tid; q <= E.nlast; q += CV_TILE_THREADS) s_eos[q] = E.tab[q];
-        tab = s_eos;
-    }
-    __syncthreads();
-    EosT<EK> eos(E, tab);
-
     const int ty = tp.ty, tz = tp.tz, HY = tp.HY, HZ = tp.HZ, hy = tp.hy, hz = tp.hz;
     const int my = tid / tz, mz = tid - (tid / tz) * tz;       // own cell in the tile
     const bool lane_in_tile = tid < ty * tz;
-
-    for (int64_t work = blockIdx.x; work < (MARCH ? tp.nwork : tp.ntiles); work += gridDim.x) {
+    {
         const int kt = (int)(work % tp.ntk);
         const int64_t r = work / tp.ntk;
         const int jt = (int)(r % tp.ntj);
@@ -553,6 +555,25 @@ __global__ __launch_bounds__(CV_TILE_THREADS, MARCH ? 3 : 2) void k_visc_tiled(P
         __syncthreads();   // LDS reused by the next plane / tile
       }
     }
+}
+
+template <bool BULK, int STEP, int EK, bool SPEC, bool MARCH>
+__global__ __launch_bounds__(CV_TILE_THREADS, MARCH ? 3 : 2) void k_visc_tiled(PhysP p, EosP E, StageP st, TileP tp,
+                                                                              StagePtrs q) {
+    __shared__ float s_c[CF_N][CV_TILE_MAXC];
+    __shared__ float s_f[15][CV_TILE_MAXF];
+    __shared__ float4 s_eos[CV_EOS_LDS];
+    const int tid = threadIdx.x;
+    // EOS table into LDS (octave table small enough), else global
+    const float4 *tab = E.tab;
+    if (CV_TILE_EOS_IN_LDS && !MARCH && EK == 1 && E.nlast + 1 <= CV_EOS_LDS) {
+        for (int k = tid; k <= E.nlast; k += CV_TILE_THREADS) s_eos[k] = E.tab[k];
+        tab = s_eos;
+    }
+    __syncthreads();
+    EosT<EK> eos(E, tab);
+    for (int64_t work = blockIdx.x; work < (MARCH ? tp.nwork : tp.ntiles); work += gridDim.x)
+        tile_work<BULK, STEP, EK, SPEC, MARCH>(p, eos, st, tp, q, work, s_c, s_f);
 }
 
 }  // namespace cv

@@ -203,6 +203,39 @@ int cv_hypersf_get(cv_handle *h, float *sf, float *txyz, float *pisf, float *bks
 /* count <- 0 and drop the snapshot. */
 int cv_hypersf_reset(cv_handle *h);
 
+/* ---- event batches -----------------------------------------------------------------
+ * An ensemble of events with their own grids, stepped in lockstep by one persistent
+ * launch per RK stage (the reference runs one CLVisc per event, pyvisc/ebe.py; this is
+ * the batched replacement of that loop).  All events share the physics parameters of
+ * `base` (dims replace nx, ny, nz), the EOS, TAU0 and DT; each keeps its own fields, tile
+ * and march chunk.  A step is a captured graph (two counter resets, two stage launches,
+ * a step-counter update) replayed per step; results are identical to stepping each
+ * event alone with cv_visc_steps_async. */
+typedef struct cv_batch cv_batch;
+
+/* dims: int32[3 * nevents] = (nx, ny, nz) per event.  Every event is a viscous handle
+ * (flags as in base; the test flags and the FP64 root are rejected). */
+int cv_batch_create(const cv_params *base, const cv_eos *eos, int nevents, const int32_t *dims,
+                    int device, cv_batch **out);
+int cv_batch_destroy(cv_batch *b);
+
+/* The single-event handle of one event, for the per-event entry points (fields, initial
+ * state, max e, tile and chunk, hypersurface).  Valid until cv_batch_destroy; queued
+ * batch work has finished when it returns. */
+int cv_batch_handle(cv_batch *b, int event, cv_handle **out);
+
+/* 0 leaves an event out of the following steps (e.g. after its freeze-out). */
+int cv_batch_set_active(cv_batch *b, const int32_t *active);
+
+/* nsteps RK2 steps of every active event from loop index `loop` (cv_batch_info; it
+ * starts at 0 and advances with the steps, cv_batch_set_loop moves it). */
+int cv_batch_steps_async(cv_batch *b, int64_t nsteps);
+int cv_batch_set_loop(cv_batch *b, int64_t loop);
+int cv_batch_sync(cv_batch *b);
+int cv_batch_last_elapsed_ms(cv_batch *b, float *ms);
+/* events, loop index of the next step, work items (tile column x x-chunk) per stage */
+int cv_batch_info(cv_batch *b, int32_t *nevents, int64_t *loop, int64_t *nwork);
+
 #ifdef __cplusplus
 }
 #endif
