@@ -42,8 +42,9 @@ def log(*a):
     sys.stderr.flush()
 
 
-def make_workload(bulk=True):
-    from clvisc_amd import _lib, ic
+def make_workload(bulk=True, ic=True):
+    from clvisc_amd import _lib
+    from clvisc_amd import ic as ic_mod
     from clvisc_amd.eos import Eos
     eos = Eos('crossover')
     tab = eos.create_table()
@@ -58,9 +59,11 @@ def make_workload(bulk=True):
         p.zetaos_max, p.zetaos_tpeak, p.zetaos_width = BULK
     p.efrz = float(eos.f_ed(0.137))
     p.nxskip = p.nyskip = p.nzskip = 1
+    if not ic:
+        return p, tab, None
     t0 = time.time()
-    pos = ic.woods_saxon_positions(400, 20240601)
-    ev = ic.hotspots(NX, NY, NZ, DX, DY, DZ, positions=pos, emax=50.0)
+    pos = ic_mod.woods_saxon_positions(400, 20240601)
+    ev = ic_mod.hotspots(NX, NY, NZ, DX, DY, DZ, positions=pos, emax=50.0)
     log('IC built in %.1fs' % (time.time() - t0))
     return p, tab, ev
 
@@ -90,6 +93,113 @@ def cpu_baseline(p, tab, ev, nthreads):
                       % (NX, NY, nz, steps, dt)}
 
 
+def ensemble_events():
+    """64 seeded events: 56 boost-invariant N x N grids, N = 161 .. 301 (odd), and eight
+    161 x 161 x 41 grids; 50 .. 400 hotspots each (event e: 50 + 350 e / 63), seed 1000 + e."""
+    dims = []
+    for i in range(56):
+        n = 161 + 2 * int(round(i * 70 / 55.0))
+        dims.append((n, n, 1))
+    dims += [(161, 161, 41)] * 8
+    nhot = [50 + int(round(350 * e / 63.0)) for e in range(64)]
+    return dims, nhot
+
+
+def ensemble_ic(d, nhot, seed):
+    from clvisc_amd import ic
+    pos = ic.woods_saxon_positions(nhot, seed)
+    return ic.hotspots(d[0], d[1], d[2], DX, DY, DZ, positions=pos, emax=50.0)
+
+
+def run_ensemble(args, p, tab, name):
+    """Batched event-by-event workload: one persistent launch per RK stage over all
+    events (graph-replayed steps) vs the same events stepped one after another."""
+    from clvisc_amd import _lib
+    from clvisc_amd.autotune import Autotuner
+    dims, nhot = ensemble_events()
+    t0 = time.time()
+    evs = [ensemble_ic(d, nhot[e], 1000 + e) for e, d in enumerate(dims)]
+    log('ensemble IC built in %.1fs: %d events, %d cells' % (time.time() - t0, len(dims),
+                                                             sum(a * b * c for a, b, c in dims)))
+    batch = _lib.Batch(p, tab, dims)
+    for e in range(len(dims)):
+        h = batch.handle(e)
+        h.visc_initialize(TAU0)
+        h.ideal_stage(1, np.float32(TAU0))
+        h.set('ev1', evs[e])
+    tuned = None
+    if args.autotune:
+        t0 = time.time()
+        tuner = Autotuner(p, tab, path=args.autotune_cache or None)
+        reps = {}
+        for e, d in enumerate(dims):
+            # one representative per family: 2D grids in steps of 20 cells, the 3D shape
+            rep = d if d[2] > 1 else (min(301, 161 + 20 * int(round((d[0] - 161) / 20.0))),) * 2 + (1,)
+            if rep not in reps:
+                reps[rep] = tuner.tune(rep, ensemble_ic(rep, 200, 7))
+            ty, tz, xc, _ = reps[rep]
+            if d[2] == 1:
+                nt = -(-rep[1] // ty)
+                ty = -(-d[1] // nt)
+            h = batch.handle(e)
+            h.set_stage_kernel(2, ty, tz)
+            h.set_march_chunk(xc)
+        tuned = {'%dx%dx%d' % k: list(v[:3]) for k, v in reps.items()}
+        log('autotuned %d shapes in %.1fs: %s' % (len(reps), time.time() - t0, tuned))
+    ncell = sum(a * b * c for a, b, c in dims)
+    batch.steps(args.warmup)
+    batch.sync()
+    t0 = time.perf_counter()
+    batch.steps(args.steps, sync=False)
+    batch.sync()
+    wall = time.perf_counter() - t0
+    value = ncell * args.steps / wall
+    log('batched: %d steps %.3fs -> %.3f ms/step, %.4g cell-updates/s (device %.3f ms)'
+        % (args.steps, wall, 1000 * wall / args.steps, value, batch.last_elapsed_ms()))
+    edmax = max(batch.handle(e).max_ed(1) for e in range(len(dims)))
+    # the same events, one launch sequence per event and step
+    singles = []
+    for e, d in enumerate(dims):
+        q = _lib.CvParams.from_buffer_copy(p)
+        q.nx, q.ny, q.nz = d
+        hs = _lib.Handle(q, tab, viscous=True)
+        hs.visc_initialize(TAU0)
+        hs.ideal_stage(1, np.float32(TAU0))
+        hs.set('ev1', evs[e])
+        singles.append(hs)
+    nseq = max(2, args.steps // 4)
+    for loop in range(args.warmup):
+        for hs in singles:
+            hs.visc_steps(loop, 1, sync=False)
+    for hs in singles:
+        hs.sync()
+    t0 = time.perf_counter()
+    for loop in range(args.warmup, args.warmup + nseq):
+        for hs in singles:
+            hs.visc_steps(loop, 1, sync=False)
+    for hs in singles:
+        hs.sync()
+    seq = ncell * nseq / (time.perf_counter() - t0)
+    log('one event after another: %.4g cell-updates/s (%d steps)' % (seq, nseq))
+    for hs in singles:
+        hs.close()
+    out = {
+        'metric': 'cell-updates/s, batched event-by-event viscous KT+IS (64 uneven events)',
+        'value': value, 'unit': 'cell-updates/s', 'n_gpus': 1,
+        'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': 1000.0 * wall / args.steps,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': '64 events: 56 boost-invariant N^2 (N=161..301) + 8 x 161x161x41, 50-400 '
+                               'Woods-Saxon hotspots each, viscous + bulk, crossover EOS; one persistent '
+                               'launch per RK stage, graph-replayed steps',
+                   'cells': ncell, 'device': name, 'autotuned': tuned,
+                   'sequential_value': seq, 'batched_over_sequential': value / seq,
+                   'physical': bool(np.isfinite(edmax) and 0.0 < edmax <= 50.0)},
+    }
+    print(json.dumps(out))
+    batch.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -98,6 +208,10 @@ def main():
     ap.add_argument('--no-bulk', action='store_true', help='shear viscosity only')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-threads', type=int, default=0)
+    ap.add_argument('--workload', choices=['single', 'ensemble'], default='single',
+                    help='single: one 385x385x115 event (the headline); ensemble: 64 uneven events batched')
+    ap.add_argument('--autotune', action='store_true', help='ensemble: tune tiles and chunks per grid shape')
+    ap.add_argument('--autotune-cache', default='', help='ensemble: JSON file of tuned shapes')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU only: the hydro step of one event runs on one MI355X '
@@ -106,6 +220,12 @@ def main():
 
     from clvisc_amd import _lib
     bulk = not args.no_bulk
+    if args.workload == 'ensemble':
+        p, tab, _ = make_workload(bulk, ic=False)
+        name, cus = _lib.device_info(0)
+        log('device: %s, %d CUs' % (name, cus))
+        run_ensemble(args, p, tab, name)
+        return
     p, tab, ev = make_workload(bulk)
     name, cus = _lib.device_info(0)
     log('device: %s, %d CUs' % (name, cus))
