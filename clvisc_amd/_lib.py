@@ -107,11 +107,24 @@ def lib():
             'cv_batch_sync': [vp],
             'cv_batch_last_elapsed_ms': [vp, C.POINTER(f32)],
             'cv_batch_info': [vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)],
+            'cv_mlp_create': [C.c_int, C.c_uint64, C.POINTER(vp)],
+            'cv_mlp_destroy': [vp],
+            'cv_mlp_get_params': [vp, vp, C.c_int],
+            'cv_mlp_set_params': [vp, vp, C.c_int],
+            'cv_mlp_grad': [vp, vp, vp, i64, vp, C.POINTER(f32)],
+            'cv_mlp_set_data': [vp, vp, vp, i64],
+            'cv_mlp_train': [vp, C.c_int, C.c_int, f32, vp],
+            'cv_mlp_forward': [vp, vp, i64, vp],
+            'cv_root_eval': [vp, vp, vp, vp, vp, i64, vp, vp, vp],
+            'cv_set_root_seed': [vp, vp, C.c_int],
+            'cv_get_root_iters': [vp, vp, i64],
         }
         for name, args in sig.items():
             fn = getattr(L, name)
             fn.argtypes = args
             fn.restype = C.c_int
+        L.cv_mlp_nparam.argtypes = []
+        L.cv_mlp_nparam.restype = C.c_int
         _lib = L
     return _lib
 
@@ -132,7 +145,9 @@ def exported_symbols():
             'cv_hypersf_find', 'cv_hypersf_count', 'cv_hypersf_get', 'cv_hypersf_reset',
             'cv_batch_create', 'cv_batch_destroy', 'cv_batch_handle', 'cv_batch_set_active',
             'cv_batch_steps_async', 'cv_batch_set_loop', 'cv_batch_sync', 'cv_batch_last_elapsed_ms',
-            'cv_batch_info']
+            'cv_batch_info', 'cv_mlp_nparam', 'cv_mlp_create', 'cv_mlp_destroy', 'cv_mlp_get_params',
+            'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward',
+            'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters']
 
 
 def make_eos(table):
@@ -264,6 +279,24 @@ class Handle(object):
         check(self.L.cv_last_elapsed_ms(self.h, C.byref(out)))
         return out.value
 
+    # -- learned root-solve seed
+    def set_root_seed(self, mlp, track_iters=True):
+        check(self.L.cv_set_root_seed(self.h, mlp.m if mlp is not None else None, int(track_iters)))
+
+    def root_iters(self):
+        out = np.empty(self.n, np.int32)
+        check(self.L.cv_get_root_iters(self.h, out.ctypes.data, self.n))
+        return out
+
+    def root_eval(self, T00, M, T03, mlp=None):
+        """(e, |v|, iterations) of the root solve of the given states, plain or seeded."""
+        a = [np.ascontiguousarray(x, np.float32).reshape(-1) for x in (T00, M, T03)]
+        n = a[0].size
+        e, v, it = np.empty(n, np.float32), np.empty(n, np.float32), np.empty(n, np.int32)
+        check(self.L.cv_root_eval(self.h, mlp.m if mlp is not None else None, a[0].ctypes.data, a[1].ctypes.data,
+                                  a[2].ctypes.data, n, e.ctypes.data, v.ctypes.data, it.ctypes.data))
+        return e, v, it
+
     # -- freeze-out hypersurface
     def hypersf_reserve(self, capacity):
         check(self.L.cv_hypersf_reserve(self.h, int(capacity)))
@@ -354,6 +387,61 @@ class Batch(object):
                 h.h = None
             self.L.cv_batch_destroy(self.b)
             self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mlp(object):
+    """Owner of the learned root-solve seed network (cv_mlp)."""
+
+    def __init__(self, device=0, seed=1234):
+        self.L = lib()
+        self.nparam = self.L.cv_mlp_nparam()
+        m = C.c_void_p()
+        check(self.L.cv_mlp_create(device, C.c_uint64(seed), C.byref(m)))
+        self.m = m
+
+    def params(self):
+        out = np.empty(self.nparam, np.float32)
+        check(self.L.cv_mlp_get_params(self.m, out.ctypes.data, self.nparam))
+        return out
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, np.float32).reshape(-1)
+        check(self.L.cv_mlp_set_params(self.m, a.ctypes.data, a.size))
+
+    def grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 4)
+        T = np.ascontiguousarray(T, np.float32).reshape(-1, 2)
+        g = np.empty(self.nparam, np.float32)
+        loss = C.c_float()
+        check(self.L.cv_mlp_grad(self.m, X.ctypes.data, T.ctypes.data, X.shape[0], g.ctypes.data, C.byref(loss)))
+        return g, loss.value
+
+    def set_data(self, X, T):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 4)
+        T = np.ascontiguousarray(T, np.float32).reshape(-1, 2)
+        check(self.L.cv_mlp_set_data(self.m, X.ctypes.data, T.ctypes.data, X.shape[0]))
+
+    def train(self, steps, batch=4096, lr=1e-3):
+        hist = np.empty(max(1, steps), np.float32)
+        check(self.L.cv_mlp_train(self.m, int(steps), int(batch), float(lr), hist.ctypes.data))
+        return hist[:steps]
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, 4)
+        Y = np.empty((X.shape[0], 2), np.float32)
+        check(self.L.cv_mlp_forward(self.m, X.ctypes.data, X.shape[0], Y.ctypes.data))
+        return Y
+
+    def close(self):
+        if getattr(self, 'm', None):
+            self.L.cv_mlp_destroy(self.m)
+            self.m = None
 
     def __del__(self):
         try:

@@ -59,6 +59,11 @@ struct cv_handle {
     TileP tile;
     int tile_grid = 0;
     int march_xc = 0;         // x planes per march chunk (0: automatic)
+    // learned root-solve seed of the ideal stage (cv_set_root_seed)
+    struct cv_mlp *seed_mlp = nullptr;
+    bool track_iters = false;
+    float *seed_buf = nullptr;  // 10 n floats: T00, M, T03, features (4), outputs (2), seed
+    int *iters = nullptr;
     // freeze-out surface: fields at the previous surface time, appended elements
     float *sf_ev = nullptr, *sf_pi = nullptr, *sf_bk = nullptr;
     float sf_tau_old = 0.0f;
@@ -266,6 +271,7 @@ extern "C" int cv_destroy(cv_handle *h) {
     (void)hipFree(h->sf_ev); (void)hipFree(h->sf_pi); (void)hipFree(h->sf_bk);
     (void)hipFree(h->sf_out); (void)hipFree(h->sf_txyz); (void)hipFree(h->sf_pisf); (void)hipFree(h->sf_bksf);
     (void)hipFree(h->sf_count);
+    (void)hipFree(h->seed_buf); (void)hipFree(h->iters);
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
     if (h->ev_b) (void)hipEventDestroy(h->ev_b);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -377,10 +383,13 @@ extern "C" int cv_visc_initialize(cv_handle *h, float tau) {
 }
 
 // ---- launch helpers (queue only) --------------------------------------------
+static int launch_ideal_seeded(cv_handle *h, const StageP &st, float *ev_st, float *ev1, float *ev_out);
+
 static int launch_ideal(cv_handle *h, int step, float tau, float *ev_st, float *ev1, float *ev_out) {
     StageP st = stage_params(h, step, tau);
+    if (h->seed_mlp) return launch_ideal_seeded(h, st, ev_st, ev1, ev_out);
     hipLaunchKernelGGL(k_ideal_stage, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, st,
-                       ev_st, ev1, h->Src, ev_out);
+                       ev_st, ev1, h->Src, ev_out, nullptr, nullptr, h->track_iters ? h->iters : nullptr);
     HIP_TRY(hipGetLastError());
     return CV_OK;
 }
@@ -1021,5 +1030,298 @@ extern "C" int cv_batch_info(cv_batch *b, int32_t *nevents, int64_t *loop, int64
             if (b->active[e]) w += b->ev[e]->tile.nwork;
         *nwork = w;
     }
+    return CV_OK;
+}
+
+// ---- learned root-solve seed -------------------------------------------------------
+#include "cv_mlp.h"
+
+struct cv_mlp {
+    int device = 0;
+    float *P = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr, *loss = nullptr;
+    float *part = nullptr, *part_loss = nullptr;
+    int64_t part_blocks = 0;
+    float *X = nullptr, *T = nullptr;       // training set
+    int64_t ndata = 0;
+    int64_t adam_t = 0;
+    hipStream_t stream = nullptr;
+};
+
+static uint64_t mlp_rng(uint64_t &s) {             // SplitMix64
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+extern "C" int cv_mlp_nparam(void) { return MLP_NPARAM; }
+
+extern "C" int cv_mlp_create(int device, uint64_t seed, cv_mlp **out) {
+    if (!out) return fail(CV_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(device));
+    cv_mlp *m = new cv_mlp();
+    m->device = device;
+    bool ok = hipMalloc((void **)&m->P, sizeof(float) * MLP_NPARAM) == hipSuccess &&
+              hipMalloc((void **)&m->m1, sizeof(float) * MLP_NPARAM) == hipSuccess &&
+              hipMalloc((void **)&m->m2, sizeof(float) * MLP_NPARAM) == hipSuccess &&
+              hipMalloc((void **)&m->grad, sizeof(float) * MLP_NPARAM) == hipSuccess &&
+              hipMalloc((void **)&m->loss, sizeof(float)) == hipSuccess &&
+              hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) == hipSuccess;
+    if (!ok) {
+        cv_mlp_destroy(m);
+        return fail(CV_ERR_HIP, "cv_mlp_create: allocation failed");
+    }
+    // uniform(-a, a) with a = sqrt(6 / (fan_in + fan_out)); zero biases
+    std::vector<float> h(MLP_NPARAM, 0.0f);
+    uint64_t s = seed;
+    auto fill = [&](int off, int fo, int fi) {
+        double a = sqrt(6.0 / (fi + fo));
+        for (int q = 0; q < fo * fi; q++) h[off + q] = (float)((2.0 * ((mlp_rng(s) >> 11) * (1.0 / 9007199254740992.0)) - 1.0) * a);
+    };
+    fill(MLP_OFF_W1, MLP_H, MLP_IN);
+    fill(MLP_OFF_W2, MLP_H, MLP_H);
+    fill(MLP_OFF_W3, MLP_OUT, MLP_H);
+    std::vector<float> z(MLP_NPARAM, 0.0f);
+    HIP_TRY(hipMemcpy(m->P, h.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->m1, z.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->m2, z.data(), sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice));
+    *out = m;
+    return CV_OK;
+}
+
+extern "C" int cv_mlp_destroy(cv_mlp *m) {
+    if (!m) return CV_OK;
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    (void)hipFree(m->P); (void)hipFree(m->m1); (void)hipFree(m->m2); (void)hipFree(m->grad); (void)hipFree(m->loss);
+    (void)hipFree(m->part); (void)hipFree(m->part_loss); (void)hipFree(m->X); (void)hipFree(m->T);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return CV_OK;
+}
+
+extern "C" int cv_mlp_get_params(cv_mlp *m, float *host, int n) {
+    if (!m || !host || n != MLP_NPARAM) return fail(CV_ERR_ARG, "cv_mlp_get_params: need %d floats", MLP_NPARAM);
+    HIP_TRY(hipSetDevice(m->device));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    HIP_TRY(hipMemcpy(host, m->P, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return CV_OK;
+}
+
+extern "C" int cv_mlp_set_params(cv_mlp *m, const float *host, int n) {
+    if (!m || !host || n != MLP_NPARAM) return fail(CV_ERR_ARG, "cv_mlp_set_params: need %d floats", MLP_NPARAM);
+    HIP_TRY(hipSetDevice(m->device));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    HIP_TRY(hipMemcpy(m->P, host, sizeof(float) * n, hipMemcpyHostToDevice));
+    m->adam_t = 0;
+    HIP_TRY(hipMemset(m->m1, 0, sizeof(float) * MLP_NPARAM));
+    HIP_TRY(hipMemset(m->m2, 0, sizeof(float) * MLP_NPARAM));
+    return CV_OK;
+}
+
+static int mlp_parts(cv_mlp *m, int64_t nblk) {
+    if (m->part_blocks >= nblk) return CV_OK;
+    (void)hipFree(m->part); (void)hipFree(m->part_loss);
+    m->part = nullptr; m->part_loss = nullptr; m->part_blocks = 0;
+    HIP_TRY(hipMalloc((void **)&m->part, sizeof(float) * MLP_NPARAM * nblk));
+    HIP_TRY(hipMalloc((void **)&m->part_loss, sizeof(float) * nblk));
+    m->part_blocks = nblk;
+    return CV_OK;
+}
+
+static const int64_t MLP_SAMPLES_PER_BLOCK = 16 * MLP_WAVES;
+
+// gradient of the loss on device arrays X, T (n samples) into m->grad, loss into m->loss;
+// adam: also one Adam step
+static int mlp_grad_dev(cv_mlp *m, const float *X, const float *T, int64_t n, int adam, float lr) {
+    const int64_t nblk = (n + MLP_SAMPLES_PER_BLOCK - 1) / MLP_SAMPLES_PER_BLOCK;
+    int rc = mlp_parts(m, nblk);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)nblk), dim3(64 * MLP_WAVES), 0, m->stream, m->P, X, T, n,
+                       1.0f / (float)n, m->part, m->part_loss);
+    HIP_TRY(hipGetLastError());
+    float c1 = 1.0f, c2 = 1.0f;
+    const float b1 = 0.9f, b2 = 0.999f;
+    if (adam) {
+        m->adam_t++;
+        c1 = (float)(1.0 / (1.0 - pow((double)b1, (double)m->adam_t)));
+        c2 = (float)(1.0 / (1.0 - pow((double)b2, (double)m->adam_t)));
+    }
+    hipLaunchKernelGGL(k_mlp_reduce_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, m->stream, m->part,
+                       m->part_loss, (int)nblk, m->grad, m->loss, m->P, m->m1, m->m2, lr, b1, b2, 1e-8f, c1, c2, adam);
+    HIP_TRY(hipGetLastError());
+    return CV_OK;
+}
+
+extern "C" int cv_mlp_grad(cv_mlp *m, const float *X, const float *T, int64_t n, float *grad, float *loss) {
+    if (!m || !X || !T || !grad || !loss || n < 1) return fail(CV_ERR_ARG, "cv_mlp_grad: bad argument");
+    HIP_TRY(hipSetDevice(m->device));
+    float *dX = nullptr, *dT = nullptr;
+    HIP_TRY(hipMalloc((void **)&dX, sizeof(float) * MLP_IN * n));
+    hipError_t e = hipMalloc((void **)&dT, sizeof(float) * MLP_OUT * n);
+    if (e == hipSuccess) e = hipMemcpy(dX, X, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dT, T, sizeof(float) * MLP_OUT * n, hipMemcpyHostToDevice);
+    int rc = e == hipSuccess ? mlp_grad_dev(m, dX, dT, n, 0, 0.0f) : fail(CV_ERR_HIP, "%s", hipGetErrorString(e));
+    if (!rc) {
+        e = hipStreamSynchronize(m->stream);
+        if (e == hipSuccess) e = hipMemcpy(grad, m->grad, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(loss, m->loss, sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(CV_ERR_HIP, "cv_mlp_grad: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(dX); (void)hipFree(dT);
+    return rc;
+}
+
+extern "C" int cv_mlp_set_data(cv_mlp *m, const float *X, const float *T, int64_t n) {
+    if (!m || !X || !T || n < 1) return fail(CV_ERR_ARG, "cv_mlp_set_data: bad argument");
+    HIP_TRY(hipSetDevice(m->device));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    (void)hipFree(m->X); (void)hipFree(m->T);
+    m->X = m->T = nullptr; m->ndata = 0;
+    HIP_TRY(hipMalloc((void **)&m->X, sizeof(float) * MLP_IN * n));
+    HIP_TRY(hipMalloc((void **)&m->T, sizeof(float) * MLP_OUT * n));
+    HIP_TRY(hipMemcpy(m->X, X, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m->T, T, sizeof(float) * MLP_OUT * n, hipMemcpyHostToDevice));
+    m->ndata = n;
+    return CV_OK;
+}
+
+// steps Adam steps on consecutive batches of the training set (wrapping around); the loss
+// of every step goes to loss_hist (may be NULL)
+extern "C" int cv_mlp_train(cv_mlp *m, int steps, int batch, float lr, float *loss_hist) {
+    if (!m || steps < 0 || batch < 1) return fail(CV_ERR_ARG, "cv_mlp_train: bad argument");
+    if (m->ndata < batch) return fail(CV_ERR_STATE, "cv_mlp_train: training set smaller than a batch");
+    HIP_TRY(hipSetDevice(m->device));
+    std::vector<float> hist(steps > 0 ? steps : 1);
+    float *d_hist = nullptr;
+    if (loss_hist && steps > 0) HIP_TRY(hipMalloc((void **)&d_hist, sizeof(float) * steps));
+    int64_t off = 0;
+    int rc = CV_OK;
+    for (int s = 0; s < steps && !rc; s++) {
+        if (off + batch > m->ndata) off = 0;
+        rc = mlp_grad_dev(m, m->X + off * MLP_IN, m->T + off * MLP_OUT, batch, 1, lr);
+        if (!rc && d_hist) {
+            hipError_t e = hipMemcpyAsync(d_hist + s, m->loss, sizeof(float), hipMemcpyDeviceToDevice, m->stream);
+            if (e != hipSuccess) rc = fail(CV_ERR_HIP, "%s", hipGetErrorString(e));
+        }
+        off += batch;
+    }
+    hipError_t e = hipStreamSynchronize(m->stream);
+    if (!rc && e != hipSuccess) rc = fail(CV_ERR_HIP, "cv_mlp_train: %s", hipGetErrorString(e));
+    if (!rc && d_hist) {
+        e = hipMemcpy(loss_hist, d_hist, sizeof(float) * steps, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = fail(CV_ERR_HIP, "%s", hipGetErrorString(e));
+    }
+    (void)hipFree(d_hist);
+    return rc;
+}
+
+extern "C" int cv_mlp_forward(cv_mlp *m, const float *X, int64_t n, float *Y) {
+    if (!m || !X || !Y || n < 1) return fail(CV_ERR_ARG, "cv_mlp_forward: bad argument");
+    HIP_TRY(hipSetDevice(m->device));
+    float *dX = nullptr, *dY = nullptr;
+    HIP_TRY(hipMalloc((void **)&dX, sizeof(float) * MLP_IN * n));
+    hipError_t e = hipMalloc((void **)&dY, sizeof(float) * MLP_OUT * n);
+    if (e == hipSuccess) e = hipMemcpy(dX, X, sizeof(float) * MLP_IN * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        const int64_t nblk = (n + MLP_SAMPLES_PER_BLOCK - 1) / MLP_SAMPLES_PER_BLOCK;
+        hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)nblk), dim3(64 * MLP_WAVES), 0, m->stream, m->P, dX, n, dY);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e == hipSuccess) e = hipMemcpy(Y, dY, sizeof(float) * MLP_OUT * n, hipMemcpyDeviceToHost);
+    (void)hipFree(dX); (void)hipFree(dY);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_mlp_forward: %s", hipGetErrorString(e));
+    return CV_OK;
+}
+
+// root solve of host states (T^tt, |T^ti|, T^t eta) with the handle's EOS: plain
+// (m == NULL) or from the network's |v| seed; outputs e, |v|, iterations
+extern "C" int cv_root_eval(cv_handle *h, cv_mlp *m, const float *T00, const float *M, const float *T03, int64_t n,
+                            float *e_out, float *v_out, int32_t *iters) {
+    if (!h || !T00 || !M || !T03 || !e_out || !v_out || !iters || n < 1) return fail(CV_ERR_ARG, "cv_root_eval: bad argument");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    float *buf = nullptr;
+    int *dit = nullptr;
+    // T00, M, T03, X (4), Y (2), e, v
+    HIP_TRY(hipMalloc((void **)&buf, sizeof(float) * n * 11));
+    hipError_t e = hipMalloc((void **)&dit, sizeof(int) * n);
+    float *dT = buf, *dM = buf + n, *d3 = buf + 2 * n, *dX = buf + 3 * n, *dY = buf + 7 * n, *de = buf + 9 * n,
+          *dv = buf + 10 * n;
+    if (e == hipSuccess) e = hipMemcpy(dT, T00, sizeof(float) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dM, M, sizeof(float) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d3, T03, sizeof(float) * n, hipMemcpyHostToDevice);
+    const float *seed = nullptr;
+    if (e == hipSuccess && m) {
+        hipLaunchKernelGGL(k_seed_features, grid1d(n), dim3(256), 0, h->stream, dT, dM, d3, n, dX);
+        const int64_t nblk = (n + MLP_SAMPLES_PER_BLOCK - 1) / MLP_SAMPLES_PER_BLOCK;
+        hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)nblk), dim3(64 * MLP_WAVES), 0, h->stream, m->P, dX, n, dY);
+        // |v| is the second output of each pair: pass every other float via a strided copy
+        hipLaunchKernelGGL(k_take_col, grid1d(n), dim3(256), 0, h->stream, dY, n, 2, 1, de);
+        seed = de;
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        float *seed_buf = nullptr;
+        if (seed) {           // keep the seed apart from the outputs
+            e = hipMalloc((void **)&seed_buf, sizeof(float) * n);
+            if (e == hipSuccess) e = hipMemcpyAsync(seed_buf, seed, sizeof(float) * n, hipMemcpyDeviceToDevice, h->stream);
+        }
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_root_eval, grid1d(n), dim3(256), 0, h->stream, h->eos, dT, dM, seed_buf, n, de, dv, dit);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        (void)hipFree(seed_buf);
+    }
+    if (e == hipSuccess) e = hipMemcpy(e_out, de, sizeof(float) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(v_out, dv, sizeof(float) * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(iters, dit, sizeof(int) * n, hipMemcpyDeviceToHost);
+    (void)hipFree(buf); (void)hipFree(dit);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_root_eval: %s", hipGetErrorString(e));
+    return CV_OK;
+}
+
+// ideal stage with the learned seed: conserved state (dry run), features, network, seeded
+// root solve (the stage proper)
+static int launch_ideal_seeded(cv_handle *h, const StageP &st, float *ev_st, float *ev1, float *ev_out) {
+    const int64_t n = h->n;
+    float *cons = h->seed_buf, *X = cons + 3 * n, *Y = cons + 7 * n, *seed = cons + 9 * n;
+    hipLaunchKernelGGL(k_ideal_stage, grid1d(n), dim3(256), 0, h->stream, h->phys, h->eos, st, ev_st, ev1, h->Src,
+                       ev_out, cons, nullptr, nullptr);
+    hipLaunchKernelGGL(k_seed_features, grid1d(n), dim3(256), 0, h->stream, cons, cons + n, cons + 2 * n, n, X);
+    const int64_t nblk = (n + MLP_SAMPLES_PER_BLOCK - 1) / MLP_SAMPLES_PER_BLOCK;
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)nblk), dim3(64 * MLP_WAVES), 0, h->stream, h->seed_mlp->P, X, n, Y);
+    hipLaunchKernelGGL(k_take_col, grid1d(n), dim3(256), 0, h->stream, Y, n, 2, 1, seed);
+    hipLaunchKernelGGL(k_ideal_stage, grid1d(n), dim3(256), 0, h->stream, h->phys, h->eos, st, ev_st, ev1, h->Src,
+                       ev_out, nullptr, seed, h->track_iters ? h->iters : nullptr);
+    HIP_TRY(hipGetLastError());
+    return CV_OK;
+}
+
+extern "C" int cv_set_root_seed(cv_handle *h, cv_mlp *m, int track_iters) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    if (m && (h->prm.flags & CV_FLAG_FP64_ROOT))
+        return fail(CV_ERR_UNSUPPORTED, "cv_set_root_seed: the seed is for the FP32 root solve");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (m && !h->seed_buf) HIP_TRY(hipMalloc((void **)&h->seed_buf, sizeof(float) * 10 * h->n));
+    if (track_iters && !h->iters) {
+        HIP_TRY(hipMalloc((void **)&h->iters, sizeof(int) * h->n));
+        HIP_TRY(hipMemset(h->iters, 0, sizeof(int) * h->n));
+    }
+    h->seed_mlp = m;
+    h->track_iters = track_iters != 0;
+    return CV_OK;
+}
+
+extern "C" int cv_get_root_iters(cv_handle *h, int32_t *host, int64_t n) {
+    if (!h || !host) return fail(CV_ERR_ARG, "null argument");
+    if (!h->iters) return fail(CV_ERR_STATE, "iteration counts are not tracked (cv_set_root_seed)");
+    if (n != h->n) return fail(CV_ERR_ARG, "cv_get_root_iters: %lld cells", (long long)h->n);
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(host, h->iters, sizeof(int) * n, hipMemcpyDeviceToHost));
     return CV_OK;
 }

@@ -198,12 +198,15 @@ __device__ __forceinline__ float max_prop_speed(const PhysP &p, const EOS &eos, 
 
 // rootFinding_newton (helper.h:128-167) with P_eff = P + bulk
 // bulk: Pi added to P(e); clamp0: effective pressure P + Pi held >= 0 (bulk runs only:
-// a negative effective pressure would allow |v| > 1 in v = M / (T00 + P + Pi))
+// a negative effective pressure would allow |v| > 1 in v = M / (T00 + P + Pi)).
+// v0: start of the iteration (the reference starts at the middle of [0, 1]; a learned
+// seed starts elsewhere, the bracket and the stopping rule stay the same).
 template <class EOS>
-__device__ __forceinline__ float root_newton(const EOS &eos, float T00, float M, float bulk, bool clamp0 = false) {
+__device__ __forceinline__ float root_newton_from(const EOS &eos, float T00, float M, float bulk, bool clamp0,
+                                                  float v0, int *iters) {
     float vl = 0.0f, vh = 1.0f;
     const float dpe = 1.0f / 3.0f;
-    float v = 0.5f * (vl + vh);
+    float v = v0;
     float ed = T00 - M * v;
     float pr = eos.P(ed) + bulk;
     if (clamp0) pr = rmax(pr, 0.0f);
@@ -231,7 +234,12 @@ __device__ __forceinline__ float root_newton(const EOS &eos, float T00, float M,
         df = (T00 + pr) - M * v * dpe;
         if (f > 0.0f) vh = v; else vl = v;
     }
+    if (iters) *iters = i;
     return T00 - M * v;
+}
+template <class EOS>
+__device__ __forceinline__ float root_newton(const EOS &eos, float T00, float M, float bulk, bool clamp0 = false) {
+    return root_newton_from(eos, T00, M, bulk, clamp0, 0.5f * (0.0f + 1.0f), nullptr);
 }
 
 // FP64 twin of the root solve (cells with M close to T00); table values stay FP32.

@@ -160,9 +160,14 @@ __device__ __forceinline__ void ideal_axis(const PhysP &p, const Eos &eos, const
 // ---------------------------------------------------------------------------
 // CLIdeal.stepUpdate(step): christoffel + 3 sweeps + update_ev, one cell per thread
 // ---------------------------------------------------------------------------
+// cons_out (optional): write (T^tt, |T^ti|, T^t eta) of the root solve's input to three
+// planes and change nothing else (the input of the learned seed).  seed_v (optional): start
+// the root solve at this |v|.  iters_out (optional): iterations of the root solve.
 __global__ __launch_bounds__(256) void k_ideal_stage(PhysP p, EosP E, StageP st,
                                                      const float *__restrict__ ev_st,
-                                                     const float *ev1, float *Src, float *ev_out) {
+                                                     const float *ev1, float *Src, float *ev_out,
+                                                     float *cons_out = nullptr, const float *seed_v = nullptr,
+                                                     int *iters_out = nullptr) {
     int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= p.n) return;
     Eos eos(E, E.tab);
@@ -183,7 +188,7 @@ __global__ __launch_bounds__(256) void k_ideal_stage(PhysP p, EosP E, StageP st,
     ideal_axis<0>(p, eos, st, cl, ev_st, s);
     ideal_axis<1>(p, eos, st, cl, ev_st, s);
     ideal_axis<2>(p, eos, st, cl, ev_st, s);
-    st4(Src, p.n, I, make_float4(s[0], s[1], s[2], s[3]));
+    if (!cons_out) st4(Src, p.n, I, make_float4(s[0], s[1], s[2], s[3]));
     // update_ev (kernel_ideal.cl:165-231)
     float4 e = ld4(ev1, p.n, I);
     float pressure = eos.P(e.x);
@@ -217,15 +222,40 @@ __global__ __launch_bounds__(256) void k_ideal_stage(PhysP p, EosP E, StageP st,
         T01 *= sc; T02 *= sc; T03 *= sc;
         M = SC * T00;
     }
+    if (cons_out) {
+        cons_out[I] = T00; cons_out[p.n + I] = M; cons_out[2 * p.n + I] = T03;
+        return;
+    }
     float ed_find;
+    int it = 0;
     if ((p.flags & 8) && M > p.fp64_vmin * T00)
         ed_find = (float)root_newton_d(E, E.tab, (double)T00, (double)M, 0.0);
+    else if (seed_v)
+        ed_find = root_newton_from(eos, T00, M, 0.0f, false, fminf(fmaxf(seed_v[I], 0.0f), 0.9999f), &it);
     else
-        ed_find = root_newton(eos, T00, M, 0.0f);
+        ed_find = root_newton_from(eos, T00, M, 0.0f, false, 0.5f * (0.0f + 1.0f), &it);
+    if (iters_out) iters_out[I] = it;
     ed_find = rmax(0.0f, ed_find);
     float pr = eos.P(ed_find);
     float repv = rcp(rmax(CV_ACU, T00 + pr));
     st4(ev_out, p.n, I, make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv));
+}
+
+// Root solve of given conserved states (T^tt, |T^ti|) from the middle of [0, 1] or from a
+// seed |v|: e, |v| = M / (T^tt + P(e)) and the iteration count (evaluation of the seed).
+__global__ __launch_bounds__(256) void k_root_eval(EosP E, const float *__restrict__ T00, const float *__restrict__ M,
+                                                   const float *__restrict__ seed_v, int64_t n, float *e_out,
+                                                   float *v_out, int *iters) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    Eos eos(E, E.tab);
+    const float t = T00[q], m = M[q];
+    int it = 0;
+    const float v0 = seed_v ? fminf(fmaxf(seed_v[q], 0.0f), 0.9999f) : 0.5f * (0.0f + 1.0f);
+    float e = rmax(0.0f, root_newton_from(eos, t, m, 0.0f, false, v0, &it));
+    e_out[q] = e;
+    v_out[q] = m * rcp(rmax(CV_ACU, t + eos.P(e)));
+    iters[q] = it;
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,359 @@
+// cv_mlp.h -- the learned seed of the conserved-to-primitive root solve: a 4-64-64-2 MLP
+// (tanh) trained and evaluated on the device with BF16 MFMA (FP32 accumulate).
+//
+// Arithmetic (what the host check restates): every matrix product takes BF16-rounded
+// operands (round to nearest even) and accumulates in FP32; biases, tanh, the loss, the
+// gradients' element-wise factors and Adam are FP32.  Layer 1 (K = 4) and layer 3 (N = 2)
+// are VALU products; layer 2 (64 x 64) and its two backward products are
+// v_mfma_f32_16x16x16_bf16.  A wave handles 16 samples, so every 64 x 64 product of a wave
+// is 16 MFMAs.
+//
+// MFMA 16x16x16 operand layout (lane l of 64, i = 0..3):
+//   A (16 x 16, rows x k):  row = l % 16, k = 4 (l / 16) + i
+//   B (16 x 16, k x cols):  col = l % 16, k = 4 (l / 16) + i
+//   C (16 x 16, rows x cols): col = l % 16, row = 4 (l / 16) + i
+// so a C tile of one product is directly the A operand of a product that contracts over
+// its rows (the weight gradient) and the B operand of one that keeps them.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cv {
+
+#define MLP_IN 4
+#define MLP_H 64
+#define MLP_OUT 2
+// flat parameter vector: W1[64][4], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]
+#define MLP_OFF_W1 0
+#define MLP_OFF_B1 (MLP_OFF_W1 + MLP_H * MLP_IN)
+#define MLP_OFF_W2 (MLP_OFF_B1 + MLP_H)
+#define MLP_OFF_B2 (MLP_OFF_W2 + MLP_H * MLP_H)
+#define MLP_OFF_W3 (MLP_OFF_B2 + MLP_H)
+#define MLP_OFF_B3 (MLP_OFF_W3 + MLP_OUT * MLP_H)
+#define MLP_NPARAM (MLP_OFF_B3 + MLP_OUT)
+#define MLP_WAVES 4                      // waves per workgroup, 16 samples each
+
+typedef short mlp_bf16x4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);           // round to nearest even (finite inputs)
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16r(float x) { return __uint_as_float((uint32_t)bf16_bits(x) << 16); }
+
+__device__ __forceinline__ mlp_f32x4 mfma16(mlp_bf16x4 a, mlp_bf16x4 b, mlp_f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ mlp_bf16x4 pack4(float a, float b, float c, float d) {
+    mlp_bf16x4 r;
+    r[0] = (short)bf16_bits(a); r[1] = (short)bf16_bits(b); r[2] = (short)bf16_bits(c); r[3] = (short)bf16_bits(d);
+    return r;
+}
+
+// One wave: forward of 16 samples.  x[16][4] in LDS (already features), params FP32.
+// Outputs in C layout per lane: h1[n][i], h2[n][i] (n = column tile 0..3, i = row 0..3 of
+// the lane), y[i][o] for the lane's rows (replicated over the 16 lanes of a row group).
+struct MlpFwd {
+    float h1[4][4], h2[4][4];
+    float y[4][MLP_OUT];
+};
+
+// sum over the 16 lanes that share l / 16 (lanes of one row group)
+__device__ __forceinline__ float sum16(float v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    v += __shfl_xor(v, 8, 64);
+    return v;
+}
+// sum over the 4 lanes that share l % 16 (same column, the four row groups)
+__device__ __forceinline__ float sum_rowgroups(float v) {
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+// hbuf: per-wave LDS scratch of 16 x 64 floats (transposes C layout to A layout)
+__device__ __forceinline__ void mlp_forward_wave(const float *__restrict__ P, const float xs[16][MLP_IN],
+                                                 float (*hbuf)[MLP_H], MlpFwd &f) {
+    const int l = threadIdx.x & 63;
+    const int c0 = l & 15, rg = l >> 4;
+    // layer 1 (VALU, K = 4): z1[row][col], row = 4 rg + i, col = 16 n + c0
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const int col = 16 * n + c0;
+        const float w0 = bf16r(P[MLP_OFF_W1 + col * MLP_IN + 0]), w1 = bf16r(P[MLP_OFF_W1 + col * MLP_IN + 1]);
+        const float w2 = bf16r(P[MLP_OFF_W1 + col * MLP_IN + 2]), w3 = bf16r(P[MLP_OFF_W1 + col * MLP_IN + 3]);
+        const float b = P[MLP_OFF_B1 + col];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = 4 * rg + i;
+            float z = bf16r(xs[row][0]) * w0;
+            z = z + bf16r(xs[row][1]) * w1;
+            z = z + bf16r(xs[row][2]) * w2;
+            z = z + bf16r(xs[row][3]) * w3;
+            f.h1[n][i] = tanhf(z + b);
+        }
+    }
+    // C layout -> LDS -> A layout for layer 2
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) hbuf[4 * rg + i][16 * n + c0] = f.h1[n][i];
+    __syncthreads();
+    mlp_f32x4 acc[4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) acc[n] = mlp_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int kb = 16 * s + 4 * rg;
+        mlp_bf16x4 a = pack4(hbuf[c0][kb + 0], hbuf[c0][kb + 1], hbuf[c0][kb + 2], hbuf[c0][kb + 3]);
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            // B[k][c] = W2[16 n + c][k]
+            const float *w = P + MLP_OFF_W2 + (16 * n + c0) * MLP_H + kb;
+            mlp_bf16x4 b = pack4(w[0], w[1], w[2], w[3]);
+            acc[n] = mfma16(a, b, acc[n]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const float b = P[MLP_OFF_B2 + 16 * n + c0];
+#pragma unroll
+        for (int i = 0; i < 4; i++) f.h2[n][i] = tanhf(acc[n][i] + b);
+    }
+    // layer 3 (VALU, N = 2): lane partial over its 4 columns, then over the 16 lanes
+#pragma unroll
+    for (int o = 0; o < MLP_OUT; o++) {
+        float w[4];
+#pragma unroll
+        for (int n = 0; n < 4; n++) w[n] = bf16r(P[MLP_OFF_W3 + o * MLP_H + 16 * n + c0]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            float s = bf16r(f.h2[0][i]) * w[0];
+            s = s + bf16r(f.h2[1][i]) * w[1];
+            s = s + bf16r(f.h2[2][i]) * w[2];
+            s = s + bf16r(f.h2[3][i]) * w[3];
+            f.y[i][o] = sum16(s) + P[MLP_OFF_B3 + o];
+        }
+    }
+}
+
+// Training: forward + backward of a batch; workgroup g writes its partial gradient to
+// part[g][MLP_NPARAM] and its partial loss to part_loss[g].  Loss = mean over samples of
+// sum_o (y_o - t_o)^2.
+__global__ __launch_bounds__(64 * MLP_WAVES) void k_mlp_grad(const float *__restrict__ P, const float *__restrict__ X,
+                                                           const float *__restrict__ T, int64_t nsamp, float inv_b,
+                                                           float *part, float *part_loss) {
+    __shared__ float s_x[MLP_WAVES][16][MLP_IN];
+    __shared__ float s_h[MLP_WAVES][16][MLP_H];
+    __shared__ float s_g[MLP_NPARAM];
+    __shared__ float s_loss[MLP_WAVES];
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int c0 = l & 15, rg = l >> 4;
+    for (int q = threadIdx.x; q < MLP_NPARAM; q += blockDim.x) s_g[q] = 0.0f;
+    const int64_t base = ((int64_t)blockIdx.x * MLP_WAVES + wv) * 16;
+    for (int q = l; q < 16 * MLP_IN; q += 64) {
+        const int64_t r = base + q / MLP_IN;
+        s_x[wv][q / MLP_IN][q % MLP_IN] = r < nsamp ? X[r * MLP_IN + q % MLP_IN] : 0.0f;
+    }
+    __syncthreads();
+    MlpFwd f;
+    mlp_forward_wave(P, s_x[wv], s_h[wv], f);
+    // dy = 2 (y - t) / B for valid samples
+    float dy[4][MLP_OUT];
+    float loss = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int64_t r = base + 4 * rg + i;
+#pragma unroll
+        for (int o = 0; o < MLP_OUT; o++) {
+            const float t = r < nsamp ? T[r * MLP_OUT + o] : 0.0f;
+            const float d = f.y[i][o] - t;
+            dy[i][o] = r < nsamp ? 2.0f * d * inv_b : 0.0f;
+            if (c0 == 0 && r < nsamp) loss += d * d;
+        }
+    }
+    loss = sum_rowgroups(loss);
+    // layer 3 gradients: dW3[o][col] = sum_rows dy * bf16(h2); db3 = sum_rows dy
+#pragma unroll
+    for (int o = 0; o < MLP_OUT; o++) {
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            float s = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 4; i++) s = s + dy[i][o] * bf16r(f.h2[n][i]);
+            s = sum_rowgroups(s);
+            if (rg == 0) atomicAdd(&s_g[MLP_OFF_W3 + o * MLP_H + 16 * n + c0], s);
+        }
+        // dy is the same on the 16 lanes of a row group: lane 0 of each group holds its rows
+        float sb = dy[0][o] + dy[1][o] + dy[2][o] + dy[3][o];
+        sb = sum_rowgroups(sb);
+        if (l == 0) atomicAdd(&s_g[MLP_OFF_B3 + o], sb);
+    }
+    // dz2 = (dy W3) (1 - h2^2), C layout
+    float dz2[4][4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const float w0 = bf16r(P[MLP_OFF_W3 + 0 * MLP_H + 16 * n + c0]);
+        const float w1 = bf16r(P[MLP_OFF_W3 + 1 * MLP_H + 16 * n + c0]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float dh = dy[i][0] * w0 + dy[i][1] * w1;
+            dz2[n][i] = dh * (1.0f - f.h2[n][i] * f.h2[n][i]);
+        }
+    }
+    // db2 = sum_rows dz2; dW2 = dz2^T h1 (MFMA: A = dz2 tile m (C layout), B = h1 tile n)
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        float s = dz2[n][0] + dz2[n][1] + dz2[n][2] + dz2[n][3];
+        s = sum_rowgroups(s);
+        if (rg == 0) atomicAdd(&s_g[MLP_OFF_B2 + 16 * n + c0], s);
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        mlp_bf16x4 a = pack4(dz2[m][0], dz2[m][1], dz2[m][2], dz2[m][3]);
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            mlp_bf16x4 b = pack4(f.h1[n][0], f.h1[n][1], f.h1[n][2], f.h1[n][3]);
+            mlp_f32x4 acc = mfma16(a, b, mlp_f32x4{0.f, 0.f, 0.f, 0.f});
+            // acc: rows = W2 row index 16 m + 4 rg + i, col = 16 n + c0
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                atomicAdd(&s_g[MLP_OFF_W2 + (16 * m + 4 * rg + i) * MLP_H + 16 * n + c0], acc[i]);
+        }
+    }
+    // dh1 = dz2 W2: A = dz2 in A layout (via LDS), B[k][c] = W2[k][16 n + c]
+#pragma unroll
+    for (int n = 0; n < 4; n++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) s_h[wv][4 * rg + i][16 * n + c0] = dz2[n][i];
+    __syncthreads();
+    mlp_f32x4 dh1[4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) dh1[n] = mlp_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int kb = 16 * s + 4 * rg;
+        mlp_bf16x4 a = pack4(s_h[wv][c0][kb + 0], s_h[wv][c0][kb + 1], s_h[wv][c0][kb + 2], s_h[wv][c0][kb + 3]);
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            const float *w = P + MLP_OFF_W2 + 16 * n + c0;
+            mlp_bf16x4 b = pack4(w[(kb + 0) * MLP_H], w[(kb + 1) * MLP_H], w[(kb + 2) * MLP_H], w[(kb + 3) * MLP_H]);
+            dh1[n] = mfma16(a, b, dh1[n]);
+        }
+    }
+    // dz1 = dh1 (1 - h1^2); dW1[col][k] = sum_rows dz1 * bf16(x); db1 = sum_rows dz1
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        float dz1[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) dz1[i] = dh1[n][i] * (1.0f - f.h1[n][i] * f.h1[n][i]);
+        float sb = dz1[0] + dz1[1] + dz1[2] + dz1[3];
+        sb = sum_rowgroups(sb);
+        if (rg == 0) atomicAdd(&s_g[MLP_OFF_B1 + 16 * n + c0], sb);
+#pragma unroll
+        for (int k = 0; k < MLP_IN; k++) {
+            float s = 0.0f;
+#pragma unroll
+            for (int i = 0; i < 4; i++) s = s + dz1[i] * bf16r(s_x[wv][4 * rg + i][k]);
+            s = sum_rowgroups(s);
+            if (rg == 0) atomicAdd(&s_g[MLP_OFF_W1 + (16 * n + c0) * MLP_IN + k], s);
+        }
+    }
+    if (l == 0) s_loss[wv] = loss;
+    __syncthreads();
+    for (int q = threadIdx.x; q < MLP_NPARAM; q += blockDim.x) part[(int64_t)blockIdx.x * MLP_NPARAM + q] = s_g[q];
+    if (threadIdx.x == 0) {
+        float s = 0.0f;
+        for (int w = 0; w < MLP_WAVES; w++) s += s_loss[w];
+        part_loss[blockIdx.x] = s * inv_b;
+    }
+}
+
+// Sum the workgroup partials (fixed order) into grad, then (if adam) one Adam step on P.
+__global__ void k_mlp_reduce_adam(const float *__restrict__ part, const float *__restrict__ part_loss, int nblk,
+                                  float *grad, float *loss_out, float *P, float *m1, float *m2, float lr,
+                                  float beta1, float beta2, float eps, float c1, float c2, int adam) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q == 0) {
+        float s = 0.0f;
+        for (int b = 0; b < nblk; b++) s += part_loss[b];
+        *loss_out = s;
+    }
+    if (q >= MLP_NPARAM) return;
+    float g = 0.0f;
+    for (int b = 0; b < nblk; b++) g += part[(int64_t)b * MLP_NPARAM + q];
+    grad[q] = g;
+    if (adam) {
+        const float a = beta1 * m1[q] + (1.0f - beta1) * g;
+        const float v = beta2 * m2[q] + (1.0f - beta2) * g * g;
+        m1[q] = a;
+        m2[q] = v;
+        P[q] = P[q] - lr * (a * c1) / (sqrtf(v * c2) + eps);
+    }
+}
+
+// Inference: y[n][2] for x[n][4].
+__global__ __launch_bounds__(64 * MLP_WAVES) void k_mlp_forward(const float *__restrict__ P, const float *__restrict__ X,
+                                                              int64_t nsamp, float *Y) {
+    __shared__ float s_x[MLP_WAVES][16][MLP_IN];
+    __shared__ float s_h[MLP_WAVES][16][MLP_H];
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int c0 = l & 15, rg = l >> 4;
+    const int64_t base = ((int64_t)blockIdx.x * MLP_WAVES + wv) * 16;
+    for (int q = l; q < 16 * MLP_IN; q += 64) {
+        const int64_t r = base + q / MLP_IN;
+        s_x[wv][q / MLP_IN][q % MLP_IN] = r < nsamp ? X[r * MLP_IN + q % MLP_IN] : 0.0f;
+    }
+    __syncthreads();
+    MlpFwd f;
+    mlp_forward_wave(P, s_x[wv], s_h[wv], f);
+    if (c0 == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int64_t r = base + 4 * rg + i;
+            if (r < nsamp)
+                for (int o = 0; o < MLP_OUT; o++) Y[r * MLP_OUT + o] = f.y[i][o];
+        }
+    }
+}
+
+}  // namespace cv
+
+namespace cv {
+
+// Input features of the seed network for the conserved state (T^tt, |T^ti|, T^t eta):
+// log2 T^tt / 16, |T^ti| / T^tt, T^t eta / T^tt, and the octave of T^tt (the table-row
+// feature) / 24.  Outputs: e / T^tt and |v|.
+__device__ __forceinline__ void seed_features(float T00, float M, float T03, float x[MLP_IN]) {
+    const float t = fmaxf(T00, 1e-30f);
+    int ex;
+    (void)frexpf(t, &ex);
+    x[0] = log2f(t) * (1.0f / 16.0f);
+    x[1] = M / t;
+    x[2] = T03 / t;
+    x[3] = (float)(ex - 1) * (1.0f / 24.0f);
+}
+
+__global__ void k_seed_features(const float *__restrict__ T00, const float *__restrict__ M,
+                                const float *__restrict__ T03, int64_t n, float *X) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    float x[MLP_IN];
+    seed_features(T00[q], M[q], T03[q], x);
+    for (int k = 0; k < MLP_IN; k++) X[q * MLP_IN + k] = x[k];
+}
+
+}  // namespace cv
+
+namespace cv {
+// out[q] = in[q * stride + col]
+__global__ void k_take_col(const float *__restrict__ in, int64_t n, int stride, int col, float *out) {
+    int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) out[q] = in[q * stride + col];
+}
+}  // namespace cv

@@ -236,6 +236,39 @@ int cv_batch_last_elapsed_ms(cv_batch *b, float *ms);
 /* events, loop index of the next step, work items (tile column x x-chunk) per stage */
 int cv_batch_info(cv_batch *b, int32_t *nevents, int64_t *loop, int64_t *nwork);
 
+/* ---- learned seed of the conserved-to-primitive root solve ---------------------------
+ * No reference counterpart: the reference starts every root solve at |v| = 0.5
+ * (helper.h:128-167).  A 4-64-64-2 MLP (tanh) maps the features (log2 T^tt / 16,
+ * |T^ti| / T^tt, T^t eta / T^tt, octave of T^tt / 24) to (e / T^tt, |v|); the root solve
+ * starts at the predicted |v| and keeps its bracket [0, 1] and stopping rule, so the
+ * network can only change where the iteration starts.  Training and inference run on the
+ * device: BF16 operands on MFMA, FP32 accumulation, FP32 master weights and Adam.
+ * Parameter vector (MLP_NPARAM = 4610 floats): W1[64][4], b1[64], W2[64][64], b2[64],
+ * W3[2][64], b3[2]. */
+typedef struct cv_mlp cv_mlp;
+int cv_mlp_nparam(void);
+int cv_mlp_create(int device, uint64_t seed, cv_mlp **out);   /* Glorot-uniform weights, zero biases */
+int cv_mlp_destroy(cv_mlp *m);
+int cv_mlp_get_params(cv_mlp *m, float *host, int n);
+int cv_mlp_set_params(cv_mlp *m, const float *host, int n);   /* also resets Adam */
+/* Gradient of the loss mean_samples sum_o (y_o - t_o)^2 at the current parameters
+ * (X: n x 4 features, T: n x 2 targets); no update. */
+int cv_mlp_grad(cv_mlp *m, const float *X, const float *T, int64_t n, float *grad, float *loss);
+/* Training set to the device; cv_mlp_train runs Adam (beta 0.9, 0.999, eps 1e-8) on
+ * consecutive batches of it (wrapping), the loss of each step into loss_hist. */
+int cv_mlp_set_data(cv_mlp *m, const float *X, const float *T, int64_t n);
+int cv_mlp_train(cv_mlp *m, int steps, int batch, float lr, float *loss_hist);
+int cv_mlp_forward(cv_mlp *m, const float *X, int64_t n, float *Y);
+/* Root solve of given states with the handle's EOS, plain (m NULL) or seeded: e, |v|,
+ * iterations. */
+int cv_root_eval(cv_handle *h, cv_mlp *m, const float *T00, const float *M, const float *T03, int64_t n,
+                 float *e_out, float *v_out, int32_t *iters);
+/* Ideal stages of the handle (cv_ideal_stage, cv_ideal_steps_async) seed their root solve
+ * with m (NULL: the reference's start); track_iters keeps the iteration count per cell
+ * of the last stage for cv_get_root_iters. */
+int cv_set_root_seed(cv_handle *h, cv_mlp *m, int track_iters);
+int cv_get_root_iters(cv_handle *h, int32_t *host, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -280,10 +280,15 @@ static real max_prop_speed(const orc_params *p, const real *edv, real vk) {
 
 /* rootFinding_newton (helper.h:128-167) with P_eff(e) = P(e) + bulk; clamp0 (bulk
  * runs) holds P_eff >= 0 so that v = M / (T00 + P_eff) stays below 1. */
+static real root_newton_v0(const orc_params *p, real T00, real M, real bulk, int clamp0, real v0, int *iters);
 static real root_newton(const orc_params *p, real T00, real M, real bulk, int clamp0, int *iters) {
+    return root_newton_v0(p, T00, M, bulk, clamp0, RC(0.5f) * (RC(0.0f) + RC(1.0f)), iters);
+}
+/* the same iteration from a given start v0 (the learned seed; no reference counterpart) */
+static real root_newton_v0(const orc_params *p, real T00, real M, real bulk, int clamp0, real v0, int *iters) {
     real vl = RC(0.0f), vh = RC(1.0f);
     real dpe = RC(1.0f) / RC(3.0f);
-    real v = RC(0.5f) * (vl + vh);
+    real v = v0;
     real ed = T00 - M * v;
     real pr = eos_P(p, ed) + bulk;
     if (clamp0) pr = r_max(pr, RC(0.0f));
@@ -420,6 +425,16 @@ void orc_root(const orc_params *p, const real *T00, const real *M, int64_t n, re
     for (int64_t i = 0; i < n; i++) {
         int it = 0;
         ed[i] = solve_ed(p, T00[i], M[i], RC(0.0f), 0, &it);
+        if (iters) iters[i] = it;
+    }
+}
+void orc_root_from(const orc_params *p, const real *T00, const real *M, const real *v0, int64_t n, real *ed,
+                   int32_t *iters) {
+    for (int64_t i = 0; i < n; i++) {
+        int it = 0;
+        real s = v0[i] > RC(0.0f) ? v0[i] : RC(0.0f);
+        s = s < RC(0.9999f) ? s : RC(0.9999f);
+        ed[i] = root_newton_v0(p, T00[i], M[i], RC(0.0f), 0, s, &it);
         if (iters) iters[i] = it;
     }
 }

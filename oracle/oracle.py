@@ -226,6 +226,19 @@ def root_solve(params, T00, M, precision='f32'):
     return ed, it
 
 
+def root_solve_from(params, T00, M, v0, precision='f32'):
+    """The root solve started at v0 (clamped to [0, 0.9999]), as the seeded device solve."""
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    T00 = np.ascontiguousarray(T00, dt)
+    M = np.ascontiguousarray(M, dt)
+    v0 = np.ascontiguousarray(v0, dt)
+    ed = np.zeros(T00.size, dt)
+    it = np.zeros(T00.size, np.int32)
+    L.orc_root_from(C.byref(params), _ptr(T00), _ptr(M), _ptr(v0), C.c_int64(T00.size), _ptr(ed), _ptr(it))
+    return ed, it
+
+
 def max_prop_speed(params, ev, vk, precision='f32'):
     L = lib(precision)
     dt = np.float32 if precision == 'f32' else np.float64
