@@ -37,6 +37,23 @@ ALGO_BYTES = {False: [192, 244], True: [204, 260]}
 LAUNCH_NAMES = ['visc_stage1', 'visc_stage2']
 
 
+PROFILE = os.path.join(HERE, 'profiles', 'k_visc_tiled_visc_bulk_385x385x115.md')
+
+
+def profiled_traffic(kernel):
+    """HBM-side bytes per dispatch of a stage kernel from the committed rocprofv3 counter
+    summary (FETCH_SIZE x 2.00 + WRITE_SIZE, calibrated on a device copy), or None."""
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == 'traffic_bytes' and w[1] == kernel:
+                    return float(w[2])
+    except OSError:
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr)
     sys.stderr.flush()
@@ -200,6 +217,69 @@ def run_ensemble(args, p, tab, name):
     batch.close()
 
 
+def run_surrogate(args, name):
+    """Learned root-solve seed on the ideal Gubser workload (201 x 201 x 1, 200 steps): states
+    from the plain run, device training (BF16 MFMA), then the seeded run; iterations per cell,
+    field agreement with the plain run and with the analytic flow."""
+    from clvisc_amd import ic, surrogate
+    from clvisc_amd.eos import Eos
+    nsteps = args.steps if args.steps != 100 else 200
+    t0 = time.perf_counter()
+    states, it_plain = surrogate.gubser_run_states(nsteps=nsteps, nx=201, every=1)
+    t_plain = time.perf_counter() - t0
+    log('plain run: %d steps in %.2fs (with state collection), %d states, mean iterations %.3f'
+        % (nsteps, t_plain, states[0].size, it_plain.mean()))
+    t0 = time.perf_counter()
+    mlp, hist = surrogate.train_seed(states, steps=2000, batch=4096, lr=3e-3, nmax=5_000_000)
+    t_train = time.perf_counter() - t0
+    log('trained 2000 steps of 4096 in %.2fs: loss %.4g -> %.4g' % (t_train, hist[:20].mean(), hist[-20:].mean()))
+    t0 = time.perf_counter()
+    st_seed, it_seed, h_seed = surrogate.gubser_run_states(nsteps=nsteps, nx=201, every=1, seed_mlp=mlp,
+                                                           return_handle=True)
+    t_seed = time.perf_counter() - t0
+    log('seeded run: %d steps in %.2fs, mean iterations %.3f' % (nsteps, t_seed, it_seed.mean()))
+    eos = Eos('ideal_gas')
+    ncell = 201 * 201
+    e_p, e_s = states[3][-ncell:], st_seed[3][-ncell:]
+    big = e_p > 1e-3 * e_p.max()
+    field_rel = float((np.abs(e_s[big] - e_p[big]) / e_p[big]).max())
+    tau = 1.0 + nsteps * 0.02
+    ea, norm = ic.gubser_ideal(201, 201, 1, 0.08, 0.08, 0.15, tau, q=1.0)
+    _, norm0 = ic.gubser_ideal(201, 201, 1, 0.08, 0.08, 0.15, 1.0, q=1.0, e0=float(eos.f_ed(0.3)))
+    ea = ea[:, 0] * norm0
+    x = (np.arange(201) - 100) * 0.08
+    X, Y = np.meshgrid(x, x, indexing='ij')
+    r = np.sqrt(X * X + Y * Y).ravel() < 4.0
+    l1 = lambda e: float(np.abs(e[r] - ea[r]).sum() / ea[r].sum())
+    log('field: max rel |e_seeded - e_plain| %.3g; L1 vs analytic (r < 4 fm) plain %.4g seeded %.4g'
+        % (field_rel, l1(e_p), l1(e_s)))
+    # plain vs seeded step time (device, 50 steps each, no state collection)
+    h_seed.set_root_seed(None, track_iters=False)
+    h_seed.ideal_steps(nsteps, 50, sync=True)
+    ms_plain = h_seed.last_elapsed_ms() / 50
+    h_seed.set_root_seed(mlp, track_iters=False)
+    h_seed.ideal_steps(nsteps + 50, 50, sync=True)
+    ms_seed = h_seed.last_elapsed_ms() / 50
+    log('step time: plain %.4f ms, seeded %.4f ms' % (ms_plain, ms_seed))
+    out = {
+        'metric': 'root-solve iterations per cell, learned seed vs plain (ideal Gubser 201x201, 200 steps)',
+        'value': float(it_seed.mean()), 'unit': 'iterations/cell', 'n_gpus': 1,
+        'steps': nsteps, 'warmup': 0, 'ms_per_step': ms_seed,
+        'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32 (network bf16 MFMA, fp32 accumulate)', 'data': 'synthetic',
+        'config': {'workload': 'ideal Gubser flow 201x201x1, q=1/fm, T0=0.3 GeV, dtau 0.02, 200 steps; MLP '
+                               '4-64-64-2 tanh trained 2000 steps x 4096 on <=5e6 states of the plain run',
+                   'device': name, 'plain_iterations': float(it_plain.mean()),
+                   'seeded_max_iterations': int(it_seed.max()), 'plain_max_iterations': int(it_plain.max()),
+                   'train_seconds': t_train, 'loss_first': float(hist[:20].mean()),
+                   'loss_last': float(hist[-20:].mean()),
+                   'max_rel_e_seeded_vs_plain': field_rel, 'l1_vs_analytic_plain': l1(e_p),
+                   'l1_vs_analytic_seeded': l1(e_s), 'ms_per_step_plain': ms_plain},
+    }
+    print(json.dumps(out))
+    h_seed.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -208,8 +288,9 @@ def main():
     ap.add_argument('--no-bulk', action='store_true', help='shear viscosity only')
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-threads', type=int, default=0)
-    ap.add_argument('--workload', choices=['single', 'ensemble'], default='single',
-                    help='single: one 385x385x115 event (the headline); ensemble: 64 uneven events batched')
+    ap.add_argument('--workload', choices=['single', 'ensemble', 'surrogate'], default='single',
+                    help='single: one 385x385x115 event (the headline); ensemble: 64 uneven events batched; '
+                         'surrogate: learned root-solve seed on the ideal Gubser workload')
     ap.add_argument('--autotune', action='store_true', help='ensemble: tune tiles and chunks per grid shape')
     ap.add_argument('--autotune-cache', default='', help='ensemble: JSON file of tuned shapes')
     args = ap.parse_args()
@@ -220,6 +301,11 @@ def main():
 
     from clvisc_amd import _lib
     bulk = not args.no_bulk
+    if args.workload == 'surrogate':
+        name, cus = _lib.device_info(0)
+        log('device: %s, %d CUs' % (name, cus))
+        run_surrogate(args, name)
+        return
     if args.workload == 'ensemble':
         p, tab, _ = make_workload(bulk, ic=False)
         name, cus = _lib.device_info(0)
@@ -264,7 +350,7 @@ def main():
     dom = int(np.argmax(prof))
     roof = {'bound': 'hbm', 'kernel': LAUNCH_NAMES[dom], 'achieved': round(per[dom], 2),
             'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': round(per[dom] / HBM_PEAK_GBS, 5),
-            'traffic': None,
+            'traffic': profiled_traffic(LAUNCH_NAMES[dom]) if bulk else None,
             'step_algorithmic_bytes_per_cell': sum(abytes),
             'step_frac': round(sum(abytes) * value / 1e9 / HBM_PEAK_GBS, 5)}
     cpu = None
