@@ -215,3 +215,26 @@ def test_gubser_visc_analytic_501():
            np.abs(pi[m, 9] / pa[m, 9] - 1).max(), np.abs(pi[m, 4] - pa[m, 4]).max() / np.abs(pa[m, 4]).max())
     _say('  r<2: e max rel %.3g, vx max abs %.3g, pizz max rel %.3g, pixx max abs/scale %.3g' % err)
     assert err[0] < 0.01 and err[1] < 0.003 and err[2] < 0.04 and err[3] < 0.03
+
+
+def test_full_size_benchmark_grid_bitwise():
+    """The 385x385x115 benchmark workload (viscous + bulk, default marching kernel) for two
+    steps against the oracle on the same grid: identical fields."""
+    import bench
+    _say('case: full-size 385x385x115 viscous+bulk, 2 steps vs oracle')
+    p, tab, ev = bench.make_workload(True)
+    h = _lib.Handle(p, tab, viscous=True)
+    _say('  stage kernel %s march chunk %s' % (h.stage_kernel(), h.march_chunk()))
+    h.visc_initialize(p.tau0)
+    h.ideal_stage(1, np.float32(p.tau0))
+    h.set('ev1', ev)
+    h.visc_steps(0, 2)
+    o = O.Oracle(orc_params(p, tab, nthreads=16), 'f32', viscous=True)
+    o.load_ini(ev)
+    o.visc_steps(2)
+    for name, r in (('ev1', o.ev[1]), ('pi1', o.pi[1]), ('bulk1', o.bulk[1]), ('ev0', o.ev[0]), ('udiff', o.udiff)):
+        g = h.get(name)
+        frac = bit_equal_frac(g, r)
+        _say('  %-6s bit-equal %.8f' % (name, frac))
+        assert max_rel(g, r).max() <= TOL
+    h.close()
