@@ -217,6 +217,87 @@ def run_ensemble(args, p, tab, name):
     batch.close()
 
 
+MATCHED_REL = 1e-4     # learned form counts as matched when max rel |e_learned - e_plain| <= this
+
+
+def run_learned(args, p, tab, ev, h_plain, loop_end):
+    """The learned form of the same workload: the split stage (stage kernel, then network
+    seed and root solve in their own kernels) seeded by the 4-64-64-2 network trained on the
+    split stage's own states of this run.  Same initial state and step count as the plain
+    run that ended at loop_end on h_plain; returns the 'learned' object of the JSON line."""
+    from clvisc_amd import _lib, surrogate
+    ncell = NX * NY * NZ
+    hl = _lib.Handle(p, tab, viscous=True)
+    hl.visc_initialize(TAU0)
+    hl.ideal_stage(1, np.float32(TAU0))
+    hl.set('ev1', ev)
+    w = args.warmup
+    t0 = time.perf_counter()
+    states = surrogate.visc_run_states(hl, 0, w, every=max(1, w // 2))
+    t_states = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mlp, hist = surrogate.train_seed(states, steps=2000, batch=4096, lr=3e-3, nmax=5_000_000)
+    t_train = time.perf_counter() - t0
+    log('learned: %d states from %d split steps (%.1fs); trained 2000 x 4096 in %.2fs, loss %.4g -> %.4g'
+        % (states[0].size, w, t_states, t_train, hist[:20].mean(), hist[-20:].mean()))
+    del states
+    hl.set_root_seed(mlp, track_iters=False)
+    nrun = loop_end - w
+    hl.sync()
+    t0 = time.perf_counter()
+    hl.visc_steps(w, nrun, sync=False)
+    hl.sync()
+    wall = time.perf_counter() - t0
+    value = ncell * nrun / wall
+    # matched error: the fields of the two runs after the same steps from the same start
+    e_p, e_l = h_plain.get('ev1')[:, 0], hl.get('ev1')[:, 0]
+    big = e_p > 1e-3 * e_p.max()
+    rel = float((np.abs(e_l[big] - e_p[big]) / e_p[big]).max())
+    # iterations per cell of one more step each (tracked), at the same loop index
+    hl.set_root_seed(mlp, track_iters=True)
+    hl.visc_steps(loop_end, 1)
+    it_l = hl.root_iters()
+    h_plain.set_root_split(True)
+    h_plain.set_root_seed(None, track_iters=True)
+    h_plain.visc_steps(loop_end, 1)
+    it_p = h_plain.root_iters()
+    # the split form without the seed (the network's share of the learned step time)
+    nsplit = max(2, min(20, nrun))
+    h_plain.set_root_seed(None, track_iters=False)
+    h_plain.sync()
+    t0 = time.perf_counter()
+    h_plain.visc_steps(loop_end + 1, nsplit, sync=False)
+    h_plain.sync()
+    split_value = ncell * nsplit / (time.perf_counter() - t0)
+    log('learned: %d steps %.3fs -> %.3f ms/step, %.4g cell-updates/s; split without seed %.4g; '
+        'iterations plain %.3f learned %.3f; max rel e vs plain %.3g'
+        % (nrun, wall, 1000 * wall / nrun, value, split_value, it_p.mean(), it_l.mean(), rel))
+    # FP32 sensitivity of the plain run itself: the same steps from the initial field with
+    # e scaled by (1 + 2^-23) (one ulp), compared with the plain run at loop_end
+    hl.set_root_seed(None, track_iters=False)
+    hl.set_root_split(False)
+    hl.visc_initialize(TAU0)
+    hl.ideal_stage(1, np.float32(TAU0))
+    evp = np.array(ev, np.float32, copy=True)
+    evp[:, 0] *= np.float32(1.0 + 2.0 ** -23)
+    hl.set('ev1', evp)
+    del evp
+    hl.visc_steps(0, loop_end)
+    e_u = hl.get('ev1')[:, 0]
+    rel_ulp = float((np.abs(e_u[big] - e_p[big]) / e_p[big]).max())
+    log('plain run from the initial field perturbed by one ulp: max rel e vs plain %.3g after %d steps'
+        % (rel_ulp, loop_end))
+    hl.close()
+    mlp.close()
+    return {'value': value, 'unit': 'cell-updates/s', 'ms_per_step': 1000.0 * wall / nrun, 'steps': nrun,
+            'max_rel_e_vs_plain': rel, 'matched': bool(rel <= MATCHED_REL), 'matched_tolerance': MATCHED_REL,
+            'plain_one_ulp_ic_max_rel_e': rel_ulp,
+            'iterations_plain': float(it_p.mean()), 'iterations_learned': float(it_l.mean()),
+            'split_unseeded_value': split_value, 'train_seconds': t_train,
+            'loss_first': float(hist[:20].mean()), 'loss_last': float(hist[-20:].mean()),
+            'learned_over_plain': None}
+
+
 def run_surrogate(args, name):
     """Learned root-solve seed on the ideal Gubser workload (201 x 201 x 1, 200 steps): states
     from the plain run, device training (BF16 MFMA), then the seeded run; iterations per cell,
@@ -287,6 +368,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=5)
     ap.add_argument('--no-bulk', action='store_true', help='shear viscosity only')
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-learned', action='store_true',
+                    help='single: skip the learned (network-seeded root solve) form of the run')
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--workload', choices=['single', 'ensemble', 'surrogate'], default='single',
                     help='single: one 385x385x115 event (the headline); ensemble: 64 uneven events batched; '
@@ -353,6 +436,10 @@ def main():
             'traffic': profiled_traffic(LAUNCH_NAMES[dom]) if bulk else None,
             'step_algorithmic_bytes_per_cell': sum(abytes),
             'step_frac': round(sum(abytes) * value / 1e9 / HBM_PEAK_GBS, 5)}
+    learned = None
+    if not args.no_learned:
+        learned = run_learned(args, p, tab, ev, h, loop)
+        learned['learned_over_plain'] = learned['value'] / value
     cpu = None
     if not args.no_cpu_baseline:
         nth = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -372,6 +459,7 @@ def main():
                    'grid': [NX, NY, NZ], 'device': name, 'physical': ok,
                    'edmax_end': float(edmax)},
         'roofline': roof,
+        'learned': learned,
         'cpu_baseline': cpu,
     }
     print(json.dumps(out))

@@ -21,8 +21,8 @@ CV_FLAG_SF_FP64 = 0x10
 
 FIELDS = {'ev0': 0, 'ev1': 1, 'ev2': 2, 'pi0': 3, 'pi1': 4, 'pi2': 5,
           'bulk0': 6, 'bulk1': 7, 'bulk2': 8, 'src': 9, 'is_src': 10,
-          'udiff': 11, 'goodcell': 12}
-NCOMP = {0: 4, 1: 4, 2: 4, 3: 10, 4: 10, 5: 10, 6: 1, 7: 1, 8: 1, 9: 4, 10: 10, 11: 4, 12: 1}
+          'udiff': 11, 'goodcell': 12, 'root_cons': 13, 'root_seed': 14}
+NCOMP = {0: 4, 1: 4, 2: 4, 3: 10, 4: 10, 5: 10, 6: 1, 7: 1, 8: 1, 9: 4, 10: 10, 11: 4, 12: 1, 13: 4, 14: 1}
 
 
 class CvParams(C.Structure):
@@ -117,6 +117,7 @@ def lib():
             'cv_mlp_forward': [vp, vp, i64, vp],
             'cv_root_eval': [vp, vp, vp, vp, vp, i64, vp, vp, vp],
             'cv_set_root_seed': [vp, vp, C.c_int],
+            'cv_set_root_split': [vp, C.c_int],
             'cv_get_root_iters': [vp, vp, i64],
         }
         for name, args in sig.items():
@@ -147,7 +148,7 @@ def exported_symbols():
             'cv_batch_steps_async', 'cv_batch_set_loop', 'cv_batch_sync', 'cv_batch_last_elapsed_ms',
             'cv_batch_info', 'cv_mlp_nparam', 'cv_mlp_create', 'cv_mlp_destroy', 'cv_mlp_get_params',
             'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward',
-            'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters']
+            'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split']
 
 
 def make_eos(table):
@@ -282,6 +283,11 @@ class Handle(object):
     # -- learned root-solve seed
     def set_root_seed(self, mlp, track_iters=True):
         check(self.L.cv_set_root_seed(self.h, mlp.m if mlp is not None else None, int(track_iters)))
+
+    def set_root_split(self, on=True):
+        """Viscous stages in split form (stage, then seed and root solve in their own
+        kernels); the fields root_cons / root_seed hold the last stage's solve input."""
+        check(self.L.cv_set_root_split(self.h, int(on)))
 
     def root_iters(self):
         out = np.empty(self.n, np.int32)

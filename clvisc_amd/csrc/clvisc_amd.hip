@@ -14,6 +14,19 @@
 #include "cv_kernels.h"
 #include "cv_tiled.h"
 #include "cv_hypersf.h"
+#include "cv_mlp.h"
+
+// learned root-solve seed network (parameters, Adam moments, training set)
+struct cv_mlp {
+    int device = 0;
+    float *P = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr, *loss = nullptr;
+    float *part = nullptr, *part_loss = nullptr;
+    int64_t part_blocks = 0;
+    float *X = nullptr, *T = nullptr;       // training set
+    int64_t ndata = 0;
+    int64_t adam_t = 0;
+    hipStream_t stream = nullptr;
+};
 
 using namespace cv;
 
@@ -63,6 +76,8 @@ struct cv_handle {
     struct cv_mlp *seed_mlp = nullptr;
     bool track_iters = false;
     float *seed_buf = nullptr;  // 10 n floats: T00, M, T03, features (4), outputs (2), seed
+                                // (split viscous stage: cons planes 0..3, seed at 9 n)
+    bool root_split = false;    // viscous stages in split form (cv_set_root_split)
     int *iters = nullptr;
     // freeze-out surface: fields at the previous surface time, appended elements
     float *sf_ev = nullptr, *sf_pi = nullptr, *sf_bk = nullptr;
@@ -289,6 +304,8 @@ static int field_ptr(cv_handle *h, int f, float **ptr, int *ncomp) {
     case CV_IS_SRC: *ptr = h->IS_src; *ncomp = 10; break;
     case CV_UDIFF: *ptr = h->udiff; *ncomp = 4; break;
     case CV_GOODCELL: *ptr = h->good; *ncomp = 1; break;
+    case CV_ROOT_CONS: *ptr = h->seed_buf; *ncomp = 4; break;
+    case CV_ROOT_SEED: *ptr = h->seed_buf ? h->seed_buf + 9 * h->n : nullptr; *ncomp = 1; break;
     default: return fail(CV_ERR_ARG, "unknown field id %d", f);
     }
     if (!*ptr) return fail(CV_ERR_STATE, "field %d is not allocated for this handle (ideal or no bulk)", f);
@@ -423,6 +440,35 @@ static int launch_kt(cv_handle *h, int step, float tau, float *ev_st, float *pi_
     return CV_OK;
 }
 
+// second half of a split viscous stage: the learned seed (when set) from the conserved
+// input the stage wrote, then the root solve and velocities
+static int launch_visc_root(cv_handle *h, int step, const float *ev1, const float *bk_out, float *ev_out,
+                            float *udiff_out) {
+    const int64_t n = h->n;
+    const float *cons = h->seed_buf;
+    float *seed = nullptr;
+    if (h->seed_mlp) {
+        seed = h->seed_buf + 9 * n;
+        const int64_t nblk = (n + 16 * MLP_WAVES - 1) / (16 * MLP_WAVES);
+        hipLaunchKernelGGL(k_mlp_seed_cons, dim3((unsigned)nblk), dim3(64 * MLP_WAVES), 0, h->stream, h->seed_mlp->P,
+                           cons, n, seed);
+    }
+    int *it = h->track_iters ? h->iters : nullptr;
+#define CV_ROOT_LAUNCH(B, S, K) \
+    hipLaunchKernelGGL((k_visc_root<B, S, K>), grid1d(n), dim3(256), 0, h->stream, h->phys, h->eos, cons, bk_out, \
+                       ev1, seed, ev_out, udiff_out, it)
+#define CV_ROOT_SEL(B, S) do { if (h->eos.kind == 1) CV_ROOT_LAUNCH(B, S, 1); else CV_ROOT_LAUNCH(B, S, 0); } while (0)
+    if (h->bulk) {
+        if (step == 1) CV_ROOT_SEL(true, 1); else CV_ROOT_SEL(true, 2);
+    } else {
+        if (step == 1) CV_ROOT_SEL(false, 1); else CV_ROOT_SEL(false, 2);
+    }
+#undef CV_ROOT_SEL
+#undef CV_ROOT_LAUNCH
+    HIP_TRY(hipGetLastError());
+    return CV_OK;
+}
+
 // fused IS + KT stage of the step path (k_visc_stage)
 static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *pi_st, float *bk_st,
                         float *ev1, float *ev2, float *pi1, float *bk1, float *pi_out, float *bk_out,
@@ -431,10 +477,13 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
     if (h->kernel == 1 || h->kernel == 2) {
         const bool march = h->kernel == 2;
         dim3 g(march ? (unsigned)h->tile.nwork : (unsigned)h->tile_grid), b(CV_TILE_THREADS);
+        const bool split = h->root_split;
         StagePtrs sp = {ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src,
-                        h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out};
+                        h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out,
+                        split ? h->seed_buf : nullptr};
 #define CV_TILED_ARGS h->phys, h->eos, st, h->tile, sp
-        const bool spec = (h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST | CV_FLAG_FP64_ROOT)) != 0;
+        const bool spec = split ||
+                          (h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST | CV_FLAG_FP64_ROOT)) != 0;
         const int ek = h->eos.kind;
 #define CV_TILED_LAUNCH2(B, S, K, X, M) hipLaunchKernelGGL((k_visc_tiled<B, S, K, X, M>), g, b, 0, h->stream, CV_TILED_ARGS)
 #define CV_TILED_LAUNCH(B, S, K, X) \
@@ -452,8 +501,10 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
 #undef CV_TILED_LAUNCH2
 #undef CV_TILED_ARGS
         HIP_TRY(hipGetLastError());
+        if (split) return launch_visc_root(h, step, ev1, bk_out, ev_out, udiff_out);
         return CV_OK;
     }
+    if (h->root_split) return fail(CV_ERR_UNSUPPORTED, "split root solve needs a tiled stage kernel (kind 1 or 2)");
     dim3 g = grid1d(h->n), b(256);
 #define CV_STAGE_ARGS h->phys, h->eos, st, ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src, \
                       h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out
@@ -1034,18 +1085,6 @@ extern "C" int cv_batch_info(cv_batch *b, int32_t *nevents, int64_t *loop, int64
 }
 
 // ---- learned root-solve seed -------------------------------------------------------
-#include "cv_mlp.h"
-
-struct cv_mlp {
-    int device = 0;
-    float *P = nullptr, *m1 = nullptr, *m2 = nullptr, *grad = nullptr, *loss = nullptr;
-    float *part = nullptr, *part_loss = nullptr;
-    int64_t part_blocks = 0;
-    float *X = nullptr, *T = nullptr;       // training set
-    int64_t ndata = 0;
-    int64_t adam_t = 0;
-    hipStream_t stream = nullptr;
-};
 
 static uint64_t mlp_rng(uint64_t &s) {             // SplitMix64
     uint64_t z = (s += 0x9E3779B97F4A7C15ull);
@@ -1313,6 +1352,19 @@ extern "C" int cv_set_root_seed(cv_handle *h, cv_mlp *m, int track_iters) {
     }
     h->seed_mlp = m;
     h->track_iters = track_iters != 0;
+    if (h->viscous && m) h->root_split = true;
+    return CV_OK;
+}
+
+extern "C" int cv_set_root_split(cv_handle *h, int on) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    if (!h->viscous) return fail(CV_ERR_STATE, "cv_set_root_split: viscous handle required");
+    if (!on && h->seed_mlp) return fail(CV_ERR_STATE, "cv_set_root_split: a seed network is set (cv_set_root_seed)");
+    if (on && h->kernel == 0) return fail(CV_ERR_UNSUPPORTED, "cv_set_root_split: needs a tiled stage kernel (kind 1 or 2)");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (on && !h->seed_buf) HIP_TRY(hipMalloc((void **)&h->seed_buf, sizeof(float) * 10 * h->n));
+    h->root_split = on != 0;
     return CV_OK;
 }
 

@@ -78,10 +78,10 @@ __global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchS
         StagePtrs q;
         if (STEP == 1) {
             q = StagePtrs{E1, P1, B1, E1, E2, P1, B1, ev.udiff, ev.IS_src, ev.bsrc, ev.Src, P2, B2, E2, ev.good,
-                          nullptr};
+                          nullptr, nullptr};
         } else {
             q = StagePtrs{E2, P2, B2, E1, E2, P1, B1, ev.udiff, ev.IS_src, ev.bsrc, ev.Src, P0, B0, E0, ev.good,
-                          ev.udiff};
+                          ev.udiff, nullptr};
         }
         tile_work<BULK, STEP, EK, false, true>(ev.p, eos, st, ev.tp, q, w - ev.work0, s_c, s_f);
     }

@@ -339,6 +339,44 @@ __device__ __forceinline__ void seed_features(float T00, float M, float T03, flo
     x[3] = (float)(ex - 1) * (1.0f / 24.0f);
 }
 
+// Seed of the split viscous root solve: the conserved input as the stage wrote it (four
+// planes T^tt, T^tx, T^ty, T^t eta before the |T^ti| <= 0.999 T^tt clamp, as k_visc_root
+// reads them), the clamp, the features, the network; seed_v = the |v| output.
+__global__ __launch_bounds__(64 * MLP_WAVES) void k_mlp_seed_cons(const float *__restrict__ P,
+                                                                const float *__restrict__ cons, int64_t n,
+                                                                float *seed_v) {
+    __shared__ float s_x[MLP_WAVES][16][MLP_IN];
+    __shared__ float s_h[MLP_WAVES][16][MLP_H];
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int c0 = l & 15, rg = l >> 4;
+    const int64_t base = ((int64_t)blockIdx.x * MLP_WAVES + wv) * 16;
+    if (l < 16) {
+        const int64_t r = base + l;
+        float x[MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (r < n) {
+            float T00 = cons[r], T01 = cons[n + r], T02 = cons[2 * n + r], T03 = cons[3 * n + r];
+            float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
+            if (M > T00) {
+                T03 *= (0.999f * T00) * (1.0f / M);
+                M = 0.999f * T00;
+            }
+            seed_features(T00, M, T03, x);
+        }
+#pragma unroll
+        for (int k = 0; k < MLP_IN; k++) s_x[wv][l][k] = x[k];
+    }
+    __syncthreads();
+    MlpFwd f;
+    mlp_forward_wave(P, s_x[wv], s_h[wv], f);
+    if (c0 == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int64_t r = base + 4 * rg + i;
+            if (r < n) seed_v[r] = f.y[i][1];
+        }
+    }
+}
+
 __global__ void k_seed_features(const float *__restrict__ T00, const float *__restrict__ M,
                                 const float *__restrict__ T03, int64_t n, float *X) {
     int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

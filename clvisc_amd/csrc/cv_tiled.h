@@ -115,6 +115,10 @@ struct StagePtrs {
     const float *ev1, *ev2, *pi1, *bk1;        // d_ev[1], d_ev[2], d_pi[1], Pi[1]
     const float *udiff_in;
     float *IS_src, *bsrc, *Src, *pi_out, *bk_out, *ev_out, *good, *udiff_out;
+    // split root solve (SPEC kernels only): when set, the stage writes the root solve's
+    // input (T^tt, T^tx, T^ty, T^t eta before the |T^ti| <= 0.999 T^tt clamp) to four planes
+    // here instead of solving; k_visc_root finishes the cell (ev_out, udiff_out)
+    float *cons_out;
 };
 
 // One work item of the stage.  All threads of the workgroup call it (it synchronises).
@@ -524,6 +528,12 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
             float T01 = fabsf(T0m[1]) < CV_ACU ? 0.0f : T0m[1] * st.r_tau;
             float T02 = fabsf(T0m[2]) < CV_ACU ? 0.0f : T0m[2] * st.r_tau;
             float T03 = fabsf(T0m[3]) < CV_ACU ? 0.0f : T0m[3] * st.r_tau;
+            if (SPEC && q_.cons_out) {
+                stp(q_.cons_out, bo, nb, 0, T00);
+                stp(q_.cons_out, bo, nb, 1, T01);
+                stp(q_.cons_out, bo, nb, 2, T02);
+                stp(q_.cons_out, bo, nb, 3, T03);
+            } else {
             float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
             if (M > T00) {
                 float sc = (0.999f * T00) * rcp(M);
@@ -551,6 +561,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
 #pragma unroll
                 for (int c = 0; c < 4; c++) stp(udiff_out, bo, nb, c, un[c] - u_old[c]);
             }
+            }
         }
         __syncthreads();   // LDS reused by the next plane / tile
       }
@@ -574,6 +585,58 @@ __global__ __launch_bounds__(CV_TILE_THREADS, MARCH ? 3 : 2) void k_visc_tiled(P
     EosT<EK> eos(E, tab);
     for (int64_t work = blockIdx.x; work < (MARCH ? tp.nwork : tp.ntiles); work += gridDim.x)
         tile_work<BULK, STEP, EK, SPEC, MARCH>(p, eos, st, tp, q, work, s_c, s_f);
+}
+
+// The end of update_ev for the split stage (kernel_visc.cl:322-393 from the clamp on):
+// reads the conserved input the stage wrote to cons (four planes), Pi_new (bk_out) and,
+// at STEP 2, d_ev[1] for udiff; solves for e from seed_v (when given) or from the
+// reference's start; writes ev_out, udiff_out (STEP 2) and the iteration count.  The same
+// FP32 operations as the fused stage, so the split stage without a seed is bit-identical.
+template <bool BULK, int STEP, int EK>
+__global__ __launch_bounds__(256) void k_visc_root(PhysP p, EosP E, const float *__restrict__ cons,
+                                                   const float *__restrict__ bk_out, const float *__restrict__ ev1,
+                                                   const float *__restrict__ seed_v, float *ev_out, float *udiff_out,
+                                                   int *iters_out) {
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= p.n) return;
+    const int64_t n = p.n;
+    EosT<EK> eos(E, E.tab);
+    float T00 = cons[I], T01 = cons[n + I], T02 = cons[2 * n + I], T03 = cons[3 * n + I];
+    const float bout = BULK ? bk_out[I] : 0.0f;
+    float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
+    if (M > T00) {
+        float sc = (0.999f * T00) * rcp(M);
+        T01 *= sc; T02 *= sc; T03 *= sc;
+        M = 0.999f * T00;
+    }
+    float ed_find;
+    int it = 0;
+    if (seed_v)
+        ed_find = root_newton_from(eos, T00, M, bout, BULK, fminf(fmaxf(seed_v[I], 0.0f), 0.9999f), &it);
+    else if ((p.flags & 8) && M > p.fp64_vmin * T00)
+        ed_find = (float)root_newton_d(E, E.tab, (double)T00, (double)M, (double)bout, BULK);
+    else
+        ed_find = root_newton_from(eos, T00, M, bout, BULK, 0.5f * (0.0f + 1.0f), &it);
+    if (iters_out) iters_out[I] = it;
+    if (ed_find < CV_ACU) { T01 = 0.0f; T02 = 0.0f; T03 = 0.0f; }
+    ed_find = rmax(0.0f, ed_find);
+    float pr = eos.P(ed_find) + bout;
+    if (BULK) pr = rmax(pr, 0.0f);
+    float repv = rcp(rmax(CV_ACU, T00 + pr));
+    float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
+    ev_out[I] = en.x;
+    ev_out[n + I] = en.y;
+    ev_out[2 * n + I] = en.z;
+    ev_out[3 * n + I] = en.w;
+    if (STEP == 2) {
+        const float v1x = ev1[n + I], v1y = ev1[2 * n + I], v1z = ev1[3 * n + I];
+        const float g1 = gamma3(v1x, v1y, v1z);
+        const float u_old[4] = {g1 * 1.0f, g1 * v1x, g1 * v1y, g1 * v1z};
+        float gn = gamma3(en.y, en.z, en.w);
+        float un[4] = {gn * 1.0f, gn * en.y, gn * en.z, gn * en.w};
+#pragma unroll
+        for (int c = 0; c < 4; c++) udiff_out[c * n + I] = un[c] - u_old[c];
+    }
 }
 
 }  // namespace cv

@@ -73,6 +73,42 @@ def gubser_run_states(nsteps=200, nx=201, every=1, device=0, seed_mlp=None, retu
     return out
 
 
+def clamp_cons(cons):
+    """(T00, M, T03) of the split stage's conserved input (n x 4, before the clamp) after the
+    |T^ti| <= 0.999 T^tt clamp of the root solve (k_visc_root / k_mlp_seed_cons)."""
+    c = np.asarray(cons, np.float32).reshape(-1, 4)
+    T00, T03 = c[:, 0].copy(), c[:, 3].copy()
+    M = np.sqrt(c[:, 1] * c[:, 1] + c[:, 2] * c[:, 2] + c[:, 3] * c[:, 3]).astype(np.float32)
+    big = M > T00
+    sc = (np.float32(0.999) * T00[big]) * (np.float32(1.0) / M[big])
+    T03[big] = T03[big] * sc
+    M[big] = np.float32(0.999) * T00[big]
+    return T00, M, T03
+
+
+def split_states(h):
+    """(T00, M, T03, e, |v|) of every cell of the last split viscous stage of handle h:
+    its conserved input (field root_cons) and the solution it stepped to (ev1 after a full
+    step).  The training data of the viscous seed."""
+    T00, M, T03 = clamp_cons(h.get('root_cons'))
+    ev = h.get('ev1')
+    v = np.sqrt((ev[:, 1:].astype(np.float64) ** 2).sum(1)).astype(np.float32)
+    return T00, M, T03, ev[:, 0].copy(), v
+
+
+def visc_run_states(h, n0, nsteps, every=1):
+    """Run nsteps viscous steps of handle h (split form, no seed) from loop index n0,
+    collecting split_states every `every` steps."""
+    h.set_root_split(True)
+    cols = [[], [], [], [], []]
+    for k in range(nsteps):
+        h.visc_steps(n0 + k, 1)
+        if (k + 1) % every == 0:
+            for c, a in zip(cols, split_states(h)):
+                c.append(a)
+    return tuple(np.concatenate(c) for c in cols)
+
+
 def train_seed(states, steps=2000, batch=4096, lr=2e-3, nmax=5_000_000, seed=1234, device=0, lr_decay=True):
     """Train the seed network on (a random subset of at most nmax) states.  Returns
     (mlp, loss history)."""

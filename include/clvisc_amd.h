@@ -85,7 +85,10 @@ enum cv_field {
     CV_IS_SRC = 10,                            /* visc.d_IS_src      (visc.py:43)       */
     CV_UDIFF = 11,                             /* visc.d_udiff       (visc.py:83)       */
     CV_GOODCELL = 12,                          /* visc.d_goodcell    (visc.py:87-88)    */
-    CV_NFIELDS = 13
+    CV_ROOT_CONS = 13,                         /* split root solve: its conserved input
+                                                  (T^tt, T^tx, T^ty, T^t eta), no counterpart */
+    CV_ROOT_SEED = 14,                         /* split root solve: the learned seed |v|    */
+    CV_NFIELDS = 15
 };
 
 /* Last error message of the calling thread (never NULL). */
@@ -267,6 +270,13 @@ int cv_root_eval(cv_handle *h, cv_mlp *m, const float *T00, const float *M, cons
  * with m (NULL: the reference's start); track_iters keeps the iteration count per cell
  * of the last stage for cv_get_root_iters. */
 int cv_set_root_seed(cv_handle *h, cv_mlp *m, int track_iters);
+/* Viscous stages of the tiled kernels (kinds 1, 2) in split form: the stage writes the
+ * root solve's conserved input (field CV_ROOT_CONS) and k_visc_root finishes the cells
+ * (update_ev from the clamp on, kernel_visc.cl:322-393), seeded by the network set with
+ * cv_set_root_seed (field CV_ROOT_SEED) or from the reference's start.  Without a seed the
+ * split step is bit-identical to the fused one.  A network set on a viscous handle turns
+ * the split on by itself; on = 0 with a network set is an error. */
+int cv_set_root_split(cv_handle *h, int on);
 int cv_get_root_iters(cv_handle *h, int32_t *host, int64_t n);
 
 #ifdef __cplusplus

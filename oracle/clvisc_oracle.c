@@ -766,6 +766,28 @@ void orc_visc_along(const orc_params *p, int axis, real *Src, const real *ev, co
     }
 }
 
+/* update_ev (visc) from the clamp |T^ti| <= 0.999 T^tt on, kernel_visc.cl:365-392: the
+ * root solve (from *v0 when given) and the velocities, written to o[0..3] */
+static void visc_finish(const orc_params *p, real T00, real T01, real T02, real T03, real bnew, const real *v0,
+                        int *it, real *o) {
+    real M = (real)sqrt((double)(T01 * T01 + T02 * T02 + T03 * T03));
+    real SC = RC(0.999f);
+    if (M > T00) {
+        T01 *= DIV(SC * T00, M);
+        T02 *= DIV(SC * T00, M);
+        T03 *= DIV(SC * T00, M);
+        M = SC * T00;
+    }
+    real ed_find = v0 ? root_newton_v0(p, T00, M, bnew, p->bulk_on, *v0, it)
+                      : solve_ed(p, T00, M, bnew, p->bulk_on, it);
+    if (ed_find < ACU) { T01 = RC(0.0f); T02 = RC(0.0f); T03 = RC(0.0f); }
+    ed_find = r_max(RC(0.0f), ed_find);
+    real pr = eos_P(p, ed_find) + bnew;
+    if (p->bulk_on) pr = r_max(pr, RC(0.0f));
+    real epv = r_max(ACU, T00 + pr);
+    o[0] = ed_find; o[1] = DIV(T01, epv); o[2] = DIV(T02, epv); o[3] = DIV(T03, epv);
+}
+
 /* update_ev (visc), kernel_visc.cl:322-393; bulk1/bulk2 are Pi at the start point
  * and the freshly updated Pi (NULL without the bulk extension). */
 void orc_visc_update_ev(const orc_params *p, real *evnew, const real *ev1, const real *pi1,
@@ -793,25 +815,28 @@ void orc_visc_update_ev(const orc_params *p, real *evnew, const real *ev1, const
         real T01 = ((real)fabs((double)T0m[1]) < ACU) ? RC(0.0f) : DIV(T0m[1], tau);
         real T02 = ((real)fabs((double)T0m[2]) < ACU) ? RC(0.0f) : DIV(T0m[2], tau);
         real T03 = ((real)fabs((double)T0m[3]) < ACU) ? RC(0.0f) : DIV(T0m[3], tau);
-        real M = (real)sqrt((double)(T01 * T01 + T02 * T02 + T03 * T03));
-        real SC = RC(0.999f);
-        if (M > T00) {
-            T01 *= DIV(SC * T00, M);
-            T02 *= DIV(SC * T00, M);
-            T03 *= DIV(SC * T00, M);
-            M = SC * T00;
-        }
-        real bnew = bulk_at(bulk2, I);
         int it = 0;
-        real ed_find = solve_ed(p, T00, M, bnew, p->bulk_on, &it);
+        visc_finish(p, T00, T01, T02, T03, bulk_at(bulk2, I), NULL, &it, evnew + 4 * I);
         if (iters) iters[I] = it;
-        if (ed_find < ACU) { T01 = RC(0.0f); T02 = RC(0.0f); T03 = RC(0.0f); }
-        ed_find = r_max(RC(0.0f), ed_find);
-        real pr = eos_P(p, ed_find) + bnew;
-        if (p->bulk_on) pr = r_max(pr, RC(0.0f));
-        real epv = r_max(ACU, T00 + pr);
-        real *o = evnew + 4 * I;
-        o[0] = ed_find; o[1] = DIV(T01, epv); o[2] = DIV(T02, epv); o[3] = DIV(T03, epv);
+    }
+}
+
+/* The end of update_ev (visc) for given conserved inputs (T^tt, T^ti as update_ev has them
+ * before the clamp, 4 per cell) and Pi_new (NULL: no bulk): the split form of the stage.
+ * v0 (NULL: the reference's start) seeds the root solve, clamped to [0, 0.9999]. */
+void orc_visc_root_from(const orc_params *p, const real *cons, const real *bulk, const real *v0, int64_t n,
+                        real *ev_out, int32_t *iters) {
+    for (int64_t I = 0; I < n; I++) {
+        const real *c = cons + 4 * I;
+        int it = 0;
+        real s, *sp = NULL;
+        if (v0) {
+            s = v0[I] > RC(0.0f) ? v0[I] : RC(0.0f);
+            s = s < RC(0.9999f) ? s : RC(0.9999f);
+            sp = &s;
+        }
+        visc_finish(p, c[0], c[1], c[2], c[3], bulk ? bulk[I] : RC(0.0f), sp, &it, ev_out + 4 * I);
+        if (iters) iters[I] = it;
     }
 }
 

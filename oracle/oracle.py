@@ -239,6 +239,24 @@ def root_solve_from(params, T00, M, v0, precision='f32'):
     return ed, it
 
 
+def visc_root_from(params, cons, bulk=None, v0=None, precision='f32'):
+    """The end of the viscous update_ev (kernel_visc.cl:365-392) for conserved inputs cons
+    (n x 4: T^tt, T^tx, T^ty, T^t eta before the 0.999 clamp) and Pi_new (bulk, or None):
+    the split form of the stage.  v0 (None: the reference's start 0.5) seeds the root solve.
+    Returns (ev n x 4, iterations)."""
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    cons = np.ascontiguousarray(np.asarray(cons, dt).reshape(-1, 4))
+    n = cons.shape[0]
+    b = None if bulk is None else np.ascontiguousarray(bulk, dt).reshape(-1)
+    s = None if v0 is None else np.ascontiguousarray(v0, dt).reshape(-1)
+    ev = np.zeros((n, 4), dt)
+    it = np.zeros(n, np.int32)
+    L.orc_visc_root_from(C.byref(params), _ptr(cons), None if b is None else _ptr(b), None if s is None else _ptr(s),
+                         C.c_int64(n), _ptr(ev), _ptr(it))
+    return ev, it
+
+
 def max_prop_speed(params, ev, vk, precision='f32'):
     L = lib(precision)
     dt = np.float32 if precision == 'f32' else np.float64

@@ -1,0 +1,135 @@
+"""Split viscous stage: the stage kernel writes the root solve's conserved input and
+k_visc_root finishes the cells (update_ev from the clamp on), optionally seeded by the
+learned network.  Without a seed it is bit-identical to the fused stage (and so to the
+oracle); with a seed the device solve equals the oracle's solve from the same seeds."""
+import sys
+
+import numpy as np
+import pytest
+
+from clvisc_amd import _lib, ic, surrogate
+from clvisc_amd.eos import Eos
+from oracle import mlp_oracle as MO
+from oracle import oracle as O
+from helpers import cv_params, orc_params, bit_equal_frac
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ('ev0', 'ev1', 'ev2', 'pi0', 'pi1', 'pi2', 'udiff', 'goodcell')
+
+
+def _say(msg):
+    print(msg)
+    sys.stdout.flush()
+
+
+def _case(nx, ny, nz, bulk=None, nhot=12, seed=3):
+    eos = Eos('crossover')
+    tab = eos.create_table()
+    p = cv_params(nx, ny, nz, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08, bulk=bulk)
+    pos = ic.woods_saxon_positions(nhot, seed, R=0.6, a=0.1)
+    ev = ic.hotspots(nx, ny, nz, 0.08, 0.08, 0.15, positions=pos, emax=50.0)
+    return p, tab, ev
+
+
+def _start(p, tab, ev, kernel=None):
+    h = _lib.Handle(p, tab, viscous=True)
+    if kernel is not None:
+        h.set_stage_kernel(*kernel)
+    h.visc_initialize(p.tau0)
+    h.ideal_stage(1, np.float32(p.tau0))
+    h.set('ev1', ev)
+    return h
+
+
+@pytest.mark.parametrize('kernel,bulk', [((2, 0, 0), None), ((2, 0, 0), (0.05, 0.155, 0.02)),
+                                         ((1, 0, 0), (0.05, 0.155, 0.02)), ((2, 7, 32), None)])
+def test_split_without_seed_equals_fused(kernel, bulk):
+    p, tab, ev = _case(29, 31, 11, bulk=bulk)
+    _say('case: split vs fused stage, 29x31x11, kernel %s, bulk %s, 20 steps' % (kernel, bulk is not None))
+    hf = _start(p, tab, ev, kernel)
+    hs = _start(p, tab, ev, kernel)
+    hs.set_root_split(True)
+    hf.visc_steps(0, 20)
+    hs.visc_steps(0, 20)
+    names = FIELDS + (('bulk0', 'bulk1', 'bulk2') if bulk else ())
+    for name in names:
+        a, b = hs.get(name), hf.get(name)
+        _say('  %-8s bit-equal %.8f' % (name, bit_equal_frac(a, b)))
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    hf.close()
+    hs.close()
+
+
+def test_split_seeded_root_matches_oracle():
+    """A seeded step with a random network (arbitrary seeds in [0, 0.9999]): the seeds equal
+    the network oracle's on the same features (to accumulation rounding) and the solve from
+    those seeds equals the oracle's bit for bit, iteration counts included."""
+    p, tab, ev = _case(33, 29, 13, bulk=(0.05, 0.155, 0.02))
+    _say('case: seeded split stage vs oracle, 33x29x13 viscous+bulk')
+    h = _start(p, tab, ev)
+    h.visc_steps(0, 5)
+    rng = np.random.default_rng(21)
+    P = (rng.standard_normal(MO.NPARAM) * 0.3).astype(np.float32)
+    m = _lib.Mlp()
+    m.set_params(P)
+    h.set_root_seed(m, track_iters=True)
+    h.visc_steps(5, 1)
+    cons = h.get('root_cons')
+    seed = h.get('root_seed')
+    bk = h.get('bulk1')
+    g = h.get('ev1')
+    it = h.root_iters()
+    T00, M, T03 = surrogate.clamp_cons(cons)
+    _, _, Y = MO.forward(P, MO.features(T00, M, T03), bf16=True)
+    err = np.abs(seed - Y[:, 1]).max()
+    _say('  seed vs network oracle: max abs %.3g; seeds in [%.3f, %.3f]' % (err, seed.min(), seed.max()))
+    assert err < 2e-5 * max(1.0, np.abs(Y[:, 1]).max())
+    op = orc_params(p, tab)
+    r, itr = O.visc_root_from(op, cons, bk, seed)
+    _say('  ev1 bit-equal %.8f; iterations mean %.3f (oracle %.3f)' % (bit_equal_frac(g, r), it.mean(), itr.mean()))
+    np.testing.assert_array_equal(g, r)
+    np.testing.assert_array_equal(it, itr)
+    # the same stage unseeded (split, reference start): oracle's plain solve
+    h2 = _start(p, tab, ev)
+    h2.visc_steps(0, 5)
+    h2.set_root_split(True)
+    h2.set_root_seed(None, track_iters=True)
+    h2.visc_steps(5, 1)
+    r2, itr2 = O.visc_root_from(op, h2.get('root_cons'), h2.get('bulk1'), None)
+    np.testing.assert_array_equal(h2.get('ev1'), r2)
+    np.testing.assert_array_equal(h2.root_iters(), itr2)
+    h.close()
+    h2.close()
+    m.close()
+
+
+def test_trained_seed_viscous_run_close_to_plain():
+    """Seed trained on the split stage's own states; the seeded run needs fewer iterations
+    and stays within the root solve's tolerance of the plain run."""
+    p, tab, ev = _case(41, 41, 15, bulk=(0.05, 0.155, 0.02), nhot=20)
+    _say('case: trained seed, viscous+bulk 41x41x15, 30 steps seeded vs plain')
+    h = _start(p, tab, ev)
+    states = surrogate.visc_run_states(h, 0, 30, every=2)
+    h.close()
+    mlp, hist = surrogate.train_seed(states, steps=900, batch=4096, lr=3e-3)
+    _say('  loss %.4g -> %.4g' % (hist[:20].mean(), hist[-20:].mean()))
+    hp = _start(p, tab, ev)
+    hp.set_root_split(True)
+    hp.set_root_seed(None, track_iters=True)
+    hs = _start(p, tab, ev)
+    hs.set_root_seed(mlp, track_iters=True)
+    itp, its = [], []
+    for n in range(30):
+        hp.visc_steps(n, 1)
+        hs.visc_steps(n, 1)
+        itp.append(hp.root_iters().mean())
+        its.append(hs.root_iters().mean())
+    ep, es = hp.get('ev1')[:, 0], hs.get('ev1')[:, 0]
+    msk = ep > 1e-3 * ep.max()
+    rel = np.abs(es[msk] - ep[msk]) / ep[msk]
+    _say('  iterations plain %.3f seeded %.3f; max rel e diff %.3g' % (np.mean(itp), np.mean(its), rel.max()))
+    assert np.mean(its) < np.mean(itp)
+    assert rel.max() < 1e-3
+    for x in (hp, hs, mlp):
+        x.close()

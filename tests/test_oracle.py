@@ -406,3 +406,61 @@ def test_slope_closed_form_equals_nested_minmod(prec):
                      np.sign(c - b) * np.minimum(np.abs(c - a), np.minimum(th * np.abs(c - b), th * np.abs(b - a))),
                      0)
     assert not np.array_equal(wrong, ref)
+
+
+# --- split form of the viscous update_ev (root solve in its own pass) -------------------
+def _cons_of(e, v, dirn, P, Pi=0.0):
+    """T^tt, T^ti of a perfect fluid with pressure P + Pi (the map the solve inverts)."""
+    w = (e + P + Pi) / (1.0 - v * v)
+    T00 = w - (P + Pi)
+    Ti = (w * v)[:, None] * dirn
+    return np.column_stack([T00, Ti])
+
+
+@pytest.mark.parametrize('seeded', [False, True])
+def test_visc_root_from_inverts_the_conserved_map(ideal_tab, seeded):
+    """(e, v) -> T^{tau mu} -> visc_root_from recovers e and v^i, plain and seeded (a seed
+    at the exact |v| must not move the converged answer); with bulk Pi in the pressure too."""
+    rng = np.random.default_rng(11)
+    n = 5000
+    e = rng.uniform(0.5, 20.0, n)
+    v = rng.uniform(0.0, 0.95, n)
+    dirn = rng.standard_normal((n, 3))
+    dirn /= np.linalg.norm(dirn, axis=1, keepdims=True)
+    for Pi, bulk in ((0.0, None), (-0.05, (0.05, 0.155, 0.015))):
+        p = params(ideal_tab, bulk=bulk)
+        cons = _cons_of(e, v, dirn, e / 3.0, Pi * e / 3.0)
+        bk = None if bulk is None else (Pi * e / 3.0).astype(np.float32)
+        ev, it = O.visc_root_from(p, cons.astype(np.float32), bk, v if seeded else None)
+        assert np.abs(ev[:, 0] - e).max() / e.min() < 2e-3
+        assert np.median(np.abs(ev[:, 0] - e) / e) < 1e-5
+        assert np.abs(ev[:, 1:] - v[:, None] * dirn).max() < 1e-3
+        if seeded:
+            assert it.mean() < 2.5
+
+
+def test_visc_root_from_clamp_and_dilute_cells(ideal_tab):
+    """|T^ti| > T^tt is scaled to 0.999 T^tt (subluminal result), and e < acu zeroes v."""
+    p = params(ideal_tab)
+    cons = np.array([[1.0, 2.0, 0.0, 0.0],       # M > T00: clamped
+                     [1e-9, 0.0, 0.0, 0.0],      # vacuum
+                     [2.0, 0.0, 0.0, 0.0]], np.float32)
+    ev, _ = O.visc_root_from(p, cons)
+    assert np.linalg.norm(ev[0, 1:]) < 1.0 and ev[0, 0] > 0
+    assert np.all(ev[1, 1:] == 0.0)
+    assert ev[2, 0] == pytest.approx(2.0, rel=1e-5) and np.all(ev[2, 1:] == 0.0)
+
+
+def test_visc_root_from_plant_fails(ideal_tab):
+    """A mistake planted on the CPU (pressure without Pi, i.e. bulk ignored) fails the
+    inversion check above."""
+    rng = np.random.default_rng(12)
+    n = 500
+    e = rng.uniform(0.5, 20.0, n)
+    v = rng.uniform(0.0, 0.9, n)
+    dirn = np.tile([1.0, 0.0, 0.0], (n, 1))
+    p = params(ideal_tab, bulk=(0.05, 0.155, 0.015))
+    Pi = -0.2 * e / 3.0
+    cons = _cons_of(e, v, dirn, e / 3.0, Pi).astype(np.float32)
+    ev_bad, _ = O.visc_root_from(p, cons, np.zeros(n, np.float32))
+    assert np.abs(ev_bad[:, 0] - e).max() / e.min() > 1e-2
