@@ -298,6 +298,144 @@ def run_learned(args, p, tab, ev, h_plain, loop_end):
             'learned_over_plain': None}
 
 
+def run_combined(args, p, tab, name):
+    """Combined workload: `events` batched 385x385x115 viscous + bulk events (400 hotspots each,
+    their own seeds), FP64 root solve for cells with |T^ti| > 0.99 T^tt, evolved until every
+    event's max e drops below e(T = 0.137 GeV) (checked every ntskip = 20 steps, as
+    CLVisc.evolve does), freeze-out surface found every ntskip steps (nxskip = nyskip = 3,
+    nzskip = 1), FP64 energy balance of one step in every ntskip.  Finished events leave
+    the batch.  The plain GPU form (the same events stepped one after another) is timed
+    over a window of steps."""
+    from clvisc_amd import _lib
+    from clvisc_amd import ic as ic_mod
+    nev = args.events
+    ntskip = 20
+    p.flags |= _lib.CV_FLAG_FP64_ROOT
+    p.fp64_root_vmin = 0.99
+    p.nxskip, p.nyskip, p.nzskip = 3, 3, 1
+    efrz = p.efrz
+    ncell = NX * NY * NZ
+    dims = [(NX, NY, NZ)] * nev
+    t_start = time.perf_counter()
+    batch = _lib.Batch(p, tab, dims)
+    hs = [batch.handle(e) for e in range(nev)]
+    seeds = [20240601 + e for e in range(nev)]
+    for e, h in enumerate(hs):
+        pos = ic_mod.woods_saxon_positions(400, seeds[e])
+        ev = ic_mod.hotspots(NX, NY, NZ, DX, DY, DZ, positions=pos, emax=50.0)
+        h.visc_initialize(TAU0)
+        h.ideal_stage(1, np.float32(TAU0))
+        h.set('ev1', ev)
+        h.hypersf_reserve(args.sf_capacity)
+        h.hypersf_snapshot(np.float32(TAU0))
+    del ev
+    log('combined: %d events of %dx%dx%d set up in %.1fs (e_frz %.4g GeV/fm^3)'
+        % (nev, NX, NY, NZ, time.perf_counter() - t_start, efrz))
+    tau_of = lambda n: float(np.float32(TAU0 + n * DT))
+    active = [1] * nev
+    steps_done = [0] * nev
+    finish_tau = [None] * nev
+    balance = []                       # (event, loop, relative residual)
+    energy = [[] for _ in range(nev)]  # (tau, E) per event
+    loop = 0
+    dev_ms = 0.0
+    t_loop = time.perf_counter()
+    t_last = t_loop
+    while sum(active) and loop < args.max_loops:
+        # one step with the FP64 energy balance of the active events around it
+        pre = {e: hs[e].energy_sum(tau_of(loop), 1) for e in range(nev) if active[e]}
+        batch.steps(1)
+        dev_ms += batch.last_elapsed_ms()
+        for e, (E0, W0) in pre.items():
+            E1, _ = hs[e].energy_sum(tau_of(loop + 1), 1)
+            _, Ws = hs[e].energy_sum(1.0, 2)
+            balance.append((e, loop, (E1 - E0 + 0.5 * DT * (W0 + Ws)) / E0))
+            energy[e].append((tau_of(loop), E0))
+        batch.steps(ntskip - 1)
+        dev_ms += batch.last_elapsed_ms()
+        loop += ntskip
+        changed = False
+        for e in range(nev):
+            if not active[e]:
+                continue
+            steps_done[e] += ntskip
+            h = hs[e]
+            h.hypersf_find(np.float32(tau_of(loop)))
+            if h.max_ed(1) < efrz:
+                active[e] = 0
+                finish_tau[e] = tau_of(loop)
+                changed = True
+        if changed:
+            batch.set_active(active)
+        now = time.perf_counter()
+        if now - t_last > 10.0 or not sum(active):
+            t_last = now
+            log('  tau %.2f fm (loop %d): %d events active, %d surface elements, %.1fs'
+                % (tau_of(loop), loop, sum(active), sum(h.hypersf_count()[0] for h in hs), now - t_loop))
+    wall = time.perf_counter() - t_loop
+    updates = float(ncell) * sum(steps_done)
+    value = updates / wall
+    counts = [h.hypersf_count() for h in hs]
+    nsf = sum(c[0] for c in counts)
+    overflow = [e for e, c in enumerate(counts) if c[0] > c[1]]
+    # emit the surface: copy each event's elements to the host (and write them when asked)
+    t0 = time.perf_counter()
+    dsum = 0.0
+    for e, h in enumerate(hs):
+        if counts[e][0] > counts[e][1]:
+            continue
+        sf, tx, pisf, bksf = h.hypersf_get()
+        dsum += float(np.abs(sf[:, :4].astype(np.float64)).sum())
+        if args.sf_out:
+            from clvisc_amd import hypersf as hsf
+            d = os.path.join(args.sf_out, 'event%d' % e)
+            os.makedirs(d, exist_ok=True)
+            hsf.write_hypersf(d, sf, tx, 0.137)
+    t_emit = time.perf_counter() - t0
+    res = np.array([b[2] for b in balance])
+    log('combined: %d loops, %.1fs wall (%.1fs device steps) -> %.4g cell-updates/s; %d surface elements '
+        '(copied in %.1fs); energy balance per step |res| max %.3g median %.3g'
+        % (loop, wall, dev_ms / 1e3, value, nsf, t_emit, np.abs(res).max(), np.median(np.abs(res))))
+    # the plain GPU form: the same events one after another, a window of steps each
+    for e in range(nev):
+        active[e] = 0
+    batch.set_active(active)
+    nwin = 10
+    t0 = time.perf_counter()
+    for e, h in enumerate(hs):
+        h.visc_steps(loop, nwin, sync=False)
+    for h in hs:
+        h.sync()
+    seq = float(ncell) * nev * nwin / (time.perf_counter() - t0)
+    step_value = updates / (dev_ms / 1e3)
+    abytes = sum(ALGO_BYTES[True])
+    out = {
+        'metric': 'cell-updates/s, combined: %d batched 385x385x115 viscous+bulk events to freeze-out' % nev,
+        'value': value, 'unit': 'cell-updates/s', 'n_gpus': 1,
+        'steps': int(sum(steps_done)), 'warmup': 0, 'ms_per_step': 1000.0 * wall / max(1, loop),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': value / REF_CELL_UPDATES,
+        'dtype': 'f32 (f64 root for |T^ti| > 0.99 T^tt, f64 energy sums)', 'data': 'synthetic',
+        'config': {'workload': '%d events 385x385x115, 400 Woods-Saxon hotspots each (seeds %d..%d), '
+                               'eta/s 0.08, bulk zeta/s peak 0.05, crossover EOS, until max T < 0.137 GeV, '
+                               'surface every 20 steps (skip 3,3,1)' % (nev, seeds[0], seeds[-1]),
+                   'device': name, 'events': nev, 'cells_per_event': ncell, 'loops': loop,
+                   'finish_tau': finish_tau, 'surface_elements': nsf, 'surface_overflow_events': overflow,
+                   'surface_abs_dsigma_sum': dsum,
+                   'energy_balance_max_abs': float(np.abs(res).max()),
+                   'energy_balance_median_abs': float(np.median(np.abs(res))),
+                   'energy_first_last': [[energy[e][0][1], energy[e][-1][1]] for e in range(min(nev, 4))],
+                   'step_only_value': step_value, 'plain_sequential_value': seq,
+                   'batched_over_plain': step_value / seq},
+        'roofline': {'bound': 'hbm', 'kernel': 'batch_stage (both stages)',
+                     'achieved': round(abytes * step_value / 1e9, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(abytes * step_value / 1e9 / HBM_PEAK_GBS, 5), 'traffic': None,
+                     'step_algorithmic_bytes_per_cell': abytes},
+        'cpu_baseline': None,
+    }
+    print(json.dumps(out))
+    batch.close()
+
+
 def run_surrogate(args, name):
     """Learned root-solve seed on the ideal Gubser workload (201 x 201 x 1, 200 steps): states
     from the plain run, device training (BF16 MFMA), then the seeded run; iterations per cell,
@@ -371,7 +509,11 @@ def main():
     ap.add_argument('--no-learned', action='store_true',
                     help='single: skip the learned (network-seeded root solve) form of the run')
     ap.add_argument('--cpu-threads', type=int, default=0)
-    ap.add_argument('--workload', choices=['single', 'ensemble', 'surrogate'], default='single',
+    ap.add_argument('--events', type=int, default=16, help='combined: number of 385x385x115 events')
+    ap.add_argument('--max-loops', type=int, default=3000, help='combined: step limit')
+    ap.add_argument('--sf-capacity', type=int, default=4_000_000, help='combined: surface elements per event')
+    ap.add_argument('--sf-out', default='', help='combined: write hypersf.dat / sf_txyz.dat per event here')
+    ap.add_argument('--workload', choices=['single', 'ensemble', 'surrogate', 'combined'], default='single',
                     help='single: one 385x385x115 event (the headline); ensemble: 64 uneven events batched; '
                          'surrogate: learned root-solve seed on the ideal Gubser workload')
     ap.add_argument('--autotune', action='store_true', help='ensemble: tune tiles and chunks per grid shape')
@@ -388,6 +530,12 @@ def main():
         name, cus = _lib.device_info(0)
         log('device: %s, %d CUs' % (name, cus))
         run_surrogate(args, name)
+        return
+    if args.workload == 'combined':
+        p, tab, _ = make_workload(bulk, ic=False)
+        name, cus = _lib.device_info(0)
+        log('device: %s, %d CUs' % (name, cus))
+        run_combined(args, p, tab, name)
         return
     if args.workload == 'ensemble':
         p, tab, _ = make_workload(bulk, ic=False)

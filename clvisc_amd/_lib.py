@@ -85,6 +85,7 @@ def lib():
             'cv_ideal_steps_async': [vp, i64, i64],
             'cv_visc_steps_async': [vp, i64, i64],
             'cv_max_ed': [vp, C.c_int, C.POINTER(f32)],
+            'cv_energy_sum': [vp, C.c_int, C.c_double, vp],
             'cv_sync': [vp],
             'cv_last_elapsed_ms': [vp, C.POINTER(f32)],
             'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
@@ -148,7 +149,7 @@ def exported_symbols():
             'cv_batch_steps_async', 'cv_batch_set_loop', 'cv_batch_sync', 'cv_batch_last_elapsed_ms',
             'cv_batch_info', 'cv_mlp_nparam', 'cv_mlp_create', 'cv_mlp_destroy', 'cv_mlp_get_params',
             'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward',
-            'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split']
+            'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum']
 
 
 def make_eos(table):
@@ -243,6 +244,13 @@ class Handle(object):
         check(self.L.cv_visc_steps_async(self.h, loop0, nsteps))
         if sync:
             self.sync()
+
+    def energy_sum(self, tau, which=1):
+        """(E, W) in double: E = sum tau (T^tautau + pi^tautau) dV, W = sum tau^2 T^etaeta dV
+        (with pi); dE/dtau = -W plus the boundary flux."""
+        out = np.zeros(2, np.float64)
+        check(self.L.cv_energy_sum(self.h, which, float(tau), out.ctypes.data))
+        return float(out[0]), float(out[1])
 
     def max_ed(self, which=1):
         out = C.c_float()

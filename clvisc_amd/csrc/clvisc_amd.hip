@@ -64,6 +64,7 @@ struct cv_handle {
     float *bk[3] = {nullptr, nullptr, nullptr};
     float *Src = nullptr, *IS_src = nullptr, *bsrc = nullptr, *udiff = nullptr, *good = nullptr;
     unsigned int *d_max = nullptr;
+    double *d_esum = nullptr;     // FP64 energy sums: block partials, then the two totals
     hipStream_t stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     bool timed = false;
@@ -282,7 +283,7 @@ extern "C" int cv_destroy(cv_handle *h) {
     for (int k = 0; k < 3; k++) { (void)hipFree(h->ev[k]); (void)hipFree(h->pi[k]); (void)hipFree(h->bk[k]); }
     (void)hipFree(h->Src); (void)hipFree(h->IS_src); (void)hipFree(h->bsrc);
     (void)hipFree(h->udiff); (void)hipFree(h->good);
-    (void)hipFree(h->d_eos); (void)hipFree(h->d_max);
+    (void)hipFree(h->d_eos); (void)hipFree(h->d_max); (void)hipFree(h->d_esum);
     (void)hipFree(h->sf_ev); (void)hipFree(h->sf_pi); (void)hipFree(h->sf_bk);
     (void)hipFree(h->sf_out); (void)hipFree(h->sf_txyz); (void)hipFree(h->sf_pisf); (void)hipFree(h->sf_bksf);
     (void)hipFree(h->sf_count);
@@ -695,6 +696,25 @@ extern "C" int cv_max_ed(cv_handle *h, int which, float *out) {
     return CV_OK;
 }
 
+extern "C" int cv_energy_sum(cv_handle *h, int which, double tau, double *out) {
+    if (!h || !out) return fail(CV_ERR_ARG, "null argument");
+    if (which < 0 || which > 2) return fail(CV_ERR_ARG, "which must be 0, 1 or 2");
+    if (!(tau > 0.0)) return fail(CV_ERR_ARG, "cv_energy_sum: tau must be positive");
+    HIP_TRY(hipSetDevice(h->device));
+    if (!h->d_esum) HIP_TRY(hipMalloc((void **)&h->d_esum, sizeof(double) * (2 * CV_ESUM_BLOCKS + 2)));
+    const float *pi = h->viscous ? h->pi[which] : nullptr;
+    const float *bk = h->bulk ? h->bk[which] : nullptr;
+    hipLaunchKernelGGL(k_energy_sum, dim3(CV_ESUM_BLOCKS), dim3(256), 0, h->stream, h->phys, h->eos, h->ev[which], pi,
+                       bk, tau, h->d_esum);
+    const double dV = (double)h->prm.dx * (double)h->prm.dy * (double)h->prm.dz;
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, h->stream, h->d_esum, CV_ESUM_BLOCKS, dV,
+                       h->d_esum + 2 * CV_ESUM_BLOCKS);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, h->d_esum + 2 * CV_ESUM_BLOCKS, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CV_OK;
+}
+
 extern "C" int cv_sync(cv_handle *h) {
     if (!h) return fail(CV_ERR_ARG, "null handle");
     HIP_TRY(hipSetDevice(h->device));
@@ -842,6 +862,7 @@ struct cv_batch {
     int device = 0;
     int bulk = 0;
     int ek = 1;
+    bool fp64_root = false;                // CV_FLAG_FP64_ROOT: the SPEC stage kernels
     std::vector<cv_handle *> ev;           // one single-event state per event
     std::vector<int32_t> active;
     hipStream_t stream = nullptr;
@@ -872,10 +893,11 @@ extern "C" int cv_batch_create(const cv_params *base, const cv_eos *eos, int nev
     cv_batch *b = new cv_batch();
     b->device = device;
     b->bulk = (base->flags & CV_FLAG_BULK) ? 1 : 0;
-    if (base->flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST | CV_FLAG_FP64_ROOT)) {
+    if (base->flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST)) {
         delete b;
-        return fail(CV_ERR_UNSUPPORTED, "cv_batch_create: test flags and the FP64 root are single-event only");
+        return fail(CV_ERR_UNSUPPORTED, "cv_batch_create: the test flags are single-event only");
     }
+    b->fp64_root = (base->flags & CV_FLAG_FP64_ROOT) != 0;
     for (int e = 0; e < nevents; e++) {
         cv_params p = *base;
         p.nx = dims[3 * e]; p.ny = dims[3 * e + 1]; p.nz = dims[3 * e + 2];
@@ -1002,17 +1024,19 @@ static int batch_build(cv_batch *b) {
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     dim3 g(b->grid), blk(CV_TILE_THREADS);
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_ctr);
-#define CV_BATCH_LAUNCH(B, S)                                                                                  \
-    do {                                                                                                       \
-        if (b->ek == 1) hipLaunchKernelGGL((k_batch_stage<B, S, 1>), g, blk, 0, b->stream, b->d_state,         \
-                                           b->d_events, b->ev[0]->eos, b->d_ctr);                              \
-        else hipLaunchKernelGGL((k_batch_stage<B, S, 0>), g, blk, 0, b->stream, b->d_state, b->d_events,       \
-                                b->ev[0]->eos, b->d_ctr);                                                      \
+#define CV_BATCH_LAUNCH2(B, S, K, X) \
+    hipLaunchKernelGGL((k_batch_stage<B, S, K, X>), g, blk, 0, b->stream, b->d_state, b->d_events, b->ev[0]->eos, \
+                       b->d_ctr)
+#define CV_BATCH_LAUNCH(B, S)                                                              \
+    do {                                                                                   \
+        if (b->ek == 1) { if (b->fp64_root) CV_BATCH_LAUNCH2(B, S, 1, true); else CV_BATCH_LAUNCH2(B, S, 1, false); } \
+        else { if (b->fp64_root) CV_BATCH_LAUNCH2(B, S, 0, true); else CV_BATCH_LAUNCH2(B, S, 0, false); }            \
     } while (0)
     if (b->bulk) CV_BATCH_LAUNCH(true, 1); else CV_BATCH_LAUNCH(false, 1);
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_ctr);
     if (b->bulk) CV_BATCH_LAUNCH(true, 2); else CV_BATCH_LAUNCH(false, 2);
 #undef CV_BATCH_LAUNCH
+#undef CV_BATCH_LAUNCH2
     hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, b->stream, b->d_state);
     hipError_t e1 = hipStreamEndCapture(b->stream, &b->graph);
     if (e1 != hipSuccess) return fail(CV_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e1));

@@ -35,7 +35,8 @@ __global__ void k_batch_advance(BatchState *bs) {
     if (threadIdx.x == 0 && blockIdx.x == 0) bs->step = bs->step + 1;
 }
 
-template <bool BULK, int STEP, int EK>
+// SPEC: the FP64 root-solve branch for fast cells (CV_FLAG_FP64_ROOT) is compiled in.
+template <bool BULK, int STEP, int EK, bool SPEC>
 __global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchState *__restrict__ bs,
                                                                      const BatchEvent *__restrict__ evt, EosP E,
                                                                      unsigned long long *ctr) {
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchS
             q = StagePtrs{E2, P2, B2, E1, E2, P1, B1, ev.udiff, ev.IS_src, ev.bsrc, ev.Src, P0, B0, E0, ev.good,
                           ev.udiff, nullptr};
         }
-        tile_work<BULK, STEP, EK, false, true>(ev.p, eos, st, ev.tp, q, w - ev.work0, s_c, s_f);
+        tile_work<BULK, STEP, EK, SPEC, true>(ev.p, eos, st, ev.tp, q, w - ev.work0, s_c, s_f);
     }
 }
 

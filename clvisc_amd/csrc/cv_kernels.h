@@ -1020,6 +1020,51 @@ __global__ __launch_bounds__(256) void k_max_ed(int64_t n, const float *__restri
     }
 }
 
+// FP64 energy balance of the conserved form Q^tau = tau (T^tautau + pi^tautau), whose only
+// source besides the flux divergence is -T~^etaeta (kt_src_christoffel, kernel_visc.cl:11-40):
+// part[b] = (sum tau T^tautau, sum T~^etaeta) over the cells of block b, in double from the
+// FP32 fields (P from the FP32 table; bulk Pi added to P).  Fixed grid and order: the sums
+// are reproducible run to run.  k_energy_final adds the partials in block order.
+#define CV_ESUM_BLOCKS 1024
+__global__ __launch_bounds__(256) void k_energy_sum(PhysP p, EosP E, const float *__restrict__ ev,
+                                                    const float *__restrict__ pi, const float *__restrict__ bk,
+                                                    double tau, double *part) {
+    Eos eos(E, E.tab);
+    const int64_t n = p.n;
+    double sE = 0.0, sW = 0.0;
+    for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n; I += (int64_t)gridDim.x * blockDim.x) {
+        const float e = ev[I];
+        const double vx = ev[n + I], vy = ev[2 * n + I], vz = ev[3 * n + I];
+        const double pr = (double)eos.P(e) + (bk ? (double)bk[I] : 0.0);
+        const double g2 = 1.0 / (1.0 - (vx * vx + vy * vy + vz * vz));
+        const double w = ((double)e + pr) * g2;
+        const double p00 = pi ? (double)pi[I] : 0.0, p33 = pi ? (double)pi[9 * n + I] : 0.0;
+        sE += tau * (w - pr + p00);
+        sW += w * vz * vz + pr + p33;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sE += __shfl_xor(sE, off, 64);
+        sW += __shfl_xor(sW, off, 64);
+    }
+    __shared__ double s[2][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) { s[0][wv] = sE; s[1][wv] = sW; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = ((s[0][0] + s[0][1]) + s[0][2]) + s[0][3];
+        part[2 * blockIdx.x + 1] = ((s[1][0] + s[1][1]) + s[1][2]) + s[1][3];
+    }
+}
+
+__global__ void k_energy_final(const double *__restrict__ part, int nb, double dV, double *out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < nb; q++) { a += part[2 * q]; b += part[2 * q + 1]; }
+        out[0] = a * dV;
+        out[1] = b * dV;
+    }
+}
+
 // out[q][c] = field plane c at cell idx[q] (SoA planes -> AoS rows); idx checked on the host
 __global__ __launch_bounds__(256) void k_gather(int64_t n, int ncomp, const float *__restrict__ f,
                                                 const int64_t *__restrict__ idx, int64_t m, float *out) {

@@ -143,6 +143,14 @@ int cv_visc_steps_async(cv_handle *h, int64_t loop0, int64_t nsteps);
  * ideal.py:257-262). */
 int cv_max_ed(cv_handle *h, int which, float *out);
 
+/* FP64 energy balance of slot `which` at proper time tau (no counterpart in the reference):
+ * out[0] = sum over cells of tau (T^tautau + pi^tautau) dx dy deta (the conserved Q^tau of
+ * kernel_visc.cl:322-393 summed), out[1] = sum of T~^etaeta = tau^2 T^etaeta (+ pi~^etaeta)
+ * dx dy deta, the only non-flux source of Q^tau (kt_src_christoffel, kernel_visc.cl:11-40):
+ * dE/dtau = -W plus the flux through the grid boundary.  Summed in double from the FP32
+ * fields in a fixed order (reproducible). */
+int cv_energy_sum(cv_handle *h, int which, double tau, double *out);
+
 /* Profiling: runs one viscous step (as cv_visc_steps_async with nsteps = 1) with HIP
  * events around each of its launches, waits, and writes the launch durations in
  * milliseconds to ms[0..nms-1] in launch order; returns the number of launches in
@@ -217,7 +225,8 @@ int cv_hypersf_reset(cv_handle *h);
 typedef struct cv_batch cv_batch;
 
 /* dims: int32[3 * nevents] = (nx, ny, nz) per event.  Every event is a viscous handle
- * (flags as in base; the test flags and the FP64 root are rejected). */
+ * (flags as in base; the test flags are rejected; CV_FLAG_FP64_ROOT selects the stage
+ * kernels with the FP64 root branch). */
 int cv_batch_create(const cv_params *base, const cv_eos *eos, int nevents, const int32_t *dims,
                     int device, cv_batch **out);
 int cv_batch_destroy(cv_batch *b);

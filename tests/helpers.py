@@ -58,3 +58,22 @@ def bit_equal_frac(a, b):
     a = np.asarray(a, np.float32)
     b = np.asarray(b, np.float32)
     return float(np.mean(a.view(np.uint32) == b.view(np.uint32)))
+
+
+def energy_sums(oparams, ev, pi, bulk, tau, dV):
+    """(E, W) in float64 from AoS fields: E = sum tau (T^tautau + pi^tautau) dV and
+    W = sum (tau^2 T^etaeta + pi~^etaeta) dV, with P from the oracle's FP32 table lookup
+    (+ bulk Pi).  The balance dE/dtau = -W holds up to the boundary flux."""
+    ev = np.asarray(ev, np.float32).reshape(-1, 4)
+    P = O.eos_funcs(oparams, ev[:, 0])[:, 0].astype(np.float64)
+    if bulk is not None:
+        P = P + np.asarray(bulk, np.float32).reshape(-1).astype(np.float64)
+    e = ev[:, 0].astype(np.float64)
+    v = ev[:, 1:].astype(np.float64)
+    g2 = 1.0 / (1.0 - (v * v).sum(1))
+    w = (e + P) * g2
+    p00 = np.zeros_like(e) if pi is None else np.asarray(pi, np.float32).reshape(-1, 10)[:, 0].astype(np.float64)
+    p33 = np.zeros_like(e) if pi is None else np.asarray(pi, np.float32).reshape(-1, 10)[:, 9].astype(np.float64)
+    E = (tau * (w - P + p00)).sum() * dV
+    W = (w * v[:, 2] * v[:, 2] + P + p33).sum() * dV
+    return float(E), float(W)

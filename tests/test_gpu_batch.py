@@ -83,3 +83,29 @@ def test_batch_matches_single_events(bulk):
     for e, hs in enumerate(singles):
         _compare(batch.handle(e), hs, 'event %d (mask) %s' % (e, DIMS[e]))
     batch.close()
+
+
+def test_batch_fp64_root_matches_single_events():
+    """CV_FLAG_FP64_ROOT (FP64 root solve where M > fp64_root_vmin T^tt): the batch selects
+    the stage kernels with that branch and stays bit-identical to the single events."""
+    tab = Eos('crossover').create_table()
+    p = cv_params(1, 1, 1, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08,
+                  bulk=(0.05, 0.155, 0.02), flags=_lib.CV_FLAG_FP64_ROOT, fp64_root_vmin=0.3)
+    dims = DIMS[:3]
+    _say('case: batch of %d events with the FP64 root (vmin 0.3), bulk' % len(dims))
+    batch = _lib.Batch(p, tab, dims)
+    singles = []
+    for e, d in enumerate(dims):
+        ev = _ic(d, e)
+        _init(batch.handle(e), ev, p.tau0)
+        q = _lib.CvParams.from_buffer_copy(p)
+        q.nx, q.ny, q.nz = d
+        hs = _lib.Handle(q, tab, viscous=True)
+        _init(hs, ev, p.tau0)
+        singles.append(hs)
+    batch.steps(9)
+    for hs in singles:
+        hs.visc_steps(0, 9)
+    for e, hs in enumerate(singles):
+        _compare(batch.handle(e), hs, 'event %d %s' % (e, dims[e]))
+    batch.close()

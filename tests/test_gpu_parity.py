@@ -129,6 +129,16 @@ def test_visc_steps_2d_boost_invariant(kernel, chunk):
     _visc_compare(p, tab, ev, 15, 'viscous+bulk 2d', kernel, chunk)
 
 
+@pytest.mark.parametrize('kernel', [(0, 0, 0), (2, 0, 0)])
+def test_visc_steps_fp64_root(kernel):
+    """CV_FLAG_FP64_ROOT with a low threshold (M > 0.3 T^tt) so that many cells take the
+    double-precision root solve: identical to the oracle's FP64 branch."""
+    p, tab, ev = _visc_case(27, 25, 9, bulk=(0.05, 0.155, 0.02))
+    p.flags |= _lib.CV_FLAG_FP64_ROOT
+    p.fp64_root_vmin = 0.3
+    _visc_compare(p, tab, ev, 20, 'viscous+bulk, FP64 root for M > 0.3 T00', kernel)
+
+
 def test_visc_steps_bulk_3d():
     p, tab, ev = _visc_case(27, 25, 9, bulk=(0.05, 0.155, 0.02))
     _visc_compare(p, tab, ev, 20, 'viscous+bulk hotspots')

@@ -464,3 +464,41 @@ def test_visc_root_from_plant_fails(ideal_tab):
     cons = _cons_of(e, v, dirn, e / 3.0, Pi).astype(np.float32)
     ev_bad, _ = O.visc_root_from(p, cons, np.zeros(n, np.float32))
     assert np.abs(ev_bad[:, 0] - e).max() / e.min() > 1e-2
+
+
+@pytest.mark.parametrize('bulk', [None, (0.05, 0.155, 0.02)])
+def test_energy_balance_of_the_viscous_step(cross, bulk):
+    """Q^tau = tau (T^tautau + pi^tautau) changes in one RK2 step by dt/2 (S(U_n) + S(U*)), where
+    the sources sum to -W (-tau^2 T^etaeta, kt_src_christoffel) when the boundary is vacuum
+    and U* is the stage-1 state (d_ev[2], d_pi[2]; the reference evaluates stage 1 at the old
+    tau, so U* is not U_{n+1}).  With boost-invariant hotspots well inside the grid the FP64
+    balance E_{n+1} - E_n + dt/2 (W_n + W*) vanishes to FP32 rounding.  Pins the FP64 energy
+    sums (helpers.energy_sums; cv_energy_sum evaluates the same sums) and the source sign;
+    E without pi^tautau or W without P fails it."""
+    from helpers import energy_sums
+    _, tab = cross
+    nx = 101
+    p = params(tab, nx, nx, 1, dt=0.02, tau0=0.6, etaos_xmin=0.0, etaos_ymin=0.08, bulk=bulk)
+    pos = ic.woods_saxon_positions(8, 5, R=0.6, a=0.1)
+    ev = ic.hotspots(nx, nx, 1, 0.08, 0.08, 0.3, positions=pos, emax=30.0)
+    o = O.Oracle(p, 'f32', viscous=True)
+    o.load_ini(ev)
+    dV = 0.08 * 0.08 * 0.3
+    bk = lambda k: None if bulk is None else o.bulk[k]
+    res, res_nopi, res_noP = [], [], []
+    for n in range(12):
+        E0, W0 = energy_sums(p, o.ev[1], o.pi[1], bk(1), float(o.tau), dV)
+        E0x, _ = energy_sums(p, o.ev[1], None, bk(1), float(o.tau), dV)
+        o.visc_steps(1)
+        E1, _ = energy_sums(p, o.ev[1], o.pi[1], bk(1), float(o.tau), dV)
+        E1x, _ = energy_sums(p, o.ev[1], None, bk(1), float(o.tau), dV)
+        _, Ws = energy_sums(p, o.ev[2], o.pi[2], bk(2), 1.0, dV)
+        # W without P (planted on the CPU): the pressure part of W is P summed over cells
+        Pn = O.eos_funcs(p, o.ev[1][:, 0])[:, 0].astype(np.float64).sum() * dV
+        res.append((E1 - E0 + 0.5 * p.dt * (W0 + Ws)) / E0)
+        res_nopi.append((E1x - E0x + 0.5 * p.dt * (W0 + Ws)) / E0)
+        res_noP.append((E1 - E0 + 0.5 * p.dt * (W0 + Ws - 2 * Pn)) / E0)
+    res = np.array(res)
+    assert np.abs(res).max() < 1e-6, res
+    assert np.abs(res_noP).max() > 1e-3
+    assert np.abs(res_nopi).max() > 1e-5
