@@ -274,20 +274,20 @@ def run_learned(args, p, tab, ev, h_plain, loop_end):
         % (nrun, wall, 1000 * wall / nrun, value, split_value, it_p.mean(), it_l.mean(), rel))
     # FP32 sensitivity of the plain run itself: the same steps from the initial field with
     # e scaled by (1 + 2^-23) (one ulp), compared with the plain run at loop_end
-    hl.set_root_seed(None, track_iters=False)
-    hl.set_root_split(False)
-    hl.visc_initialize(TAU0)
-    hl.ideal_stage(1, np.float32(TAU0))
+    hl.close()
+    hu = _lib.Handle(p, tab, viscous=True)
+    hu.visc_initialize(TAU0)
+    hu.ideal_stage(1, np.float32(TAU0))
     evp = np.array(ev, np.float32, copy=True)
     evp[:, 0] *= np.float32(1.0 + 2.0 ** -23)
-    hl.set('ev1', evp)
+    hu.set('ev1', evp)
     del evp
-    hl.visc_steps(0, loop_end)
-    e_u = hl.get('ev1')[:, 0]
+    hu.visc_steps(0, loop_end)
+    e_u = hu.get('ev1')[:, 0]
     rel_ulp = float((np.abs(e_u[big] - e_p[big]) / e_p[big]).max())
     log('plain run from the initial field perturbed by one ulp: max rel e vs plain %.3g after %d steps'
         % (rel_ulp, loop_end))
-    hl.close()
+    hu.close()
     mlp.close()
     return {'value': value, 'unit': 'cell-updates/s', 'ms_per_step': 1000.0 * wall / nrun, 'steps': nrun,
             'max_rel_e_vs_plain': rel, 'matched': bool(rel <= MATCHED_REL), 'matched_tolerance': MATCHED_REL,

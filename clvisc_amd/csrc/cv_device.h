@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #define CV_ACU 1.0e-7f
+#define CV_BULK_EMIN 1.0e-3f   // bulk Pi is switched off below this e [GeV/fm^3]
 #define CV_THETA 1.1f
 #define CV_HBARC 0.19733f
 
@@ -57,6 +58,14 @@ struct StageP {
 __device__ __forceinline__ float rcp(float b) { return 1.0f / b; }     // correctly rounded
 __device__ __forceinline__ float rmax(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float rmin(float a, float b) { return b < a ? b : a; }
+// Regulation of the updated bulk pressure (the bulk Pi extension): |Pi| <= P(e)/2 and
+// Pi = 0 in dilute cells (e < CV_BULK_EMIN).  pr = P(e), e = the stage's e.
+__device__ __forceinline__ float bulk_regulate(float b, float pr, float e) {
+    const float h = 0.5f * pr;
+    b = rmin(b, h);
+    b = rmax(b, -h);
+    return e < CV_BULK_EMIN ? 0.0f : b;
+}
 __device__ __forceinline__ float rsign(float x) {
     return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : x);
 }

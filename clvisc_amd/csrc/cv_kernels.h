@@ -469,7 +469,7 @@ __global__ __launch_bounds__(256) void k_is_stage(PhysP p, EosP E, StageP st,
         bs += bsrc_;
         float b_old = bk1[I] * u_old[0];
         bout = (b_old + (bs * p.dt) * st.r_step - p.dt * oot_b * zetav * theta) * rcp(u_new[0] + p.dt * oot_b);
-        bout = rmax(bout, -eos.P(e2.x));   // no negative effective pressure
+        bout = bulk_regulate(bout, eos.P(e2.x), e2.x);
         if (fabsf(bout) > max_abs) max_abs = fabsf(bout);
     }
     float pre = eos.P(e2.x);
@@ -903,7 +903,7 @@ __global__ __launch_bounds__(256, CV_STAGE_WAVES) void k_visc_stage(PhysP p, Eos
         b_start = STEP == 1 ? b2 : bk1[I];
         float b_old = b_start * u_old[0];
         bout = (b_old + (bs * p.dt) * st.r_step - p.dt * oot_b * zetav * theta) * rcp(u_new[0] + p.dt * oot_b);
-        bout = rmax(bout, -eos.P(e2.x));   // no negative effective pressure
+        bout = bulk_regulate(bout, eos.P(e2.x), e2.x);
         if (fabsf(bout) > max_abs) max_abs = fabsf(bout);
     }
     {

@@ -488,7 +488,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
                 b_start = STEP == 1 ? b2 : ldo(bk1, bo);
                 float b_old = b_start * u_old[0];
                 bout = (b_old + (bs * p.dt) * st.r_step - p.dt * oot_b * zetav * theta) * rcp(u_new[0] + p.dt * oot_b);
-                bout = rmax(bout, -eos.P(e2.x));   // no negative effective pressure
+                bout = bulk_regulate(bout, eos.P(e2.x), e2.x);
                 if (fabsf(bout) > max_abs) max_abs = fabsf(bout);
             }
             {

@@ -1065,7 +1065,11 @@ void orc_update_pimn(const orc_params *p, real *pinew, real *goodcell, const rea
             real b_old = bulk1[I] * u_old[0];
             bout = DIV(b_old + DIV(bulk_src[I] * DT, (real)step) - DT * one_over_taubulk * zetav * theta,
                        u_new[0] + DT * one_over_taubulk);
-            bout = r_max(bout, -eos_P(p, e2[0]));   /* no negative effective pressure */
+            /* regulation (extension): |Pi| <= P/2, Pi = 0 where e < 1e-3 GeV/fm^3 */
+            real hp = RC(0.5f) * eos_P(p, e2[0]);
+            bout = r_min(bout, hp);
+            bout = r_max(bout, -hp);
+            if (e2[0] < RC(1e-3f)) bout = RC(0.0f);
             if ((real)fabs((double)bout) > max_pimn_abs) max_pimn_abs = (real)fabs((double)bout);
         }
         real pre = eos_P(p, e2[0]);

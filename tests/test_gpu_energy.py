@@ -31,7 +31,7 @@ def test_energy_sum_and_balance(nz, bulk):
     h.ideal_stage(1, np.float32(p.tau0))
     h.set('ev1', ev)
     op = orc_params(p, tab)
-    dV = 0.08 * 0.08 * dz
+    dV = float(p.dx) * float(p.dy) * float(p.dz)     # the device's FP32 spacings, multiplied in double
     bk = lambda k: h.get('bulk%d' % k) if bulk else None
     tau = lambda n: float(np.float32(p.tau0 + n * p.dt))
     res = []
