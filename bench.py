@@ -511,7 +511,7 @@ def main():
     ap.add_argument('--cpu-threads', type=int, default=0)
     ap.add_argument('--events', type=int, default=16, help='combined: number of 385x385x115 events')
     ap.add_argument('--max-loops', type=int, default=3000, help='combined: step limit')
-    ap.add_argument('--sf-capacity', type=int, default=4_000_000, help='combined: surface elements per event')
+    ap.add_argument('--sf-capacity', type=int, default=6_000_000, help='combined: surface elements per event')
     ap.add_argument('--sf-out', default='', help='combined: write hypersf.dat / sf_txyz.dat per event here')
     ap.add_argument('--workload', choices=['single', 'ensemble', 'surrogate', 'combined'], default='single',
                     help='single: one 385x385x115 event (the headline); ensemble: 64 uneven events batched; '

@@ -66,6 +66,23 @@ __device__ __forceinline__ float bulk_regulate(float b, float pr, float e) {
     b = rmax(b, -h);
     return e < CV_BULK_EMIN ? 0.0f : b;
 }
+// Second regulation of the bulk extension (zeta/s > 0 only): pi^{mu nu} and Pi scaled down
+// together so that the largest |component| is at most the enthalpy e + P (the reference's
+// bad-cell test against T^tautau alone lets |pi| reach gamma^2 (e + P) in the fast dilute
+// periphery, and the bulk pressure makes that region unstable).  max_abs follows.
+template <bool BULK>
+__device__ __forceinline__ void enthalpy_cap(float zmax, float out[10], float &bout, float &max_abs, float e, float pr) {
+    if (BULK && zmax > 0.0f) {
+        const float lim = e + pr;
+        if (max_abs > lim) {
+            const float sc = lim * (1.0f / max_abs);
+#pragma unroll
+            for (int m = 0; m < 10; m++) out[m] = out[m] * sc;
+            bout = bout * sc;
+            max_abs = lim;
+        }
+    }
+}
 __device__ __forceinline__ float rsign(float x) {
     return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : x);
 }

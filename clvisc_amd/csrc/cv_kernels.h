@@ -473,6 +473,7 @@ __global__ __launch_bounds__(256) void k_is_stage(PhysP p, EosP E, StageP st,
         if (fabsf(bout) > max_abs) max_abs = fabsf(bout);
     }
     float pre = eos.P(e2.x);
+    enthalpy_cap<BULK>(p.zmax, out, bout, max_abs, e2.x, pre);
     float T00 = (e2.x + pre) * u_new[0] * u_new[0] - pre;
     float gc = 1.0f;
     if (max_abs > T00) {
@@ -908,6 +909,7 @@ __global__ __launch_bounds__(256, CV_STAGE_WAVES) void k_visc_stage(PhysP p, Eos
     }
     {
         float pre = eos.P(e2.x);
+        enthalpy_cap<BULK>(p.zmax, out, bout, max_abs, e2.x, pre);
         float T00 = (e2.x + pre) * u_new[0] * u_new[0] - pre;
         float gc = 1.0f;
         if (max_abs > T00) {

@@ -493,6 +493,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
             }
             {
                 float pre = eos.P(e2.x);
+                enthalpy_cap<BULK>(p.zmax, out, bout, max_abs, e2.x, pre);
                 float T00 = (e2.x + pre) * u_new[0] * u_new[0] - pre;
                 float gc = 1.0f;
                 if (max_abs > T00) {

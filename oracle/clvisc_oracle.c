@@ -1073,6 +1073,17 @@ void orc_update_pimn(const orc_params *p, real *pinew, real *goodcell, const rea
             if ((real)fabs((double)bout) > max_pimn_abs) max_pimn_abs = (real)fabs((double)bout);
         }
         real pre = eos_P(p, e2[0]);
+        /* bulk extension with zeta/s > 0: pi and Pi scaled together so that the largest
+         * |component| is at most e + P (no reference counterpart) */
+        if (p->bulk_on && bulknew && p->zetaos_max > 0.0) {
+            real lim = e2[0] + pre;
+            if (max_pimn_abs > lim) {
+                real sc = DIV(lim, max_pimn_abs);
+                for (int m = 0; m < 10; m++) out[m] = out[m] * sc;
+                bout = bout * sc;
+                max_pimn_abs = lim;
+            }
+        }
         real T00 = (e2[0] + pre) * u_new[0] * u_new[0] - pre;
         goodcell[I] = RC(1.0f);
         if (max_pimn_abs > T00) {

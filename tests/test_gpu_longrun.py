@@ -1,6 +1,8 @@
 """Long evolutions of the 385x385x115 benchmark event to freeze-out on the device (the
 oracle is bit-identical over the steps it can afford; these check the evolution itself):
-max e decreases between checks and drops below e(T = 0.137 GeV) in finite time, for the
+max e never exceeds 4x the lowest max e before it (the reference's algorithm has short
+transient spikes in the dilute periphery, up to ~2x; the unregulated bulk extension ran away
+by orders of magnitude) and drops below e(T = 0.137 GeV) in finite time, for the
 reference's shear-only physics and with the regulated bulk pressure."""
 import sys
 
@@ -32,8 +34,10 @@ def test_benchmark_event_reaches_freeze_out(bulk):
         loop += 20
         hist.append(h.max_ed(1))
     tau = bench.TAU0 + loop * bench.DT
-    _say('  max e every 20 steps: %s' % ' '.join('%.3g' % m for m in hist[::5]))
+    _say('  max e every 20 steps: %s' % ' '.join('%.3g' % m for m in hist))
+    growth = max(hist[k] / min(hist[:k]) for k in range(1, len(hist)))
     _say('  frozen out at tau %.2f fm after %d steps' % (tau, loop))
     h.close()
     assert hist[-1] < p.efrz
-    assert all(b <= a for a, b in zip(hist, hist[1:])), hist
+    _say('  largest max e over the lowest before it: %.3f' % growth)
+    assert growth <= 4.0, hist
