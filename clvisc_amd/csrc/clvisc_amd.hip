@@ -65,6 +65,8 @@ struct cv_handle {
     float *Src = nullptr, *IS_src = nullptr, *bsrc = nullptr, *udiff = nullptr, *good = nullptr;
     unsigned int *d_max = nullptr;
     double *d_esum = nullptr;     // FP64 energy sums: block partials, then the two totals
+    FixEntry *d_fix = nullptr;    // deferred FP64 root solves (CV_FLAG_FP64_ROOT, tiled kernels)
+    unsigned *d_fix_ctr = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     bool timed = false;
@@ -253,6 +255,11 @@ extern "C" int cv_create(const cv_params *p, const cv_eos *eos, int device, int 
             for (int k = 0; ok && k < 3; k++) ok = alloc(&h->bk[k], n);
             ok = ok && alloc(&h->bsrc, n);
         }
+        // deferred FP64 root solves of the tiled stage kernels: n entries always suffice
+        if (ok && (p->flags & CV_FLAG_FP64_ROOT))
+            ok = hipMalloc((void **)&h->d_fix, sizeof(FixEntry) * n) == hipSuccess &&
+                 hipMalloc((void **)&h->d_fix_ctr, sizeof(unsigned)) == hipSuccess &&
+                 hipMemset(h->d_fix_ctr, 0, sizeof(unsigned)) == hipSuccess;
     }
     {
         hipDeviceProp_t prop;
@@ -284,6 +291,7 @@ extern "C" int cv_destroy(cv_handle *h) {
     (void)hipFree(h->Src); (void)hipFree(h->IS_src); (void)hipFree(h->bsrc);
     (void)hipFree(h->udiff); (void)hipFree(h->good);
     (void)hipFree(h->d_eos); (void)hipFree(h->d_max); (void)hipFree(h->d_esum);
+    (void)hipFree(h->d_fix); (void)hipFree(h->d_fix_ctr);
     (void)hipFree(h->sf_ev); (void)hipFree(h->sf_pi); (void)hipFree(h->sf_bk);
     (void)hipFree(h->sf_out); (void)hipFree(h->sf_txyz); (void)hipFree(h->sf_pisf); (void)hipFree(h->sf_bksf);
     (void)hipFree(h->sf_count);
@@ -481,28 +489,48 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
         const bool split = h->root_split;
         StagePtrs sp = {ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src,
                         h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out,
-                        split ? h->seed_buf : nullptr};
+                        split ? h->seed_buf : nullptr, h->d_fix, h->d_fix_ctr};
 #define CV_TILED_ARGS h->phys, h->eos, st, h->tile, sp
-        const bool spec = split ||
-                          (h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST | CV_FLAG_FP64_ROOT)) != 0;
+        // optional branches: none, the FP64 root alone, or all of them
+        const bool tests = (h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST)) != 0;
+        const bool fp64 = (h->prm.flags & CV_FLAG_FP64_ROOT) != 0;
+        const int spec = (split || tests) ? CV_SPEC_ALL : (fp64 ? CV_SPEC_FP64 : 0);
+        // FP64 cells are deferred to k_fp64_fix unless the split stage solves every cell later
+        const bool defer = fp64 && !split;
+        if (defer) hipLaunchKernelGGL(k_fix_reset, dim3(1), dim3(64), 0, h->stream, h->d_fix_ctr);
         const int ek = h->eos.kind;
 #define CV_TILED_LAUNCH2(B, S, K, X, M) hipLaunchKernelGGL((k_visc_tiled<B, S, K, X, M>), g, b, 0, h->stream, CV_TILED_ARGS)
 #define CV_TILED_LAUNCH(B, S, K, X) \
         do { if (march) CV_TILED_LAUNCH2(B, S, K, X, true); else CV_TILED_LAUNCH2(B, S, K, X, false); } while (0)
-#define CV_TILED_SEL(B, S)                                      \
-        if (ek == 1) { if (spec) CV_TILED_LAUNCH(B, S, 1, true); else CV_TILED_LAUNCH(B, S, 1, false); } \
-        else { if (spec) CV_TILED_LAUNCH(B, S, 0, true); else CV_TILED_LAUNCH(B, S, 0, false); }
+#define CV_TILED_SEL_K(B, S, K)                                                                  \
+        do { if (spec == CV_SPEC_ALL) CV_TILED_LAUNCH(B, S, K, CV_SPEC_ALL);                         \
+             else if (spec == CV_SPEC_FP64) CV_TILED_LAUNCH(B, S, K, CV_SPEC_FP64);                  \
+             else CV_TILED_LAUNCH(B, S, K, 0); } while (0)
+#define CV_TILED_SEL(B, S) \
+        if (ek == 1) CV_TILED_SEL_K(B, S, 1); else CV_TILED_SEL_K(B, S, 0);
         if (h->bulk) {
             if (step == 1) { CV_TILED_SEL(true, 1) } else { CV_TILED_SEL(true, 2) }
         } else {
             if (step == 1) { CV_TILED_SEL(false, 1) } else { CV_TILED_SEL(false, 2) }
         }
 #undef CV_TILED_SEL
+#undef CV_TILED_SEL_K
 #undef CV_TILED_LAUNCH
 #undef CV_TILED_LAUNCH2
 #undef CV_TILED_ARGS
         HIP_TRY(hipGetLastError());
         if (split) return launch_visc_root(h, step, ev1, bk_out, ev_out, udiff_out);
+        if (defer) {
+            dim3 fg(4 * (h->cus > 0 ? h->cus : 256)), fb(256);
+#define CV_FIX_LAUNCH(B, S, K) hipLaunchKernelGGL((k_fp64_fix<B, S, K>), fg, fb, 0, h->stream, h->phys, h->eos, \
+                                                  h->d_fix, h->d_fix_ctr, ev1, ev_out, udiff_out)
+#define CV_FIX_SEL(B, S) do { if (ek == 1) CV_FIX_LAUNCH(B, S, 1); else CV_FIX_LAUNCH(B, S, 0); } while (0)
+            if (h->bulk) { if (step == 1) CV_FIX_SEL(true, 1); else CV_FIX_SEL(true, 2); }
+            else { if (step == 1) CV_FIX_SEL(false, 1); else CV_FIX_SEL(false, 2); }
+#undef CV_FIX_SEL
+#undef CV_FIX_LAUNCH
+            HIP_TRY(hipGetLastError());
+        }
         return CV_OK;
     }
     if (h->root_split) return fail(CV_ERR_UNSUPPORTED, "split root solve needs a tiled stage kernel (kind 1 or 2)");
@@ -999,6 +1027,7 @@ static int batch_build(cv_batch *b) {
         t.work0 = w;
         for (int k = 0; k < 3; k++) { t.ev[k] = h->ev[k]; t.pi[k] = h->pi[k]; t.bk[k] = h->bk[k]; }
         t.Src = h->Src; t.IS_src = h->IS_src; t.bsrc = h->bsrc; t.udiff = h->udiff; t.good = h->good;
+        t.fix = h->d_fix; t.fix_ctr = h->d_fix_ctr;
         w += h->tile.nwork;
     }
     if (w >= ((int64_t)1 << 31)) return fail(CV_ERR_CAPACITY, "batch: %lld work items exceed 2^31", (long long)w);
@@ -1029,12 +1058,29 @@ static int batch_build(cv_batch *b) {
                        b->d_ctr)
 #define CV_BATCH_LAUNCH(B, S)                                                              \
     do {                                                                                   \
-        if (b->ek == 1) { if (b->fp64_root) CV_BATCH_LAUNCH2(B, S, 1, true); else CV_BATCH_LAUNCH2(B, S, 1, false); } \
-        else { if (b->fp64_root) CV_BATCH_LAUNCH2(B, S, 0, true); else CV_BATCH_LAUNCH2(B, S, 0, false); }            \
+        if (b->ek == 1) { if (b->fp64_root) CV_BATCH_LAUNCH2(B, S, 1, CV_SPEC_FP64); else CV_BATCH_LAUNCH2(B, S, 1, 0); } \
+        else { if (b->fp64_root) CV_BATCH_LAUNCH2(B, S, 0, CV_SPEC_FP64); else CV_BATCH_LAUNCH2(B, S, 0, 0); }            \
     } while (0)
-    if (b->bulk) CV_BATCH_LAUNCH(true, 1); else CV_BATCH_LAUNCH(false, 1);
+    // deferred FP64 root solves of each stage, per event
+    const int nact = b->nact;
+    dim3 fg(std::max(1, 4 * (b->ev[0]->cus > 0 ? b->ev[0]->cus : 256) / std::max(1, nact)), nact), fb(256);
+#define CV_BFIX_LAUNCH(B, S, K) hipLaunchKernelGGL((k_batch_fp64_fix<B, S, K>), fg, fb, 0, b->stream, b->d_state, \
+                                                   b->d_events, b->ev[0]->eos)
+#define CV_BATCH_FIX(B, S)                                                                                    \
+    do {                                                                                                      \
+        if (b->fp64_root) { if (b->ek == 1) CV_BFIX_LAUNCH(B, S, 1); else CV_BFIX_LAUNCH(B, S, 0); }          \
+    } while (0)
+    if (b->fp64_root)
+        hipLaunchKernelGGL(k_batch_fix_reset, dim3((nact + 63) / 64), dim3(64), 0, b->stream, b->d_state, b->d_events);
+    if (b->bulk) { CV_BATCH_LAUNCH(true, 1); CV_BATCH_FIX(true, 1); }
+    else { CV_BATCH_LAUNCH(false, 1); CV_BATCH_FIX(false, 1); }
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_ctr);
-    if (b->bulk) CV_BATCH_LAUNCH(true, 2); else CV_BATCH_LAUNCH(false, 2);
+    if (b->fp64_root)
+        hipLaunchKernelGGL(k_batch_fix_reset, dim3((nact + 63) / 64), dim3(64), 0, b->stream, b->d_state, b->d_events);
+    if (b->bulk) { CV_BATCH_LAUNCH(true, 2); CV_BATCH_FIX(true, 2); }
+    else { CV_BATCH_LAUNCH(false, 2); CV_BATCH_FIX(false, 2); }
+#undef CV_BATCH_FIX
+#undef CV_BFIX_LAUNCH
 #undef CV_BATCH_LAUNCH
 #undef CV_BATCH_LAUNCH2
     hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, b->stream, b->d_state);

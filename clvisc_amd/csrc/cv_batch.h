@@ -17,6 +17,8 @@ struct BatchEvent {
     int64_t work0;                     // first work item of this event in the list
     float *ev[3], *pi[3], *bk[3];      // physical buffers of slots 0..2 at step parity 0
     float *Src, *IS_src, *bsrc, *udiff, *good;
+    FixEntry *fix;                     // deferred FP64 root solves (CV_FLAG_FP64_ROOT), n entries
+    unsigned *fix_ctr;
 };
 
 struct BatchState {
@@ -35,8 +37,8 @@ __global__ void k_batch_advance(BatchState *bs) {
     if (threadIdx.x == 0 && blockIdx.x == 0) bs->step = bs->step + 1;
 }
 
-// SPEC: the FP64 root-solve branch for fast cells (CV_FLAG_FP64_ROOT) is compiled in.
-template <bool BULK, int STEP, int EK, bool SPEC>
+// SPEC: 0 or CV_SPEC_FP64 (the FP64 root-solve branch for fast cells, CV_FLAG_FP64_ROOT).
+template <bool BULK, int STEP, int EK, int SPEC>
 __global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchState *__restrict__ bs,
                                                                      const BatchEvent *__restrict__ evt, EosP E,
                                                                      unsigned long long *ctr) {
@@ -79,13 +81,33 @@ __global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchS
         StagePtrs q;
         if (STEP == 1) {
             q = StagePtrs{E1, P1, B1, E1, E2, P1, B1, ev.udiff, ev.IS_src, ev.bsrc, ev.Src, P2, B2, E2, ev.good,
-                          nullptr, nullptr};
+                          nullptr, nullptr, ev.fix, ev.fix_ctr};
         } else {
             q = StagePtrs{E2, P2, B2, E1, E2, P1, B1, ev.udiff, ev.IS_src, ev.bsrc, ev.Src, P0, B0, E0, ev.good,
-                          ev.udiff, nullptr};
+                          ev.udiff, nullptr, ev.fix, ev.fix_ctr};
         }
         tile_work<BULK, STEP, EK, SPEC, true>(ev.p, eos, st, ev.tp, q, w - ev.work0, s_c, s_f);
     }
+}
+
+// Deferred FP64 root solves of the active events (blockIdx.y = entry of the event table),
+// with the slots of this step's parity as k_batch_stage uses them.
+template <bool BULK, int STEP, int EK>
+__global__ __launch_bounds__(256) void k_batch_fp64_fix(const BatchState *__restrict__ bs,
+                                                        const BatchEvent *__restrict__ evt, EosP E) {
+    if ((int)blockIdx.y >= bs->nevents) return;
+    const BatchEvent &ev = evt[blockIdx.y];
+    const int par = (int)(bs->step & 1);
+    float *E0 = ev.ev[par], *E1 = ev.ev[1 - par], *E2 = ev.ev[2];
+    float *out = STEP == 1 ? E2 : E0;
+    const unsigned cnt = *ev.fix_ctr;
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x)
+        fp64_fix_cell<BULK, STEP, EK>(ev.p, E, ev.fix[q], E1, out, ev.udiff);
+}
+
+__global__ void k_batch_fix_reset(const BatchState *__restrict__ bs, const BatchEvent *__restrict__ evt) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < bs->nevents) *evt[e].fix_ctr = 0u;
 }
 
 }  // namespace cv

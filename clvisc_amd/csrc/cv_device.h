@@ -302,8 +302,10 @@ __device__ __forceinline__ double eos_P_d(const EosP &E, const float4 *tab, doub
     return 0.3327 * eps - 0.3223 * pow(eps, 0.4585) - 0.003906 * eps * exp(-0.05697 * eps);
 }
 
-__device__ __forceinline__ double root_newton_d(const EosP &E, const float4 *tab, double T00, double M, double bulk,
-                                                bool clamp0 = false) {
+// Out of line: few cells take it, and inlined its double registers would spill in the
+// stage kernels that carry the branch.
+__device__ __noinline__ double root_newton_d(const EosP &E, const float4 *tab, double T00, double M, double bulk,
+                                             bool clamp0 = false) {
     double vl = 0.0, vh = 1.0, dpe = 1.0 / 3.0;
     double v = 0.5;
     double ed = T00 - M * v;

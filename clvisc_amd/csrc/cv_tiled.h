@@ -109,28 +109,45 @@ __device__ __forceinline__ void face_flux(const PhysP &p, const EOS &eos, float 
     }
 }
 
+// A cell whose root solve runs in FP64 (CV_FLAG_FP64_ROOT, M > fp64_vmin T^tt), recorded by
+// the stage kernel after the |T^ti| <= 0.999 T^tt clamp and finished by k_fp64_fix.
+struct FixEntry {
+    uint32_t cell;
+    float T00, T01, T02, T03, M, bulk, pad;
+};
+
 // Field pointers of one RK stage (the buffers of CLVisc.IS_stepUpdate / visc_stepUpdate).
 struct StagePtrs {
     const float *ev_st, *pi_st, *bk_st;        // d_ev[step], d_pi[step], Pi[step]: the stencil
     const float *ev1, *ev2, *pi1, *bk1;        // d_ev[1], d_ev[2], d_pi[1], Pi[1]
     const float *udiff_in;
     float *IS_src, *bsrc, *Src, *pi_out, *bk_out, *ev_out, *good, *udiff_out;
-    // split root solve (SPEC kernels only): when set, the stage writes the root solve's
+    // split root solve (SPEC & CV_SPEC_SPLIT kernels only): when set, the stage writes the root solve's
     // input (T^tt, T^tx, T^ty, T^t eta before the |T^ti| <= 0.999 T^tt clamp) to four planes
     // here instead of solving; k_visc_root finishes the cell (ev_out, udiff_out)
     float *cons_out;
+    // deferred FP64 root solves (SPEC & CV_SPEC_FP64 kernels): list of n entries and its count
+    FixEntry *fix;
+    unsigned *fix_ctr;
 };
 
+// Optional branches of the stage kernels (template mask SPEC; flags and pointers are still
+// tested at run time).  Each costs registers in the whole kernel, so a kernel carries only
+// what its caller asks for.
+#define CV_SPEC_TESTS 1      // GUBSER_VISC_TEST / RIEMANN_TEST
+#define CV_SPEC_FP64 2       // FP64 root solve for fast cells
+#define CV_SPEC_SPLIT 4      // split root solve (StagePtrs::cons_out)
+#define CV_SPEC_ALL 7
+
 // One work item of the stage.  All threads of the workgroup call it (it synchronises).
-// EK: EOS table kind (0 row/col, 1 octave); SPEC: the GUBSER_VISC_TEST / RIEMANN_TEST /
-// FP64-root branches are compiled in (flags are still tested at run time).
+// EK: EOS table kind (0 row/col, 1 octave); SPEC: mask of the optional branches above.
 // MARCH = false: a work item is one tile of one x-plane; every thread evaluates its two x
 // faces from its own x-stencil (cells i-2..i+2, read from global memory).
 // MARCH = true: a work item is one tile column over a chunk of xc consecutive x-planes; the
 // workgroup walks the chunk in increasing x and each thread carries the x face F(i-1/2),
 // the slopes of its cell i and u^mu of cell i-1 from one plane to the next, so every x face
 // and every x slope is evaluated once (the chunk's first plane starts from the full stencil).
-template <bool BULK, int STEP, int EK, bool SPEC, bool MARCH>
+template <bool BULK, int STEP, int EK, int SPEC, bool MARCH>
 __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, const StageP &st, const TileP &tp,
                                           const StagePtrs &q_, int64_t work, float (*s_c)[CV_TILE_MAXC],
                                           float (*s_f)[CV_TILE_MAXF]) {
@@ -247,7 +264,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
                 float Tzz = (ec.x + pressure) * u0 * u0 * ec.w * ec.w + pressure + q[9];
                 float Ttz = (ec.x + pressure) * u0 * u0 * ec.w + q[3];
                 t[0] = t[0] - Tzz; t[1] = t[1] - 0.0f; t[2] = t[2] - 0.0f; t[3] = t[3] - Ttz;
-                if (SPEC && (p.flags & 1)) {
+                if ((SPEC & CV_SPEC_TESTS) && (p.flags & 1)) {
                     float Ttt = (ec.x + pressure) * u0 * u0 - pressure;
                     float Ttx = (ec.x + pressure) * u0 * u0 * ec.y;
                     float Tty = (ec.x + pressure) * u0 * u0 * ec.z;
@@ -439,7 +456,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
             float etav = local_etaos * cpTs.w * CV_HBARC;
             float one_over_taupi = temperature * rcp(3.0f * rmax(CV_ACU, local_etaos) * CV_HBARC);
             float coef_pipi = 0.0f;
-            const bool gubser = SPEC && (p.flags & 2);
+            const bool gubser = (SPEC & CV_SPEC_TESTS) && (p.flags & 2);
             if (gubser) {
                 float q4 = sqrtf(sqrtf(ed_step));
                 float ETAOS = p.etaos_ymin;
@@ -529,7 +546,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
             float T01 = fabsf(T0m[1]) < CV_ACU ? 0.0f : T0m[1] * st.r_tau;
             float T02 = fabsf(T0m[2]) < CV_ACU ? 0.0f : T0m[2] * st.r_tau;
             float T03 = fabsf(T0m[3]) < CV_ACU ? 0.0f : T0m[3] * st.r_tau;
-            if (SPEC && q_.cons_out) {
+            if ((SPEC & CV_SPEC_SPLIT) && q_.cons_out) {
                 stp(q_.cons_out, bo, nb, 0, T00);
                 stp(q_.cons_out, bo, nb, 1, T01);
                 stp(q_.cons_out, bo, nb, 2, T02);
@@ -541,11 +558,13 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
                 T01 *= sc; T02 *= sc; T03 *= sc;
                 M = 0.999f * T00;
             }
-            float ed_find;
-            if (SPEC && (p.flags & 8) && M > p.fp64_vmin * T00)
-                ed_find = (float)root_newton_d(E, tab, (double)T00, (double)M, (double)bout, BULK);
-            else
-                ed_find = root_newton(eos, T00, M, bout, BULK);
+            if ((SPEC & CV_SPEC_FP64) && (p.flags & 8) && M > p.fp64_vmin * T00) {
+                // FP64 root solve deferred to k_fp64_fix (kept out of this kernel's registers);
+                // each cell is recorded at most once per stage, so n entries always suffice
+                const unsigned slot = atomicAdd(q_.fix_ctr, 1u);
+                q_.fix[slot] = FixEntry{bo >> 2, T00, T01, T02, T03, M, bout, 0.0f};
+            } else {
+            float ed_find = root_newton(eos, T00, M, bout, BULK);
             if (ed_find < CV_ACU) { T01 = 0.0f; T02 = 0.0f; T03 = 0.0f; }
             ed_find = rmax(0.0f, ed_find);
             float pr = eos.P(ed_find) + bout;
@@ -563,13 +582,14 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
                 for (int c = 0; c < 4; c++) stp(udiff_out, bo, nb, c, un[c] - u_old[c]);
             }
             }
+            }
         }
         __syncthreads();   // LDS reused by the next plane / tile
       }
     }
 }
 
-template <bool BULK, int STEP, int EK, bool SPEC, bool MARCH>
+template <bool BULK, int STEP, int EK, int SPEC, bool MARCH>
 __global__ __launch_bounds__(CV_TILE_THREADS, MARCH ? 3 : 2) void k_visc_tiled(PhysP p, EosP E, StageP st, TileP tp,
                                                                               StagePtrs q) {
     __shared__ float s_c[CF_N][CV_TILE_MAXC];
@@ -638,6 +658,49 @@ __global__ __launch_bounds__(256) void k_visc_root(PhysP p, EosP E, const float 
 #pragma unroll
         for (int c = 0; c < 4; c++) udiff_out[c * n + I] = un[c] - u_old[c];
     }
+}
+
+// The end of update_ev for one deferred cell: FP64 root solve, then the same FP32 tail as
+// the stage kernel (kernel_visc.cl:375-392), so the fields are those of the inline solve.
+template <bool BULK, int STEP, int EK>
+__device__ __forceinline__ void fp64_fix_cell(const PhysP &p, const EosP &E, const FixEntry &f,
+                                              const float *__restrict__ ev1, float *ev_out, float *udiff_out) {
+    EosT<EK> eos(E, E.tab);
+    const int64_t n = p.n, I = f.cell;
+    float T01 = f.T01, T02 = f.T02, T03 = f.T03;
+    float ed_find = (float)root_newton_d(E, E.tab, (double)f.T00, (double)f.M, (double)f.bulk, BULK);
+    if (ed_find < CV_ACU) { T01 = 0.0f; T02 = 0.0f; T03 = 0.0f; }
+    ed_find = rmax(0.0f, ed_find);
+    float pr = eos.P(ed_find) + f.bulk;
+    if (BULK) pr = rmax(pr, 0.0f);
+    float repv = rcp(rmax(CV_ACU, f.T00 + pr));
+    float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
+    ev_out[I] = en.x;
+    ev_out[n + I] = en.y;
+    ev_out[2 * n + I] = en.z;
+    ev_out[3 * n + I] = en.w;
+    if (STEP == 2) {
+        const float v1x = ev1[n + I], v1y = ev1[2 * n + I], v1z = ev1[3 * n + I];
+        const float g1 = gamma3(v1x, v1y, v1z);
+        const float u_old[4] = {g1 * 1.0f, g1 * v1x, g1 * v1y, g1 * v1z};
+        float gn = gamma3(en.y, en.z, en.w);
+        float un[4] = {gn * 1.0f, gn * en.y, gn * en.z, gn * en.w};
+#pragma unroll
+        for (int c = 0; c < 4; c++) udiff_out[c * n + I] = un[c] - u_old[c];
+    }
+}
+
+template <bool BULK, int STEP, int EK>
+__global__ __launch_bounds__(256) void k_fp64_fix(PhysP p, EosP E, const FixEntry *__restrict__ fix,
+                                                  const unsigned *__restrict__ ctr, const float *__restrict__ ev1,
+                                                  float *ev_out, float *udiff_out) {
+    const unsigned cnt = *ctr;
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += gridDim.x * blockDim.x)
+        fp64_fix_cell<BULK, STEP, EK>(p, E, fix[q], ev1, ev_out, udiff_out);
+}
+
+__global__ void k_fix_reset(unsigned *ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *ctr = 0u;
 }
 
 }  // namespace cv
