@@ -491,10 +491,11 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
                         h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out,
                         split ? h->seed_buf : nullptr, h->d_fix, h->d_fix_ctr};
 #define CV_TILED_ARGS h->phys, h->eos, st, h->tile, sp
-        // optional branches: none, the FP64 root alone, or all of them
+        // optional branches: none, the FP64 root alone, the split alone (k_visc_root solves the
+        // FP64 cells of a split stage), or all of them (test flags)
         const bool tests = (h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST)) != 0;
         const bool fp64 = (h->prm.flags & CV_FLAG_FP64_ROOT) != 0;
-        const int spec = (split || tests) ? CV_SPEC_ALL : (fp64 ? CV_SPEC_FP64 : 0);
+        const int spec = tests ? CV_SPEC_ALL : (split ? CV_SPEC_SPLIT : (fp64 ? CV_SPEC_FP64 : 0));
         // FP64 cells are deferred to k_fp64_fix unless the split stage solves every cell later
         const bool defer = fp64 && !split;
         if (defer) hipLaunchKernelGGL(k_fix_reset, dim3(1), dim3(64), 0, h->stream, h->d_fix_ctr);
@@ -505,6 +506,7 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
 #define CV_TILED_SEL_K(B, S, K)                                                                  \
         do { if (spec == CV_SPEC_ALL) CV_TILED_LAUNCH(B, S, K, CV_SPEC_ALL);                         \
              else if (spec == CV_SPEC_FP64) CV_TILED_LAUNCH(B, S, K, CV_SPEC_FP64);                  \
+             else if (spec == CV_SPEC_SPLIT) CV_TILED_LAUNCH(B, S, K, CV_SPEC_SPLIT);                \
              else CV_TILED_LAUNCH(B, S, K, 0); } while (0)
 #define CV_TILED_SEL(B, S) \
         if (ek == 1) CV_TILED_SEL_K(B, S, 1); else CV_TILED_SEL_K(B, S, 0);

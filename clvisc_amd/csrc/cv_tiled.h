@@ -560,8 +560,15 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
             }
             if ((SPEC & CV_SPEC_FP64) && (p.flags & 8) && M > p.fp64_vmin * T00) {
                 // FP64 root solve deferred to k_fp64_fix (kept out of this kernel's registers);
-                // each cell is recorded at most once per stage, so n entries always suffice
-                const unsigned slot = atomicAdd(q_.fix_ctr, 1u);
+                // each cell is recorded at most once per stage, so n entries always suffice.
+                // One atomic per wavefront: the deferring lanes take consecutive slots.
+                const uint64_t mask = __ballot(1);
+                const int lane = (int)__lane_id();
+                const int leader = __ffsll((unsigned long long)mask) - 1;
+                unsigned base = 0u;
+                if (lane == leader) base = atomicAdd(q_.fix_ctr, (unsigned)__popcll(mask));
+                base = __shfl(base, leader, 64);
+                const unsigned slot = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
                 q_.fix[slot] = FixEntry{bo >> 2, T00, T01, T02, T03, M, bout, 0.0f};
             } else {
             float ed_find = root_newton(eos, T00, M, bout, BULK);
