@@ -86,6 +86,7 @@ def lib():
             'cv_visc_steps_async': [vp, i64, i64],
             'cv_max_ed': [vp, C.c_int, C.POINTER(f32)],
             'cv_energy_sum': [vp, C.c_int, C.c_double, vp],
+            'cv_slice_sums': [vp, C.c_int, vp],
             'cv_sync': [vp],
             'cv_last_elapsed_ms': [vp, C.POINTER(f32)],
             'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
@@ -149,7 +150,7 @@ def exported_symbols():
             'cv_batch_steps_async', 'cv_batch_set_loop', 'cv_batch_sync', 'cv_batch_last_elapsed_ms',
             'cv_batch_info', 'cv_mlp_nparam', 'cv_mlp_create', 'cv_mlp_destroy', 'cv_mlp_get_params',
             'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward',
-            'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum']
+            'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums']
 
 
 def make_eos(table):
@@ -251,6 +252,13 @@ class Handle(object):
         out = np.zeros(2, np.float64)
         check(self.L.cv_energy_sum(self.h, which, float(tau), out.ctypes.data))
         return float(out[0]), float(out[1])
+
+    def slice_sums(self, which=1):
+        """(NZ, 6) float64 per-eta-slice sums of Txx - Tyy, Txx + Tyy, T^tx, u0 s, u0 e |v_T|,
+        u0 e over the (x, y) plane (cv_slice_sums)."""
+        out = np.zeros((self.params.nz, 6), np.float64)
+        check(self.L.cv_slice_sums(self.h, which, out.ctypes.data))
+        return out
 
     def max_ed(self, which=1):
         out = C.c_float()

@@ -1,9 +1,12 @@
 """BulkInfo: slices and bulk observables along the evolution, with the attributes and
 output files of the reference's pyvisc/bulkinfo.py:26-340.
 
-Slices come from the device field proxies (``DeviceField.get()``); the observables
-(eccentricities, <v_r>, total entropy) are the reference's NumPy formulas
-(pyvisc/bulkinfo.py:203-266).
+The reference copies the whole field to the host at every call and evaluates the
+eccentricities of every eta slice in NumPy (pyvisc/bulkinfo.py:129-260).  Here the slices
+are gathered on the device (``DeviceField.gather``: the three lines and two planes, not the
+field) and the per-slice sums behind eccp, <v_r> and the entropy come from one device
+reduction (``cv_slice_sums``); ``device_reductions=False`` keeps the reference's host path
+(whole-field copy, NumPy formulas) for comparison.
 """
 from __future__ import annotations
 
@@ -16,9 +19,11 @@ import numpy as np
 class BulkInfo(object):
     """ed(x), ed(y), ed(eta), T, v slices, eccentricities vs rapidity (bulkinfo.py:26)."""
 
-    def __init__(self, cfg, eos):
+    def __init__(self, cfg, eos, device_reductions=True):
         self.cfg = cfg
         self.eos = eos
+        self.device_reductions = device_reductions
+        self.slice_sums = []          # per call: (NZ, 6) device sums (cv_slice_sums)
         NX, NY, NZ = cfg.NX, cfg.NY, cfg.NZ
         self.ex, self.ey, self.ez = [], [], []
         self.vx, self.vy, self.vz = [], [], []
@@ -39,17 +44,40 @@ class BulkInfo(object):
         self.time.append(tau)
         self.edmax.append(edmax)
         NX, NY, NZ = self.cfg.NX, self.cfg.NY, self.cfg.NZ
-        bulk = d_ev1.get().reshape(NX, NY, NZ, 4)
-        self._ecc_vs_rapidity(bulk)
         i0, j0, k0 = NX // 2, NY // 2, NZ // 2
-        evx0, evy0, evz0 = bulk[:, j0, k0], bulk[i0, :, k0], bulk[i0, j0, :]
+        if self.device_reductions and getattr(d_ev1, 'field_id', None) in (0, 1, 2):
+            self._get_device(d_ev1, i0, j0, k0)
+        else:
+            bulk = d_ev1.get().reshape(NX, NY, NZ, 4)
+            self._ecc_vs_rapidity(bulk)
+            self._slices(bulk[:, j0, k0], bulk[i0, :, k0], bulk[i0, j0, :], bulk[:, :, k0], bulk[:, j0, :])
+        if d_pi is not None:
+            self._pimn(d_pi)
+
+    def _get_device(self, d_ev1, i0, j0, k0):
+        """Lines and planes gathered on the device; eccentricities from cv_slice_sums."""
+        NX, NY, NZ = self.cfg.NX, self.cfg.NY, self.cfg.NZ
+        ii, jj, kk = np.arange(NX), np.arange(NY), np.arange(NZ)
+        idx = lambda i, j, k: (np.asarray(i) * NY + np.asarray(j)) * NZ + np.asarray(k)
+        gx, gy = np.meshgrid(ii, jj, indexing='ij')
+        hx, hz = np.meshgrid(ii, kk, indexing='ij')
+        cells = [idx(ii, j0, k0), idx(i0, jj, k0), idx(i0, j0, kk), idx(gx, gy, k0).reshape(-1),
+                 idx(hx, j0, hz).reshape(-1)]
+        rows = d_ev1.gather(np.concatenate(cells))
+        parts = np.split(rows, np.cumsum([c.size for c in cells])[:-1])
+        self._slices(parts[0], parts[1], parts[2], parts[3].reshape(NX, NY, 4), parts[4].reshape(NX, NZ, 4))
+        s = d_ev1.handle.slice_sums(d_ev1.field_id)
+        self.slice_sums.append(s)
+        self.ecc1_vs_rapidity.append(s[:, 2] / s[:, 1])
+        self.ecc2_vs_rapidity.append(s[:, 0] / s[:, 1])
+
+    def _slices(self, evx0, evy0, evz0, evxy, evxz):
         self.ex.append(evx0[:, 0].copy())
         self.ey.append(evy0[:, 0].copy())
         self.ez.append(evz0[:, 0].copy())
         self.vx.append(evx0[:, 1].copy())
         self.vy.append(evy0[:, 2].copy())
         self.vz.append(evz0[:, 3].copy())
-        evxy, evxz = bulk[:, :, k0], bulk[:, j0, :]
         self.exy.append(evxy[..., 0].copy())
         self.vx_xy.append(evxy[..., 1].copy())
         self.vy_xy.append(evxy[..., 2].copy())
@@ -57,15 +85,18 @@ class BulkInfo(object):
         self.vx_xz.append(evxz[..., 1].copy())
         self.vy_xz.append(evxz[..., 2].copy())
         self.vz_xz.append(evxz[..., 3].copy())
-        if d_pi is not None:
-            # the reference reads cell gid_x*NY*(NZ/2) + gid_y*(NZ/2) + NZ/2 (bulkinfo.py:112),
-            # which is the mid-eta plane only for NZ == 1; kept as written
-            gx, gy = np.meshgrid(np.arange(NX), np.arange(NY), indexing='ij')
-            oid = (gx * NY * (NZ // 2) + gy * (NZ // 2) + NZ // 2).reshape(-1)
-            pi = d_pi.gather(oid)
-            self.pixx_xy.append(pi[:, 4].reshape(NX, NY))
-            self.piyy_xy.append(pi[:, 7].reshape(NX, NY))
-            self.pitx_xy.append(pi[:, 1].reshape(NX, NY))
+
+    def _pimn(self, d_pi):
+        """pi^xx, pi^yy, pi^tx of the get_pimn kernel's cells (bulkinfo.py:105-121)."""
+        NX, NY, NZ = self.cfg.NX, self.cfg.NY, self.cfg.NZ
+        # the reference reads cell gid_x*NY*(NZ/2) + gid_y*(NZ/2) + NZ/2 (bulkinfo.py:112),
+        # which is the mid-eta plane only for NZ == 1; kept as written
+        gx, gy = np.meshgrid(np.arange(NX), np.arange(NY), indexing='ij')
+        oid = (gx * NY * (NZ // 2) + gy * (NZ // 2) + NZ // 2).reshape(-1)
+        pi = d_pi.gather(oid)
+        self.pixx_xy.append(pi[:, 4].reshape(NX, NY))
+        self.piyy_xy.append(pi[:, 7].reshape(NX, NY))
+        self.pitx_xy.append(pi[:, 1].reshape(NX, NY))
 
     def eccp(self, ed, vx, vy, vz=0.0, pixx=None, piyy=None, pitx=None):
         """(<T^0x>/<Txx+Tyy>, <Txx-Tyy>/<Txx+Tyy>) (bulkinfo.py:203-229)."""

@@ -745,6 +745,29 @@ extern "C" int cv_energy_sum(cv_handle *h, int which, double tau, double *out) {
     return CV_OK;
 }
 
+extern "C" int cv_slice_sums(cv_handle *h, int which, double *out) {
+    if (!h || !out) return fail(CV_ERR_ARG, "null argument");
+    if (which < 0 || which > 2) return fail(CV_ERR_ARG, "which must be 0, 1 or 2");
+    HIP_TRY(hipSetDevice(h->device));
+    const int nz = h->prm.nz;
+    const int64_t nrow = (int64_t)h->prm.nx * h->prm.ny;
+    const int nb = (int)std::min<int64_t>(1024, nrow);
+    const int64_t rpb = (nrow + nb - 1) / nb;
+    const size_t nq = (size_t)nz * CV_SLICE_Q;
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(double) * (nq * (size_t)nb + nq)));
+    hipLaunchKernelGGL(k_slice_sums, dim3(nb), dim3(128), 0, h->stream, h->phys, h->eos, h->ev[which], rpb, d);
+    hipLaunchKernelGGL(k_slice_final, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, h->stream, d, nb, nz,
+                       d + nq * (size_t)nb);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d + nq * (size_t)nb, sizeof(double) * nq, hipMemcpyDeviceToHost,
+                                            h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_slice_sums: %s", hipGetErrorString(e));
+    return CV_OK;
+}
+
 extern "C" int cv_sync(cv_handle *h) {
     if (!h) return fail(CV_ERR_ARG, "null handle");
     HIP_TRY(hipSetDevice(h->device));

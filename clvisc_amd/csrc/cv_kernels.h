@@ -1067,6 +1067,51 @@ __global__ void k_energy_final(const double *__restrict__ part, int nb, double d
     }
 }
 
+// BulkInfo observables per eta slice (bulkinfo.py:203-260, ecc_vs_rapidity :247-260): for
+// every k, sums over the (x, y) plane of Txx - Tyy, Txx + Tyy and T^tx (eccp), u0 s (total
+// entropy), u0 e |v_T| and u0 e (<v_r>), with e floored at 1e-10 and v^2 > 1 replaced by
+// 0.999999 as the reference's NumPy code does; P, s from the FP32 table, sums in double.
+// Block b covers rows (i, j) [b R, (b + 1) R); thread k of the block walks them at its k, so
+// a warp reads consecutive cells.  part[b][k][q]; k_slice_final adds the blocks in order.
+#define CV_SLICE_Q 6
+__global__ __launch_bounds__(128) void k_slice_sums(PhysP p, EosP E, const float *__restrict__ ev,
+                                                    int64_t rows_per_block, double *part) {
+    Eos eos(E, E.tab);
+    const int64_t n = p.n, nrow = (int64_t)p.nx * p.ny;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+    const int64_t r1 = r0 + rows_per_block < nrow ? r0 + rows_per_block : nrow;
+    for (int k = threadIdx.x; k < p.nz; k += blockDim.x) {
+        double s[CV_SLICE_Q] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int64_t r = r0; r < r1; r++) {
+            const int64_t I = r * p.nz + k;
+            const float ef = ev[I] < 1.0e-10f ? 1.0e-10f : ev[I];
+            const double e = ef, vx = ev[n + I], vy = ev[2 * n + I], vz = ev[3 * n + I];
+            const double pr = (double)eos.P(ef);
+            double vr2 = vx * vx + vy * vy + vz * vz;
+            if (vr2 > 1.0) vr2 = 0.999999;
+            const double u0 = 1.0 / sqrt(1.0 - vr2);
+            const double w = (e + pr) * u0 * u0;
+            const double txx = w * vx * vx + pr, tyy = w * vy * vy + pr;
+            s[0] += txx - tyy;
+            s[1] += txx + tyy;
+            s[2] += w * vx;
+            s[3] += u0 * (double)eos.S(ef);
+            s[4] += u0 * e * sqrt(vx * vx + vy * vy);
+            s[5] += u0 * e;
+        }
+#pragma unroll
+        for (int q = 0; q < CV_SLICE_Q; q++) part[((int64_t)blockIdx.x * p.nz + k) * CV_SLICE_Q + q] = s[q];
+    }
+}
+
+__global__ void k_slice_final(const double *__restrict__ part, int nb, int nz, double *out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nz * CV_SLICE_Q) return;
+    double a = 0.0;
+    for (int b = 0; b < nb; b++) a += part[(int64_t)b * nz * CV_SLICE_Q + t];
+    out[t] = a;
+}
+
 // out[q][c] = field plane c at cell idx[q] (SoA planes -> AoS rows); idx checked on the host
 __global__ __launch_bounds__(256) void k_gather(int64_t n, int ncomp, const float *__restrict__ f,
                                                 const int64_t *__restrict__ idx, int64_t m, float *out) {

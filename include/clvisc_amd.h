@@ -151,6 +151,13 @@ int cv_max_ed(cv_handle *h, int which, float *out);
  * fields in a fixed order (reproducible). */
 int cv_energy_sum(cv_handle *h, int which, double tau, double *out);
 
+/* BulkInfo observables of slot `which` per eta slice, replacing the whole-field copy of
+ * BulkInfo.ecc_vs_rapidity (pyvisc/bulkinfo.py:247-260) and the plane sums of eccp /
+ * mean_vr / total_entropy (bulkinfo.py:203-245): out[k * 6 + q] = sum over the (x, y) plane
+ * at eta index k of q = 0: Txx - Tyy, 1: Txx + Tyy, 2: T^tx, 3: u0 s, 4: u0 e |v_T|,
+ * 5: u0 e (e floored at 1e-10, v^2 > 1 -> 0.999999 as there).  Double sums, fixed order. */
+int cv_slice_sums(cv_handle *h, int which, double *out);
+
 /* Profiling: runs one viscous step (as cv_visc_steps_async with nsteps = 1) with HIP
  * events around each of its launches, waits, and writes the launch durations in
  * milliseconds to ms[0..nms-1] in launch order; returns the number of launches in
