@@ -25,6 +25,21 @@ FIELDS = {'ev0': 0, 'ev1': 1, 'ev2': 2, 'pi0': 3, 'pi1': 4, 'pi2': 5,
 NCOMP = {0: 4, 1: 4, 2: 4, 3: 10, 4: 10, 5: 10, 6: 1, 7: 1, 8: 1, 9: 4, 10: 10, 11: 4, 12: 1, 13: 4, 14: 1}
 
 
+class CvGlauber(C.Structure):
+    """cv_glauber: the [glauber] keys of hydro.info (include/clvisc_amd.h)."""
+    _fields_ = [(k, C.c_float) for k in ('num_nucleons', 'sqrts', 'ro0', 'ra', 'eta', 'si0', 'b', 'edmax', 'hwn',
+                                         'eta_flat', 'eta_gw')] + [('bjorken', C.c_int32)]
+
+    @classmethod
+    def from_cfg(cls, cfg, bjorken=False):
+        g = cls()
+        g.num_nucleons, g.sqrts, g.ro0 = cfg.NumOfNucleons, cfg.SQRTS, cfg.NucleonDensity
+        g.ra, g.eta, g.si0, g.b = cfg.Ra, cfg.Eta, cfg.Si0, cfg.ImpactParameter
+        g.edmax, g.hwn, g.eta_flat, g.eta_gw = cfg.Edmax, cfg.Hwn, cfg.Eta_flat, cfg.Eta_gw
+        g.bjorken = int(bjorken)
+        return g
+
+
 class CvParams(C.Structure):
     _fields_ = [
         ('nx', C.c_int32), ('ny', C.c_int32), ('nz', C.c_int32), ('flags', C.c_int32),
@@ -87,6 +102,8 @@ def lib():
             'cv_max_ed': [vp, C.c_int, C.POINTER(f32)],
             'cv_energy_sum': [vp, C.c_int, C.c_double, vp],
             'cv_slice_sums': [vp, C.c_int, vp],
+            'cv_glauber_ini': [vp, vp, C.c_int],
+            'cv_glauber_nbinary': [vp, vp, f32, f32, f32, f32, C.c_int, C.c_int, vp],
             'cv_sync': [vp],
             'cv_last_elapsed_ms': [vp, C.POINTER(f32)],
             'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
@@ -150,7 +167,8 @@ def exported_symbols():
             'cv_batch_steps_async', 'cv_batch_set_loop', 'cv_batch_sync', 'cv_batch_last_elapsed_ms',
             'cv_batch_info', 'cv_mlp_nparam', 'cv_mlp_create', 'cv_mlp_destroy', 'cv_mlp_get_params',
             'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward',
-            'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums']
+            'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums', 'cv_glauber_ini',
+            'cv_glauber_nbinary']
 
 
 def make_eos(table):
@@ -259,6 +277,15 @@ class Handle(object):
         out = np.zeros((self.params.nz, 6), np.float64)
         check(self.L.cv_slice_sums(self.h, which, out.ctypes.data))
         return out
+
+    def glauber_ini(self, glauber, which=1):
+        """Optical Glauber initial condition into slot `which` (cv_glauber_ini)."""
+        check(self.L.cv_glauber_ini(self.h, C.byref(glauber), which))
+
+    def glauber_nbinary(self, glauber, xlow, ylow, dx, dy, nx, ny):
+        out = np.empty(nx * ny, np.float32)
+        check(self.L.cv_glauber_nbinary(self.h, C.byref(glauber), xlow, ylow, dx, dy, nx, ny, out.ctypes.data))
+        return out.reshape(nx, ny)
 
     def max_ed(self, which=1):
         out = C.c_float()

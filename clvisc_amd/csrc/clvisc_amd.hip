@@ -15,6 +15,7 @@
 #include "cv_tiled.h"
 #include "cv_hypersf.h"
 #include "cv_mlp.h"
+#include "cv_ic.h"
 
 // learned root-solve seed network (parameters, Adam moments, training set)
 struct cv_mlp {
@@ -765,6 +766,47 @@ extern "C" int cv_slice_sums(cv_handle *h, int which, double *out) {
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_slice_sums: %s", hipGetErrorString(e));
+    return CV_OK;
+}
+
+static int check_glauber(const cv_glauber *g) {
+    if (!(g->num_nucleons >= 1.0f) || !(g->ra > 0.0f) || !(g->eta > 0.0f) || !(g->ro0 > 0.0f) ||
+        !(g->si0 > 0.0f) || !(g->edmax >= 0.0f) || !(g->eta_gw > 0.0f))
+        return fail(CV_ERR_ARG, "cv_glauber: NumOfNucleons >= 1 and positive Ra, Eta, NucleonDensity, Si0, Eta_gw "
+                                "and Edmax >= 0 required");
+    return CV_OK;
+}
+
+extern "C" int cv_glauber_ini(cv_handle *h, const cv_glauber *g, int which) {
+    if (!h || !g) return fail(CV_ERR_ARG, "null argument");
+    if (which < 0 || which > 2) return fail(CV_ERR_ARG, "which must be 0, 1 or 2");
+    int rc = check_glauber(g);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    const int64_t nxy = (int64_t)h->prm.nx * h->prm.ny;
+    hipLaunchKernelGGL(k_glauber_ini, dim3((unsigned)((nxy + 255) / 256)), dim3(256), 0, h->stream, h->phys, *g,
+                       h->ev[which]);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CV_OK;
+}
+
+extern "C" int cv_glauber_nbinary(cv_handle *h, const cv_glauber *g, float xlow, float ylow, float dx, float dy,
+                                  int nx, int ny, float *out) {
+    if (!h || !g || !out || nx < 1 || ny < 1 || (int64_t)nx * ny > (1 << 28))
+        return fail(CV_ERR_ARG, "cv_glauber_nbinary: bad argument");
+    int rc = check_glauber(g);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(h->device));
+    float *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(float) * (size_t)nx * ny));
+    hipLaunchKernelGGL(k_glauber_nbinary, dim3((unsigned)(((int64_t)nx * ny + 255) / 256)), dim3(256), 0, h->stream,
+                       *g, xlow, ylow, dx, dy, nx, ny, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, sizeof(float) * (size_t)nx * ny, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_glauber_nbinary: %s", hipGetErrorString(e));
     return CV_OK;
 }
 

@@ -111,9 +111,18 @@ class CLVisc(object):
         outpath = os.path.join(self.cfg.fPathOut, 'bad_cell_masks_tau%s.txt' % self.ideal.tau)
         np.savetxt(outpath, frz_sf + good_cells, header=comments)
 
-    # -- initial conditions outside this hydro path --------------------------------
-    def optical_glauber_ini(self, *args, **kwargs):
-        raise NotImplementedError('optical Glauber IC is not part of this hydro path; load_ini the IC')
+    # -- initial conditions ------------------------------------------------------------
+    def optical_glauber_ini(self, system='Au+Au', cent_min=None, cent_max=None, save_binary_collisions=False):
+        """Optical Glauber IC on the device into d_ev[1] (pyvisc/visc.py:151-171); with a
+        centrality range the impact parameter becomes weight_mean_b of that class and is
+        written into the configuration, as there."""
+        from .glauber import Glauber, weight_mean_b
+        if cent_min is not None and cent_max is not None:
+            self.cfg.ImpactParameter = weight_mean_b(cent_min, cent_max, system)
+        glauber = Glauber(self.cfg, None, None, self.compile_options, self.ideal.d_ev[1])
+        if save_binary_collisions:
+            glauber.save_nbinary(self.cfg)
+        return glauber
 
     def create_ini_from_partons(self, *args, **kwargs):
         raise NotImplementedError('parton smearing IC is not part of this hydro path; load_ini the IC')

@@ -91,6 +91,23 @@ enum cv_field {
     CV_NFIELDS = 15
 };
 
+/* Optical Glauber model (pyvisc/kernel/kernel_glauber.cl, pyvisc/glauber.py): the
+ * [glauber] section of hydro.info.  bjorken = 1 is the BJORKEN_SCALING define (H(eta) = 1). */
+typedef struct cv_glauber {
+    float num_nucleons;     /* NumOfNucleons */
+    float sqrts;            /* SQRTS (GeV) */
+    float ro0;              /* NucleonDensity (fm^-3) */
+    float ra;               /* Ra (fm) */
+    float eta;              /* Eta: Woods-Saxon diffuseness (fm) */
+    float si0;              /* Si0: inelastic cross section (fm^2) */
+    float b;                /* ImpactParameter (fm) */
+    float edmax;            /* Edmax (GeV/fm^3) at b = 0 */
+    float hwn;              /* Hwn: wounded-nucleon fraction */
+    float eta_flat;         /* Eta_flat */
+    float eta_gw;           /* Eta_gw */
+    int32_t bjorken;
+} cv_glauber;
+
 /* Last error message of the calling thread (never NULL). */
 const char *cv_last_error(void);
 
@@ -157,6 +174,16 @@ int cv_energy_sum(cv_handle *h, int which, double tau, double *out);
  * at eta index k of q = 0: Txx - Tyy, 1: Txx + Tyy, 2: T^tx, 3: u0 s, 4: u0 e |v_T|,
  * 5: u0 e (e floored at 1e-10, v^2 > 1 -> 0.999999 as there).  Double sums, fixed order. */
 int cv_slice_sums(cv_handle *h, int which, double *out);
+
+/* ---- initial conditions on the device ------------------------------------------------
+ * glauber_ini (kernel_glauber.cl:82-102, Glauber.__init__ pyvisc/glauber.py:63-70): slot
+ * `which` <- (Edmax e_T(x, y) / e_T(0, 0) H(eta_s), 0, 0, 0). */
+int cv_glauber_ini(cv_handle *h, const cv_glauber *g, int which);
+/* num_of_binary_collisions (kernel_glauber.cl:104-121, Glauber.save_nbinary
+ * pyvisc/glauber.py:95-105): out[i * ny + j] = Si0 T(x - b/2, y) T(x + b/2, y) on the grid
+ * x = xlow + i dx, y = ylow + j dy (host array of nx * ny floats). */
+int cv_glauber_nbinary(cv_handle *h, const cv_glauber *g, float xlow, float ylow, float dx, float dy, int nx,
+                       int ny, float *out);
 
 /* Profiling: runs one viscous step (as cv_visc_steps_async with nsteps = 1) with HIP
  * events around each of its launches, waits, and writes the launch durations in

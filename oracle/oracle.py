@@ -319,3 +319,56 @@ def hypersf(nx, ny, nz, dx, dy, dz, nxskip, nyskip, nzskip, efrz, ev_old, ev_new
                   C.c_int64(cap), C.c_int(int(viscous)))
     n = min(cnt.value, cap)
     return (cnt.value, sf[:n], tx[:n], (ps[:n] if viscous else None), (bs[:n] if with_bulk else None))
+
+
+# -- optical Glauber (oracle/glauber_oracle.c) ------------------------------------------
+class OrcGlauber(C.Structure):
+    _fields_ = [(k, C.c_float) for k in ('num_nucleons', 'sqrts', 'ro0', 'ra', 'eta', 'si0', 'b', 'edmax', 'hwn',
+                                         'eta_flat', 'eta_gw')] + [('bjorken', C.c_int32)]
+
+
+def glauber_params(cfg, bjorken=False):
+    """OrcGlauber from the [glauber] keys of a configuration."""
+    g = OrcGlauber()
+    g.num_nucleons, g.sqrts, g.ro0 = cfg.NumOfNucleons, cfg.SQRTS, cfg.NucleonDensity
+    g.ra, g.eta, g.si0, g.b = cfg.Ra, cfg.Eta, cfg.Si0, cfg.ImpactParameter
+    g.edmax, g.hwn, g.eta_flat, g.eta_gw = cfg.Edmax, cfg.Hwn, cfg.Eta_flat, cfg.Eta_gw
+    g.bjorken = int(bjorken)
+    return g
+
+
+def glauber_thickness(g, x, y, precision='f32'):
+    L = lib(precision)
+    rt = C.c_float if precision == 'f32' else C.c_double
+    L.orc_glauber_thickness.restype = rt
+    L.orc_glauber_thickness.argtypes = [C.c_void_p, rt, rt]
+    return L.orc_glauber_thickness(C.byref(g), x, y)
+
+
+def glauber_nw(g, ta, tb, precision='f32'):
+    L = lib(precision)
+    rt = C.c_float if precision == 'f32' else C.c_double
+    L.orc_glauber_nw.restype = rt
+    L.orc_glauber_nw.argtypes = [C.c_void_p, rt, rt]
+    return L.orc_glauber_nw(C.byref(g), ta, tb)
+
+
+def glauber_ini(g, nx, ny, nz, dx, dy, dz, precision='f32'):
+    """(nx*ny*nz, 4) AoS field of glauber_ini."""
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    rt = C.c_float if precision == 'f32' else C.c_double
+    ev = np.zeros((nx * ny * nz, 4), dt)
+    L.orc_glauber_ini.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, rt, rt, rt, C.c_void_p]
+    L.orc_glauber_ini(C.byref(g), nx, ny, nz, dx, dy, dz, _ptr(ev))
+    return ev
+
+
+def glauber_nbinary(g, xlow, ylow, dx, dy, nx, ny, precision='f32'):
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    rt = C.c_float if precision == 'f32' else C.c_double
+    out = np.zeros(nx * ny, dt)
+    L.orc_glauber_nbinary.argtypes = [C.c_void_p, rt, rt, rt, rt, C.c_int, C.c_int, C.c_void_p]
+    L.orc_glauber_nbinary(C.byref(g), xlow, ylow, dx, dy, nx, ny, _ptr(out))
+    return out.reshape(nx, ny)
