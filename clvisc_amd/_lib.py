@@ -104,6 +104,8 @@ def lib():
             'cv_slice_sums': [vp, C.c_int, vp],
             'cv_glauber_ini': [vp, vp, C.c_int],
             'cv_glauber_nbinary': [vp, vp, f32, f32, f32, f32, C.c_int, C.c_int, vp],
+            'cv_smear_partons': [vp, vp, i64, f32, f32, f32, C.c_int],
+            'cv_longitudinal_profile': [vp, C.c_int, vp],
             'cv_sync': [vp],
             'cv_last_elapsed_ms': [vp, C.POINTER(f32)],
             'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
@@ -168,7 +170,7 @@ def exported_symbols():
             'cv_batch_info', 'cv_mlp_nparam', 'cv_mlp_create', 'cv_mlp_destroy', 'cv_mlp_get_params',
             'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward',
             'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums', 'cv_glauber_ini',
-            'cv_glauber_nbinary']
+            'cv_glauber_nbinary', 'cv_smear_partons', 'cv_longitudinal_profile']
 
 
 def make_eos(table):
@@ -286,6 +288,18 @@ class Handle(object):
         out = np.empty(nx * ny, np.float32)
         check(self.L.cv_glauber_nbinary(self.h, C.byref(glauber), xlow, ylow, dx, dy, nx, ny, out.ctypes.data))
         return out.reshape(nx, ny)
+
+    def smear_partons(self, p4x4, sigr=0.6, sigz=0.6, kfactor=1.0, which=1):
+        """Gaussian-smeared parton list (rows E px py pz t x y z) into slot `which`."""
+        a = np.ascontiguousarray(np.asarray(p4x4, np.float32).reshape(-1, 8))
+        check(self.L.cv_smear_partons(self.h, a.ctypes.data if a.size else None, a.shape[0], sigr, sigz,
+                                      kfactor, which))
+
+    def longitudinal_profile(self, profile, which=1):
+        a = np.ascontiguousarray(profile, np.float32).reshape(-1)
+        if a.size != self.params.nz:
+            raise ValueError('profile needs NZ = %d values' % self.params.nz)
+        check(self.L.cv_longitudinal_profile(self.h, which, a.ctypes.data))
 
     def max_ed(self, which=1):
         out = C.c_float()

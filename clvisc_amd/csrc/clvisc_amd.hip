@@ -810,6 +810,63 @@ extern "C" int cv_glauber_nbinary(cv_handle *h, const cv_glauber *g, float xlow,
     return CV_OK;
 }
 
+extern "C" int cv_smear_partons(cv_handle *h, const float *p4x4, int64_t npartons, float sigr, float sigz,
+                                float kfactor, int which) {
+    if (!h || (!p4x4 && npartons > 0) || npartons < 0) return fail(CV_ERR_ARG, "cv_smear_partons: bad argument");
+    if (which < 0 || which > 2) return fail(CV_ERR_ARG, "which must be 0, 1 or 2");
+    if (!(sigr > 0.0f) || !(sigz > 0.0f)) return fail(CV_ERR_ARG, "cv_smear_partons: SIGR, SIGZ must be positive");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    // the reference's constants (kernel_gaussian_smearing_new.cl:3-7) in double, rounded once
+    const double c = 1.0 / (2.0 * (double)sigr * (double)sigr);
+    const double cz = 1.0 / (2.0 * (double)sigz * (double)sigz);
+    const double pif = (double)3.14159265358979323846f;      // M_PI_F
+    const double w = (c / pif) * (sqrt(cz / pif) / (double)(float)h->prm.tau0);   // TAU0 is a float define
+    float *dp = nullptr;
+    SmearParton *ds = nullptr;
+    const int64_t np1 = npartons > 0 ? npartons : 1;
+    HIP_TRY(hipMalloc((void **)&dp, sizeof(float) * 8 * np1));
+    hipError_t e = hipMalloc((void **)&ds, sizeof(SmearParton) * np1);
+    if (e == hipSuccess && npartons > 0) e = hipMemcpy(dp, p4x4, sizeof(float) * 8 * npartons, hipMemcpyHostToDevice);
+    if (e == hipSuccess && npartons > 0) {
+        hipLaunchKernelGGL(k_smear_prep, dim3((unsigned)((npartons + 255) / 256)), dim3(256), 0, h->stream, dp,
+                           npartons, ds);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_smear, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, ds, npartons, (float)c,
+                           (float)w, kfactor, h->ev[which]);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(dp);
+    (void)hipFree(ds);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_smear_partons: %s", hipGetErrorString(e));
+    return CV_OK;
+}
+
+extern "C" int cv_longitudinal_profile(cv_handle *h, int which, const float *profile) {
+    if (!h || !profile) return fail(CV_ERR_ARG, "null argument");
+    if (which < 0 || which > 2) return fail(CV_ERR_ARG, "which must be 0, 1 or 2");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const int64_t nxy = (int64_t)h->prm.nx * h->prm.ny;
+    float *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(float) * (4 * nxy + h->prm.nz)));
+    hipError_t e = hipMemcpy(d + 4 * nxy, profile, sizeof(float) * h->prm.nz, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_long_profile_mid, dim3((unsigned)((nxy + 255) / 256)), dim3(256), 0, h->stream, h->phys,
+                           h->ev[which], d);
+        hipLaunchKernelGGL(k_long_profile, grid1d(h->n), dim3(256), 0, h->stream, h->phys, d, d + 4 * nxy,
+                           h->ev[which]);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_longitudinal_profile: %s", hipGetErrorString(e));
+    return CV_OK;
+}
+
 extern "C" int cv_sync(cv_handle *h) {
     if (!h) return fail(CV_ERR_ARG, "null handle");
     HIP_TRY(hipSetDevice(h->device));

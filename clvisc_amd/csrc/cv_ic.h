@@ -79,4 +79,108 @@ __global__ __launch_bounds__(256) void k_glauber_nbinary(cv_glauber g, float xlo
     out[t] = gl_nb(g, ta, tb);
 }
 
+// ---------------------------------------------------------------------------------------
+// Gaussian smearing of a parton list (kernel_gaussian_smearing_new.cl:11-120, used by
+// SmearingP4X4 / CLVisc.smear_from_p4x4): every cell sums, over the partons closer than 10
+// sigma, w exp(-d^2 / (2 SIGR^2)) (m_T cosh(Y - eta_s), p_x, p_y, m_T sinh(Y - eta_s)), with
+// d the (x, y, eta_s) distance (the eta_s difference also scaled by SIGR, as there) and
+// w = 1/(2 pi SIGR^2) sqrt(1/(2 pi SIGZ^2)) / TAU0; then KFACTOR, the fixed-point root solve
+// of helper.h:112-125 for e, and (e, 0, 0, 0) into the field.  The per-parton quantities
+// (eta_s, m_T, Y) do not depend on the cell and are computed once per parton.
+struct SmearParton {
+    float x, y, etas, mt, Y, px, py, pad;
+};
+
+__global__ __launch_bounds__(256) void k_smear_prep(const float *__restrict__ p4x4, int64_t np_, SmearParton *out) {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= np_) return;
+    const float *q = p4x4 + 8 * m;           // E px py pz t x y z
+    const float E = q[0], px = q[1], py = q[2], pz = q[3], t = q[4], x = q[5], y = q[6], z = q[7];
+    SmearParton s;
+    s.x = x;
+    s.y = y;
+    s.etas = 0.5f * logf(rmax(t + z, CV_ACU)) - 0.5f * logf(rmax(t - z, CV_ACU));
+    s.mt = sqrtf(rmax(E * E - pz * pz, px * px + py * py));
+    s.Y = 0.5f * (logf(rmax(E + pz, CV_ACU)) - logf(rmax(E - pz, CV_ACU)));
+    s.px = px;
+    s.py = py;
+    s.pad = 0.0f;
+    out[m] = s;
+}
+
+#define CV_SMEAR_CHUNK 512
+// one thread per cell; partons staged through LDS in chunks, summed in list order
+__global__ __launch_bounds__(256) void k_smear(PhysP p, EosP E, const SmearParton *__restrict__ par, int64_t np_,
+                                               float one_o_2sigr2, float w, float kfactor, float *ev) {
+    __shared__ SmearParton s_p[CV_SMEAR_CHUNK];
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = I < p.n;
+    const int64_t ij = live ? I / p.nz : 0;
+    const int k = live ? (int)(I - ij * p.nz) : 0;
+    const int i = (int)(ij / p.ny), j = (int)(ij - (int64_t)(ij / p.ny) * p.ny);
+    const float gx = (float)(i - p.nx / 2) * p.dx, gy = (float)(j - p.ny / 2) * p.dy;
+    const float ge = (float)(k - p.nz / 2) * p.dz;
+    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
+    for (int64_t n = 0; n < np_; n += CV_SMEAR_CHUNK) {
+        const int cnt = (int)(np_ - n < CV_SMEAR_CHUNK ? np_ - n : CV_SMEAR_CHUNK);
+        __syncthreads();
+        for (int m = threadIdx.x; m < cnt; m += blockDim.x) s_p[m] = par[n + m];
+        __syncthreads();
+        if (live) {
+            for (int m = 0; m < cnt; m++) {
+                const SmearParton &q = s_p[m];
+                const float d1 = gx - q.x, d2 = gy - q.y, d3 = ge - q.etas;
+                const float dist = one_o_2sigr2 * (d1 * d1 + d2 * d2 + d3 * d3);
+                if (dist < 50.0f) {
+                    const float delta = w * expf(-dist);
+                    t0 = t0 + delta * (q.mt * coshf(q.Y - ge));
+                    t1 = t1 + delta * q.px;
+                    t2 = t2 + delta * q.py;
+                    t3 = t3 + delta * (q.mt * sinhf(q.Y - ge));
+                }
+            }
+        }
+    }
+    if (!live) return;
+    t0 = kfactor * t0; t1 = kfactor * t1; t2 = kfactor * t2; t3 = kfactor * t3;
+    const float K2 = t1 * t1 + t2 * t2 + t3 * t3;
+    const float K0 = t0 < CV_ACU ? CV_ACU : t0;
+    Eos eos(E, E.tab);
+    // fixed-point root solve (helper.h:112-125)
+    float E1 = K0, E0;
+    int it = 0;
+    for (;;) {
+        E0 = E1;
+        E1 = K0 - K2 / (K0 + eos.P(E1));
+        it++;
+        if (it > 100 || fabsf(E1 - E0) / rmax(fabsf(E1), CV_ACU) < CV_ACU) break;
+    }
+    const int64_t n = p.n;
+    ev[I] = rmax(0.0f, E1);
+    ev[n + I] = 0.0f;
+    ev[2 * n + I] = 0.0f;
+    ev[3 * n + I] = 0.0f;
+}
+
+// change_longitudinal_profile (kernel_gaussian_smearing_new.cl:124-138): every cell of a
+// column becomes (profile[k] e_mid, v_mid) of the column's mid-rapidity cell.  Two passes so
+// that the mid-rapidity cell is read before it is rewritten (the reference's single pass
+// races on it when profile[NZ/2] != 1).  mid: 4 planes of NX*NY.
+__global__ __launch_bounds__(256) void k_long_profile_mid(PhysP p, const float *__restrict__ ev, float *mid) {
+    const int64_t nxy = (int64_t)p.nx * p.ny;
+    const int64_t ij = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ij >= nxy) return;
+    const int64_t I = ij * p.nz + p.nz / 2;
+    for (int c = 0; c < 4; c++) mid[c * nxy + ij] = ev[c * p.n + I];
+}
+__global__ __launch_bounds__(256) void k_long_profile(PhysP p, const float *__restrict__ mid,
+                                                      const float *__restrict__ prof, float *ev) {
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= p.n) return;
+    const int64_t nxy = (int64_t)p.nx * p.ny;
+    const int64_t ij = I / p.nz;
+    ev[I] = prof[I - ij * p.nz] * mid[ij];
+    for (int c = 1; c < 4; c++) ev[c * p.n + I] = mid[c * nxy + ij];
+}
+
 }  // namespace cv

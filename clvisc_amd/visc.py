@@ -124,11 +124,22 @@ class CLVisc(object):
             glauber.save_nbinary(self.cfg)
         return glauber
 
-    def create_ini_from_partons(self, *args, **kwargs):
-        raise NotImplementedError('parton smearing IC is not part of this hydro path; load_ini the IC')
+    def create_ini_from_partons(self, fname, SIGR=0.6, SIGZ=0.6, KFACTOR=1.0):
+        """Gaussian-smeared partons read from a text file (one header line, rows E px py pz
+        t x y z) into d_ev[1] (pyvisc/visc.py:176-183, Smearing pyvisc/smearing.py:23-47)."""
+        dat = np.loadtxt(fname, skiprows=1, dtype=np.float32, ndmin=2)
+        self.handle.smear_partons(dat, SIGR, SIGZ, KFACTOR, which=1)
 
-    def smear_from_p4x4(self, *args, **kwargs):
-        raise NotImplementedError('parton smearing IC is not part of this hydro path; load_ini the IC')
+    def smear_from_p4x4(self, p4x4, SIGR=0.6, SIGZ=0.6, KFACTOR=1.0, force_bjorken=False,
+                        longitudinal_profile=None):
+        """Gaussian-smeared partons p4x4 (n x 8: E px py pz t x y z) into d_ev[1], then the
+        optional longitudinal profile (pyvisc/visc.py:185-195, SmearingP4X4
+        pyvisc/smearing.py:63-107): force_bjorken copies mid-rapidity to every eta_s."""
+        self.handle.smear_partons(p4x4, SIGR, SIGZ, KFACTOR, which=1)
+        if force_bjorken:
+            self.handle.longitudinal_profile(np.ones(self.cfg.NZ, np.float32), which=1)
+        elif longitudinal_profile is not None:
+            self.handle.longitudinal_profile(longitudinal_profile, which=1)
 
     def get_vorticity(self, save_data=False):
         raise NotImplementedError('vorticity is not part of this hydro path')

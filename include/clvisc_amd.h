@@ -184,6 +184,15 @@ int cv_glauber_ini(cv_handle *h, const cv_glauber *g, int which);
  * x = xlow + i dx, y = ylow + j dy (host array of nx * ny floats). */
 int cv_glauber_nbinary(cv_handle *h, const cv_glauber *g, float xlow, float ylow, float dx, float dy, int nx,
                        int ny, float *out);
+/* Gaussian smearing of partons (smearing kernel, kernel_gaussian_smearing_new.cl:11-120;
+ * SmearingP4X4 pyvisc/smearing.py:63-90): p4x4 = npartons rows of (E, px, py, pz, t, x, y,
+ * z) (the reference's real8); slot `which` <- (e, 0, 0, 0) with e from the smeared
+ * T^{tau mu} by the fixed-point root solve of helper.h:112-125. */
+int cv_smear_partons(cv_handle *h, const float *p4x4, int64_t npartons, float sigr, float sigz, float kfactor,
+                     int which);
+/* change_longitudinal_profile (kernel_gaussian_smearing_new.cl:124-138): every cell of an
+ * (x, y) column <- (profile[k] e_mid, v_mid) of its mid-rapidity cell; profile: NZ floats. */
+int cv_longitudinal_profile(cv_handle *h, int which, const float *profile);
 
 /* Profiling: runs one viscous step (as cv_visc_steps_async with nsteps = 1) with HIP
  * events around each of its launches, waits, and writes the launch durations in

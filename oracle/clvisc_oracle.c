@@ -1186,3 +1186,74 @@ void orc_visc_steps(const orc_params *p, orc_state *s, int64_t loop0, int64_t ns
 }
 
 int orc_real_size(void) { return (int)sizeof(real); }
+
+/* ------------------------------------------------------------------------- */
+/* Gaussian smearing of partons, kernel_gaussian_smearing_new.cl:11-120      */
+/* ------------------------------------------------------------------------- */
+/* p4x4: np rows (E, px, py, pz, t, x, y, z); one_o_2sigr2 and w (= w1 w2) as the device
+ * rounds them; ev: AoS rows of 4.  Per parton: eta_s = (log max(t+z, acu) - log max(t-z,
+ * acu))/2, m_T = sqrt(max(E^2 - pz^2, px^2 + py^2)), Y = (log max(E+pz, acu) -
+ * log max(E-pz, acu))/2; per cell the partons are summed in list order; then KFACTOR and
+ * the fixed-point root solve of helper.h:112-125. */
+void orc_smear(const orc_params *p, const real *p4x4, int64_t np_, real one_o_2sigr2, real w, real kfactor,
+               real *ev) {
+    int nx = p->nx, ny = p->ny, nz = p->nz;
+    real *pre = (real *)malloc(sizeof(real) * 7 * (np_ > 0 ? np_ : 1));
+    for (int64_t m = 0; m < np_; m++) {
+        const real *q = p4x4 + 8 * m;
+        real E = q[0], px = q[1], py = q[2], pz = q[3], t = q[4], x = q[5], y = q[6], z = q[7];
+        real *o = pre + 7 * m;
+        o[0] = x; o[1] = y;
+#if ORC_DOUBLE
+        o[2] = RC(0.5f) * log(r_max(t + z, ACU)) - RC(0.5f) * log(r_max(t - z, ACU));
+        o[3] = sqrt(r_max(E * E - pz * pz, px * px + py * py));
+        o[4] = RC(0.5f) * (log(r_max(E + pz, ACU)) - log(r_max(E - pz, ACU)));
+#else
+        o[2] = 0.5f * logf(r_max(t + z, ACU)) - 0.5f * logf(r_max(t - z, ACU));
+        o[3] = sqrtf(r_max(E * E - pz * pz, px * px + py * py));
+        o[4] = 0.5f * (logf(r_max(E + pz, ACU)) - logf(r_max(E - pz, ACU)));
+#endif
+        o[5] = px; o[6] = py;
+    }
+    set_threads(p);
+    #pragma omp parallel for schedule(static)
+    for (int64_t I = 0; I < (int64_t)nx * ny * nz; I++) {
+        int64_t ij = I / nz;
+        int k = (int)(I - ij * nz), i = (int)(ij / ny), j = (int)(ij - (ij / ny) * ny);
+        real gx = (real)(i - nx / 2) * RC(p->dx), gy = (real)(j - ny / 2) * RC(p->dy), ge = (real)(k - nz / 2) * RC(p->dz);
+        real t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+        for (int64_t m = 0; m < np_; m++) {
+            const real *q = pre + 7 * m;
+            real d1 = gx - q[0], d2 = gy - q[1], d3 = ge - q[2];
+            real dist = one_o_2sigr2 * (d1 * d1 + d2 * d2 + d3 * d3);
+            if (dist < RC(50.0f)) {
+#if ORC_DOUBLE
+                real delta = w * exp(-dist);
+                t0 = t0 + delta * (q[3] * cosh(q[4] - ge));
+                t3 = t3 + delta * (q[3] * sinh(q[4] - ge));
+#else
+                real delta = w * expf(-dist);
+                t0 = t0 + delta * (q[3] * coshf(q[4] - ge));
+                t3 = t3 + delta * (q[3] * sinhf(q[4] - ge));
+#endif
+                t1 = t1 + delta * q[5];
+                t2 = t2 + delta * q[6];
+            }
+        }
+        t0 = kfactor * t0; t1 = kfactor * t1; t2 = kfactor * t2; t3 = kfactor * t3;
+        real K2 = t1 * t1 + t2 * t2 + t3 * t3;
+        real K0 = t0 < ACU ? ACU : t0;
+        real E1 = K0, E0;
+        int it = 0;
+        for (;;) {
+            E0 = E1;
+            E1 = K0 - K2 / (K0 + eos_P(p, E1));
+            it++;
+            if (it > 100 || (real)fabs((double)(E1 - E0)) / r_max((real)fabs((double)E1), ACU) < ACU) break;
+        }
+        real *o = ev + 4 * I;
+        o[0] = r_max(RC(0.0f), E1);
+        o[1] = o[2] = o[3] = RC(0.0f);
+    }
+    free(pre);
+}

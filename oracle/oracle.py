@@ -372,3 +372,30 @@ def glauber_nbinary(g, xlow, ylow, dx, dy, nx, ny, precision='f32'):
     L.orc_glauber_nbinary.argtypes = [C.c_void_p, rt, rt, rt, rt, C.c_int, C.c_int, C.c_void_p]
     L.orc_glauber_nbinary(C.byref(g), xlow, ylow, dx, dy, nx, ny, _ptr(out))
     return out.reshape(nx, ny)
+
+
+def smear_constants(sigr, sigz, tau0):
+    """(1/(2 SIGR^2), w1 w2) as the device rounds them: computed in double with the float pi
+    of M_PI_F (kernel_gaussian_smearing_new.cl:3-7), then rounded to float."""
+    pif = float(np.float32(np.pi))
+    c = 1.0 / (2.0 * float(np.float32(sigr)) ** 2)
+    cz = 1.0 / (2.0 * float(np.float32(sigz)) ** 2)
+    w = (c / pif) * (np.sqrt(cz / pif) / float(np.float32(tau0)))
+    return np.float32(c), np.float32(w)
+
+
+def smear(params, p4x4, sigr, sigz, kfactor, precision='f32'):
+    """(n, 4) AoS field of the smearing kernel for a parton list (rows E px py pz t x y z)."""
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    rt = C.c_float if precision == 'f32' else C.c_double
+    a = np.ascontiguousarray(np.asarray(p4x4, dt).reshape(-1, 8))
+    c, w = smear_constants(sigr, sigz, params.tau0_cfg)
+    if precision != 'f32':
+        cd = 1.0 / (2.0 * sigr * sigr)
+        c, w = cd, (cd / np.pi) * (np.sqrt(1.0 / (2.0 * sigz * sigz) / np.pi) / params.tau0_cfg)
+    n = params.nx * params.ny * params.nz
+    ev = np.zeros((n, 4), dt)
+    L.orc_smear.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, rt, rt, rt, C.c_void_p]
+    L.orc_smear(C.byref(params), _ptr(a) if a.size else None, a.shape[0], c, w, kfactor, _ptr(ev))
+    return ev

@@ -63,3 +63,59 @@ def test_clvisc_optical_glauber_ini(tmp_path):
     assert np.abs(ev[:, 0] - ref[:, 0]).max() <= REL * ref[:, 0].max()
     assert (tmp_path / 'nbin.dat').exists()
     assert np.loadtxt(tmp_path / 'nbin.dat').shape == (66, 66)
+
+
+def _partons(n, seed):
+    rng = np.random.default_rng(seed)
+    mom = rng.normal(0, 1.2, (n, 3))
+    E = np.sqrt((mom ** 2).sum(1) + 0.3 ** 2)
+    pos = np.column_stack([np.full(n, 0.6), rng.normal(0, 1.5, (n, 2)), rng.normal(0, 0.4, n)])
+    return np.column_stack([E, mom, pos]).astype(np.float32)
+
+
+@pytest.mark.parametrize('npart', [1, 700, 1500])
+def test_smear_partons_matches_oracle(npart):
+    """Device smearing (LDS chunks of 512 partons, list order kept) against the FP32 oracle."""
+    nx, ny, nz, h = 41, 37, 21, 0.2
+    tab = Eos('crossover').create_table()
+    p = cv_params(nx, ny, nz, 0.02, h, h, h, 0.6)
+    pp = _partons(npart, npart)
+    _say('case: smearing %d partons on %dx%dx%d' % (npart, nx, ny, nz))
+    hd = _lib.Handle(p, tab, viscous=False)
+    hd.smear_partons(pp, 0.6, 0.6, 1.45, 1)
+    dev = hd.get('ev1')
+    from helpers import orc_params
+    ref = O.smear(orc_params(p, tab), pp, 0.6, 0.6, 1.45, 'f32')
+    err = np.abs(dev[:, 0] - ref[:, 0]).max() / ref[:, 0].max()
+    _say('  max |e - e_oracle| / max e = %.3g, bit-equal %.4f' % (err, np.mean(dev[:, 0] == ref[:, 0])))
+    assert err <= 1e-5
+    assert np.all(dev[:, 1:] == 0.0)
+    # change_longitudinal_profile: mid-rapidity copied with a profile
+    prof = np.exp(-((np.arange(nz) - nz // 2) * h) ** 2 / 2).astype(np.float32)
+    hd.longitudinal_profile(prof, 1)
+    e2 = hd.get('ev1')[:, 0].reshape(nx, ny, nz)
+    mid = dev[:, 0].reshape(nx, ny, nz)[:, :, nz // 2]
+    np.testing.assert_array_equal(e2, prof[None, None, :] * mid[:, :, None])
+    hd.close()
+
+
+def test_clvisc_smear_from_p4x4_and_file(tmp_path):
+    from clvisc_amd import CLVisc
+    cfg = read_config(argv=[])
+    cfg.NX, cfg.NY, cfg.NZ = 33, 31, 9
+    cfg.DX, cfg.DY, cfg.DZ = 0.2, 0.2, 0.2
+    cfg.fPathOut = str(tmp_path)
+    _say('case: CLVisc.smear_from_p4x4 / create_ini_from_partons')
+    visc = CLVisc(cfg)
+    pp = _partons(300, 11)
+    visc.smear_from_p4x4(pp, SIGR=0.5, SIGZ=0.5, KFACTOR=1.2, force_bjorken=True)
+    e = visc.ideal.d_ev[1].get()[:, 0].reshape(33, 31, 9)
+    assert np.all(e == e[:, :, 4:5])
+    fname = tmp_path / 'partons.txt'
+    np.savetxt(fname, pp, header='E px py pz t x y z')
+    visc.create_ini_from_partons(str(fname), SIGR=0.5, SIGZ=0.5, KFACTOR=1.2)
+    from helpers import orc_params
+    ref = O.smear(orc_params(visc.handle.params, visc.eos_table), np.loadtxt(fname, dtype=np.float32), 0.5, 0.5,
+                  1.2, 'f32')
+    got = visc.ideal.d_ev[1].get()
+    assert np.abs(got[:, 0] - ref[:, 0]).max() <= 1e-5 * ref[:, 0].max()
