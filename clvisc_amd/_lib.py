@@ -106,6 +106,7 @@ def lib():
             'cv_glauber_nbinary': [vp, vp, f32, f32, f32, f32, C.c_int, C.c_int, vp],
             'cv_smear_partons': [vp, vp, i64, f32, f32, f32, C.c_int],
             'cv_longitudinal_profile': [vp, C.c_int, vp],
+            'cv_sd_to_ed': [vp, C.c_int, vp, vp, i64],
             'cv_sync': [vp],
             'cv_last_elapsed_ms': [vp, C.POINTER(f32)],
             'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
@@ -170,7 +171,8 @@ def exported_symbols():
             'cv_batch_info', 'cv_mlp_nparam', 'cv_mlp_create', 'cv_mlp_destroy', 'cv_mlp_get_params',
             'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward',
             'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums', 'cv_glauber_ini',
-            'cv_glauber_nbinary', 'cv_smear_partons', 'cv_longitudinal_profile']
+            'cv_glauber_nbinary', 'cv_smear_partons', 'cv_longitudinal_profile',
+            'cv_sd_to_ed']
 
 
 def make_eos(table):
@@ -300,6 +302,12 @@ class Handle(object):
         if a.size != self.params.nz:
             raise ValueError('profile needs NZ = %d values' % self.params.nz)
         check(self.L.cv_longitudinal_profile(self.h, which, a.ctypes.data))
+
+    def sd_to_ed(self, s_nodes, e_nodes, which=1):
+        """Component e of slot `which` from entropy to energy density (cv_sd_to_ed)."""
+        s = np.ascontiguousarray(s_nodes, np.float64)
+        e = np.ascontiguousarray(e_nodes, np.float64)
+        check(self.L.cv_sd_to_ed(self.h, which, s.ctypes.data, e.ctypes.data, s.size))
 
     def max_ed(self, which=1):
         out = C.c_float()

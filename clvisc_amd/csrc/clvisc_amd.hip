@@ -867,6 +867,28 @@ extern "C" int cv_longitudinal_profile(cv_handle *h, int which, const float *pro
     return CV_OK;
 }
 
+extern "C" int cv_sd_to_ed(cv_handle *h, int which, const double *s_nodes, const double *e_nodes, int64_t m) {
+    if (!h || !s_nodes || !e_nodes || m < 2) return fail(CV_ERR_ARG, "cv_sd_to_ed: bad argument (2 nodes at least)");
+    if (which < 0 || which > 2) return fail(CV_ERR_ARG, "which must be 0, 1 or 2");
+    for (int64_t i = 1; i < m; i++)
+        if (!(s_nodes[i] > s_nodes[i - 1]))
+            return fail(CV_ERR_ARG, "cv_sd_to_ed: s nodes must increase strictly (node %lld)", (long long)i);
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    double *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(double) * 2 * m));
+    hipError_t e = hipMemcpy(d, s_nodes, sizeof(double) * m, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + m, e_nodes, sizeof(double) * m, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_sd_to_ed, grid1d(h->n), dim3(256), 0, h->stream, h->n, h->ev[which], d, d + m, m);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_sd_to_ed: %s", hipGetErrorString(e));
+    return CV_OK;
+}
+
 extern "C" int cv_sync(cv_handle *h) {
     if (!h) return fail(CV_ERR_ARG, "null handle");
     HIP_TRY(hipSetDevice(h->device));

@@ -183,4 +183,30 @@ __global__ __launch_bounds__(256) void k_long_profile(PhysP p, const float *__re
     for (int c = 1; c < 4; c++) ev[c * p.n + I] = mid[c * nxy + ij];
 }
 
+// Entropy density -> energy density of a field (from_sd_to_ed, pyvisc/ebe_trento.py:26-33:
+// InterpolatedUnivariateSpline(eos.s, eos.ed, k=1), i.e. piecewise linear in s through the
+// EOS nodes with linear extrapolation beyond the first and last segments), in double.
+__global__ __launch_bounds__(256) void k_sd_to_ed(int64_t n, float *e, const double *__restrict__ s_nodes,
+                                                  const double *__restrict__ e_nodes, int64_t m) {
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= n) return;
+    const double s = e[I];
+    int64_t lo;                              // the segment [lo, lo + 1] that s falls in (or the end one)
+    if (s <= s_nodes[0]) {
+        lo = 0;
+    } else if (s >= s_nodes[m - 1]) {
+        lo = m - 2;
+    } else {
+        lo = 0;
+        int64_t up = m - 1;                  // s_nodes[lo] <= s < s_nodes[up]
+        while (up - lo > 1) {
+            const int64_t mid = (lo + up) >> 1;
+            if (s_nodes[mid] <= s) lo = mid; else up = mid;
+        }
+    }
+    const int64_t hi = lo + 1;
+    const double t = (s - s_nodes[lo]) / (s_nodes[hi] - s_nodes[lo]);
+    e[I] = (float)(e_nodes[lo] + t * (e_nodes[hi] - e_nodes[lo]));
+}
+
 }  // namespace cv

@@ -176,3 +176,24 @@ def hotspots(nx, ny, nz, dx, dy, dz, nhot=400, width=0.5, emax=50.0, seed=202406
     ev = np.zeros((nx, ny, nz, 4), dtype=dtype)
     ev[..., 0] = (e2[:, :, None] * h[None, None, :]).astype(dtype)
     return ev.reshape(-1, 4)
+
+
+def from_sd_to_ed(entropy, eos):
+    """Entropy density -> energy density through the EOS nodes, piecewise linear in s with
+    linear extrapolation at both ends (pyvisc/ebe_trento.py:26-33, where
+    InterpolatedUnivariateSpline(eos.s, eos.ed, k=1) does this).  Host version; the device
+    version is ``sd_to_ed_on_device``."""
+    s = np.asarray(eos.s, np.float64)
+    ed = np.asarray(eos.ed, np.float64)
+    x = np.asarray(entropy, np.float64)
+    out = np.interp(x, s, ed)
+    lo, hi = x < s[0], x > s[-1]
+    out[lo] = ed[0] + (x[lo] - s[0]) / (s[1] - s[0]) * (ed[1] - ed[0])
+    out[hi] = ed[-2] + (x[hi] - s[-2]) / (s[-1] - s[-2]) * (ed[-1] - ed[-2])
+    return out
+
+
+def sd_to_ed_on_device(d_ev, eos):
+    """Convert the e component of a device field that holds an entropy density (a TRENTo
+    profile loaded with ``set``) to energy density in place (cv_sd_to_ed)."""
+    d_ev.handle.sd_to_ed(eos.s, eos.ed, d_ev.field_id)

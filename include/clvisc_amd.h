@@ -193,6 +193,10 @@ int cv_smear_partons(cv_handle *h, const float *p4x4, int64_t npartons, float si
 /* change_longitudinal_profile (kernel_gaussian_smearing_new.cl:124-138): every cell of an
  * (x, y) column <- (profile[k] e_mid, v_mid) of its mid-rapidity cell; profile: NZ floats. */
 int cv_longitudinal_profile(cv_handle *h, int which, const float *profile);
+/* Entropy density -> energy density in place, component e of slot `which` (from_sd_to_ed,
+ * pyvisc/ebe_trento.py:26-33): piecewise linear through the m EOS nodes (s_nodes strictly
+ * increasing, e_nodes), linear extrapolation at both ends, in double. */
+int cv_sd_to_ed(cv_handle *h, int which, const double *s_nodes, const double *e_nodes, int64_t m);
 
 /* Profiling: runs one viscous step (as cv_visc_steps_async with nsteps = 1) with HIP
  * events around each of its launches, waits, and writes the launch durations in

@@ -119,3 +119,29 @@ def test_clvisc_smear_from_p4x4_and_file(tmp_path):
                   1.2, 'f32')
     got = visc.ideal.d_ev[1].get()
     assert np.abs(got[:, 0] - ref[:, 0]).max() <= 1e-5 * ref[:, 0].max()
+
+
+@pytest.mark.parametrize('eos_name', ['ideal_gas', 'crossover'])
+def test_sd_to_ed_on_device(eos_name):
+    """TRENTo entropy profile -> energy density on the device (cv_sd_to_ed) against the host
+    restatement of from_sd_to_ed, including values beyond both ends of the table."""
+    from clvisc_amd.ic import from_sd_to_ed, sd_to_ed_on_device
+    from clvisc_amd.ideal import DeviceField
+    eos = Eos(eos_name)
+    nx, ny, nz = 31, 29, 3
+    p = cv_params(nx, ny, nz, 0.02, 0.2, 0.2, 0.3, 0.6)
+    h = _lib.Handle(p, eos.create_table(), viscous=False)
+    rng = np.random.default_rng(5)
+    s = np.exp(rng.uniform(np.log(1e-6), np.log(2.0 * eos.s[-1]), nx * ny * nz)).astype(np.float32)
+    s[:5] = [0.0, eos.s[0], eos.s[-1], 1.5 * eos.s[-1], eos.s[1] * 0.5]
+    ev = np.zeros((s.size, 4), np.float32)
+    ev[:, 0] = s
+    h.set('ev1', ev)
+    _say('case: sd_to_ed %s, %d cells' % (eos_name, s.size))
+    sd_to_ed_on_device(DeviceField(h, 'ev1'), eos)
+    got = h.get('ev1')[:, 0]
+    ref = from_sd_to_ed(s.astype(np.float64), eos).astype(np.float32)
+    rel = np.abs(got.astype(np.float64) - ref) / np.maximum(np.abs(ref), 1e-30)
+    _say('  max rel %.3g, bit-equal %.4f' % (rel.max(), np.mean(got == ref)))
+    assert rel.max() <= 2.4e-7
+    h.close()
