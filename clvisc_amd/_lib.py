@@ -21,8 +21,9 @@ CV_FLAG_SF_FP64 = 0x10
 
 FIELDS = {'ev0': 0, 'ev1': 1, 'ev2': 2, 'pi0': 3, 'pi1': 4, 'pi2': 5,
           'bulk0': 6, 'bulk1': 7, 'bulk2': 8, 'src': 9, 'is_src': 10,
-          'udiff': 11, 'goodcell': 12, 'root_cons': 13, 'root_seed': 14}
-NCOMP = {0: 4, 1: 4, 2: 4, 3: 10, 4: 10, 5: 10, 6: 1, 7: 1, 8: 1, 9: 4, 10: 10, 11: 4, 12: 1, 13: 4, 14: 1}
+          'udiff': 11, 'goodcell': 12, 'root_cons': 13, 'root_seed': 14, 'omega0': 15, 'omega1': 16}
+NCOMP = {0: 4, 1: 4, 2: 4, 3: 10, 4: 10, 5: 10, 6: 1, 7: 1, 8: 1, 9: 4, 10: 10, 11: 4, 12: 1, 13: 4, 14: 1,
+         15: 32, 16: 32}
 
 
 class CvGlauber(C.Structure):
@@ -107,6 +108,10 @@ def lib():
             'cv_smear_partons': [vp, vp, i64, f32, f32, f32, C.c_int],
             'cv_longitudinal_profile': [vp, C.c_int, vp],
             'cv_sd_to_ed': [vp, C.c_int, vp, vp, i64],
+            'cv_vorticity_enable': [vp],
+            'cv_vorticity': [vp, f32],
+            'cv_vorticity_mu': [vp, f32, vp, i64],
+            'cv_hypersf_get_vorticity': [vp, vp, i64],
             'cv_sync': [vp],
             'cv_last_elapsed_ms': [vp, C.POINTER(f32)],
             'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
@@ -172,7 +177,7 @@ def exported_symbols():
             'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward',
             'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums', 'cv_glauber_ini',
             'cv_glauber_nbinary', 'cv_smear_partons', 'cv_longitudinal_profile',
-            'cv_sd_to_ed']
+            'cv_sd_to_ed', 'cv_vorticity_enable', 'cv_vorticity', 'cv_vorticity_mu', 'cv_hypersf_get_vorticity']
 
 
 def make_eos(table):
@@ -308,6 +313,25 @@ class Handle(object):
         s = np.ascontiguousarray(s_nodes, np.float64)
         e = np.ascontiguousarray(e_nodes, np.float64)
         check(self.L.cv_sd_to_ed(self.h, which, s.ctypes.data, e.ctypes.data, s.size))
+
+    def vorticity_enable(self):
+        check(self.L.cv_vorticity_enable(self.h))
+
+    def vorticity(self, tau):
+        """d_omega[1] from d_ev[0] and d_ev[1] at tau (cv_vorticity)."""
+        check(self.L.cv_vorticity(self.h, float(tau)))
+
+    def vorticity_mu(self, efrz):
+        out = np.empty((self.n, 4), np.float32)
+        check(self.L.cv_vorticity_mu(self.h, float(efrz), out.ctypes.data, out.size))
+        return out
+
+    def hypersf_get_vorticity(self):
+        """(n, 32) vorticity tensors of the stored surface elements."""
+        n, _ = self.hypersf_count()
+        out = np.empty((n, 32), np.float32)
+        check(self.L.cv_hypersf_get_vorticity(self.h, out.ctypes.data if n else out.ctypes.data, n))
+        return out
 
     def max_ed(self, which=1):
         out = C.c_float()

@@ -16,6 +16,7 @@
 #include "cv_hypersf.h"
 #include "cv_mlp.h"
 #include "cv_ic.h"
+#include "cv_vort.h"
 
 // learned root-solve seed network (parameters, Adam moments, training set)
 struct cv_mlp {
@@ -88,6 +89,10 @@ struct cv_handle {
     float sf_tau_old = 0.0f;
     bool sf_have_old = false;
     float *sf_out = nullptr, *sf_txyz = nullptr, *sf_pisf = nullptr, *sf_bksf = nullptr;
+    // vorticity (cv_vorticity_enable): 32 planes per slot (d_omega[0], d_omega[1] with the
+    // shear and accT tensors), and 32 floats per surface element
+    float *om[2] = {nullptr, nullptr};
+    float *sf_om = nullptr;
     unsigned long long *sf_count = nullptr;
     int64_t sf_cap = 0;
 };
@@ -295,6 +300,7 @@ extern "C" int cv_destroy(cv_handle *h) {
     (void)hipFree(h->d_fix); (void)hipFree(h->d_fix_ctr);
     (void)hipFree(h->sf_ev); (void)hipFree(h->sf_pi); (void)hipFree(h->sf_bk);
     (void)hipFree(h->sf_out); (void)hipFree(h->sf_txyz); (void)hipFree(h->sf_pisf); (void)hipFree(h->sf_bksf);
+    (void)hipFree(h->sf_om); (void)hipFree(h->om[0]); (void)hipFree(h->om[1]);
     (void)hipFree(h->sf_count);
     (void)hipFree(h->seed_buf); (void)hipFree(h->iters);
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
@@ -316,6 +322,7 @@ static int field_ptr(cv_handle *h, int f, float **ptr, int *ncomp) {
     case CV_GOODCELL: *ptr = h->good; *ncomp = 1; break;
     case CV_ROOT_CONS: *ptr = h->seed_buf; *ncomp = 4; break;
     case CV_ROOT_SEED: *ptr = h->seed_buf ? h->seed_buf + 9 * h->n : nullptr; *ncomp = 1; break;
+    case CV_OMEGA0: case CV_OMEGA1: *ptr = h->om[f - CV_OMEGA0]; *ncomp = CV_VORT_N; break;
     default: return fail(CV_ERR_ARG, "unknown field id %d", f);
     }
     if (!*ptr) return fail(CV_ERR_STATE, "field %d is not allocated for this handle (ideal or no bulk)", f);
@@ -907,7 +914,8 @@ extern "C" int cv_last_elapsed_ms(cv_handle *h, float *ms) {
 // ---- freeze-out hypersurface ------------------------------------------------
 static void sf_free_out(cv_handle *h) {
     (void)hipFree(h->sf_out); (void)hipFree(h->sf_txyz); (void)hipFree(h->sf_pisf); (void)hipFree(h->sf_bksf);
-    h->sf_out = h->sf_txyz = h->sf_pisf = h->sf_bksf = nullptr;
+    (void)hipFree(h->sf_om);
+    h->sf_out = h->sf_txyz = h->sf_pisf = h->sf_bksf = h->sf_om = nullptr;
     h->sf_cap = 0;
 }
 
@@ -922,6 +930,7 @@ extern "C" int cv_hypersf_reserve(cv_handle *h, int64_t capacity) {
               hipMalloc((void **)&h->sf_txyz, sizeof(float) * 4 * capacity) == hipSuccess;
     if (ok && h->viscous) ok = hipMalloc((void **)&h->sf_pisf, sizeof(float) * 10 * capacity) == hipSuccess;
     if (ok && h->bulk) ok = hipMalloc((void **)&h->sf_bksf, sizeof(float) * capacity) == hipSuccess;
+    if (ok && h->om[0]) ok = hipMalloc((void **)&h->sf_om, sizeof(float) * 32 * capacity) == hipSuccess;
     if (ok && !h->sf_count) {
         ok = hipMalloc((void **)&h->sf_count, sizeof(unsigned long long)) == hipSuccess &&
              hipMemset(h->sf_count, 0, sizeof(unsigned long long)) == hipSuccess;
@@ -987,9 +996,81 @@ extern "C" int cv_hypersf_find(cv_handle *h, float tau_new) {
     auto kern = (p.flags & CV_FLAG_SF_FP64) ? k_hypersf<double> : k_hypersf<float>;
     hipLaunchKernelGGL(kern, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, h->stream, s, h->sf_ev,
                        h->ev[1], h->sf_pi, h->pi[1], h->bulk ? h->sf_bk : nullptr, h->bulk ? h->bk[1] : nullptr,
-                       h->sf_tau_old, tau_new, h->sf_count, h->sf_out, h->sf_txyz, h->sf_pisf, h->sf_bksf);
+                       h->sf_tau_old, tau_new, h->sf_count, h->sf_out, h->sf_txyz, h->sf_pisf, h->sf_bksf,
+                       h->om[0], h->om[1], h->sf_om);
     HIP_TRY(hipGetLastError());
+    // d_omega[0] <- d_omega[1] after the search (pyvisc/visc.py:392-395)
+    if (h->om[0])
+        HIP_TRY(hipMemcpyAsync(h->om[0], h->om[1], sizeof(float) * CV_VORT_N * h->n, hipMemcpyDeviceToDevice,
+                               h->stream));
     return cv_hypersf_snapshot(h, tau_new);
+}
+
+extern "C" int cv_vorticity_enable(cv_handle *h) {
+    if (!h || !h->viscous) return fail(CV_ERR_STATE, "cv_vorticity_enable: viscous handle required");
+    if (h->om[0]) return CV_OK;
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    const size_t bytes = sizeof(float) * CV_VORT_N * (size_t)h->n;
+    bool ok = hipMalloc((void **)&h->om[0], bytes) == hipSuccess && hipMalloc((void **)&h->om[1], bytes) == hipSuccess &&
+              hipMemset(h->om[0], 0, bytes) == hipSuccess && hipMemset(h->om[1], 0, bytes) == hipSuccess;
+    if (ok && h->sf_cap) ok = hipMalloc((void **)&h->sf_om, sizeof(float) * 32 * h->sf_cap) == hipSuccess;
+    if (!ok) {
+        (void)hipFree(h->om[0]); (void)hipFree(h->om[1]);
+        h->om[0] = h->om[1] = nullptr;
+        return fail(CV_ERR_HIP, "cv_vorticity_enable: allocation failed");
+    }
+    return CV_OK;
+}
+
+extern "C" int cv_vorticity(cv_handle *h, float tau) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    if (!h->om[0]) return fail(CV_ERR_STATE, "cv_vorticity: call cv_vorticity_enable first");
+    if (!(tau > 0.0f)) return fail(CV_ERR_ARG, "cv_vorticity: tau must be positive");
+    HIP_TRY(hipSetDevice(h->device));
+    if (h->eos.kind == 1)
+        hipLaunchKernelGGL(k_vorticity<1>, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, h->ev[0],
+                           h->ev[1], tau, h->om[1]);
+    else
+        hipLaunchKernelGGL(k_vorticity<0>, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, h->ev[0],
+                           h->ev[1], tau, h->om[1]);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return CV_OK;
+}
+
+extern "C" int cv_vorticity_mu(cv_handle *h, float efrz, float *host, int64_t nfloats) {
+    if (!h || !host) return fail(CV_ERR_ARG, "null argument");
+    if (!h->om[0]) return fail(CV_ERR_STATE, "cv_vorticity_mu: call cv_vorticity_enable first");
+    if (nfloats != 4 * h->n) return fail(CV_ERR_ARG, "cv_vorticity_mu: %lld floats expected", (long long)(4 * h->n));
+    HIP_TRY(hipSetDevice(h->device));
+    float *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(float) * 4 * h->n));
+    if (h->eos.kind == 1)
+        hipLaunchKernelGGL(k_vorticity_mu<1>, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, h->ev[1],
+                           h->om[1], efrz, d);
+    else
+        hipLaunchKernelGGL(k_vorticity_mu<0>, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, h->ev[1],
+                           h->om[1], efrz, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host, d, sizeof(float) * 4 * h->n, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_vorticity_mu: %s", hipGetErrorString(e));
+    return CV_OK;
+}
+
+extern "C" int cv_hypersf_get_vorticity(cv_handle *h, float *om, int64_t nelem) {
+    if (!h || !om) return fail(CV_ERR_ARG, "null argument");
+    if (!h->sf_om) return fail(CV_ERR_STATE, "cv_hypersf_get_vorticity: vorticity is not enabled");
+    int64_t c = 0, cap = 0;
+    int rc = cv_hypersf_count(h, &c, &cap);
+    if (rc) return rc;
+    if (c > cap) return fail(CV_ERR_CAPACITY, "cv_hypersf_get_vorticity: %lld elements, capacity %lld", (long long)c,
+                             (long long)cap);
+    if (nelem != c) return fail(CV_ERR_ARG, "cv_hypersf_get_vorticity: %lld elements stored", (long long)c);
+    if (c) HIP_TRY(hipMemcpy(om, h->sf_om, sizeof(float) * 32 * c, hipMemcpyDeviceToHost));
+    return CV_OK;
 }
 
 extern "C" int cv_hypersf_count(cv_handle *h, int64_t *count, int64_t *capacity) {

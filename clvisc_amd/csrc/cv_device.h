@@ -179,6 +179,10 @@ template <int K> struct EosT {
         return 18.202f * eps - 63.0218f - 4.85479f * expf(-2.72407E-11f * powf(eps, 4.54886f))
                + 65.1272f * powf(eps, -0.128012f) * expf(-0.00369624f * powf(eps, 1.18735f));
     }
+    __device__ __forceinline__ float T(float eps) const {
+        if (in_table(eps)) return raw(eps).z;
+        return (eps + P(eps)) * rcp(S(eps));
+    }
     __device__ __forceinline__ float CS2(float eps) const {
         if (in_table(eps)) return raw(eps).x;
         return 0.333333333f;

@@ -188,7 +188,8 @@ __global__ __launch_bounds__(128) void k_hypersf(SfP p, const float *__restrict_
                                                  const float *__restrict__ bk_old,
                                                  const float *__restrict__ bk_new, float tau_old_f, float tau_new_f,
                                                  unsigned long long *count, float *sf, float *txyz, float *pisf,
-                                                 float *bksf) {
+                                                 float *bksf, const float *__restrict__ om_old = nullptr,
+                                                 const float *__restrict__ om_new = nullptr, float *omsf = nullptr) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t total = (int64_t)p.gx * p.gy * p.gz;
     if (t >= total) return;
@@ -261,6 +262,14 @@ __global__ __launch_bounds__(128) void k_hypersf(SfP p, const float *__restrict_
 #pragma unroll
             for (int c = 0; c < 16; c++) acc = acc + w[c] * (R)(c < 8 ? bk_old : bk_new)[cid[c]];
             bksf[id] = (float)acc;
+        }
+    }
+    if (om_old) {      // vorticity tensors at the centroid (visc_hypersf, kernel_hypersf.cl:621-650)
+        for (int m = 0; m < 32; m++) {
+            R acc = 0;
+#pragma unroll
+            for (int c = 0; c < 16; c++) acc = acc + w[c] * (R)(c < 8 ? om_old : om_new)[(int64_t)m * p.n + cid[c]];
+            omsf[32 * id + m] = (float)acc;
         }
     }
 }

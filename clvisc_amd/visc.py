@@ -86,12 +86,17 @@ class CLVisc(object):
         self.ideal.hypersf.save_pimn(self.cfg.fPathOut, self.cfg.TFRZ, pr, set_to_zero=set_to_zero)
 
     def save(self, save_hypersf=True, save_bulk=False, save_pi=False, save_vorticity=False):
-        """pyvisc/visc.py:423-462 without the vorticity files."""
-        if save_vorticity:
-            raise NotImplementedError('vorticity on the surface is not part of this hydro path')
+        """pyvisc/visc.py:423-462: surface, pi on the surface, the vorticity files."""
         self.ideal.save(save_hypersf, save_bulk, viscous_on=True)
         if save_hypersf:
             self.save_pimn_sf()
+        if save_vorticity:
+            om = self.handle.hypersf_get_vorticity()
+            for name, a, b in (('omegamu_sf.dat', 0, 6), ('omegamu_shear1_sf.dat', 6, 22),
+                               ('omegamu_shear2_sf.dat', 22, 26), ('omegamu_accT_sf.dat', 26, 32)):
+                out_path = os.path.join(self.cfg.fPathOut, name)
+                print('vorticity omega_{mu, nu} on surface is saved to', out_path)
+                np.savetxt(out_path, om[:, a:b], fmt='%.6e', header='omega^{01, 02, 03, 12, 13, 23}')
         if save_pi and hasattr(self, 'pimn_info'):
             self.pimn_info.save()
 
@@ -142,7 +147,16 @@ class CLVisc(object):
             self.handle.longitudinal_profile(longitudinal_profile, which=1)
 
     def get_vorticity(self, save_data=False):
-        raise NotImplementedError('vorticity is not part of this hydro path')
+        """Thermal, shear and acceleration-temperature vorticity tensors of d_ev[0] -> d_ev[1]
+        into d_omega[1] on the device (pyvisc/visc.py:316-350); with save_data the vector
+        omega^mu = Omega^{mu nu} u_nu (first column T) goes to omega_mu_tau<tau>.dat."""
+        self.handle.vorticity_enable()
+        self.handle.vorticity(self.ideal.tau)
+        if save_data:
+            omega_mu = self.handle.vorticity_mu(self.ideal.efrz)
+            fname = ('omega_mu_tau%s' % self.ideal.tau).replace('.', 'p') + '.dat'
+            np.savetxt(os.path.join(os.path.abspath(self.cfg.fPathOut), fname), omega_mu,
+                       header='omega^{t, x, y, z} = Omega^{mu nu} u_{nu}')
 
     # -- main loop -----------------------------------------------------------------
     def evolve(self, max_loops=1000, save_hypersf=True, save_bulk=False, plot_bulk=False, save_pi=True,
@@ -150,7 +164,7 @@ class CLVisc(object):
         """The loop of pyvisc/visc.py:473-560.  Without force_run_to_maxloop it runs
         until max e < e_frz, as the reference does (max_loops is not consulted)."""
         if save_vorticity:
-            raise NotImplementedError('vorticity is not part of this hydro path')
+            self.handle.vorticity_enable()
         if jet_eloss_src is not None and jet_eloss_src.get('switch_on', False):
             raise NotImplementedError('the jet energy-loss source is not part of this hydro path')
         cfg = self.cfg
@@ -185,12 +199,16 @@ class CLVisc(object):
             # unchanged, so is is_finished); after is_finished the surface is taken
             # every loop
             nrun = 1 if (is_finished or (debug and loop % ntskip == 0)) else ntskip - loop % ntskip
+            if save_vorticity and loop % ntskip == 0:
+                nrun = 1            # the vorticity is taken after this step (visc.py:545-547)
             if force_run_to_maxloop:
                 nrun = min(nrun, max_loops - loop)
             self.handle.visc_steps(loop, nrun)
             self.update_time(loop + nrun - 1)
+            if save_vorticity and loop % ntskip == 0:
+                self.get_vorticity(save_data=False)
             if debug and loop % ntskip == 0:
                 self.check_bad_cell()
             loop += nrun
             print('one step: {dtime}'.format(dtime=(time() - t0) / nrun))
-        self.save(save_hypersf=save_hypersf, save_bulk=save_bulk, save_pi=save_pi)
+        self.save(save_hypersf=save_hypersf, save_bulk=save_bulk, save_pi=save_pi, save_vorticity=save_vorticity)

@@ -88,7 +88,9 @@ enum cv_field {
     CV_ROOT_CONS = 13,                         /* split root solve: its conserved input
                                                   (T^tt, T^tx, T^ty, T^t eta), no counterpart */
     CV_ROOT_SEED = 14,                         /* split root solve: the learned seed |v|    */
-    CV_NFIELDS = 15
+    CV_OMEGA0 = 15, CV_OMEGA1 = 16,            /* visc.d_omega[0..1] + d_omega_shear1/2 +
+                                                  d_omega_accT (visc.py:52-71), 32 per cell */
+    CV_NFIELDS = 17
 };
 
 /* Optical Glauber model (pyvisc/kernel/kernel_glauber.cl, pyvisc/glauber.py): the
@@ -197,6 +199,22 @@ int cv_longitudinal_profile(cv_handle *h, int which, const float *profile);
  * pyvisc/ebe_trento.py:26-33): piecewise linear through the m EOS nodes (s_nodes strictly
  * increasing, e_nodes), linear extrapolation at both ends, in double. */
 int cv_sd_to_ed(cv_handle *h, int which, const double *s_nodes, const double *e_nodes, int64_t m);
+
+/* ---- vorticity ------------------------------------------------------------------------
+ * Allocate d_omega[0..1] (32 components per cell: omega^{tx,ty,tz,xy,xz,yz}, omega_shear1
+ * (16), omega_shear2 (4), omega_accT (6); visc.py:52-71) and, with the surface, 32 floats
+ * per surface element.  Viscous handles only. */
+int cv_vorticity_enable(cv_handle *h);
+/* get_vorticity (visc.py:316-335): kernel_vorticity.cl omega / omega_shear / omega_accT from
+ * d_ev[0] (previous) and d_ev[1] at tau into d_omega[1].  After each cv_hypersf_find the
+ * tensors are interpolated to the elements and d_omega[0] <- d_omega[1] (visc.py:383-395). */
+int cv_vorticity(cv_handle *h, float tau);
+/* omega_mu (kernel_vorticity.cl:103-140, visc.py:336-350): host AoS rows (T, omega^x,
+ * omega^y, omega^eta) from d_ev[1] and d_omega[1], zero where e < 0.1 efrz. */
+int cv_vorticity_mu(cv_handle *h, float efrz, float *host, int64_t nfloats);
+/* The 32 vorticity components of each stored surface element (omegamu_sf.dat,
+ * omegamu_shear1_sf.dat, omegamu_shear2_sf.dat, omegamu_accT_sf.dat of visc.py:430-458). */
+int cv_hypersf_get_vorticity(cv_handle *h, float *om, int64_t nelem);
 
 /* Profiling: runs one viscous step (as cv_visc_steps_async with nsteps = 1) with HIP
  * events around each of its launches, waits, and writes the launch durations in

@@ -1257,3 +1257,107 @@ void orc_smear(const orc_params *p, const real *p4x4, int64_t np_, real one_o_2s
     }
     free(pre);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Vorticity, kernel_vorticity.cl: omega (:38-99), omega_shear (:143-221),     */
+/* omega_accT (:225-337), omega_mu (:103-140)                                 */
+/* ------------------------------------------------------------------------- */
+typedef struct { real ub[4], uk[4], T; } vq;
+
+static vq vort_q(const orc_params *p, const real *ev) {
+    vq q;
+    real u[4];
+    umu4(ev, u);
+    real T = eos_T(p, ev[0]);
+    static const real gm[4] = {RC(1.0f), RC(-1.0f), RC(-1.0f), RC(-1.0f)};
+    for (int c = 0; c < 4; c++) { q.uk[c] = gm[c] * u[c]; q.ub[c] = DIV(q.uk[c], T); }
+    q.T = T;
+    return q;
+}
+
+/* ev_old, ev_new: AoS rows of 4 (d_ev[0], d_ev[1]); out: n rows of 32 in the order
+ * omega(6), omega_shear1(16), omega_shear2(4), omega_accT(6).  An axis of one cell has zero
+ * partial derivative (the reference reads past the grid there); the Christoffel terms stay. */
+void orc_vorticity(const orc_params *p, const real *ev_old, const real *ev_new, real tau, real *out) {
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const real DT = RC(p->dt), DX = RC(p->dx), DY = RC(p->dy), DZ = RC(p->dz);
+    set_threads(p);
+    #pragma omp parallel for schedule(static)
+    for (int64_t I = 0; I < (int64_t)nx * ny * nz; I++) {
+        const int k = (int)(I % nz), j = (int)((I / nz) % ny), i = (int)(I / ((int64_t)nz * ny));
+        vq qo = vort_q(p, ev_old + 4 * I), qn = vort_q(p, ev_new + 4 * I);
+        const int idx[3] = {i, j, k}, dim[3] = {nx, ny, nz};
+        const int64_t stride[3] = {(int64_t)ny * nz, nz, 1};
+        const real h2[3] = {RC(2.0f) * DX, RC(2.0f) * DY, RC(2.0f) * DZ * tau};
+        const real h1[3] = {DX, DY, DZ * tau};
+        real dub[3][4], duk[3][4], dT[3];
+        for (int a = 0; a < 3; a++) {
+            if (dim[a] == 1) {
+                for (int c = 0; c < 4; c++) dub[a][c] = duk[a][c] = RC(0.0f);
+                dT[a] = RC(0.0f);
+            } else if (idx[a] != 0 && idx[a] != dim[a] - 1) {
+                vq qp = vort_q(p, ev_new + 4 * (I + stride[a])), qm = vort_q(p, ev_new + 4 * (I - stride[a]));
+                for (int c = 0; c < 4; c++) { dub[a][c] = DIV(qp.ub[c] - qm.ub[c], h2[a]); duk[a][c] = DIV(qp.uk[c] - qm.uk[c], h2[a]); }
+                dT[a] = DIV(qp.T - qm.T, h2[a]);
+            } else if (idx[a] == 0) {
+                vq qp = vort_q(p, ev_new + 4 * (I + stride[a]));
+                for (int c = 0; c < 4; c++) { dub[a][c] = DIV(qp.ub[c] - qn.ub[c], h1[a]); duk[a][c] = DIV(qp.uk[c] - qn.uk[c], h1[a]); }
+                dT[a] = DIV(qp.T - qn.T, h1[a]);
+            } else {
+                vq qm = vort_q(p, ev_new + 4 * (I - stride[a]));
+                for (int c = 0; c < 4; c++) { dub[a][c] = DIV(qn.ub[c] - qm.ub[c], h1[a]); duk[a][c] = DIV(qn.uk[c] - qm.uk[c], h1[a]); }
+                dT[a] = DIV(qn.T - qm.T, h1[a]);
+            }
+        }
+        real *o = out + 32 * I;
+        real dbt[4];
+        for (int c = 0; c < 4; c++) dbt[c] = DIV(qn.ub[c] - qo.ub[c], DT);
+        const real hc = RC(0.5f) * RC(0.19733f);
+        o[0] = hc * (dub[1][3] - dub[2][2]);
+        o[1] = hc * (dub[2][1] - dub[0][3]);
+        o[2] = hc * (dub[0][2] - dub[1][1]);
+        o[3] = hc * (dbt[3] - dub[2][0] + DIV(qn.ub[3], tau));
+        o[4] = hc * (dub[1][0] - dbt[2]);
+        o[5] = hc * (dbt[1] - dub[0][0]);
+        real dkt[4], dkz[4];
+        for (int c = 0; c < 4; c++) dkt[c] = DIV(qn.uk[c] - qo.uk[c], DT);
+        const real chr[4] = {DIV(-qn.uk[3], tau), RC(0.0f), RC(0.0f), DIV(-qn.uk[0], tau)};
+        for (int c = 0; c < 4; c++) dkz[c] = duk[2][c] + chr[c];
+        const real *rows[4] = {dkt, duk[0], duk[1], dkz};
+        for (int a = 0; a < 4; a++)
+            for (int c = 0; c < 4; c++) o[6 + 4 * a + c] = RC(0.5f) * rows[a][c];
+        real u[4], DU[4];
+        umu4(ev_new + 4 * I, u);
+        for (int c = 0; c < 4; c++) {
+            DU[c] = u[0] * dkt[c] + u[1] * duk[0][c] + u[2] * duk[1][c] + u[3] * dkz[c];
+            o[22 + c] = RC(0.5f) * DU[c];
+        }
+        const real T = qn.T;
+        const real dTt = DIV(qn.T - qo.T, DT);
+        const real gt = DU[0] - DIV(dTt, T), gx = DU[1] - DIV(dT[0], T), gy = DU[2] - DIV(dT[1], T),
+                   gz = DU[3] - DIV(dT[2], T);
+        const real *kk = qn.uk;
+        o[26] = RC(0.5f) * (kk[2] * gz - kk[3] * gy);
+        o[27] = RC(0.5f) * (-kk[1] * gz + kk[3] * gx);
+        o[28] = RC(0.5f) * (kk[1] * gy - kk[2] * gx);
+        o[29] = RC(0.5f) * (-kk[3] * gt + kk[0] * gz);
+        o[30] = RC(0.5f) * (kk[2] * gt - kk[0] * gy);
+        o[31] = RC(0.5f) * (-kk[1] * gt + kk[0] * gx);
+    }
+}
+
+/* omega_mu: out n rows of 4 = (T, omega^x, omega^y, omega^eta), zero where e < 0.1 efrz */
+void orc_vorticity_mu(const orc_params *p, const real *ev, const real *om32, real efrz, real *out) {
+    int64_t n = (int64_t)p->nx * p->ny * p->nz;
+    for (int64_t I = 0; I < n; I++) {
+        real u[4];
+        umu4(ev + 4 * I, u);
+        const real *w = om32 + 32 * I;
+        real *o = out + 4 * I;
+        o[0] = eos_T(p, ev[4 * I]);
+        o[1] = -u[0] * w[0] - u[1] * RC(0.0f) - u[2] * w[3] - u[3] * w[4];
+        o[2] = -u[0] * w[1] + u[1] * w[3] - u[2] * RC(0.0f) - u[3] * w[5];
+        o[3] = -u[0] * w[2] + u[1] * w[4] + u[2] * w[5] - u[3] * RC(0.0f);
+        if ((double)ev[4 * I] < 0.1 * (double)efrz) o[0] = o[1] = o[2] = o[3] = RC(0.0f);
+    }
+}

@@ -399,3 +399,27 @@ def smear(params, p4x4, sigr, sigz, kfactor, precision='f32'):
     L.orc_smear.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, rt, rt, rt, C.c_void_p]
     L.orc_smear(C.byref(params), _ptr(a) if a.size else None, a.shape[0], c, w, kfactor, _ptr(ev))
     return ev
+
+
+def vorticity(params, ev_old, ev_new, tau, precision='f32'):
+    """(n, 32) vorticity tensors of kernel_vorticity.cl (orc_vorticity) from AoS fields."""
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    rt = C.c_float if precision == 'f32' else C.c_double
+    a, b = (np.ascontiguousarray(np.asarray(x, dt).reshape(-1, 4)) for x in (ev_old, ev_new))
+    out = np.zeros((a.shape[0], 32), dt)
+    L.orc_vorticity.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, rt, C.c_void_p]
+    L.orc_vorticity(C.byref(params), _ptr(a), _ptr(b), tau, _ptr(out))
+    return out
+
+
+def vorticity_mu(params, ev, om32, efrz, precision='f32'):
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    rt = C.c_float if precision == 'f32' else C.c_double
+    e = np.ascontiguousarray(np.asarray(ev, dt).reshape(-1, 4))
+    w = np.ascontiguousarray(np.asarray(om32, dt).reshape(-1, 32))
+    out = np.zeros((e.shape[0], 4), dt)
+    L.orc_vorticity_mu.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, rt, C.c_void_p]
+    L.orc_vorticity_mu(C.byref(params), _ptr(e), _ptr(w), efrz, _ptr(out))
+    return out
