@@ -112,6 +112,7 @@ def lib():
             'cv_vorticity': [vp, f32],
             'cv_vorticity_mu': [vp, f32, vp, i64],
             'cv_hypersf_get_vorticity': [vp, vp, i64],
+            'cv_polarization': [C.c_int, C.c_int, vp, vp, vp, i64, vp, i64, f32, f32, vp, vp, vp],
             'cv_sync': [vp],
             'cv_last_elapsed_ms': [vp, C.POINTER(f32)],
             'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
@@ -177,7 +178,8 @@ def exported_symbols():
             'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward',
             'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums', 'cv_glauber_ini',
             'cv_glauber_nbinary', 'cv_smear_partons', 'cv_longitudinal_profile',
-            'cv_sd_to_ed', 'cv_vorticity_enable', 'cv_vorticity', 'cv_vorticity_mu', 'cv_hypersf_get_vorticity']
+            'cv_sd_to_ed', 'cv_vorticity_enable', 'cv_vorticity', 'cv_vorticity_mu', 'cv_hypersf_get_vorticity',
+            'cv_polarization']
 
 
 def make_eos(table):

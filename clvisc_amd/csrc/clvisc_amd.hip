@@ -17,6 +17,7 @@
 #include "cv_mlp.h"
 #include "cv_ic.h"
 #include "cv_vort.h"
+#include "cv_pol.h"
 
 // learned root-solve seed network (parameters, Adam moments, training set)
 struct cv_mlp {
@@ -893,6 +894,49 @@ extern "C" int cv_sd_to_ed(cv_handle *h, int which, const double *s_nodes, const
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     (void)hipFree(d);
     if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_sd_to_ed: %s", hipGetErrorString(e));
+    return CV_OK;
+}
+
+extern "C" int cv_polarization(int device, int kind, const float *sf, const float *om1, const float *om2,
+                               int64_t nsf, const float *mom, int64_t nmom, float tfrz, float mass,
+                               float *pol_lab, float *density, float *pol_lrf) {
+    if (kind < 0 || kind > 2) return fail(CV_ERR_ARG, "cv_polarization: kind must be 0, 1 or 2");
+    if (!sf || !om1 || (kind == 1 && !om2) || !mom || !pol_lab || !density || !pol_lrf || nsf < 0 || nmom < 1 ||
+        nmom > (1 << 30))
+        return fail(CV_ERR_ARG, "cv_polarization: bad argument");
+    if (!(tfrz > 0.0f) || !(mass > 0.0f)) return fail(CV_ERR_ARG, "cv_polarization: T_frz and mass must be positive");
+    HIP_TRY(hipSetDevice(device));
+    const int w1 = kind == 1 ? 16 : 6;
+    const int64_t n1 = nsf > 0 ? nsf : 1;
+    size_t fl = (size_t)n1 * (8 + w1 + (kind == 1 ? 4 : 0)) + (size_t)nmom * 13;
+    float *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(float) * fl));
+    float *dsf = d, *d1 = dsf + 8 * n1, *d2 = d1 + (size_t)w1 * n1, *dm = d2 + (kind == 1 ? 4 * n1 : 0);
+    float *dpl = dm + 4 * nmom, *dde = dpl + 4 * nmom, *dlr = dde + nmom;
+    hipError_t e = hipSuccess;
+    if (nsf > 0) {
+        e = hipMemcpy(dsf, sf, sizeof(float) * 8 * nsf, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d1, om1, sizeof(float) * w1 * nsf, hipMemcpyHostToDevice);
+        if (e == hipSuccess && kind == 1) e = hipMemcpy(d2, om2, sizeof(float) * 4 * nsf, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemcpy(dm, mom, sizeof(float) * 4 * nmom, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        PolP P;
+        P.nsf = nsf; P.tfrz = tfrz; P.beta = (float)(1.0 / (double)tfrz); P.mass = mass; P.kind = kind;
+        if (kind == 0)
+            hipLaunchKernelGGL(k_polarization<0>, dim3((unsigned)nmom), dim3(256), 0, 0, P, dsf, d1, d2, dm, dpl, dde, dlr);
+        else if (kind == 1)
+            hipLaunchKernelGGL(k_polarization<1>, dim3((unsigned)nmom), dim3(256), 0, 0, P, dsf, d1, d2, dm, dpl, dde, dlr);
+        else
+            hipLaunchKernelGGL(k_polarization<2>, dim3((unsigned)nmom), dim3(256), 0, 0, P, dsf, d1, d2, dm, dpl, dde, dlr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(pol_lab, dpl, sizeof(float) * 4 * nmom, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(density, dde, sizeof(float) * nmom, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(pol_lrf, dlr, sizeof(float) * 4 * nmom, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_polarization: %s", hipGetErrorString(e));
     return CV_OK;
 }
 

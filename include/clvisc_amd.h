@@ -216,6 +216,17 @@ int cv_vorticity_mu(cv_handle *h, float efrz, float *host, int64_t nfloats);
  * omegamu_shear1_sf.dat, omegamu_shear2_sf.dat, omegamu_accT_sf.dat of visc.py:430-458). */
 int cv_hypersf_get_vorticity(cv_handle *h, float *om, int64_t nelem);
 
+/* Lambda polarization from the freeze-out surface (pyvisc/kernel/kernel_polarization.cl,
+ * Polarization.get_themal / get_shear / get_accT of pyvisc/polarization/polarization.py):
+ * sf = nsf rows of 8 (hypersf.dat: dS_mu, v, eta_s); kind 0: om1 = omegamu_sf rows (6);
+ * kind 1: om1 = omegamu_shear1_sf rows (16), om2 = omegamu_shear2_sf rows (4); kind 2:
+ * om1 = omegamu_accT_sf rows (6).  mom = nmom rows (m_T, Y, p_x, p_y).  Outputs per
+ * momentum: pol_lab (4, = sum / (4 mass)), density, pol_lrf (4, rest frame of the Lambda).
+ * No handle: device index only. */
+int cv_polarization(int device, int kind, const float *sf, const float *om1, const float *om2, int64_t nsf,
+                    const float *mom, int64_t nmom, float tfrz, float mass, float *pol_lab, float *density,
+                    float *pol_lrf);
+
 /* Profiling: runs one viscous step (as cv_visc_steps_async with nsteps = 1) with HIP
  * events around each of its launches, waits, and writes the launch durations in
  * milliseconds to ms[0..nms-1] in launch order; returns the number of launches in

@@ -423,3 +423,19 @@ def vorticity_mu(params, ev, om32, efrz, precision='f32'):
     L.orc_vorticity_mu.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, rt, C.c_void_p]
     L.orc_vorticity_mu(C.byref(params), _ptr(e), _ptr(w), efrz, _ptr(out))
     return out
+
+
+def polarization(kind, sf, om1, om2, mom, tfrz, mass, precision='f32'):
+    """(pol_lab (m,4), density (m,), pol_lrf (m,4)) of orc_polarization (kernel_polarization.cl)."""
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    rt = C.c_float if precision == 'f32' else C.c_double
+    f = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+    sf, om1, om2, mom = f(sf), f(om1), f(om2), f(mom).reshape(-1, 4)
+    m = mom.shape[0]
+    pl, de, lr = np.zeros((m, 4), dt), np.zeros(m, dt), np.zeros((m, 4), dt)
+    L.orc_polarization.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                   rt, rt, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_polarization(kind, _ptr(sf), _ptr(om1), _ptr(om2), sf.shape[0], _ptr(mom), m, tfrz, mass, _ptr(pl),
+                       _ptr(de), _ptr(lr))
+    return pl, de, lr

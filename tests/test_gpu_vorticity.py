@@ -116,3 +116,38 @@ def test_clvisc_evolve_saves_vorticity(tmp_path):
     visc.get_vorticity(save_data=True)
     files = list(tmp_path.glob('omega_mu_tau*.dat'))
     assert files and np.loadtxt(files[0]).shape == (nx * ny * nz, 4)
+    # the polarization driver reads what evolve wrote
+    from clvisc_amd.polarization import Polarization
+    pz = Polarization(str(tmp_path), T=0.30, shear=True, accT=True)
+    mt = np.sqrt(1.0 + 1.116 ** 2)
+    pz.set_momentum(np.array([[mt, 0.0, 1.0, 0.0], [mt, 0.3, 0.0, 1.0]], np.float32))
+    for get in (pz.get_themal, pz.get_shear, pz.get_accT):
+        pol, rho, lrf = get()
+        assert pol.shape == (2, 4) and np.all(rho > 0) and np.all(np.isfinite(lrf))
+
+
+@pytest.mark.parametrize('kind', [0, 1, 2])
+def test_polarization_matches_oracle(kind):
+    """cv_polarization (one workgroup per momentum, register sums) against the oracle's
+    restatement in the reference's summation order (tree per 256 elements)."""
+    from test_polarization_oracle import _surface, _mom, T, M
+    from clvisc_amd.polarization import Polarization
+    n = 3000
+    sf = _surface(n, 11)
+    mom = _mom(64, 12)
+    rng = np.random.default_rng(13)
+    w = 16 if kind == 1 else 6
+    o1 = rng.normal(0, 0.05, (n, w)).astype(np.float32)
+    o2 = rng.normal(0, 0.05, (n, 4)).astype(np.float32)
+    _say('case: polarization kind %d, %d elements, %d momenta' % (kind, n, mom.shape[0]))
+    pz = Polarization.from_arrays(sf, omega=o1 if kind == 0 else None, omega_shear1=o1 if kind == 1 else None,
+                                  omega_shear2=o2 if kind == 1 else None, omega_accT=o1 if kind == 2 else None,
+                                  T=T, mass=M)
+    pz.set_momentum(mom)
+    pol, rho, lrf = (pz.get_themal, pz.get_shear, pz.get_accT)[kind]()
+    rp, rr, rl = O.polarization(kind, sf, o1, o2, mom, T, M, 'f32')
+    e_pol = np.abs(pol - rp).max() / np.abs(rp).max()
+    e_rho = np.abs(rho - rr).max() / np.abs(rr).max()
+    e_lrf = np.abs(lrf[:, 1:] - rl[:, 1:]).max() / np.abs(rl[:, 1:]).max()
+    _say('  rel diff: pol_lab %.3g density %.3g pol_lrf %.3g' % (e_pol, e_rho, e_lrf))
+    assert e_pol < 1e-4 and e_rho < 1e-5 and e_lrf < 1e-4
