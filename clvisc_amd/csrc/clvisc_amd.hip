@@ -467,7 +467,8 @@ static int launch_visc_root(cv_handle *h, int step, const float *ev1, const floa
     float *seed = nullptr;
     if (h->seed_mlp) {
         seed = h->seed_buf + 9 * n;
-        const int64_t nblk = (n + 16 * MLP_WAVES - 1) / (16 * MLP_WAVES);
+        const int64_t per_blk = 16 * MLP_WAVES * MLP_SEED_TILES;
+        const int64_t nblk = (n + per_blk - 1) / per_blk;
         hipLaunchKernelGGL(k_mlp_seed_cons, dim3((unsigned)nblk), dim3(64 * MLP_WAVES), 0, h->stream, h->seed_mlp->P,
                            cons, n, seed);
     }

@@ -342,37 +342,107 @@ __device__ __forceinline__ void seed_features(float T00, float M, float T03, flo
 // Seed of the split viscous root solve: the conserved input as the stage wrote it (four
 // planes T^tt, T^tx, T^ty, T^t eta before the |T^ti| <= 0.999 T^tt clamp, as k_visc_root
 // reads them), the clamp, the features, the network; seed_v = the |v| output.
+// Throughput form of the forward pass (only the |v| output): every wave packs its share of
+// the weights once (BF16 B operands of the 64x64 layer in registers, hardware round to
+// nearest even) and runs MLP_SEED_TILES tiles of 16 samples; tanh(z) = 1 - 2/(exp(2z) + 1)
+// with the hardware exp and reciprocal.  The seed only sets where the root solve starts
+// (the solve keeps the reference's bracket and stopping rule), so these approximations
+// change iteration counts, not what the solve converges to.
+#define MLP_SEED_TILES 8
+__device__ __forceinline__ float tanh_seed(float z) {
+    const float e = __expf(2.0f * z);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+__device__ __forceinline__ float bf16_hw(float x) { return (float)(__bf16)x; }
+__device__ __forceinline__ mlp_bf16x4 pack4_hw(float a, float b, float c, float d) {
+    typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+    const bf2 lo = (bf2){(__bf16)a, (__bf16)b}, hi = (bf2){(__bf16)c, (__bf16)d};
+    const uint32_t ulo = __builtin_bit_cast(uint32_t, lo), uhi = __builtin_bit_cast(uint32_t, hi);
+    mlp_bf16x4 r;
+    r[0] = (short)(ulo & 0xffffu); r[1] = (short)(ulo >> 16); r[2] = (short)(uhi & 0xffffu); r[3] = (short)(uhi >> 16);
+    return r;
+}
+
 __global__ __launch_bounds__(64 * MLP_WAVES) void k_mlp_seed_cons(const float *__restrict__ P,
                                                                 const float *__restrict__ cons, int64_t n,
                                                                 float *seed_v) {
-    __shared__ float s_x[MLP_WAVES][16][MLP_IN];
-    __shared__ float s_h[MLP_WAVES][16][MLP_H];
+    __shared__ float s_h[MLP_WAVES][16][MLP_H + 1];
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int c0 = l & 15, rg = l >> 4;
-    const int64_t base = ((int64_t)blockIdx.x * MLP_WAVES + wv) * 16;
-    if (l < 16) {
-        const int64_t r = base + l;
-        float x[MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f};
-        if (r < n) {
-            float T00 = cons[r], T01 = cons[n + r], T02 = cons[2 * n + r], T03 = cons[3 * n + r];
-            float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
-            if (M > T00) {
-                T03 *= (0.999f * T00) * (1.0f / M);
-                M = 0.999f * T00;
-            }
-            seed_features(T00, M, T03, x);
-        }
+    // this lane's weights: layer-1 columns 16 n + c0, the 64x64 layer's B operands, layer 3 (|v|)
+    float w1[4][MLP_IN], b1[4], b2[4], w3[4];
+    mlp_bf16x4 bp[4][4];
 #pragma unroll
-        for (int k = 0; k < MLP_IN; k++) s_x[wv][l][k] = x[k];
+    for (int m = 0; m < 4; m++) {
+        const int col = 16 * m + c0;
+#pragma unroll
+        for (int k = 0; k < MLP_IN; k++) w1[m][k] = bf16_hw(P[MLP_OFF_W1 + col * MLP_IN + k]);
+        b1[m] = P[MLP_OFF_B1 + col];
+        b2[m] = P[MLP_OFF_B2 + col];
+        w3[m] = bf16_hw(P[MLP_OFF_W3 + 1 * MLP_H + col]);
+#pragma unroll
+        for (int st = 0; st < 4; st++) {
+            const float *w = P + MLP_OFF_W2 + col * MLP_H + 16 * st + 4 * rg;
+            bp[st][m] = pack4_hw(w[0], w[1], w[2], w[3]);
+        }
     }
-    __syncthreads();
-    MlpFwd f;
-    mlp_forward_wave(P, s_x[wv], s_h[wv], f);
-    if (c0 == 0) {
+    const float b3 = P[MLP_OFF_B3 + 1];
+    const int64_t tile0 = ((int64_t)blockIdx.x * MLP_WAVES + wv) * MLP_SEED_TILES;
+    for (int t = 0; t < MLP_SEED_TILES; t++) {
+        const int64_t base = (tile0 + t) * 16;
+        // features of sample base + (l & 15), broadcast to the rows of this lane by shuffles
+        float x[MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f};
+        {
+            const int64_t r = base + c0;
+            if (r < n) {
+                float T00 = cons[r], T01 = cons[n + r], T02 = cons[2 * n + r], T03 = cons[3 * n + r];
+                float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
+                if (M > T00) {
+                    T03 *= (0.999f * T00) * (1.0f / M);
+                    M = 0.999f * T00;
+                }
+                seed_features(T00, M, T03, x);
+            }
+        }
+        float xr[4][MLP_IN];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int k = 0; k < MLP_IN; k++) xr[i][k] = bf16_hw(__shfl(x[k], 4 * rg + i, 64));
+        // layer 1 (VALU): C layout, row 4 rg + i, column 16 m + c0; to LDS for the A layout
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                float z = xr[i][0] * w1[m][0];
+                z = z + xr[i][1] * w1[m][1];
+                z = z + xr[i][2] * w1[m][2];
+                z = z + xr[i][3] * w1[m][3];
+                s_h[wv][4 * rg + i][16 * m + c0] = tanh_seed(z + b1[m]);
+            }
+        __syncthreads();
+        mlp_f32x4 acc[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) acc[m] = mlp_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int st = 0; st < 4; st++) {
+            const int kb = 16 * st + 4 * rg;
+            const mlp_bf16x4 a = pack4_hw(s_h[wv][c0][kb + 0], s_h[wv][c0][kb + 1], s_h[wv][c0][kb + 2],
+                                          s_h[wv][c0][kb + 3]);
+#pragma unroll
+            for (int m = 0; m < 4; m++) acc[m] = mfma16(a, bp[st][m], acc[m]);
+        }
+        __syncthreads();
+        // layer 2 activation and layer 3 (the |v| output only)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
+            float sum = bf16_hw(tanh_seed(acc[0][i] + b2[0])) * w3[0];
+            sum = sum + bf16_hw(tanh_seed(acc[1][i] + b2[1])) * w3[1];
+            sum = sum + bf16_hw(tanh_seed(acc[2][i] + b2[2])) * w3[2];
+            sum = sum + bf16_hw(tanh_seed(acc[3][i] + b2[3])) * w3[3];
+            const float y = sum16(sum) + b3;
             const int64_t r = base + 4 * rg + i;
-            if (r < n) seed_v[r] = f.y[i][1];
+            if (c0 == 0 && r < n) seed_v[r] = y;
         }
     }
 }
