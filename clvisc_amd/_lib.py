@@ -113,6 +113,7 @@ def lib():
             'cv_vorticity_mu': [vp, f32, vp, i64],
             'cv_hypersf_get_vorticity': [vp, vp, i64],
             'cv_polarization': [C.c_int, C.c_int, vp, vp, vp, i64, vp, i64, f32, f32, vp, vp, vp],
+            'cv_set_xcd_map': [vp, C.c_int],
             'cv_sync': [vp],
             'cv_last_elapsed_ms': [vp, C.POINTER(f32)],
             'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
@@ -179,7 +180,7 @@ def exported_symbols():
             'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums', 'cv_glauber_ini',
             'cv_glauber_nbinary', 'cv_smear_partons', 'cv_longitudinal_profile',
             'cv_sd_to_ed', 'cv_vorticity_enable', 'cv_vorticity', 'cv_vorticity_mu', 'cv_hypersf_get_vorticity',
-            'cv_polarization']
+            'cv_polarization', 'cv_set_xcd_map']
 
 
 def make_eos(table):
@@ -334,6 +335,9 @@ class Handle(object):
         out = np.empty((n, 32), np.float32)
         check(self.L.cv_hypersf_get_vorticity(self.h, out.ctypes.data if n else out.ctypes.data, n))
         return out
+
+    def set_xcd_map(self, nxcd):
+        check(self.L.cv_set_xcd_map(self.h, int(nxcd)))
 
     def max_ed(self, which=1):
         out = C.c_float()

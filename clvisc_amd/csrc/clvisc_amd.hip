@@ -78,6 +78,7 @@ struct cv_handle {
     TileP tile;
     int tile_grid = 0;
     int march_xc = 0;         // x planes per march chunk (0: automatic)
+    int xcd = 8;              // XCD-contiguous work mapping of the tiled kernels (MI355X: 8 XCDs; cv_set_xcd_map)
     // learned root-solve seed of the ideal stage (cv_set_root_seed)
     struct cv_mlp *seed_mlp = nullptr;
     bool track_iters = false;
@@ -126,6 +127,7 @@ static int set_tile(cv_handle *h, int ty, int tz) {
     if (h->march_xc > 0) t.xc = h->march_xc < p.nx ? h->march_xc : p.nx;
     t.nchunks = (p.nx + t.xc - 1) / t.xc;
     t.nwork = cols * t.nchunks;
+    t.xcd = h->xcd;
     h->tile = t;
     int64_t g = (int64_t)(h->cus > 0 ? h->cus : 256) * 2 * 4;
     h->tile_grid = (int)(t.ntiles < g ? t.ntiles : g);
@@ -705,6 +707,16 @@ extern "C" int cv_set_march_chunk(cv_handle *h, int xc) {
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->march_xc = xc;
     return set_tile(h, h->tile.ty, h->tile.tz);
+}
+
+extern "C" int cv_set_xcd_map(cv_handle *h, int nxcd) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    if (nxcd < 0 || nxcd > 64) return fail(CV_ERR_ARG, "cv_set_xcd_map: nxcd in [0, 64]");
+    HIP_TRY(hipSetDevice(h->device));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->xcd = nxcd;
+    h->tile.xcd = nxcd;
+    return CV_OK;
 }
 
 extern "C" int cv_get_march_chunk(cv_handle *h, int32_t *xc, int32_t *nchunks) {

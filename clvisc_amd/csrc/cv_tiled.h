@@ -41,7 +41,18 @@ struct TileP {
     int64_t ntiles;            // nx * ntj * ntk
     int xc, nchunks;           // march: x planes per chunk, chunks along x
     int64_t nwork;             // march: ntj * ntk * nchunks
+    int xcd;                   // > 0: spread work items over this many XCDs in contiguous ranges
 };
+
+// Work item of workgroup b when the grid has one workgroup per item: the hardware hands
+// workgroups to the XCDs round-robin (b % nx), so XCD x gets the contiguous range of items
+// [start(x), start(x) + count(x)) and neighbouring tiles (shared y/z halos, the march's
+// x-neighbours) meet in one XCD's L2.  A bijection of [0, total).
+__device__ __forceinline__ int64_t xcd_work(int64_t b, int64_t total, int nx) {
+    const int64_t x = b % nx, i = b / nx;
+    const int64_t per = total / nx, rem = total % nx;
+    return x * per + (x < rem ? x : rem) + i;
+}
 
 // per-cell quantities of one cell from its raw fields
 template <class EOS>
@@ -611,7 +622,12 @@ __global__ __launch_bounds__(CV_TILE_THREADS, MARCH ? 3 : 2) void k_visc_tiled(P
     }
     __syncthreads();
     EosT<EK> eos(E, tab);
-    for (int64_t work = blockIdx.x; work < (MARCH ? tp.nwork : tp.ntiles); work += gridDim.x)
+    const int64_t total = MARCH ? tp.nwork : tp.ntiles;
+    if (tp.xcd > 0 && (int64_t)gridDim.x == total) {
+        tile_work<BULK, STEP, EK, SPEC, MARCH>(p, eos, st, tp, q, xcd_work(blockIdx.x, total, tp.xcd), s_c, s_f);
+        return;
+    }
+    for (int64_t work = blockIdx.x; work < total; work += gridDim.x)
         tile_work<BULK, STEP, EK, SPEC, MARCH>(p, eos, st, tp, q, work, s_c, s_f);
 }
 

@@ -245,6 +245,11 @@ int cv_get_stage_kernel(cv_handle *h, int32_t *kind, int32_t *ty, int32_t *tz);
 /* x planes per march chunk of kind 2 (0: automatic, about 32 work items per CU). */
 int cv_set_march_chunk(cv_handle *h, int xc);
 int cv_get_march_chunk(cv_handle *h, int32_t *xc, int32_t *nchunks);
+/* Work mapping of the tiled stage kernels (no counterpart in the reference): nxcd > 0 gives
+ * each of the nxcd XCDs, to which the hardware hands workgroups round-robin, a contiguous
+ * range of work items (neighbouring tiles share an L2); 0 keeps workgroup b on item b.
+ * Default 8 (MI355X has 8 XCDs).  Results are the same either way. */
+int cv_set_xcd_map(cv_handle *h, int nxcd);
 
 /* Wait for all queued work. */
 int cv_sync(cv_handle *h);

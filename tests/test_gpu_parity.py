@@ -80,9 +80,12 @@ def _visc_case(nx, ny, nz, bulk=None, nhot=12, seed=3, eos_kind='crossover'):
     return p, tab, ev
 
 
-def _visc_compare(p, tab, ev, nsteps, label, kernel=None, chunk=None):
-    _say('case: %s %dx%dx%d, %d steps, kernel %s chunk %s' % (label, p.nx, p.ny, p.nz, nsteps, kernel, chunk))
+def _visc_compare(p, tab, ev, nsteps, label, kernel=None, chunk=None, xcd=None):
+    _say('case: %s %dx%dx%d, %d steps, kernel %s chunk %s xcd map %s'
+         % (label, p.nx, p.ny, p.nz, nsteps, kernel, chunk, xcd))
     h = _lib.Handle(p, tab, viscous=True)
+    if xcd is not None:
+        h.set_xcd_map(xcd)
     if kernel is not None:
         h.set_stage_kernel(*kernel)
     if chunk is not None:
@@ -127,6 +130,15 @@ def test_visc_steps_small_3d(kernel, chunk):
 def test_visc_steps_2d_boost_invariant(kernel, chunk):
     p, tab, ev = _visc_case(45, 41, 1, bulk=(0.05, 0.155, 0.02))
     _visc_compare(p, tab, ev, 15, 'viscous+bulk 2d', kernel, chunk)
+
+
+@pytest.mark.parametrize('kernel,chunk,xcd', [((1, 0, 0), None, 0), ((1, 8, 27), None, 3),
+                                              ((2, 0, 0), None, 0), ((2, 7, 32), 3, 3), ((2, 11, 23), 5, 16)])
+def test_visc_steps_xcd_map(kernel, chunk, xcd):
+    """The tiled kernels' workgroup -> work-item mapping (off, and XCD counts that do not
+    divide the work) changes only the schedule: the oracle's fields bit for bit."""
+    p, tab, ev = _visc_case(29, 31, 11, bulk=(0.05, 0.155, 0.02))
+    _visc_compare(p, tab, ev, 20, 'viscous+bulk hotspots', kernel, chunk, xcd)
 
 
 @pytest.mark.parametrize('kernel', [(0, 0, 0), (2, 0, 0)])
