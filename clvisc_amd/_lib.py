@@ -133,6 +133,7 @@ def lib():
             'cv_batch_set_active': [vp, vp],
             'cv_batch_steps_async': [vp, i64],
             'cv_batch_set_loop': [vp, i64],
+            'cv_batch_set_xcd_map': [vp, C.c_int],
             'cv_batch_sync': [vp],
             'cv_batch_last_elapsed_ms': [vp, C.POINTER(f32)],
             'cv_batch_info': [vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)],
@@ -174,7 +175,7 @@ def exported_symbols():
             'cv_hypersf_reserve', 'cv_hypersf_snapshot',
             'cv_hypersf_find', 'cv_hypersf_count', 'cv_hypersf_get', 'cv_hypersf_reset',
             'cv_batch_create', 'cv_batch_destroy', 'cv_batch_handle', 'cv_batch_set_active',
-            'cv_batch_steps_async', 'cv_batch_set_loop', 'cv_batch_sync', 'cv_batch_last_elapsed_ms',
+            'cv_batch_steps_async', 'cv_batch_set_loop', 'cv_batch_set_xcd_map', 'cv_batch_sync', 'cv_batch_last_elapsed_ms',
             'cv_batch_info', 'cv_mlp_nparam', 'cv_mlp_create', 'cv_mlp_destroy', 'cv_mlp_get_params',
             'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward',
             'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums', 'cv_glauber_ini',
@@ -467,6 +468,9 @@ class Batch(object):
 
     def set_loop(self, loop):
         check(self.L.cv_batch_set_loop(self.b, int(loop)))
+
+    def set_xcd_map(self, nxcd):
+        check(self.L.cv_batch_set_xcd_map(self.b, int(nxcd)))
 
     def sync(self):
         check(self.L.cv_batch_sync(self.b))

@@ -1182,7 +1182,8 @@ struct cv_batch {
     bool timed = false;
     BatchState *d_state = nullptr;
     BatchEvent *d_events = nullptr;
-    unsigned long long *d_ctr = nullptr;
+    unsigned long long *d_ctr = nullptr;   // CV_BATCH_XCD_MAX work counters
+    int nxcd = 8;                          // work ranges of the persistent stage (cv_batch_set_xcd_map)
     int64_t loop = 0;                      // loop index of the next step
     bool dirty = true;                     // event table / graph must be rebuilt
     int nact = 0;
@@ -1231,7 +1232,7 @@ extern "C" int cv_batch_create(const cv_params *base, const cv_eos *eos, int nev
               hipEventCreate(&b->ev_a) == hipSuccess && hipEventCreate(&b->ev_b) == hipSuccess &&
               hipMalloc((void **)&b->d_state, sizeof(BatchState)) == hipSuccess &&
               hipMalloc((void **)&b->d_events, sizeof(BatchEvent) * nevents) == hipSuccess &&
-              hipMalloc((void **)&b->d_ctr, sizeof(unsigned long long)) == hipSuccess;
+              hipMalloc((void **)&b->d_ctr, sizeof(unsigned long long) * CV_BATCH_XCD_MAX) == hipSuccess;
     if (!ok) {
         int rc = fail(CV_ERR_HIP, "cv_batch_create: allocation failed");
         cv_batch_destroy(b);
@@ -1322,6 +1323,7 @@ static int batch_build(cv_batch *b) {
     st.step = 0;
     st.nwork = w;
     st.nevents = (int)order.size();
+    st.nxcd = b->nxcd > 1 ? b->nxcd : 1;
     HIP_TRY(hipSetDevice(b->device));
     if (!tab.empty())
         HIP_TRY(hipMemcpyAsync(b->d_events, tab.data(), sizeof(BatchEvent) * tab.size(), hipMemcpyHostToDevice,
@@ -1399,6 +1401,17 @@ extern "C" int cv_batch_steps_async(cv_batch *b, int64_t nsteps) {
     // the device step counter advanced with the host loop, and its buffer parity with the
     // host slot swaps above: the same graph serves the next call
     b->loop += nsteps;
+    return CV_OK;
+}
+
+extern "C" int cv_batch_set_xcd_map(cv_batch *b, int nxcd) {
+    if (!b) return fail(CV_ERR_ARG, "null batch");
+    if (nxcd < 0 || nxcd > CV_BATCH_XCD_MAX)
+        return fail(CV_ERR_ARG, "cv_batch_set_xcd_map: nxcd in [0, %d]", CV_BATCH_XCD_MAX);
+    int rc = batch_settle(b);
+    if (rc) return rc;
+    b->nxcd = nxcd;
+    b->dirty = true;
     return CV_OK;
 }
 

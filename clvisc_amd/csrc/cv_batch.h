@@ -1,8 +1,11 @@
 // cv_batch.h -- one persistent launch per RK stage for an ensemble of events with their
 // own grids (event-by-event hydro, the reference's pyvisc/ebe.py running one CLVisc per
 // event).  Every event keeps its own fields, tile and march chunk; the work items of all
-// events (tile column x x-chunk) form one list that the workgroups of a persistent grid
-// take from an atomic counter, largest events first.  The step index lives in device
+// events (tile column x x-chunk) form one list, largest events first, that the workgroups
+// of a persistent grid take from atomic counters: the list is cut into nxcd contiguous
+// ranges with a counter each, workgroup b starts on range b % nxcd (the XCD the hardware
+// dispatched it to, so that neighbouring tiles share that XCD's L2) and moves on to the
+// following ranges once its own is exhausted.  The step index lives in device
 // memory so that a captured graph of one step can be replayed: tau, the stage constants
 // and the d_ev[0] / d_ev[1] buffer rotation follow from it, with the same FP32 operations
 // as the single-event path.
@@ -27,10 +30,13 @@ struct BatchState {
     int64_t step;                      // steps taken since the descriptor was built
     int64_t nwork;                     // work items of all active events
     int nevents;                       // active events (entries of the event table)
+    int nxcd;                          // work ranges / counters, 1..CV_BATCH_XCD_MAX
 };
 
+#define CV_BATCH_XCD_MAX 16
+
 __global__ void k_batch_reset(unsigned long long *ctr) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *ctr = 0ull;
+    if (threadIdx.x < CV_BATCH_XCD_MAX && blockIdx.x == 0) ctr[threadIdx.x] = 0ull;
 }
 
 __global__ void k_batch_advance(BatchState *bs) {
@@ -52,10 +58,24 @@ __global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchS
     const float tau = (float)(bs->tau0 + (double)(bs->loop0 + n + (STEP - 1)) * bs->dt);
     const int64_t nwork = bs->nwork;
     const int nev = bs->nevents;
+    const int nr = bs->nxcd;
+    const int64_t per = nwork / nr, rem = nwork % nr;
+    const int home = (int)(blockIdx.x % (unsigned)nr);
+    int k = 0;                                          // thread 0: ranges exhausted so far
     for (;;) {
         if (threadIdx.x == 0) {
-            unsigned long long t = atomicAdd(ctr, 1ull);
-            s_work = t < (unsigned long long)nwork ? (int)t : (int)nwork;
+            int64_t got = nwork;
+            for (; k < nr; ++k) {
+                const int x = (home + k) % nr;
+                const int64_t cnt = per + (x < rem ? 1 : 0);
+                if (cnt == 0) continue;
+                const unsigned long long t = atomicAdd(&ctr[x], 1ull);
+                if (t < (unsigned long long)cnt) {
+                    got = (int64_t)x * per + (x < rem ? x : rem) + (int64_t)t;
+                    break;
+                }
+            }
+            s_work = (int)got;
         }
         __syncthreads();
         const int w = __builtin_amdgcn_readfirstlane(s_work);     // uniform: event table via scalar loads

@@ -328,6 +328,11 @@ int cv_batch_sync(cv_batch *b);
 int cv_batch_last_elapsed_ms(cv_batch *b, float *ms);
 /* events, loop index of the next step, work items (tile column x x-chunk) per stage */
 int cv_batch_info(cv_batch *b, int32_t *nevents, int64_t *loop, int64_t *nwork);
+/* Work ranges of the persistent stage (no counterpart in the reference): the event list is
+ * cut into nxcd contiguous ranges with a counter each and workgroup b starts on range
+ * b % nxcd, the XCD it runs on; 0 or 1 is one shared counter.  Default 8.  Results are the
+ * same either way. */
+int cv_batch_set_xcd_map(cv_batch *b, int nxcd);
 
 /* ---- learned seed of the conserved-to-primitive root solve ---------------------------
  * No reference counterpart: the reference starts every root solve at |v| = 0.5

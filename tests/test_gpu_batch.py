@@ -42,13 +42,17 @@ def _compare(hb, hs, label):
         np.testing.assert_array_equal(g, r, err_msg='%s %s' % (label, name))
 
 
-@pytest.mark.parametrize('bulk', [False, True])
-def test_batch_matches_single_events(bulk):
+@pytest.mark.parametrize('bulk,nxcd', [(False, None), (True, None), (True, 0), (False, 5), (True, 16)])
+def test_batch_matches_single_events(bulk, nxcd):
+    """Default work ranges (8), one shared counter (0) and range counts that do not divide
+    the work list: the schedule changes, the fields do not."""
     tab = Eos('crossover').create_table()
     p = cv_params(1, 1, 1, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08,
                   bulk=(0.05, 0.155, 0.02) if bulk else None)
-    _say('case: batch of %d events %s, bulk=%s' % (len(DIMS), DIMS, bulk))
+    _say('case: batch of %d events %s, bulk=%s, work ranges %s' % (len(DIMS), DIMS, bulk, nxcd))
     batch = _lib.Batch(p, tab, DIMS)
+    if nxcd is not None:
+        batch.set_xcd_map(nxcd)
     singles = []
     for e, d in enumerate(DIMS):
         ev = _ic(d, e)
