@@ -285,9 +285,10 @@ class Handle(object):
         return float(out[0]), float(out[1])
 
     def slice_sums(self, which=1):
-        """(NZ, 6) float64 per-eta-slice sums of Txx - Tyy, Txx + Tyy, T^tx, u0 s, u0 e |v_T|,
-        u0 e over the (x, y) plane (cv_slice_sums)."""
-        out = np.zeros((self.params.nz, 6), np.float64)
+        """(NZ, 10) float64 per-eta-slice sums over the (x, y) plane (cv_slice_sums): Txx - Tyy,
+        Txx + Tyy, T^tx, u0 s, u0 e |v_T|, u0 e (bulkinfo.py's floors), then (e + P) u0^2,
+        (e + P) u0^2 v^eta, P, u0 s (kernel_bulkinfo.cl's unfloored e and gamma)."""
+        out = np.zeros((self.params.nz, 10), np.float64)
         check(self.L.cv_slice_sums(self.h, which, out.ctypes.data))
         return out
 

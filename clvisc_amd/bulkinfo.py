@@ -23,7 +23,8 @@ class BulkInfo(object):
         self.cfg = cfg
         self.eos = eos
         self.device_reductions = device_reductions
-        self.slice_sums = []          # per call: (NZ, 6) device sums (cv_slice_sums)
+        self.slice_sums = []          # per call: (NZ, 10) device sums (cv_slice_sums)
+        self.energy, self.entropy = [], []   # get_total_energy_and_entropy_on_gpu
         NX, NY, NZ = cfg.NX, cfg.NY, cfg.NZ
         self.ex, self.ey, self.ez = [], [], []
         self.vx, self.vy, self.vz = [], [], []
@@ -134,6 +135,53 @@ class BulkInfo(object):
         u0 = 1.0 / np.sqrt(1.0 - vr2)
         return (u0 * self.eos.f_S(ed)).sum() * tau * self.cfg.DX * self.cfg.DY
 
+    def _etas(self):
+        # eta of slice k as kernel_bulkinfo.cl:38 takes it: (K - NZ/2) DZ, integer NZ/2
+        return (np.arange(self.cfg.NZ) - self.cfg.NZ // 2) * float(self.cfg.DZ)
+
+    def _energy_entropy(self, tau, s):
+        vol = tau * self.cfg.DX * self.cfg.DY * self.cfg.DZ
+        eta = self._etas()
+        e_lab = np.cosh(eta) * (s[:, 6] - s[:, 8]) + np.sinh(eta) * s[:, 7]
+        return float(e_lab.sum() * vol), float(s[:, 9].sum() * vol)
+
+    @staticmethod
+    def _device_field(d_ev):
+        if getattr(d_ev, 'field_id', None) not in (0, 1, 2):
+            raise TypeError('the device reductions need a DeviceField of d_ev[0], d_ev[1] or d_ev[2]')
+        return d_ev.handle, d_ev.field_id
+
+    def get_total_energy_and_entropy_on_gpu(self, tau, d_ev):
+        """Total lab-frame energy sum((e + P) u0^2 (cosh eta + v^eta sinh eta) - P cosh eta)
+        and total entropy sum(u0 s), times tau DX DY DZ, of d_ev at tau (bulkinfo_h5.py:264-277,
+        kernel_bulkinfo.cl:18-46); appended to energy / entropy and returned.  One device
+        reduction (cv_slice_sums) in double instead of two per-cell arrays summed on the device."""
+        h, fid = self._device_field(d_ev)
+        e_total, s_total = self._energy_entropy(tau, h.slice_sums(fid))
+        self.energy.append(e_total)
+        self.entropy.append(s_total)
+        return e_total, s_total
+
+    def ecc_vs_rapidity_on_gpu(self, tau, d_ev):
+        """(eccp1, eccp2) per eta slice: T^tx / (Txx + Tyy) and (Txx - Tyy) / (Txx + Tyy) summed
+        over the plane (kernel_bulkinfo.cl:51-113, bulkinfo_h5.py:292-297).  The sums floor e at
+        1e-10 and cap v^2 > 1 as bulkinfo.py does; kernel_bulkinfo.cl's gamma differs from that
+        only for 1 - v^2 < 1e-7."""
+        h, fid = self._device_field(d_ev)
+        s = h.slice_sums(fid)
+        return s[:, 2] / s[:, 1], s[:, 0] / s[:, 1]
+
+    def total_energy_and_entropy(self, tau, bulk):
+        """Host form of get_total_energy_and_entropy_on_gpu on a (NX, NY, NZ, 4) field."""
+        ed = np.asarray(bulk[..., 0], np.float64)
+        vx, vy, vz = (np.asarray(bulk[..., c], np.float64) for c in (1, 2, 3))
+        pr = np.asarray(self.eos.f_P(ed), np.float64)
+        u0 = 1.0 / np.sqrt(np.maximum(1.0 - vx * vx - vy * vy - vz * vz, 1.0e-7))
+        eta = self._etas()[None, None, :]
+        e_lab = (ed + pr) * u0 * u0 * (np.cosh(eta) + vz * np.sinh(eta)) - pr * np.cosh(eta)
+        vol = tau * self.cfg.DX * self.cfg.DY * self.cfg.DZ
+        return float(e_lab.sum() * vol), float((u0 * np.asarray(self.eos.f_S(ed), np.float64)).sum() * vol)
+
     def _ecc_vs_rapidity(self, bulk):
         NZ = self.cfg.NZ
         ecc1 = np.empty(NZ)
@@ -187,3 +235,10 @@ class BulkInfo(object):
         np.savetxt(path_out + '/edmax.dat', np.array(list(zip(self.time, self.edmax))), header='tau, edmax')
         np.savetxt(path_out + '/vr.dat', np.array(list(zip(self.time, vr))), header='tau <vr>')
         np.savetxt(path_out + '/entropy.dat', np.array(list(zip(self.time, entropy))), header='tau  entropy')
+        if self.energy and len(self.energy) == len(self.time):
+            # avg.dat of the save_to_hdf5 variant (bulkinfo_h5.py:300-306)
+            k0 = self.cfg.NZ // 2
+            edcent = [ez[k0] for ez in self.ez]
+            np.savetxt(path_out + '/avg.dat',
+                       np.array(list(zip(self.time, ecc2, edcent, self.entropy, self.energy, vr))),
+                       header='tau, eccp, ed(0,0,0), stotal, Etotal, <vr>')

@@ -1073,7 +1073,7 @@ __global__ void k_energy_final(const double *__restrict__ part, int nb, double d
 // 0.999999 as the reference's NumPy code does; P, s from the FP32 table, sums in double.
 // Block b covers rows (i, j) [b R, (b + 1) R); thread k of the block walks them at its k, so
 // a warp reads consecutive cells.  part[b][k][q]; k_slice_final adds the blocks in order.
-#define CV_SLICE_Q 6
+#define CV_SLICE_Q 10
 __global__ __launch_bounds__(128) void k_slice_sums(PhysP p, EosP E, const float *__restrict__ ev,
                                                     int64_t rows_per_block, double *part) {
     Eos eos(E, E.tab);
@@ -1081,7 +1081,7 @@ __global__ __launch_bounds__(128) void k_slice_sums(PhysP p, EosP E, const float
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
     const int64_t r1 = r0 + rows_per_block < nrow ? r0 + rows_per_block : nrow;
     for (int k = threadIdx.x; k < p.nz; k += blockDim.x) {
-        double s[CV_SLICE_Q] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double s[CV_SLICE_Q] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int64_t r = r0; r < r1; r++) {
             const int64_t I = r * p.nz + k;
             const float ef = ev[I] < 1.0e-10f ? 1.0e-10f : ev[I];
@@ -1098,6 +1098,19 @@ __global__ __launch_bounds__(128) void k_slice_sums(PhysP p, EosP E, const float
             s[3] += u0 * (double)eos.S(ef);
             s[4] += u0 * e * sqrt(vx * vx + vy * vy);
             s[5] += u0 * e;
+            // total_energy_and_entropy (kernel_bulkinfo.cl:18-46): unfloored e, gamma with
+            // 1 - v^2 floored at 1e-7 (helper.h:46-48); the lab-frame energy of the slice is
+            // cosh(eta) (s[6] - s[8]) + sinh(eta) s[7]
+            const float er = ev[I];
+            const double prr = (double)eos.P(er);
+            double g2 = 1.0 - (vx * vx + vy * vy + vz * vz);
+            g2 = g2 > 1.0e-7 ? g2 : 1.0e-7;
+            const double ug = 1.0 / sqrt(g2);
+            const double wr = ((double)er + prr) * ug * ug;
+            s[6] += wr;
+            s[7] += wr * vz;
+            s[8] += prr;
+            s[9] += ug * (double)eos.S(er);
         }
 #pragma unroll
         for (int q = 0; q < CV_SLICE_Q; q++) part[((int64_t)blockIdx.x * p.nz + k) * CV_SLICE_Q + q] = s[q];

@@ -171,10 +171,15 @@ int cv_max_ed(cv_handle *h, int which, float *out);
 int cv_energy_sum(cv_handle *h, int which, double tau, double *out);
 
 /* BulkInfo observables of slot `which` per eta slice, replacing the whole-field copy of
- * BulkInfo.ecc_vs_rapidity (pyvisc/bulkinfo.py:247-260) and the plane sums of eccp /
- * mean_vr / total_entropy (bulkinfo.py:203-245): out[k * 6 + q] = sum over the (x, y) plane
- * at eta index k of q = 0: Txx - Tyy, 1: Txx + Tyy, 2: T^tx, 3: u0 s, 4: u0 e |v_T|,
- * 5: u0 e (e floored at 1e-10, v^2 > 1 -> 0.999999 as there).  Double sums, fixed order. */
+ * BulkInfo.ecc_vs_rapidity (pyvisc/bulkinfo.py:247-260), the plane sums of eccp /
+ * mean_vr / total_entropy (bulkinfo.py:203-245) and the total_energy_and_entropy and
+ * eccp_vs_rapidity kernels (kernel_bulkinfo.cl:18-113, bulkinfo_h5.py:264-297):
+ * out[k * 10 + q] = sum over the (x, y) plane at eta index k of q = 0: Txx - Tyy,
+ * 1: Txx + Tyy, 2: T^tx, 3: u0 s, 4: u0 e |v_T|, 5: u0 e (e floored at 1e-10, v^2 > 1 ->
+ * 0.999999 as bulkinfo.py does), and, with kernel_bulkinfo.cl's unfloored e and gamma
+ * (1 - v^2 floored at 1e-7): 6: (e + P) u0^2, 7: (e + P) u0^2 v^eta, 8: P, 9: u0 s; the
+ * slice's lab-frame energy density sum is cosh(eta_k) (q6 - q8) + sinh(eta_k) q7.  Double
+ * sums, fixed order. */
 int cv_slice_sums(cv_handle *h, int which, double *out);
 
 /* ---- initial conditions on the device ------------------------------------------------

@@ -74,3 +74,51 @@ def test_get_records_slices_and_ecc_vs_rapidity():
     assert e1.shape == (nz,) and np.ptp(e2) < 1e-12 and np.ptp(e1) < 1e-12
     ref = bi.eccp(ev[:, :, 0, 0], ev[:, :, 0, 1], ev[:, :, 0, 2], ev[:, :, 0, 3])
     assert e2[0] == ref[1] and e1[0] == ref[0]
+
+
+def test_total_energy_and_entropy_closed_forms():
+    """The lab-frame energy of total_energy_and_entropy (kernel_bulkinfo.cl:18-46) is
+    T^{t mu} dSigma_mu on the tau = const surface, dSigma_mu = tau (cosh eta, 0, 0,
+    -sinh eta) dx dy deta: checked against that contraction with T^{t mu} built from the
+    Cartesian four-velocity u^t = u0 (cosh eta + v sinh eta), u^z = u0 (sinh eta + v cosh eta)
+    on a random field; at rest and uniform it is e sum(cosh eta_k), and the entropy u0 s."""
+    eos = Eos('ideal_gas')
+    nx, ny, nz = 5, 4, 9
+    cfg = _cfg(nx, ny, nz)
+    cfg.DX, cfg.DY, cfg.DZ = 0.2, 0.3, 0.25
+    bi = BulkInfo(cfg, eos)
+    tau = 1.7
+    vol = tau * 0.2 * 0.3 * 0.25
+    eta = (np.arange(nz) - nz // 2) * 0.25
+    # at rest, uniform e
+    ev = np.zeros((nx, ny, nz, 4))
+    ev[..., 0] = 2.5
+    E, S = bi.total_energy_and_entropy(tau, ev)
+    assert E == pytest.approx(2.5 * nx * ny * np.cosh(eta).sum() * vol, rel=1e-12)
+    assert S == pytest.approx(float(eos.f_S(2.5)) * nx * ny * nz * vol, rel=1e-12)
+    # random field: the contraction
+    rng = np.random.default_rng(5)
+    ev[..., 0] = rng.uniform(0.5, 5.0, (nx, ny, nz))
+    v = rng.uniform(-0.5, 0.5, (nx, ny, nz, 3))
+    ev[..., 1:] = v
+    E, S = bi.total_energy_and_entropy(tau, ev)
+    e, pr = ev[..., 0], eos.f_P(ev[..., 0])
+    u0 = 1.0 / np.sqrt(1.0 - (v * v).sum(-1))
+    ch, sh = np.cosh(eta)[None, None, :], np.sinh(eta)[None, None, :]
+    ut = u0 * (ch + v[..., 2] * sh)
+    uz = u0 * (sh + v[..., 2] * ch)
+    Ttt = (e + pr) * ut * ut - pr
+    Ttz = (e + pr) * ut * uz
+    ref = (tau * (ch * Ttt - sh * Ttz)).sum() * 0.2 * 0.3 * 0.25
+    assert E == pytest.approx(ref, rel=1e-12)
+    assert S == pytest.approx((u0 * eos.f_S(e)).sum() * vol, rel=1e-12)
+    # the same from per-slice sums (the device reduction's ten columns)
+    s = np.zeros((nz, 10))
+    w = (e + pr) * u0 * u0
+    s[:, 6] = w.sum((0, 1))
+    s[:, 7] = (w * v[..., 2]).sum((0, 1))
+    s[:, 8] = pr.sum((0, 1))
+    s[:, 9] = (u0 * eos.f_S(e)).sum((0, 1))
+    E2, S2 = bi._energy_entropy(tau, s)
+    assert E2 == pytest.approx(ref, rel=1e-12)
+    assert S2 == pytest.approx(S, rel=1e-12)

@@ -38,6 +38,7 @@ def test_device_bulkinfo_matches_host_path():
     h.visc_steps(0, 30)
     cfg = read_config()
     cfg.NX, cfg.NY, cfg.NZ = nx, ny, nz
+    cfg.DX, cfg.DY, cfg.DZ = 0.08, 0.08, 0.15
     d_ev1 = DeviceField(h, 'ev1')
     bd = BulkInfo(cfg, eos, device_reductions=True)
     bh = BulkInfo(cfg, eos, device_reductions=False)
@@ -70,6 +71,37 @@ def test_device_bulkinfo_matches_host_path():
                (u0 * e * np.sqrt(vx * vx + vy * vy)).sum(), (u0 * e).sum()]
         for q in range(6):
             assert abs(s[k, q] - ref[q]) <= 1e-10 * max(abs(ref[q]), 1e-12) + 1e-12 * abs(ref[1]), (k, q)
+        # kernel_bulkinfo.cl's columns: unfloored e, gamma with 1 - v^2 floored at 1e-7
+        er = ev[:, :, k, 0].reshape(-1)
+        fr = O.eos_funcs(op, er).astype(np.float64)
+        ug = 1.0 / np.sqrt(np.maximum(1.0 - (vx * vx + vy * vy + vz * vz), 1e-7))
+        wr = (er + fr[:, 0]) * ug * ug
+        ref2 = [wr.sum(), (wr * vz).sum(), fr[:, 0].sum(), (ug * fr[:, 1]).sum()]
+        for q in range(4):
+            assert abs(s[k, 6 + q] - ref2[q]) <= 1e-10 * abs(ref2[q]) + 1e-12 * abs(ref2[0]), (k, 6 + q)
+    # total energy / entropy and eccp vs rapidity through the reference's method names, against
+    # the per-cell formula in float64 on the same FP32 table values
+    tau = 1.2
+    E, S = bd.get_total_energy_and_entropy_on_gpu(tau, d_ev1)
+    e = ev[..., 0].reshape(-1)
+    f = O.eos_funcs(op, e).astype(np.float64)
+    vx, vy, vz = (ev[..., c].reshape(-1).astype(np.float64) for c in (1, 2, 3))
+    ug = 1.0 / np.sqrt(np.maximum(1.0 - (vx * vx + vy * vy + vz * vz), 1e-7))
+    eta = np.tile((np.arange(nz) - nz // 2) * float(cfg.DZ), nx * ny)
+    elab = (e + f[:, 0]) * ug * ug * (np.cosh(eta) + vz * np.sinh(eta)) - f[:, 0] * np.cosh(eta)
+    vol = tau * cfg.DX * cfg.DY * cfg.DZ
+    _say('  E_lab %.8g (numpy %.8g), S %.8g (numpy %.8g)'
+         % (E, elab.sum() * vol, S, (ug * f[:, 1]).sum() * vol))
+    assert E == pytest.approx(elab.sum() * vol, rel=1e-10)
+    assert S == pytest.approx((ug * f[:, 1]).sum() * vol, rel=1e-10)
+    assert bd.energy == [E] and bd.entropy == [S]
+    # the host form (Python EOS interpolation instead of the FP32 table): close, not equal
+    Eh, Sh = bh.total_energy_and_entropy(tau, ev)
+    _say('  host form: E %.8g S %.8g' % (Eh, Sh))
+    assert Eh == pytest.approx(E, rel=1e-3) and Sh == pytest.approx(S, rel=1e-3)
+    e1, e2 = bd.ecc_vs_rapidity_on_gpu(tau, d_ev1)
+    np.testing.assert_array_equal(e1, bd.ecc1_vs_rapidity[0])
+    np.testing.assert_array_equal(e2, bd.ecc2_vs_rapidity[0])
     h.close()
 
 
