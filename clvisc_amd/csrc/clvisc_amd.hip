@@ -134,20 +134,33 @@ static int set_tile(cv_handle *h, int ty, int tz) {
     return CV_OK;
 }
 
-// default tile: least padding over (ny, nz), then most threads
+// default tile: least modelled cost per cell update.  Per plane a workgroup runs the halo
+// pass (HY x HZ cells), the y and z face passes and the interior pass (x faces, update_pimn,
+// root solve) over its 256 threads; each pass costs its VALU work per wave times the
+// wavefronts it occupies (work), and the passes a single wavefront repeats while the rest
+// of the workgroup waits at the barrier add to the critical path (crit).  Weights are the
+// passes' instruction counts per wavefront (halo cell 280, face 640, interior cell 3000).
+// Padding of (ny, nz) to whole tiles and a short z run (fewer cells per cache line on the
+// x-neighbour loads) scale the result.
 static int default_tile(cv_handle *h) {
     const cv_params &p = h->prm;
     if (p.nz == 1) return set_tile(h, p.ny >= 256 ? 256 : p.ny, 1);
+    auto waves = [](int n, int w) { return (double)((n + w - 1) / w); };
     int best_ty = 0, best_tz = 0;
     double best = 1e30;
     for (int tz = 4; tz <= 64; tz++) {
         for (int ty = 1; ty * tz <= CV_TILE_THREADS; ty++) {
-            int hy = p.ny > 1 ? 2 : 0;
-            if ((ty + 2 * hy) * (tz + 4) > CV_TILE_MAXC) continue;
-            if ((hy > 0 && (ty + 1) * tz > CV_TILE_MAXF) || ty * (tz + 1) > CV_TILE_MAXF) continue;
-            if (ty * tz < 192) continue;
-            double pad = (double)((p.ny + ty - 1) / ty * ty) * ((p.nz + tz - 1) / tz * tz) / ((double)p.ny * p.nz);
-            double score = pad * (256.0 / (ty * tz)) * (1.0 + 0.02 * ((double)(ty + 4) * (tz + 4) / (ty * tz)));
+            const int hy = p.ny > 1 ? 2 : 0;
+            const int hc = (ty + 2 * hy) * (tz + 4), yf = hy > 0 ? (ty + 1) * tz : 0, zf = ty * (tz + 1);
+            if (hc > CV_TILE_MAXC || yf > CV_TILE_MAXF || zf > CV_TILE_MAXF) continue;
+            if (ty * tz < 160) continue;
+            const double pad =
+                (double)((p.ny + ty - 1) / ty * ty) * ((p.nz + tz - 1) / tz * tz) / ((double)p.ny * p.nz);
+            const double work = 280.0 * waves(hc, 64) + 640.0 * (waves(yf, 64) + waves(zf, 64)) +
+                                3000.0 * waves(ty * tz, 64);
+            const double crit = 280.0 * waves(hc, CV_TILE_THREADS) +
+                                640.0 * (waves(yf, CV_TILE_THREADS) + waves(zf, CV_TILE_THREADS)) + 3000.0;
+            const double score = pad * (work + crit) / (ty * tz) * (1.0 + 0.5 / tz);
             if (score < best) { best = score; best_ty = ty; best_tz = tz; }
         }
     }

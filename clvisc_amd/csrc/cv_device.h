@@ -92,16 +92,14 @@ __device__ __forceinline__ float minmod(float x, float y) {
 }
 // generalised minmod slope at b from cells (a, b, c):
 //   minmod(0.5 (c-a), minmod(THETA (c-b), THETA (b-a)))   (helper.h:213-217)
-// in closed form: 0 when c-b and b-a carry different sign bits, else
-// copysign(min(0.5|c-a|, THETA|c-b|, THETA|b-a|), c-b).  For finite inputs this is the
-// reference's value bit for bit except that a zero may carry the sign of c-b (a zero
-// difference makes the min 0); products with 0.5 and signs are exact.  The operands of
-// the min are >= +0, so IEEE minNum (v_min) equals the reference's min().
+// For finite x, y, minmod(x, y) = min(|x|, |y|) (sgn x + sgn y) / 2 is the median of
+// (x, y, 0): the operand of smaller magnitude when the signs agree, else zero.  The
+// median is one of its operands, so med3 returns the reference's value bit for bit,
+// except that a zero result may carry the other sign; c-a has the sign of c-b when c-b
+// and b-a agree (the difference of two distinct floats is never zero).
 __device__ __forceinline__ float slope(float a, float b, float c) {
-    float d2 = c - b, d3 = b - a;
-    float m = fminf(0.5f * fabsf(c - a), fminf(CV_THETA * fabsf(d2), CV_THETA * fabsf(d3)));
-    uint32_t sx = __float_as_uint(d2) ^ __float_as_uint(d3);
-    return (sx & 0x80000000u) ? 0.0f : copysignf(m, d2);
+    const float inner = __builtin_amdgcn_fmed3f(CV_THETA * (c - b), CV_THETA * (b - a), 0.0f);
+    return __builtin_amdgcn_fmed3f(0.5f * (c - a), inner, 0.0f);
 }
 __device__ __forceinline__ float gamma3(float vx, float vy, float vz) {
     return rcp(sqrtf(rmax(1.0f - vx * vx - vy * vy - vz * vz, CV_ACU)));
