@@ -404,6 +404,16 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
             }
         }
         __syncthreads();
+        // STEP 2: the update's reads of d_ev[1], d_pi[1] and Pi[1] are issued here, ahead of
+        // the z differences, so that their latency overlaps that work
+        float4 e1_pre = ec;
+        float pi1_pre[10], bk1_pre = 0.0f;
+        if (STEP == 2 && active) {
+            e1_pre = ld4p(ev1, bo, nb);
+#pragma unroll
+            for (int m = 0; m < 10; m++) pi1_pre[m] = ldp(pi1, bo, nb, m);
+            if (BULK) bk1_pre = ldo(bk1, bo);
+        }
         if (active) {
             if (tp.hz > 0) {
                 const int fp_ = my * (tz + 1) + mz + 1, fm_ = my * (tz + 1) + mz;
@@ -433,7 +443,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
 
         // ---- 5. update_pimn (kernel_IS.cl:501-709) and update_ev (kernel_visc.cl:322-393) ----
         if (active) {
-            const float4 e1 = STEP == 1 ? ec : ld4p(ev1, bo, nb);
+            const float4 e1 = STEP == 1 ? ec : e1_pre;
             const float4 e2 = STEP == 2 ? ec : ld4p(ev2, bo, nb);
             float g1 = STEP == 1 ? s_c[CF_U0][cc] : gamma3(e1.y, e1.z, e1.w);
             float u_old[4] = {g1 * 1.0f, g1 * e1.y, g1 * e1.z, g1 * e1.w};
@@ -491,7 +501,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
                     int mn = IDX(mu, nu);
                     if (u[0] > 10000.0f) sigma = 0.0f;
                     float piNS = etav * sigma;
-                    float pi_old = (STEP == 1 ? q[mn] : ldp(pi1, bo, nb, mn)) * u_old[0];
+                    float pi_old = (STEP == 1 ? q[mn] : pi1_pre[mn]) * u_old[0];
                     float src = (-q[mn] * theta) * p.r_3 - (q[mn] * u[0]) * st.r_tau;
 #pragma unroll
                     for (int l = 0; l < 4; l++)
@@ -513,7 +523,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
                 float b2 = s_c[CF_BK][cc];
                 float bsrc_ = (b2 * theta) * p.r_3 - (b2 * u[0]) * st.r_tau;
                 bs += bsrc_;
-                b_start = STEP == 1 ? b2 : ldo(bk1, bo);
+                b_start = STEP == 1 ? b2 : bk1_pre;
                 float b_old = b_start * u_old[0];
                 bout = (b_old + (bs * p.dt) * st.r_step - p.dt * oot_b * zetav * theta) * rcp(u_new[0] + p.dt * oot_b);
                 bout = bulk_regulate(bout, eos.P(e2.x), e2.x);
@@ -548,7 +558,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 float g0 = c == 0 ? 1.0f : 0.0f;
-                float pim0 = STEP == 1 ? q[c] : ldp(pi1, bo, nb, c);
+                float pim0 = STEP == 1 ? q[c] : pi1_pre[c];
                 T0m[c] = ((e1.x + pressure) * g1 * u_old[c] - pressure * g0 + pim0) * st.old_time;
             }
 #pragma unroll
