@@ -404,10 +404,14 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
             }
         }
         __syncthreads();
-        // STEP 2: the update's reads of d_ev[1], d_pi[1] and Pi[1] are issued here, ahead of
-        // the z differences, so that their latency overlaps that work
+        // the update's reads (STEP 1: udiff; STEP 2: d_ev[1], d_pi[1] and Pi[1]) are issued
+        // here, ahead of the z differences, so that their latency overlaps that work
         float4 e1_pre = ec;
-        float pi1_pre[10], bk1_pre = 0.0f;
+        float pi1_pre[10], bk1_pre = 0.0f, ud_pre[4];
+        if (STEP == 1 && active) {
+#pragma unroll
+            for (int c = 0; c < 4; c++) ud_pre[c] = ldp(udiff_in, bo, nb, c);
+        }
         if (STEP == 2 && active) {
             e1_pre = ld4p(ev1, bo, nb);
 #pragma unroll
@@ -451,7 +455,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
             if (STEP == 1) {
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
-                    float ud_ = ldp(udiff_in, bo, nb, c);
+                    float ud_ = ud_pre[c];
                     udt[c] = ud_ * p.r_dt;
                     u_new[c] = u_old[c] + ud_;
                 }
