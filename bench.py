@@ -87,7 +87,7 @@ def make_workload(bulk=True, ic=True):
 
 def cpu_baseline(p, tab, ev, nthreads):
     """FP32 oracle on the host cores: the central 29 eta slices of the same initial
-    condition, 2 RK2 steps, reference-ordered unfused kernels."""
+    condition, 12 RK2 steps (about 10 s on 16 cores), reference-ordered unfused kernels."""
     from oracle import oracle as O
     from clvisc_amd import _lib
     nz = 29
@@ -98,7 +98,7 @@ def cpu_baseline(p, tab, ev, nthreads):
                        bulk=bulk, nthreads=nthreads)
     o = O.Oracle(op, 'f32', viscous=True)
     o.load_ini(sub)
-    steps = 2
+    steps = 12
     t0 = time.perf_counter()
     o.visc_steps(steps)
     dt = time.perf_counter() - t0
@@ -292,6 +292,7 @@ def run_learned(args, p, tab, ev, h_plain, loop_end):
     return {'value': value, 'unit': 'cell-updates/s', 'ms_per_step': 1000.0 * wall / nrun, 'steps': nrun,
             'max_rel_e_vs_plain': rel, 'matched': bool(rel <= MATCHED_REL), 'matched_tolerance': MATCHED_REL,
             'plain_one_ulp_ic_max_rel_e': rel_ulp,
+            'within_one_ulp_sensitivity': bool(rel <= rel_ulp),
             'iterations_plain': float(it_p.mean()), 'iterations_learned': float(it_l.mean()),
             'split_unseeded_value': split_value, 'train_seconds': t_train,
             'loss_first': float(hist[:20].mean()), 'loss_last': float(hist[-20:].mean()),
@@ -499,6 +500,18 @@ def run_surrogate(args, name):
     h_seed.close()
 
 
+def run_counter_pass(args, h):
+    """--warmup + 1 steps of the single workload, then one device-to-device copy of a
+    pi-sized field (NX NY NZ x 10 floats = 681.8 MB read and written, cv_copy_field): the
+    kernel of known size that scales FETCH_SIZE / WRITE_SIZE to bytes."""
+    h.visc_steps(0, args.warmup + 1)
+    h.sync()
+    h.copy('pi2', 'pi1')
+    h.sync()
+    log('counter pass: %d steps, then a %.1f MB field copy' % (args.warmup + 1, 40.0 * NX * NY * NZ / 1e6))
+    h.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -518,6 +531,9 @@ def main():
                          'surrogate: learned root-solve seed on the ideal Gubser workload')
     ap.add_argument('--autotune', action='store_true', help='ensemble: tune tiles and chunks per grid shape')
     ap.add_argument('--autotune-cache', default='', help='ensemble: JSON file of tuned shapes')
+    ap.add_argument('--counter-pass', action='store_true',
+                    help='single: only --warmup + 1 steps, then one 682 MB device copy (the byte '
+                         'counters\' calibration kernel); for rocprofv3 --pmc passes')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
         print('bench.py measures one GPU only: the hydro step of one event runs on one MI355X '
@@ -550,6 +566,9 @@ def main():
     h.visc_initialize(TAU0)
     h.ideal_stage(1, np.float32(TAU0))
     h.set('ev1', ev)
+    if args.counter_pass:
+        run_counter_pass(args, h)
+        return
     loop = 0
     h.visc_steps(loop, args.warmup)
     loop += args.warmup
