@@ -54,12 +54,9 @@ __device__ __forceinline__ int64_t xcd_work(int64_t b, int64_t total, int nx) {
     return x * per + (x < rem ? x : rem) + i;
 }
 
-// per-cell quantities of one cell from its raw fields
-template <class EOS>
-__device__ __forceinline__ void cell_prims(const PhysP &p, const EOS &eos, float tau, float4 e, const float pi[10],
-                                           float bk, float out[CF_N]) {
-    float u0 = gamma3(e.y, e.z, e.w);
-    float pr = eos.P(e.x) + bk;
+// per-cell quantities of one cell from its raw fields, given its u^tau and P + Pi
+__device__ __forceinline__ void cell_prims_up(float tau, float4 e, const float pi[10], float bk, float u0, float pr,
+                                              float out[CF_N]) {
     float4 T = t0m_u(e, pr, u0);
     out[CF_E] = e.x; out[CF_VX] = e.y; out[CF_VY] = e.z; out[CF_VZ] = e.w;
     out[CF_U0] = u0; out[CF_P] = pr;
@@ -70,6 +67,13 @@ __device__ __forceinline__ void cell_prims(const PhysP &p, const EOS &eos, float
 #pragma unroll
     for (int m = 0; m < 10; m++) out[CF_PI0 + m] = pi[m];
     out[CF_BK] = bk;
+}
+
+// per-cell quantities of one cell from its raw fields
+template <class EOS>
+__device__ __forceinline__ void cell_prims(const PhysP &p, const EOS &eos, float tau, float4 e, const float pi[10],
+                                           float bk, float out[CF_N]) {
+    cell_prims_up(tau, e, pi, bk, gamma3(e.y, e.z, e.w), eos.P(e.x) + bk, out);
 }
 
 // Face value between cells L and R along axis A, from the cells Lm = L-1, L, R, Rp = R+1.
@@ -185,8 +189,9 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
         const int ibeg = MARCH ? xr * tp.xc : xr;
         const int iend = MARCH ? (ibeg + tp.xc < p.nx ? ibeg + tp.xc : p.nx) : xr + 1;
         const int j0 = jt * ty, k0 = kt * tz;
-        // state carried along x by MARCH: F(i-1/2), slopes of cell i, u^mu of cell i-1
-        float xFm[15], xDAL[15], xum1[4];
+        // state carried along x by MARCH: F(i-1/2), slopes of cell i, u^mu of cell i-1, and
+        // u^tau, P + Pi of cell i+1 (evaluated as cell i+2 of the previous plane)
+        float xFm[15], xDAL[15], xum1[4], xu0n = 0.0f, xprn = 0.0f;
         if (MARCH && lane_in_tile && j0 + my < p.ny && k0 + mz < p.nz) {
             const uint32_t I0 = (uint32_t)((int64_t)ibeg * p.sx) + (uint32_t)(j0 + my) * (uint32_t)p.sy +
                                 (uint32_t)(k0 + mz);
@@ -207,6 +212,8 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
             face_flux<0, BULK, false, true>(p, eos, tau, gx, 0, 1, 2, 3, xFm, nullptr, xDAL);
 #pragma unroll
             for (int c = 0; c < 4; c++) xum1[c] = c == 0 ? xc[1][CF_U0] * 1.0f : xc[1][CF_U0] * xc[1][CF_E + c];
+            xu0n = xc[3][CF_U0];
+            xprn = xc[3][CF_P];
         }
       for (int i = ibeg; i < iend; i++) {
         const uint32_t plane = (uint32_t)((int64_t)i * p.sx);
@@ -296,7 +303,15 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
 #pragma unroll
                     for (int m = 0; m < 10; m++) pv[m] = ldp(pi_st, b2, nb, m);
                     float bkv = BULK ? ldo(bk_st, b2) : 0.0f;
-                    cell_prims(p, eos, tau, e, pv, bkv, w == 1 ? xr1 : xr2);
+                    if (w == 1) {
+                        // cell i+1 (min(i+1, nx-1)) is the previous plane's cell i+2
+                        // (min(i+2, nx-1) there), with the same stage inputs
+                        cell_prims_up(tau, e, pv, bkv, xu0n, xprn, xr1);
+                    } else {
+                        cell_prims(p, eos, tau, e, pv, bkv, xr2);
+                        xu0n = xr2[CF_U0];
+                        xprn = xr2[CF_P];
+                    }
                 }
                 auto gx = [&](int f, int c) { return c == 1 ? s_c[f][cc] : (c == 2 ? xr1[f] : xr2[f]); };
                 float Fp[15], dar[15];
