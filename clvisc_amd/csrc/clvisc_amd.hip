@@ -144,7 +144,9 @@ static int set_tile(cv_handle *h, int ty, int tz) {
 // x-neighbour loads) scale the result.
 static int default_tile(cv_handle *h) {
     const cv_params &p = h->prm;
-    if (p.nz == 1) return set_tile(h, p.ny >= 256 ? 256 : p.ny, 1);
+    // one row of the (y, eta) plane per tile: at most 252 cells, so that the halo pass
+    // (ty + 4 cells) and the y-face pass (ty + 1 faces) fit one pass of 256 threads
+    if (p.nz == 1) return set_tile(h, p.ny >= 252 ? 252 : p.ny, 1);
     auto waves = [](int n, int w) { return (double)((n + w - 1) / w); };
     int best_ty = 0, best_tz = 0;
     double best = 1e30;
