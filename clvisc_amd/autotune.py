@@ -33,8 +33,9 @@ def _valid(ny, nz, ty, tz):
     return True
 
 
-def candidates(nx, ny, nz):
-    """Tile shapes and chunk lengths worth timing for a grid."""
+def candidates(nx, ny, nz, default=None):
+    """Tile shapes and chunk lengths worth timing for a grid; ``default`` (ty, tz), the
+    library's own choice for the grid (its cost model of the wave passes), comes first."""
     tiles = []
     if nz == 1:
         for nt in range(1, 5):
@@ -49,6 +50,8 @@ def candidates(nx, ny, nz):
                     pad = (-(-ny // ty) * ty) * (-(-nz // tz) * tz) / float(ny * nz)
                     tiles.append((pad * TILE_THREADS / (ty * tz), ty, tz))
         tiles = [(ty, tz) for _, ty, tz in sorted(tiles)[:6]]
+    if default:
+        tiles = [tuple(int(x) for x in default)] + tiles
     tiles = [t for t in dict.fromkeys(tiles) if _valid(ny, nz, *t)]
     chunks = sorted(set(c for c in (2, 4, 8, 12, 16, 24, 32) if c <= nx) | {0})
     return [(ty, tz, xc) for (ty, tz) in tiles for xc in chunks]
@@ -89,7 +92,8 @@ class Autotuner(object):
                 h.ideal_stage(1, np.float32(self.params.tau0))
                 h.set('ev1', ev)
             best = None
-            for ty, tz, xc in candidates(nx, ny, nz):
+            _, dty, dtz = batch.handle(0).stage_kernel()
+            for ty, tz, xc in candidates(nx, ny, nz, default=(dty, dtz)):
                 try:
                     for e in range(copies):
                         h = batch.handle(e)

@@ -1317,6 +1317,18 @@ static int batch_build(cv_batch *b) {
     std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
         return b->ev[a]->tile.nwork * b->ev[a]->tile.xc > b->ev[c]->tile.nwork * b->ev[c]->tile.xc;
     });
+    // march chunk of the events whose chunk was not set (cv_set_march_chunk): one length for
+    // the batch, so that the whole list holds about 32 work items per CU.  A lone event's
+    // default fills the GPU by itself (1-plane chunks for a boost-invariant grid of two tile
+    // columns); in a batch the other events bring the parallelism and longer chunks save the
+    // chunk starts.  At most 16 planes, so that the list keeps enough items to balance.
+    int64_t colplanes = 0;
+    for (int e : order) {
+        const cv_handle *h = b->ev[e];
+        colplanes += (int64_t)h->tile.ntj * h->tile.ntk * h->prm.nx;
+    }
+    const int64_t target = (int64_t)(b->ev[0]->cus > 0 ? b->ev[0]->cus : 256) * 32;
+    const int xcb = (int)std::max<int64_t>(1, std::min<int64_t>(16, colplanes / std::max<int64_t>(1, target)));
     std::vector<BatchEvent> tab(order.size());
     int64_t w = 0;
     for (size_t q = 0; q < order.size(); q++) {
@@ -1324,11 +1336,16 @@ static int batch_build(cv_batch *b) {
         BatchEvent &t = tab[q];
         t.p = h->phys;
         t.tp = h->tile;
+        if (h->march_xc == 0) {
+            t.tp.xc = std::min(xcb, h->prm.nx);
+            t.tp.nchunks = (h->prm.nx + t.tp.xc - 1) / t.tp.xc;
+            t.tp.nwork = (int64_t)t.tp.ntj * t.tp.ntk * t.tp.nchunks;
+        }
         t.work0 = w;
         for (int k = 0; k < 3; k++) { t.ev[k] = h->ev[k]; t.pi[k] = h->pi[k]; t.bk[k] = h->bk[k]; }
         t.Src = h->Src; t.IS_src = h->IS_src; t.bsrc = h->bsrc; t.udiff = h->udiff; t.good = h->good;
         t.fix = h->d_fix; t.fix_ctr = h->d_fix_ctr;
-        w += h->tile.nwork;
+        w += t.tp.nwork;
     }
     if (w >= ((int64_t)1 << 31)) return fail(CV_ERR_CAPACITY, "batch: %lld work items exceed 2^31", (long long)w);
     BatchState st;
@@ -1458,10 +1475,11 @@ extern "C" int cv_batch_info(cv_batch *b, int32_t *nevents, int64_t *loop, int64
     if (nevents) *nevents = (int32_t)b->ev.size();
     if (loop) *loop = b->loop;
     if (nwork) {
+        // the list of the last build; before the first build, the events' own chunks
         int64_t w = 0;
         for (size_t e = 0; e < b->ev.size(); e++)
             if (b->active[e]) w += b->ev[e]->tile.nwork;
-        *nwork = w;
+        *nwork = b->dirty ? w : b->nwork;
     }
     return CV_OK;
 }

@@ -331,7 +331,8 @@ int cv_batch_steps_async(cv_batch *b, int64_t nsteps);
 int cv_batch_set_loop(cv_batch *b, int64_t loop);
 int cv_batch_sync(cv_batch *b);
 int cv_batch_last_elapsed_ms(cv_batch *b, float *ms);
-/* events, loop index of the next step, work items (tile column x x-chunk) per stage */
+/* events, loop index of the next step, work items (tile column x x-chunk) per stage (of the
+ * last build of the event list; before the first step, of the events' own chunks) */
 int cv_batch_info(cv_batch *b, int32_t *nevents, int64_t *loop, int64_t *nwork);
 /* Work ranges of the persistent stage (no counterpart in the reference): the event list is
  * cut into nxcd contiguous ranges with a counter each and workgroup b starts on range
