@@ -13,6 +13,7 @@
 #include "../../include/clvisc_amd.h"
 #include "cv_kernels.h"
 #include "cv_tiled.h"
+#include "cv_ideal_tiled.h"
 #include "cv_hypersf.h"
 #include "cv_mlp.h"
 #include "cv_ic.h"
@@ -440,6 +441,20 @@ static int launch_ideal_seeded(cv_handle *h, const StageP &st, float *ev_st, flo
 static int launch_ideal(cv_handle *h, int step, float tau, float *ev_st, float *ev1, float *ev_out) {
     StageP st = stage_params(h, step, tau);
     if (h->seed_mlp) return launch_ideal_seeded(h, st, ev_st, ev1, ev_out);
+    // x-marching tiles with the marching stage kernel's tile and chunks; the test flags, the
+    // FP64 root and the iteration counts take the per-cell kernel
+    if (h->kernel == 2 && h->phys.nb && !(h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_FP64_ROOT)) &&
+        !h->track_iters) {
+        dim3 g((unsigned)h->tile.nwork), b(CV_ITILE_THREADS);
+        if (h->eos.kind == 1)
+            hipLaunchKernelGGL(k_ideal_tiled<1>, g, b, 0, h->stream, h->phys, h->eos, st, h->tile, ev_st, ev1,
+                               h->Src, ev_out);
+        else
+            hipLaunchKernelGGL(k_ideal_tiled<0>, g, b, 0, h->stream, h->phys, h->eos, st, h->tile, ev_st, ev1,
+                               h->Src, ev_out);
+        HIP_TRY(hipGetLastError());
+        return CV_OK;
+    }
     hipLaunchKernelGGL(k_ideal_stage, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, st,
                        ev_st, ev1, h->Src, ev_out, nullptr, nullptr, h->track_iters ? h->iters : nullptr);
     HIP_TRY(hipGetLastError());

@@ -242,9 +242,11 @@ int cv_profile_visc_step(cv_handle *h, int64_t loop0, float *ms, int nms, int32_
  * stencil quantity re-evaluated per cell; kind 1 = tiled kernel, one workgroup per
  * ty x tz tile of an x-plane, per-cell quantities and y/z faces evaluated once in LDS;
  * kind 2 (default) = the tiled kernel marching along x over chunks of planes, every x face
- * and slope evaluated once (ty, tz <= 0 picks the default tile for the grid).  All give
- * the same FP32 results.  No reference counterpart (the reference's launch geometry is
- * BSZ, pyvisc/hydro.info). */
+ * and slope evaluated once (ty, tz <= 0 picks the default tile for the grid).  The ideal
+ * step (cv_ideal_steps_async, cv_ideal_stage) runs on x-marching tiles of the same shape
+ * with kind 2 and per cell otherwise (also with the test flags, CV_FLAG_FP64_ROOT, a root
+ * seed or iteration counts).  All give the same FP32 results.  No reference counterpart
+ * (the reference's launch geometry is BSZ, pyvisc/hydro.info). */
 int cv_set_stage_kernel(cv_handle *h, int kind, int ty, int tz);
 int cv_get_stage_kernel(cv_handle *h, int32_t *kind, int32_t *ty, int32_t *tz);
 /* x planes per march chunk of kind 2 (0: automatic, about 32 work items per CU). */

@@ -71,6 +71,35 @@ def test_ideal_steps_gubser_201():
     assert abs(h.max_ed() - o.max_ed()) == 0.0
 
 
+@pytest.mark.parametrize('eos_kind,dims,kernel,chunk', [
+    ('crossover', (29, 31, 11), (2, 0, 0), None), ('crossover', (29, 31, 11), (2, 7, 32), 3),
+    ('crossover', (29, 31, 11), (2, 11, 23), 5), ('ideal_gas', (27, 25, 9), (2, 0, 0), None),
+    ('ideal_gas', (27, 25, 9), (0, 0, 0), None), ('crossover', (45, 41, 1), (2, 64, 1), 4)])
+def test_ideal_steps_tiled_3d(eos_kind, dims, kernel, chunk):
+    """The ideal stage on x-marching tiles (kind 2; kind 0 is the per-cell kernel) against
+    the oracle's ideal steps: identical fields."""
+    nx, ny, nz = dims
+    eos = Eos(eos_kind)
+    tab = eos.create_table()
+    p = cv_params(nx, ny, nz, 0.02, 0.08, 0.08, 0.15, 0.6)
+    pos = ic.woods_saxon_positions(12, 5, R=0.6, a=0.1)
+    ev = ic.hotspots(nx, ny, nz, 0.08, 0.08, 0.15, positions=pos, emax=30.0)
+    _say('case: ideal hotspots %s %dx%dx%d, 25 steps, kernel %s chunk %s' % (eos_kind, nx, ny, nz, kernel, chunk))
+    h = _lib.Handle(p, tab, viscous=False)
+    h.set_stage_kernel(*kernel)
+    if chunk is not None:
+        h.set_march_chunk(chunk)
+    h.set('ev1', ev)
+    h.ideal_steps(0, 25)
+    o = O.Oracle(orc_params(p, tab, nthreads=16), 'f32', viscous=False)
+    o.load_ini(ev)
+    o.ideal_steps(25)
+    for name, g, r in (('ev1', h.get('ev1'), o.ev[1]), ('ev2', h.get('ev2'), o.ev[2]), ('src', h.get('src'), o.Src)):
+        _say('  %-4s bit-equal %.8f' % (name, bit_equal_frac(g, r)))
+        np.testing.assert_array_equal(g, r, err_msg=name)
+    h.close()
+
+
 def _visc_case(nx, ny, nz, bulk=None, nhot=12, seed=3, eos_kind='crossover'):
     eos = Eos(eos_kind)
     tab = eos.create_table()
