@@ -500,6 +500,109 @@ def run_surrogate(args, name):
     h_seed.close()
 
 
+def run_aux(args, p, tab, name):
+    """The device kernels around the step, on the benchmark grid (385x385x115): initial
+    conditions (optical Glauber, smearing of 20000 partons, entropy -> e), the vorticity
+    tensors and omega^mu, the freeze-out surface with pi, Pi and the 32 vorticity tensors
+    interpolated, one BulkInfo.get, and the Lambda polarization (thermal) of the surface
+    at 1024 momenta.  Each call is timed by the wall clock with the device idle before
+    and after it (best of 3 after one warm-up)."""
+    from clvisc_amd import _lib
+    from clvisc_amd import ic as ic_mod
+    from clvisc_amd.config import read_config
+    from clvisc_amd.eos import Eos
+    from clvisc_amd.bulkinfo import BulkInfo
+    from clvisc_amd.ideal import DeviceField
+    from clvisc_amd.polarization import Polarization
+    eos = Eos('crossover')
+    ncell = NX * NY * NZ
+    p.nxskip, p.nyskip, p.nzskip = 3, 3, 1          # as the combined run
+    h = _lib.Handle(p, tab, viscous=True)
+
+    def timed(fn, reps=3):
+        fn()
+        h.sync()
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            out = fn()
+            h.sync()
+            dt = time.perf_counter() - t0
+            best = dt if best is None or dt < best else best
+        return 1000.0 * best, out
+
+    res = {}
+    cfg = read_config()
+    g = _lib.CvGlauber.from_cfg(cfg)
+    res['glauber_ini_ms'], _ = timed(lambda: h.glauber_ini(g, 1))
+    rng = np.random.default_rng(7)
+    npart = 20000
+    pt = rng.exponential(0.6, npart)
+    phi = rng.uniform(0, 2 * np.pi, npart)
+    y = rng.normal(0.0, 1.5, npart)
+    mt = np.sqrt(pt * pt + 0.3 ** 2)
+    x, yy = rng.normal(0, 2.5, npart), rng.normal(0, 2.5, npart)
+    t, z = TAU0 * np.cosh(y), TAU0 * np.sinh(y)
+    partons = np.column_stack([mt * np.cosh(y), pt * np.cos(phi), pt * np.sin(phi), mt * np.sinh(y), t, x, yy, z])
+    res['smear_20000_partons_ms'], _ = timed(lambda: h.smear_partons(partons))
+    res['sd_to_ed_ms'], _ = timed(lambda: h.sd_to_ed(eos.s, eos.ed, 1))
+    # the evolution these act on: the bench's initial field, one ntskip of steps
+    pos = ic_mod.woods_saxon_positions(400, 20240601)
+    ev = ic_mod.hotspots(NX, NY, NZ, DX, DY, DZ, positions=pos, emax=50.0)
+    h.visc_initialize(TAU0)
+    h.ideal_stage(1, np.float32(TAU0))
+    h.set('ev1', ev)
+    del ev
+    h.vorticity_enable()
+    h.hypersf_reserve(args.sf_capacity)
+    h.hypersf_snapshot(np.float32(TAU0))
+    nsteps = 20
+    h.visc_steps(0, nsteps)
+    tau = np.float32(TAU0 + nsteps * DT)
+    res['vorticity_ms'], _ = timed(lambda: h.vorticity(tau))
+    t0 = time.perf_counter()
+    h.hypersf_find(tau)
+    h.sync()
+    res['hypersf_find_ms'] = 1000.0 * (time.perf_counter() - t0)
+    nsf, _ = h.hypersf_count()
+    res['vorticity_mu_with_copy_ms'], _ = timed(lambda: h.vorticity_mu(p.efrz), reps=1)
+    c = read_config()
+    c.NX, c.NY, c.NZ, c.DX, c.DY, c.DZ = NX, NY, NZ, DX, DY, DZ
+    bi = BulkInfo(c, eos, device_reductions=True)
+    d_ev1 = DeviceField(h, 'ev1')
+    res['bulkinfo_get_ms'], _ = timed(lambda: bi.get(float(tau), d_ev1, 1.0))
+    sf, _, _, _ = h.hypersf_get()
+    om = h.hypersf_get_vorticity()
+    h.close()
+    nmom = 1024
+    pz = Polarization.from_arrays(sf, omega=np.ascontiguousarray(om[:, 0:6]), T=0.137, mass=1.116)
+    ptm = np.repeat(np.linspace(0.2, 3.0, 32), 32)
+    phm = np.tile(np.linspace(0, 2 * np.pi, 32, endpoint=False), 32)
+    mtm = np.sqrt(ptm ** 2 + 1.116 ** 2)
+    pz.set_momentum(np.column_stack([mtm, np.zeros(nmom), ptm * np.cos(phm), ptm * np.sin(phm)]))
+    pz.get_themal()
+    t0 = time.perf_counter()
+    pz.get_themal()
+    res['polarization_thermal_ms'] = 1000.0 * (time.perf_counter() - t0)
+    res['surface_elements'] = int(nsf)
+    res['momenta'] = nmom
+    for k_, v_ in res.items():
+        log('  %-28s %s' % (k_, ('%.3f' % v_) if isinstance(v_, float) else v_))
+    out = {
+        'metric': 'ms per call of the device kernels around the viscous step (385x385x115)',
+        'value': res['vorticity_ms'], 'unit': 'ms', 'n_gpus': 1, 'steps': nsteps, 'warmup': 1,
+        'ms_per_step': None, 'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': 'initial conditions, vorticity, freeze-out surface, BulkInfo and Lambda '
+                               'polarization on the 385x385x115 grid after %d viscous + bulk steps' % nsteps,
+                   'cells': ncell, 'device': name, 'timings': res,
+                   'cells_per_s': {k_: ncell / (v_ * 1e-3) for k_, v_ in res.items()
+                                   if k_ in ('glauber_ini_ms', 'sd_to_ed_ms', 'vorticity_ms')},
+                   'element_momenta_per_s': nsf * nmom / (res['polarization_thermal_ms'] * 1e-3)},
+    }
+    print(json.dumps(out))
+
+
 def run_counter_pass(args, h):
     """--warmup + 1 steps of the single workload, then one device-to-device copy of a
     pi-sized field (NX NY NZ x 10 floats = 681.8 MB read and written, cv_copy_field): the
@@ -526,7 +629,7 @@ def main():
     ap.add_argument('--max-loops', type=int, default=3000, help='combined: step limit')
     ap.add_argument('--sf-capacity', type=int, default=6_000_000, help='combined: surface elements per event')
     ap.add_argument('--sf-out', default='', help='combined: write hypersf.dat / sf_txyz.dat per event here')
-    ap.add_argument('--workload', choices=['single', 'ensemble', 'surrogate', 'combined'], default='single',
+    ap.add_argument('--workload', choices=['single', 'ensemble', 'surrogate', 'combined', 'aux'], default='single',
                     help='single: one 385x385x115 event (the headline); ensemble: 64 uneven events batched; '
                          'surrogate: learned root-solve seed on the ideal Gubser workload')
     ap.add_argument('--autotune', action='store_true', help='ensemble: tune tiles and chunks per grid shape')
@@ -552,6 +655,12 @@ def main():
         name, cus = _lib.device_info(0)
         log('device: %s, %d CUs' % (name, cus))
         run_combined(args, p, tab, name)
+        return
+    if args.workload == 'aux':
+        p, tab, _ = make_workload(bulk, ic=False)
+        name, cus = _lib.device_info(0)
+        log('device: %s, %d CUs' % (name, cus))
+        run_aux(args, p, tab, name)
         return
     if args.workload == 'ensemble':
         p, tab, _ = make_workload(bulk, ic=False)
