@@ -885,7 +885,10 @@ extern "C" int cv_smear_partons(cv_handle *h, const float *p4x4, int64_t nparton
         e = hipGetLastError();
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_smear, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, ds, npartons, (float)c,
+        const cv_params &pp = h->prm;
+        dim3 g((unsigned)((pp.nx + CV_SMEAR_BI - 1) / CV_SMEAR_BI), (unsigned)((pp.ny + CV_SMEAR_BJ - 1) / CV_SMEAR_BJ),
+               (unsigned)((pp.nz + CV_SMEAR_BK - 1) / CV_SMEAR_BK));
+        hipLaunchKernelGGL(k_smear, g, dim3(256), 0, h->stream, h->phys, h->eos, ds, npartons, (float)c,
                            (float)w, kfactor, h->ev[which]);
         e = hipGetLastError();
     }

@@ -108,58 +108,110 @@ __global__ __launch_bounds__(256) void k_smear_prep(const float *__restrict__ p4
     out[m] = s;
 }
 
-#define CV_SMEAR_CHUNK 512
-// one thread per cell; partons staged through LDS in chunks, summed in list order
+// One workgroup per block of 16 x 8 x 8 cells (i, j, k; k fastest), each thread owning 4
+// cells along i.  The partons pass through LDS in chunks of CV_SMEAR_CHUNK with what the
+// block needs of each: squared distances to the block's 16 x, 8 y and 8 eta coordinates,
+// and mt cosh(Y - eta), mt sinh(Y - eta) at its 8 eta (the same expressions as per pair,
+// evaluated once per parton and block row instead of once per cell).  Every cell then sums
+// its partons in list order with the reference's distance, cut and weight.
+#define CV_SMEAR_CHUNK 128
+#define CV_SMEAR_BI 16
+#define CV_SMEAR_BJ 8
+#define CV_SMEAR_BK 8
 __global__ __launch_bounds__(256) void k_smear(PhysP p, EosP E, const SmearParton *__restrict__ par, int64_t np_,
                                                float one_o_2sigr2, float w, float kfactor, float *ev) {
-    __shared__ SmearParton s_p[CV_SMEAR_CHUNK];
-    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = I < p.n;
-    const int64_t ij = live ? I / p.nz : 0;
-    const int k = live ? (int)(I - ij * p.nz) : 0;
-    const int i = (int)(ij / p.ny), j = (int)(ij - (int64_t)(ij / p.ny) * p.ny);
-    const float gx = (float)(i - p.nx / 2) * p.dx, gy = (float)(j - p.ny / 2) * p.dy;
-    const float ge = (float)(k - p.nz / 2) * p.dz;
-    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
+    // rows padded by one word: the 8 rows a wavefront reads at one column fall in 8 banks
+    __shared__ float4 s_dx[CV_SMEAR_CHUNK][CV_SMEAR_BI / 4];
+    __shared__ float s_dy[CV_SMEAR_BJ][CV_SMEAR_CHUNK + 1], s_dz[CV_SMEAR_BK][CV_SMEAR_CHUNK + 1];
+    __shared__ float s_ch[CV_SMEAR_BK][CV_SMEAR_CHUNK + 1], s_sh[CV_SMEAR_BK][CV_SMEAR_CHUNK + 1];
+    __shared__ float s_px[CV_SMEAR_CHUNK], s_py[CV_SMEAR_CHUNK];
+    const int tid = threadIdx.x;
+    const int ap = tid / (CV_SMEAR_BJ * CV_SMEAR_BK);                 // 4 cells i0 + 4 ap + r
+    const int b = (tid / CV_SMEAR_BK) % CV_SMEAR_BJ, c = tid % CV_SMEAR_BK;
+    const int i0 = blockIdx.x * CV_SMEAR_BI, j0 = blockIdx.y * CV_SMEAR_BJ, k0 = blockIdx.z * CV_SMEAR_BK;
+    const int j = j0 + b, k = k0 + c;
+    float t0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, t1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float t2[4] = {0.0f, 0.0f, 0.0f, 0.0f}, t3[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     for (int64_t n = 0; n < np_; n += CV_SMEAR_CHUNK) {
         const int cnt = (int)(np_ - n < CV_SMEAR_CHUNK ? np_ - n : CV_SMEAR_CHUNK);
         __syncthreads();
-        for (int m = threadIdx.x; m < cnt; m += blockDim.x) s_p[m] = par[n + m];
+        // threads 0..127: x and y distances, px, py; threads 128..255: eta distances, cosh, sinh
+        const int m = tid % CV_SMEAR_CHUNK;
+        if (m < cnt) {
+            const SmearParton q = par[n + m];
+            if (tid < CV_SMEAR_CHUNK) {
+#pragma unroll
+                for (int u4 = 0; u4 < CV_SMEAR_BI / 4; u4++) {
+                    float d[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const float d1 = (float)(i0 + 4 * u4 + r - p.nx / 2) * p.dx - q.x;
+                        d[r] = d1 * d1;
+                    }
+                    s_dx[m][u4] = make_float4(d[0], d[1], d[2], d[3]);
+                }
+#pragma unroll
+                for (int u = 0; u < CV_SMEAR_BJ; u++) {
+                    const float d2 = (float)(j0 + u - p.ny / 2) * p.dy - q.y;
+                    s_dy[u][m] = d2 * d2;
+                }
+                s_px[m] = q.px;
+                s_py[m] = q.py;
+            } else {
+#pragma unroll
+                for (int u = 0; u < CV_SMEAR_BK; u++) {
+                    const float ge = (float)(k0 + u - p.nz / 2) * p.dz;
+                    const float d3 = ge - q.etas;
+                    s_dz[u][m] = d3 * d3;
+                    s_ch[u][m] = q.mt * coshf(q.Y - ge);
+                    s_sh[u][m] = q.mt * sinhf(q.Y - ge);
+                }
+            }
+        }
         __syncthreads();
-        if (live) {
-            for (int m = 0; m < cnt; m++) {
-                const SmearParton &q = s_p[m];
-                const float d1 = gx - q.x, d2 = gy - q.y, d3 = ge - q.etas;
-                const float dist = one_o_2sigr2 * (d1 * d1 + d2 * d2 + d3 * d3);
+        for (int mm = 0; mm < cnt; mm++) {
+            const float4 dxv = s_dx[mm][ap];
+            const float sy = s_dy[b][mm], sz = s_dz[c][mm], chv = s_ch[c][mm], shv = s_sh[c][mm];
+            const float px = s_px[mm], py = s_py[mm];
+            const float dx[4] = {dxv.x, dxv.y, dxv.z, dxv.w};
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float dist = one_o_2sigr2 * (dx[r] + sy + sz);
                 if (dist < 50.0f) {
                     const float delta = w * expf(-dist);
-                    t0 = t0 + delta * (q.mt * coshf(q.Y - ge));
-                    t1 = t1 + delta * q.px;
-                    t2 = t2 + delta * q.py;
-                    t3 = t3 + delta * (q.mt * sinhf(q.Y - ge));
+                    t0[r] = t0[r] + delta * chv;
+                    t1[r] = t1[r] + delta * px;
+                    t2[r] = t2[r] + delta * py;
+                    t3[r] = t3[r] + delta * shv;
                 }
             }
         }
     }
-    if (!live) return;
-    t0 = kfactor * t0; t1 = kfactor * t1; t2 = kfactor * t2; t3 = kfactor * t3;
-    const float K2 = t1 * t1 + t2 * t2 + t3 * t3;
-    const float K0 = t0 < CV_ACU ? CV_ACU : t0;
+    if (j >= p.ny || k >= p.nz) return;
     Eos eos(E, E.tab);
-    // fixed-point root solve (helper.h:112-125)
-    float E1 = K0, E0;
-    int it = 0;
-    for (;;) {
-        E0 = E1;
-        E1 = K0 - K2 / (K0 + eos.P(E1));
-        it++;
-        if (it > 100 || fabsf(E1 - E0) / rmax(fabsf(E1), CV_ACU) < CV_ACU) break;
-    }
     const int64_t n = p.n;
-    ev[I] = rmax(0.0f, E1);
-    ev[n + I] = 0.0f;
-    ev[2 * n + I] = 0.0f;
-    ev[3 * n + I] = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int i = i0 + 4 * ap + r;
+        if (i >= p.nx) break;
+        const float T0 = kfactor * t0[r], T1 = kfactor * t1[r], T2 = kfactor * t2[r], T3 = kfactor * t3[r];
+        const float K2 = T1 * T1 + T2 * T2 + T3 * T3;
+        const float K0 = T0 < CV_ACU ? CV_ACU : T0;
+        // fixed-point root solve (helper.h:112-125)
+        float E1 = K0, E0;
+        int it = 0;
+        for (;;) {
+            E0 = E1;
+            E1 = K0 - K2 / (K0 + eos.P(E1));
+            it++;
+            if (it > 100 || fabsf(E1 - E0) / rmax(fabsf(E1), CV_ACU) < CV_ACU) break;
+        }
+        const int64_t I = ((int64_t)i * p.ny + j) * p.nz + k;
+        ev[I] = rmax(0.0f, E1);
+        ev[n + I] = 0.0f;
+        ev[2 * n + I] = 0.0f;
+        ev[3 * n + I] = 0.0f;
+    }
 }
 
 // change_longitudinal_profile (kernel_gaussian_smearing_new.cl:124-138): every cell of a

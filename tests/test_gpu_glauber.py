@@ -75,7 +75,8 @@ def _partons(n, seed):
 
 @pytest.mark.parametrize('npart', [1, 700, 1500])
 def test_smear_partons_matches_oracle(npart):
-    """Device smearing (LDS chunks of 512 partons, list order kept) against the FP32 oracle."""
+    """Device smearing (blocks of 4x8x8 cells, partons through LDS in chunks of 256 with their
+    per-block distances and mt cosh, sinh; list order kept) against the FP32 oracle."""
     nx, ny, nz, h = 41, 37, 21, 0.2
     tab = Eos('crossover').create_table()
     p = cv_params(nx, ny, nz, 0.02, h, h, h, 0.6)
