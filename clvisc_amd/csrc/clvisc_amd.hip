@@ -835,10 +835,15 @@ extern "C" int cv_glauber_ini(cv_handle *h, const cv_glauber *g, int which) {
     if (rc) return rc;
     HIP_TRY(hipSetDevice(h->device));
     const int64_t nxy = (int64_t)h->prm.nx * h->prm.ny;
-    hipLaunchKernelGGL(k_glauber_ini, dim3((unsigned)((nxy + 255) / 256)), dim3(256), 0, h->stream, h->phys, *g,
-                       h->ev[which]);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    float *edxy = nullptr;
+    HIP_TRY(hipMalloc((void **)&edxy, sizeof(float) * nxy));
+    hipLaunchKernelGGL(k_glauber_columns, dim3((unsigned)((nxy + 255) / 256)), dim3(256), 0, h->stream, h->phys, *g,
+                       edxy);
+    hipLaunchKernelGGL(k_glauber_ini, grid1d(h->n), dim3(256), 0, h->stream, h->phys, *g, edxy, h->ev[which]);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(edxy);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_glauber_ini: %s", hipGetErrorString(e));
     return CV_OK;
 }
 

@@ -43,8 +43,9 @@ __device__ __forceinline__ float gl_weight_eta(const cv_glauber &g, float z) {
     return 1.0f;
 }
 
-// one thread per (i, j): e(x, y) H(eta) into the four planes of ev (velocities zero)
-__global__ __launch_bounds__(256) void k_glauber_ini(PhysP p, cv_glauber g, float *ev) {
+// e(x, y) of each column (one thread per (i, j)), then e(x, y) H(eta) into the four planes
+// of ev (velocities zero) one thread per cell, so that the field writes are coalesced
+__global__ __launch_bounds__(256) void k_glauber_columns(PhysP p, cv_glauber g, float *edxy) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)p.nx * p.ny) return;
     const int i = (int)(t / p.ny), j = (int)(t - (int64_t)(t / p.ny) * p.ny);
@@ -56,16 +57,20 @@ __global__ __launch_bounds__(256) void k_glauber_ini(PhysP p, cv_glauber g, floa
     const float b = g.b;
     const float ta = gl_thickness(g, x - 0.5f * b, y);
     const float tb = gl_thickness(g, x + 0.5f * b, y);
-    const float edxy = kfac * gl_ed_transverse(g, ta, tb);
+    edxy[t] = kfac * gl_ed_transverse(g, ta, tb);
+}
+
+__global__ __launch_bounds__(256) void k_glauber_ini(PhysP p, cv_glauber g, const float *__restrict__ edxy, float *ev) {
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= p.n) return;
+    const int64_t t = I / p.nz;
+    const int k = (int)(I - t * p.nz);
+    const float etas = (float)(k - p.nz / 2) * p.dz;
     const int64_t n = p.n;
-    for (int k = 0; k < p.nz; k++) {
-        const float etas = (float)(k - p.nz / 2) * p.dz;
-        const int64_t I = t * p.nz + k;
-        ev[I] = edxy * gl_weight_eta(g, etas);
-        ev[n + I] = 0.0f;
-        ev[2 * n + I] = 0.0f;
-        ev[3 * n + I] = 0.0f;
-    }
+    ev[I] = edxy[t] * gl_weight_eta(g, etas);
+    ev[n + I] = 0.0f;
+    ev[2 * n + I] = 0.0f;
+    ev[3 * n + I] = 0.0f;
 }
 
 __global__ __launch_bounds__(256) void k_glauber_nbinary(cv_glauber g, float xlow, float ylow, float dx, float dy,
