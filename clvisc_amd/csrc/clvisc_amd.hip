@@ -789,7 +789,7 @@ extern "C" int cv_energy_sum(cv_handle *h, int which, double tau, double *out) {
     hipLaunchKernelGGL(k_energy_sum, dim3(CV_ESUM_BLOCKS), dim3(256), 0, h->stream, h->phys, h->eos, h->ev[which], pi,
                        bk, tau, h->d_esum);
     const double dV = (double)h->prm.dx * (double)h->prm.dy * (double)h->prm.dz;
-    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, h->stream, h->d_esum, CV_ESUM_BLOCKS, dV,
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, h->stream, h->d_esum, CV_ESUM_BLOCKS, dV,
                        h->d_esum + 2 * CV_ESUM_BLOCKS);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, h->d_esum + 2 * CV_ESUM_BLOCKS, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -809,8 +809,7 @@ extern "C" int cv_slice_sums(cv_handle *h, int which, double *out) {
     double *d = nullptr;
     HIP_TRY(hipMalloc((void **)&d, sizeof(double) * (nq * (size_t)nb + nq)));
     hipLaunchKernelGGL(k_slice_sums, dim3(nb), dim3(128), 0, h->stream, h->phys, h->eos, h->ev[which], rpb, d);
-    hipLaunchKernelGGL(k_slice_final, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, h->stream, d, nb, nz,
-                       d + nq * (size_t)nb);
+    hipLaunchKernelGGL(k_slice_final, dim3((unsigned)nq), dim3(256), 0, h->stream, d, nb, nz, d + nq * (size_t)nb);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out, d + nq * (size_t)nb, sizeof(double) * nq, hipMemcpyDeviceToHost,
                                             h->stream);

@@ -1026,7 +1026,7 @@ __global__ __launch_bounds__(256) void k_max_ed(int64_t n, const float *__restri
 // source besides the flux divergence is -T~^etaeta (kt_src_christoffel, kernel_visc.cl:11-40):
 // part[b] = (sum tau T^tautau, sum T~^etaeta) over the cells of block b, in double from the
 // FP32 fields (P from the FP32 table; bulk Pi added to P).  Fixed grid and order: the sums
-// are reproducible run to run.  k_energy_final adds the partials in block order.
+// are reproducible run to run.  k_energy_final adds the partials in a fixed order.
 #define CV_ESUM_BLOCKS 1024
 __global__ __launch_bounds__(256) void k_energy_sum(PhysP p, EosP E, const float *__restrict__ ev,
                                                     const float *__restrict__ pi, const float *__restrict__ bk,
@@ -1058,12 +1058,26 @@ __global__ __launch_bounds__(256) void k_energy_sum(PhysP p, EosP E, const float
     }
 }
 
-__global__ void k_energy_final(const double *__restrict__ part, int nb, double dV, double *out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double a = 0.0, b = 0.0;
-        for (int q = 0; q < nb; q++) { a += part[2 * q]; b += part[2 * q + 1]; }
-        out[0] = a * dV;
-        out[1] = b * dV;
+// one workgroup of 256: each thread adds partials q = tid, tid + 256, ... in order, then a
+// fixed tree over the 256 (the same order on every call)
+__global__ __launch_bounds__(256) void k_energy_final(const double *__restrict__ part, int nb, double dV,
+                                                      double *out) {
+    __shared__ double sa[256], sb[256];
+    double a = 0.0, b = 0.0;
+    for (int q = threadIdx.x; q < nb; q += 256) { a += part[2 * q]; b += part[2 * q + 1]; }
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sa[threadIdx.x] = sa[threadIdx.x] + sa[threadIdx.x + w];
+            sb[threadIdx.x] = sb[threadIdx.x] + sb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sa[0] * dV;
+        out[1] = sb[0] * dV;
     }
 }
 
@@ -1072,7 +1086,7 @@ __global__ void k_energy_final(const double *__restrict__ part, int nb, double d
 // entropy), u0 e |v_T| and u0 e (<v_r>), with e floored at 1e-10 and v^2 > 1 replaced by
 // 0.999999 as the reference's NumPy code does; P, s from the FP32 table, sums in double.
 // Block b covers rows (i, j) [b R, (b + 1) R); thread k of the block walks them at its k, so
-// a warp reads consecutive cells.  part[b][k][q]; k_slice_final adds the blocks in order.
+// a warp reads consecutive cells.  part[b][k][q]; k_slice_final adds them in a fixed order.
 #define CV_SLICE_Q 10
 __global__ __launch_bounds__(128) void k_slice_sums(PhysP p, EosP E, const float *__restrict__ ev,
                                                     int64_t rows_per_block, double *part) {
@@ -1117,12 +1131,20 @@ __global__ __launch_bounds__(128) void k_slice_sums(PhysP p, EosP E, const float
     }
 }
 
-__global__ void k_slice_final(const double *__restrict__ part, int nb, int nz, double *out) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nz * CV_SLICE_Q) return;
+// one workgroup of 256 per output: each thread sums partials b = tid, tid + 256, ... in
+// order, then a fixed tree over the 256 (the same order on every call)
+__global__ __launch_bounds__(256) void k_slice_final(const double *__restrict__ part, int nb, int nz, double *out) {
+    __shared__ double s[256];
+    const int t = blockIdx.x;                       // output index, < nz * CV_SLICE_Q
     double a = 0.0;
-    for (int b = 0; b < nb; b++) a += part[(int64_t)b * nz * CV_SLICE_Q + t];
-    out[t] = a;
+    for (int b = threadIdx.x; b < nb; b += 256) a += part[(int64_t)b * nz * CV_SLICE_Q + t];
+    s[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[t] = s[0];
 }
 
 // out[q][c] = field plane c at cell idx[q] (SoA planes -> AoS rows); idx checked on the host
