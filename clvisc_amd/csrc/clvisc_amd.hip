@@ -148,6 +148,9 @@ static int default_tile(cv_handle *h) {
     // one row of the (y, eta) plane per tile: at most 252 cells, so that the halo pass
     // (ty + 4 cells) and the y-face pass (ty + 1 faces) fit one pass of 256 threads
     if (p.nz == 1) return set_tile(h, p.ny >= 252 ? 252 : p.ny, 1);
+    // likewise one eta row when ny == 1 (the halo and the z-face pass then hold tz + 4 cells
+    // and tz + 1 faces)
+    if (p.ny == 1) return set_tile(h, 1, p.nz >= 252 ? 252 : p.nz);
     auto waves = [](int n, int w) { return (double)((n + w - 1) / w); };
     int best_ty = 0, best_tz = 0;
     double best = 1e30;

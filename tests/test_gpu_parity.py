@@ -100,6 +100,45 @@ def test_ideal_steps_tiled_3d(eos_kind, dims, kernel, chunk):
     h.close()
 
 
+@pytest.mark.parametrize('dims', [(33, 1, 17), (1, 29, 13), (1, 1, 41), (27, 25, 2), (2, 3, 9)])
+@pytest.mark.parametrize('viscous', [True, False])
+def test_steps_thin_grids(dims, viscous):
+    """Grids with one (or two) cells along an axis, and the 1x1xN longitudinal expansion of
+    pyvisc/tests/test_longitudinal_expansion.py: the default kernels against the oracle."""
+    nx, ny, nz = dims
+    eos = Eos('crossover')
+    tab = eos.create_table()
+    p = cv_params(nx, ny, nz, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08,
+                  bulk=(0.05, 0.155, 0.02) if viscous else None)
+    x, y, z = ic.grid_coords(nx, ny, nz, 0.08, 0.08, 0.15)
+    X, Y, Z = np.meshgrid(x, y, z, indexing='ij')
+    ev = np.zeros((nx, ny, nz, 4), np.float32)
+    ev[..., 0] = 20.0 * np.exp(-(X ** 2 + Y ** 2) / 0.5 - Z ** 2 / 2.0) + 0.01
+    ev = ev.reshape(-1, 4)
+    _say('case: thin grid %dx%dx%d %s, 15 steps' % (nx, ny, nz, 'viscous+bulk' if viscous else 'ideal'))
+    h = _lib.Handle(p, tab, viscous=viscous)
+    _say('  stage kernel %s march chunk %s' % (h.stage_kernel(), h.march_chunk()))
+    o = O.Oracle(orc_params(p, tab, nthreads=8), 'f32', viscous=viscous)
+    o.load_ini(ev)
+    if viscous:
+        h.visc_initialize(p.tau0)
+        h.ideal_stage(1, np.float32(p.tau0))
+        h.set('ev1', ev)
+        h.visc_steps(0, 15)
+        o.visc_steps(15)
+        pairs = (('ev1', o.ev[1]), ('pi1', o.pi[1]), ('bulk1', o.bulk[1]), ('udiff', o.udiff))
+    else:
+        h.set('ev1', ev)
+        h.ideal_steps(0, 15)
+        o.ideal_steps(15)
+        pairs = (('ev1', o.ev[1]), ('src', o.Src))
+    for name, r in pairs:
+        g = h.get(name)
+        _say('  %-6s bit-equal %.8f' % (name, bit_equal_frac(g, r)))
+        np.testing.assert_array_equal(g, r, err_msg=name)
+    h.close()
+
+
 def _visc_case(nx, ny, nz, bulk=None, nhot=12, seed=3, eos_kind='crossover'):
     eos = Eos(eos_kind)
     tab = eos.create_table()
