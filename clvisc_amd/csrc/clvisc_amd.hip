@@ -97,6 +97,9 @@ struct cv_handle {
     float *om[2] = {nullptr, nullptr};
     float *sf_om = nullptr;
     unsigned long long *sf_count = nullptr;
+    uint32_t *sf_list = nullptr;         // cubes that cross EFRZ (pass 1 of the search)
+    unsigned *sf_nlist = nullptr;
+    int64_t sf_list_cap = 0;
     int64_t sf_cap = 0;
 };
 
@@ -323,7 +326,7 @@ extern "C" int cv_destroy(cv_handle *h) {
     (void)hipFree(h->sf_ev); (void)hipFree(h->sf_pi); (void)hipFree(h->sf_bk);
     (void)hipFree(h->sf_out); (void)hipFree(h->sf_txyz); (void)hipFree(h->sf_pisf); (void)hipFree(h->sf_bksf);
     (void)hipFree(h->sf_om); (void)hipFree(h->om[0]); (void)hipFree(h->om[1]);
-    (void)hipFree(h->sf_count);
+    (void)hipFree(h->sf_count); (void)hipFree(h->sf_list); (void)hipFree(h->sf_nlist);
     (void)hipFree(h->seed_buf); (void)hipFree(h->iters);
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
     if (h->ev_b) (void)hipEventDestroy(h->ev_b);
@@ -1090,8 +1093,29 @@ extern "C" int cv_hypersf_find(cv_handle *h, float tau_new) {
     s.cap = h->sf_cap;
     s.viscous = h->viscous;
     int64_t total = (int64_t)s.gx * s.gy * s.gz;
-    auto kern = (p.flags & CV_FLAG_SF_FP64) ? k_hypersf<double> : k_hypersf<float>;
-    hipLaunchKernelGGL(kern, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, h->stream, s, h->sf_ev,
+    if (total >= ((int64_t)1 << 32)) return fail(CV_ERR_CAPACITY, "cv_hypersf_find: %lld cubes", (long long)total);
+    if (total > h->sf_list_cap) {
+        HIP_TRY(hipStreamSynchronize(h->stream));
+        (void)hipFree(h->sf_list);
+        h->sf_list = nullptr;
+        h->sf_list_cap = 0;
+        if (hipMalloc((void **)&h->sf_list, sizeof(uint32_t) * total) != hipSuccess)
+            return fail(CV_ERR_HIP, "cv_hypersf_find: allocation of the cube list failed");
+        h->sf_list_cap = total;
+    }
+    if (!h->sf_nlist && hipMalloc((void **)&h->sf_nlist, sizeof(unsigned)) != hipSuccess) {
+        h->sf_nlist = nullptr;
+        return fail(CV_ERR_HIP, "cv_hypersf_find: allocation failed");
+    }
+    // pass 1: list the crossing cubes; pass 2: their elements, grid-stride over the list
+    hipLaunchKernelGGL(k_fix_reset, dim3(1), dim3(64), 0, h->stream, h->sf_nlist);
+    const bool fp64 = (p.flags & CV_FLAG_SF_FP64) != 0;
+    auto mark = fp64 ? k_hypersf_mark<double> : k_hypersf_mark<float>;
+    hipLaunchKernelGGL(mark, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, h->stream, s, h->sf_ev, h->ev[1],
+                       h->sf_list, h->sf_nlist);
+    auto kern = fp64 ? k_hypersf<double> : k_hypersf<float>;
+    const int64_t g2 = std::min<int64_t>((total + 127) / 128, (int64_t)(h->cus > 0 ? h->cus : 256) * 16);
+    hipLaunchKernelGGL(kern, dim3((unsigned)g2), dim3(128), 0, h->stream, s, h->sf_list, h->sf_nlist, h->sf_ev,
                        h->ev[1], h->sf_pi, h->pi[1], h->bulk ? h->sf_bk : nullptr, h->bulk ? h->bk[1] : nullptr,
                        h->sf_tau_old, tau_new, h->sf_count, h->sf_out, h->sf_txyz, h->sf_pisf, h->sf_bksf,
                        h->om[0], h->om[1], h->sf_om);

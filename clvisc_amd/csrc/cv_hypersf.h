@@ -176,12 +176,70 @@ template <> __device__ __forceinline__ float sinh_<float>(float x) { return sinh
 template <> __device__ __forceinline__ double cosh_<double>(double x) { return cosh(x); }
 template <> __device__ __forceinline__ double sinh_<double>(double x) { return sinh(x); }
 
-// One thread per cube (i, j, k) of the thinned grid, k fastest.  ev_old/ev_new,
+// Cubes (i, j, k) of the thinned grid, k fastest, in two passes: k_hypersf_mark lists the
+// cubes that cross EFRZ (one thread per cube), k_hypersf builds their elements (one thread
+// per listed cube, so that the wavefronts of the second pass are full).  ev_old/ev_new,
 // pi_old/pi_new, bk_old/bk_new: SoA planes of n cells (pi only when viscous, bk only
 // with bulk viscosity).  R is the precision of everything computed from the corner
 // values; the stored element is rounded to float.
+
+// Corner cells and energy densities of cube t of the skipped grid; false if the cube lies
+// outside the grid or all its 16 corners are on one side of EFRZ (the early exit of
+// get_hypersf, kernel_hypersf.cl:439-455).
 template <typename R>
-__global__ __launch_bounds__(128) void k_hypersf(SfP p, const float *__restrict__ ev_old,
+__device__ __forceinline__ bool sf_cube_corners(const SfP &p, int64_t t, const float *__restrict__ ev_old,
+                                                const float *__restrict__ ev_new, int64_t cid[16], R ed[16]) {
+    const int k = (int)(t % p.gz);
+    const int j = (int)((t / p.gz) % p.gy);
+    const int i = (int)(t / ((int64_t)p.gz * p.gy));
+    if (!((i + 1) * p.nxskip < p.nx && (j + 1) * p.nyskip < p.ny && (k + 1) * p.nzskip < p.nz)) return false;
+    const R EFRZ = (R)p.efrz;
+#pragma unroll
+    for (int c = 0; c < 16; c++) {
+        int ii = (i + (c & 1)) * p.nxskip, jj = (j + ((c >> 1) & 1)) * p.nyskip, kk = (k + ((c >> 2) & 1)) * p.nzskip;
+        cid[c] = ((int64_t)ii * p.ny + jj) * p.nz + kk;
+        ed[c] = (R)(c < 8 ? ev_old : ev_new)[cid[c]];
+    }
+    return !((ed[0] - EFRZ) * (ed[15] - EFRZ) > 0 && (ed[1] - EFRZ) * (ed[14] - EFRZ) > 0 &&
+             (ed[2] - EFRZ) * (ed[13] - EFRZ) > 0 && (ed[3] - EFRZ) * (ed[12] - EFRZ) > 0 &&
+             (ed[4] - EFRZ) * (ed[11] - EFRZ) > 0 && (ed[5] - EFRZ) * (ed[10] - EFRZ) > 0 &&
+             (ed[6] - EFRZ) * (ed[9] - EFRZ) > 0 && (ed[7] - EFRZ) * (ed[8] - EFRZ) > 0);
+}
+
+// Pass 1: the cubes that cross EFRZ, listed (one atomic per wavefront, slots by ballot).
+template <typename R>
+__global__ __launch_bounds__(256) void k_hypersf_mark(SfP p, const float *__restrict__ ev_old,
+                                                      const float *__restrict__ ev_new, uint32_t *list,
+                                                      unsigned *nlist) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)p.gx * p.gy * p.gz;
+    bool cross = false;
+    if (t < total) {
+        int64_t cid[16];
+        R ed[16];
+        cross = sf_cube_corners<R>(p, t, ev_old, ev_new, cid, ed);
+    }
+    const uint64_t mask = __ballot(cross);
+    if (mask == 0ull) return;                          // uniform over the wavefront
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    unsigned base = 0u;
+    if (lane == leader) base = atomicAdd(nlist, (unsigned)__popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (cross) list[base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull))] = (uint32_t)t;
+}
+
+// Pass 2: one element per listed cube (grid-stride over the list).
+template <typename R>
+__device__ void sf_cube(const SfP &p, int64_t t, const float *__restrict__ ev_old, const float *__restrict__ ev_new,
+                        const float *__restrict__ pi_old, const float *__restrict__ pi_new,
+                        const float *__restrict__ bk_old, const float *__restrict__ bk_new, float tau_old_f,
+                        float tau_new_f, unsigned long long *count, float *sf, float *txyz, float *pisf, float *bksf,
+                        const float *__restrict__ om_old, const float *__restrict__ om_new, float *omsf);
+
+template <typename R>
+__global__ __launch_bounds__(128) void k_hypersf(SfP p, const uint32_t *__restrict__ list,
+                                                 const unsigned *__restrict__ nlist, const float *__restrict__ ev_old,
                                                  const float *__restrict__ ev_new,
                                                  const float *__restrict__ pi_old,
                                                  const float *__restrict__ pi_new,
@@ -190,27 +248,25 @@ __global__ __launch_bounds__(128) void k_hypersf(SfP p, const float *__restrict_
                                                  unsigned long long *count, float *sf, float *txyz, float *pisf,
                                                  float *bksf, const float *__restrict__ om_old = nullptr,
                                                  const float *__restrict__ om_new = nullptr, float *omsf = nullptr) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t total = (int64_t)p.gx * p.gy * p.gz;
-    if (t >= total) return;
+    const unsigned m = *nlist;
+    for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < m; q += gridDim.x * blockDim.x)
+        sf_cube<R>(p, (int64_t)list[q], ev_old, ev_new, pi_old, pi_new, bk_old, bk_new, tau_old_f, tau_new_f, count,
+                   sf, txyz, pisf, bksf, om_old, om_new, omsf);
+}
+
+template <typename R>
+__device__ void sf_cube(const SfP &p, int64_t t, const float *__restrict__ ev_old, const float *__restrict__ ev_new,
+                        const float *__restrict__ pi_old, const float *__restrict__ pi_new,
+                        const float *__restrict__ bk_old, const float *__restrict__ bk_new, float tau_old_f,
+                        float tau_new_f, unsigned long long *count, float *sf, float *txyz, float *pisf, float *bksf,
+                        const float *__restrict__ om_old, const float *__restrict__ om_new, float *omsf) {
     const int k = (int)(t % p.gz);
     const int j = (int)((t / p.gz) % p.gy);
     const int i = (int)(t / ((int64_t)p.gz * p.gy));
-    if (!((i + 1) * p.nxskip < p.nx && (j + 1) * p.nyskip < p.ny && (k + 1) * p.nzskip < p.nz)) return;
-    const R EFRZ = (R)p.efrz;
     int64_t cid[16];
     R ed[16];
-#pragma unroll
-    for (int c = 0; c < 16; c++) {
-        int ii = (i + (c & 1)) * p.nxskip, jj = (j + ((c >> 1) & 1)) * p.nyskip, kk = (k + ((c >> 2) & 1)) * p.nzskip;
-        cid[c] = ((int64_t)ii * p.ny + jj) * p.nz + kk;
-        ed[c] = (R)(c < 8 ? ev_old : ev_new)[cid[c]];
-    }
-    if ((ed[0] - EFRZ) * (ed[15] - EFRZ) > 0 && (ed[1] - EFRZ) * (ed[14] - EFRZ) > 0 &&
-        (ed[2] - EFRZ) * (ed[13] - EFRZ) > 0 && (ed[3] - EFRZ) * (ed[12] - EFRZ) > 0 &&
-        (ed[4] - EFRZ) * (ed[11] - EFRZ) > 0 && (ed[5] - EFRZ) * (ed[10] - EFRZ) > 0 &&
-        (ed[6] - EFRZ) * (ed[9] - EFRZ) > 0 && (ed[7] - EFRZ) * (ed[8] - EFRZ) > 0)
-        return;
+    if (!sf_cube_corners<R>(p, t, ev_old, ev_new, cid, ed)) return;
+    const R EFRZ = (R)p.efrz;
     V4<R> all[32];
     int nint = sf_all_intersections<R>(ed, EFRZ, all);
     V4<R> ef = sf_energy_flow<R>(ed, EFRZ);
