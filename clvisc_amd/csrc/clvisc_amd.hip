@@ -97,6 +97,7 @@ struct cv_handle {
     float *om[2] = {nullptr, nullptr};
     float *sf_om = nullptr;
     unsigned long long *sf_count = nullptr;
+    double *d_slice = nullptr;           // cv_slice_sums partials and sums (first call)
     uint32_t *sf_list = nullptr;         // cubes that cross EFRZ (pass 1 of the search)
     unsigned *sf_nlist = nullptr;
     int64_t sf_list_cap = 0;
@@ -326,7 +327,7 @@ extern "C" int cv_destroy(cv_handle *h) {
     (void)hipFree(h->sf_ev); (void)hipFree(h->sf_pi); (void)hipFree(h->sf_bk);
     (void)hipFree(h->sf_out); (void)hipFree(h->sf_txyz); (void)hipFree(h->sf_pisf); (void)hipFree(h->sf_bksf);
     (void)hipFree(h->sf_om); (void)hipFree(h->om[0]); (void)hipFree(h->om[1]);
-    (void)hipFree(h->sf_count); (void)hipFree(h->sf_list); (void)hipFree(h->sf_nlist);
+    (void)hipFree(h->sf_count); (void)hipFree(h->sf_list); (void)hipFree(h->sf_nlist); (void)hipFree(h->d_slice);
     (void)hipFree(h->seed_buf); (void)hipFree(h->iters);
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
     if (h->ev_b) (void)hipEventDestroy(h->ev_b);
@@ -812,15 +813,18 @@ extern "C" int cv_slice_sums(cv_handle *h, int which, double *out) {
     const int nb = (int)std::min<int64_t>(1024, nrow);
     const int64_t rpb = (nrow + nb - 1) / nb;
     const size_t nq = (size_t)nz * CV_SLICE_Q;
-    double *d = nullptr;
-    HIP_TRY(hipMalloc((void **)&d, sizeof(double) * (nq * (size_t)nb + nq)));
+    // nb and nz are fixed by the grid, so one buffer per handle serves every call
+    if (!h->d_slice && hipMalloc((void **)&h->d_slice, sizeof(double) * (nq * (size_t)nb + nq)) != hipSuccess) {
+        h->d_slice = nullptr;
+        return fail(CV_ERR_HIP, "cv_slice_sums: allocation failed");
+    }
+    double *d = h->d_slice;
     hipLaunchKernelGGL(k_slice_sums, dim3(nb), dim3(128), 0, h->stream, h->phys, h->eos, h->ev[which], rpb, d);
     hipLaunchKernelGGL(k_slice_final, dim3((unsigned)nq), dim3(256), 0, h->stream, d, nb, nz, d + nq * (size_t)nb);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out, d + nq * (size_t)nb, sizeof(double) * nq, hipMemcpyDeviceToHost,
                                             h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    (void)hipFree(d);
     if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_slice_sums: %s", hipGetErrorString(e));
     return CV_OK;
 }
