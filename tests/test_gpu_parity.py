@@ -174,6 +174,43 @@ def test_large_grid_tiled_equals_per_cell():
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize('viscous', [False, True])
+def test_riemann_mode_matches_oracle(viscous):
+    """CV_FLAG_RIEMANN_TEST (Cartesian mode without the Christoffel terms, pyvisc/ideal.py:164-165;
+    the per-cell ideal kernel and the tiled viscous kernel's test-branch instance): a Riemann
+    step along x against the oracle in the same mode, identical fields."""
+    nx, ny, nz = 41, 9, 7
+    eos = Eos('ideal_gas')
+    tab = eos.create_table()
+    p = cv_params(nx, ny, nz, 0.01, 0.08, 0.08, 0.08, 1.0, etaos_xmin=0.0, etaos_ymin=0.08,
+                  flags=_lib.CV_FLAG_RIEMANN_TEST)
+    ev = np.zeros((nx, ny, nz, 4), np.float32)
+    ev[: nx // 2, ..., 0] = 10.0
+    ev[nx // 2:, ..., 0] = 1.0
+    ev = ev.reshape(-1, 4)
+    _say('case: riemann mode %dx%dx%d %s, 20 steps' % (nx, ny, nz, 'viscous' if viscous else 'ideal'))
+    h = _lib.Handle(p, tab, viscous=viscous)
+    o = O.Oracle(orc_params(p, tab, nthreads=8), 'f32', viscous=viscous)
+    o.load_ini(ev)
+    if viscous:
+        h.visc_initialize(p.tau0)
+        h.ideal_stage(1, np.float32(p.tau0))
+        h.set('ev1', ev)
+        h.visc_steps(0, 20)
+        o.visc_steps(20)
+        pairs = (('ev1', o.ev[1]), ('pi1', o.pi[1]))
+    else:
+        h.set('ev1', ev)
+        h.ideal_steps(0, 20)
+        o.ideal_steps(20)
+        pairs = (('ev1', o.ev[1]), ('src', o.Src))
+    for name, r in pairs:
+        g = h.get(name)
+        _say('  %-4s bit-equal %.8f' % (name, bit_equal_frac(g, r)))
+        np.testing.assert_array_equal(g, r, err_msg=name)
+    h.close()
+
+
 def _visc_case(nx, ny, nz, bulk=None, nhot=12, seed=3, eos_kind='crossover'):
     eos = Eos(eos_kind)
     tab = eos.create_table()
