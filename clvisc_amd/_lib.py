@@ -18,6 +18,8 @@ CV_FLAG_GUBSER_VISC_TEST = 0x2
 CV_FLAG_BULK = 0x4
 CV_FLAG_FP64_ROOT = 0x8
 CV_FLAG_SF_FP64 = 0x10
+CV_FLAG_PIMN_OMEGA = 0x20
+CV_FLAG_OMEGA_OMEGA = 0x40
 
 FIELDS = {'ev0': 0, 'ev1': 1, 'ev2': 2, 'pi0': 3, 'pi1': 4, 'pi2': 5,
           'bulk0': 6, 'bulk1': 7, 'bulk2': 8, 'src': 9, 'is_src': 10,

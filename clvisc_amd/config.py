@@ -130,8 +130,8 @@ def read_config(path=None, argv=None):
                             type=type(value[0]), default=value[0], help=value[1])
     for flag, help_ in (('riemann_test', 'switch on riemann test (Christoffel terms removed)'),
                         ('gubser_visc_test', '2nd order viscous gubser test'),
-                        ('pimn_omega_coupling', 'pi^{mu nu} - vorticity coupling (not available)'),
-                        ('omega_omega_coupling', 'vorticity - vorticity coupling (not available)'),
+                        ('pimn_omega_coupling', 'pi^{mu nu} - vorticity coupling in the IS source'),
+                        ('omega_omega_coupling', 'vorticity - vorticity coupling in the IS source'),
                         ('save_to_hdf5', 'accepted for compatibility; bulk info is written as text'),
                         ('opencl_interactive', 'accepted for compatibility (no effect)'),
                         ('hypersf_fp64', 'freeze-out surface geometry in FP64 instead of the FP32 of the reference')):

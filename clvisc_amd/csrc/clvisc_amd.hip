@@ -542,7 +542,8 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
 #define CV_TILED_ARGS h->phys, h->eos, st, h->tile, sp
         // optional branches: none, the FP64 root alone, the split alone (k_visc_root solves the
         // FP64 cells of a split stage), or all of them (test flags)
-        const bool tests = (h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST)) != 0;
+        const bool tests = (h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST | CV_FLAG_PIMN_OMEGA |
+                                            CV_FLAG_OMEGA_OMEGA)) != 0;
         const bool fp64 = (h->prm.flags & CV_FLAG_FP64_ROOT) != 0;
         const int spec = tests ? CV_SPEC_ALL : (split ? CV_SPEC_SPLIT : (fp64 ? CV_SPEC_FP64 : 0));
         // FP64 cells are deferred to k_fp64_fix unless the split stage solves every cell later
@@ -1274,9 +1275,9 @@ extern "C" int cv_batch_create(const cv_params *base, const cv_eos *eos, int nev
     cv_batch *b = new cv_batch();
     b->device = device;
     b->bulk = (base->flags & CV_FLAG_BULK) ? 1 : 0;
-    if (base->flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST)) {
+    if (base->flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST | CV_FLAG_PIMN_OMEGA | CV_FLAG_OMEGA_OMEGA)) {
         delete b;
-        return fail(CV_ERR_UNSUPPORTED, "cv_batch_create: the test flags are single-event only");
+        return fail(CV_ERR_UNSUPPORTED, "cv_batch_create: the test flags and the vorticity couplings are single-event only");
     }
     b->fp64_root = (base->flags & CV_FLAG_FP64_ROOT) != 0;
     for (int e = 0; e < nevents; e++) {

@@ -340,6 +340,70 @@ __device__ float pipi(int lam, int mu, int nu, const float *pim, const float *u)
     return first + second;
 }
 
+// Vorticity couplings (PIMUNU_OMEGA_COUPLING, OMEGA_OMEGA_COUPLING, kernel_IS.cl:349-470):
+// Omega^{mu nu} = 1/2 Delta^mu_a Delta^nu_b omega^{ab} and the projected products
+// pi^<mu_lam Omega^nu>lam, Omega^<mu_lam Omega^nu>lam, in the reference's operand order.
+__device__ float omega_proj(int mu, int nu, const float *om, const float *u) {
+    const float ul[4] = {u[0], -u[1], -u[2], -u[3]};
+    float D[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) D[a][b] = (a == b ? 1.0f : 0.0f) - u[a] * ul[b];
+    const float A[4][4] = {{0.0f, om[0], om[1], om[2]}, {-om[0], 0.0f, om[3], om[4]},
+                           {-om[1], -om[3], 0.0f, om[5]}, {-om[2], -om[4], -om[5], 0.0f}};
+    float temp[4] = {dot4(D[nu], A[0]), dot4(D[nu], A[1]), dot4(D[nu], A[2]), dot4(D[nu], A[3])};
+    return 0.5f * dot4(D[mu], temp);
+}
+__device__ __forceinline__ void omega_row(int lam, const float *Om, float B[4]) {
+    if (lam == 0) { B[0] = 0.0f; B[1] = Om[0]; B[2] = Om[1]; B[3] = Om[2]; }
+    else if (lam == 1) { B[0] = Om[0]; B[1] = 0.0f; B[2] = Om[3]; B[3] = Om[4]; }
+    else if (lam == 2) { B[0] = Om[1]; B[1] = -Om[3]; B[2] = 0.0f; B[3] = Om[5]; }
+    else { B[0] = Om[2]; B[1] = -Om[4]; B[2] = -Om[5]; B[3] = 0.0f; }
+}
+__device__ float pi_omega(int lam, int mu, int nu, const float *pim, const float *Om, const float *u) {
+    const float g[4] = {1.0f, -1.0f, -1.0f, -1.0f};
+    float A[4] = {pim[IDX(0, lam)], -pim[IDX(1, lam)], -pim[IDX(2, lam)], -pim[IDX(3, lam)]};
+    float B[4];
+    omega_row(lam, Om, B);
+    float D[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) D[a][b] = (a == b ? g[a] : 0.0f) - u[a] * u[b];
+    float first = 0.5f * (dot4(D[mu], A) * dot4(D[nu], B) + dot4(D[mu], B) * dot4(D[nu], A));
+    float temp[4] = {dot4(D[0], A), dot4(D[1], A), dot4(D[2], A), dot4(D[3], A)};
+    float second = -1.0f / 3.0f * ((mu == nu ? g[mu] : 0.0f) - u[mu] * u[nu]) * dot4(temp, B);
+    return first + second;
+}
+__device__ float omega_omega(int lam, int mu, int nu, const float *Om, const float *u) {
+    const float g[4] = {1.0f, -1.0f, -1.0f, -1.0f};
+    float B[4];
+    omega_row(lam, Om, B);
+    float D[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) D[a][b] = (a == b ? g[a] : 0.0f) - u[a] * u[b];
+    float first = dot4(D[mu], B) * dot4(D[nu], B);
+    float temp[4] = {dot4(D[0], B), dot4(D[1], B), dot4(D[2], B), dot4(D[3], B)};
+    float second = -1.0f / 3.0f * ((mu == nu ? g[mu] : 0.0f) - u[mu] * u[nu]) * dot4(temp, B);
+    return first + second;
+}
+// the six projected components (Omega^{01}, 02, 03, 12, 13, 23) from the derivatives of u
+__device__ void omega_cell(const float *udt, const float *udx, const float *udy, const float *udz, const float *u,
+                           float Om[6]) {
+    const float om[6] = {udt[1] - udx[0], udt[2] - udy[0], udt[3] - udz[0],
+                         udx[2] - udy[1], udx[3] - udz[1], udy[3] - udz[2]};
+    Om[0] = omega_proj(0, 1, om, u); Om[1] = omega_proj(0, 2, om, u); Om[2] = omega_proj(0, 3, om, u);
+    Om[3] = omega_proj(1, 2, om, u); Om[4] = omega_proj(1, 3, om, u); Om[5] = omega_proj(2, 3, om, u);
+}
+// the coupling terms of update_pimn's source for component (mu, nu) (kernel_IS.cl:636-683)
+__device__ __forceinline__ float couple_src(float src, int flags, int mu, int nu, const float *pim, const float *Om,
+                                            const float *u, float one_over_taupi) {
+    if (flags & CV_FLAG_PIMN_OMEGA)
+        src += 2.0f * (pi_omega(0, mu, nu, pim, Om, u) - pi_omega(1, mu, nu, pim, Om, u)
+                       - pi_omega(2, mu, nu, pim, Om, u) - pi_omega(3, mu, nu, pim, Om, u));
+    if (flags & CV_FLAG_OMEGA_OMEGA)
+        src -= 1.0f * one_over_taupi * (omega_omega(0, mu, nu, Om, u) - omega_omega(1, mu, nu, Om, u)
+                                        - omega_omega(2, mu, nu, Om, u) - omega_omega(3, mu, nu, Om, u));
+    return src;
+}
+
 // ---------------------------------------------------------------------------
 // CLVisc.IS_stepUpdate(step): one cell per thread
 //   ev_st/pi_st/bk_st : d_ev[step], d_pi[step] (stencil)
@@ -431,6 +495,9 @@ __global__ __launch_bounds__(256) void k_is_stage(PhysP p, EosP E, StageP st,
     float pi2[10];
 #pragma unroll
     for (int m = 0; m < 10; m++) pi2[m] = pi_st[(int64_t)m * p.n + I];
+    const bool cpl = (p.flags & (CV_FLAG_PIMN_OMEGA | CV_FLAG_OMEGA_OMEGA)) != 0;
+    float Om[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (cpl) omega_cell(udt, udx, udy, udz, u, Om);
     const float gd[4] = {1.0f, -1.0f, -1.0f, -1.0f};
     float max_abs = 0.0f;
     float out[10];
@@ -453,6 +520,7 @@ __global__ __launch_bounds__(256) void k_is_stage(PhysP p, EosP E, StageP st,
             if (gubser)
                 src -= one_over_taupi * coef_pipi * (pipi(0, mu, nu, pi2, u) - pipi(1, mu, nu, pi2, u)
                                                      - pipi(2, mu, nu, pi2, u) - pipi(3, mu, nu, pi2, u));
+            if (cpl) src = couple_src(src, p.flags, mu, nu, pi2, Om, u, one_over_taupi);
             s[mn] += src;
             pi_old = (pi_old + (s[mn] * p.dt) * st.r_step + p.dt * one_over_taupi * piNS) * rden;
             if (fabsf(pi_old) > max_abs) max_abs = fabsf(pi_old);
@@ -865,6 +933,9 @@ __global__ __launch_bounds__(256, CV_STAGE_WAVES) void k_visc_stage(PhysP p, Eos
         coef_pipi = p.lam1 * rcp(ed_step);
         etav = ((ETAOS * 4.0f) * p.r_3) * (sqrtf(ed_step) * q4);
     }
+    const bool cpl = (p.flags & (CV_FLAG_PIMN_OMEGA | CV_FLAG_OMEGA_OMEGA)) != 0;
+    float Om[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (cpl) omega_cell(udt, udx, udy, udz, u, Om);
     const float gd[4] = {1.0f, -1.0f, -1.0f, -1.0f};
     float max_abs = 0.0f;
     float out[10];
@@ -887,6 +958,7 @@ __global__ __launch_bounds__(256, CV_STAGE_WAVES) void k_visc_stage(PhysP p, Eos
             if (gubser)
                 src -= one_over_taupi * coef_pipi * (pipi(0, mu, nu, q, u) - pipi(1, mu, nu, q, u)
                                                      - pipi(2, mu, nu, q, u) - pipi(3, mu, nu, q, u));
+            if (cpl) src = couple_src(src, p.flags, mu, nu, q, Om, u, one_over_taupi);
             s[mn] += src;
             pi_old = (pi_old + (s[mn] * p.dt) * st.r_step + p.dt * one_over_taupi * piNS) * rden;
             if (fabsf(pi_old) > max_abs) max_abs = fabsf(pi_old);

@@ -149,7 +149,7 @@ struct StagePtrs {
 // Optional branches of the stage kernels (template mask SPEC; flags and pointers are still
 // tested at run time).  Each costs registers in the whole kernel, so a kernel carries only
 // what its caller asks for.
-#define CV_SPEC_TESTS 1      // GUBSER_VISC_TEST / RIEMANN_TEST
+#define CV_SPEC_TESTS 1      // GUBSER_VISC_TEST / RIEMANN_TEST / the vorticity couplings
 #define CV_SPEC_FP64 2       // FP64 root solve for fast cells
 #define CV_SPEC_SPLIT 4      // split root solve (StagePtrs::cons_out)
 #define CV_SPEC_ALL 7
@@ -497,6 +497,10 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
             float one_over_taupi = temperature * rcp(3.0f * rmax(CV_ACU, local_etaos) * CV_HBARC);
             float coef_pipi = 0.0f;
             const bool gubser = (SPEC & CV_SPEC_TESTS) && (p.flags & 2);
+            // vorticity couplings (test-branch instances only; production kernels carry neither)
+            const bool cpl = (SPEC & CV_SPEC_TESTS) && (p.flags & (CV_FLAG_PIMN_OMEGA | CV_FLAG_OMEGA_OMEGA));
+            float Om[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            if (cpl) omega_cell(udt, udx, udy, udz, u, Om);
             if (gubser) {
                 float q4 = sqrtf(sqrtf(ed_step));
                 float ETAOS = p.etaos_ymin;
@@ -528,6 +532,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
                     if (gubser)
                         src -= one_over_taupi * coef_pipi * (pipi(0, mu, nu, q, u) - pipi(1, mu, nu, q, u)
                                                              - pipi(2, mu, nu, q, u) - pipi(3, mu, nu, q, u));
+                    if (cpl) src = couple_src(src, p.flags, mu, nu, q, Om, u, one_over_taupi);
                     s[mn] += src;
                     pi_old = (pi_old + (s[mn] * p.dt) * st.r_step + p.dt * one_over_taupi * piNS) * rden;
                     if (fabsf(pi_old) > max_abs) max_abs = fabsf(pi_old);

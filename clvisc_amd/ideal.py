@@ -65,6 +65,11 @@ def _params_from_cfg(cfg, efrz, viscous):
         p.fp64_root_vmin = vmin
     if getattr(cfg, 'hypersf_fp64', False):
         flags |= _lib.CV_FLAG_SF_FP64
+    # the IS vorticity couplings of update_pimn (pyvisc/visc.py:116-121, kernel_IS.cl:636-683)
+    if viscous and getattr(cfg, 'pimn_omega_coupling', False):
+        flags |= _lib.CV_FLAG_PIMN_OMEGA
+    if viscous and getattr(cfg, 'omega_omega_coupling', False):
+        flags |= _lib.CV_FLAG_OMEGA_OMEGA
     p.flags = flags
     p.efrz = float(efrz)
     p.nxskip, p.nyskip, p.nzskip = int(cfg.nxskip), int(cfg.nyskip), int(cfg.nzskip)
@@ -81,9 +86,6 @@ class CLIdeal(object):
             os.makedirs(path)
         if not getattr(self.cfg, 'use_float32', True):
             raise ValueError('use_float32=False: the device path computes in FP32 only')
-        for flag in ('pimn_omega_coupling', 'omega_omega_coupling'):
-            if getattr(self.cfg, flag, False):
-                raise NotImplementedError('%s is not part of this hydro path' % flag)
         self.determine_float_size(self.cfg)
         self.gpu_id = gpu_id
         self.size = self.cfg.NX * self.cfg.NY * self.cfg.NZ

@@ -45,6 +45,8 @@ extern "C" {
 #define CV_FLAG_FP64_ROOT        0x8   /* FP64 root solve where M > fp64_root_vmin * T00 */
 #define CV_FLAG_SF_FP64          0x10  /* freeze-out surface geometry in FP64 (exact where FP32
                                           loses the tie-breaking shift of coplanar intersections) */
+#define CV_FLAG_PIMN_OMEGA       0x20  /* -D PIMUNU_OMEGA_COUPLING (pyvisc/visc.py:117-118) */
+#define CV_FLAG_OMEGA_OMEGA      0x40  /* -D OMEGA_OMEGA_COUPLING (pyvisc/visc.py:120-121) */
 
 /* Hydro parameters: the reference's -D compile options (pyvisc/ideal.py:146-171,
  * pyvisc/visc.py:108-121) as run-time values. */

@@ -99,6 +99,7 @@ typedef struct orc_params {
     float fp64_root_vmin;
     int32_t nthreads;          /* OpenMP threads, 0 = runtime default */
     double tau0_cfg, dt_cfg;   /* TAU0, DT as the Python config holds them (double) */
+    int32_t couplings;         /* 1: -D PIMUNU_OMEGA_COUPLING, 2: -D OMEGA_OMEGA_COUPLING */
 } orc_params;
 
 static int64_t ncells(const orc_params *p) { return (int64_t)p->nx * p->ny * p->nz; }
@@ -975,6 +976,73 @@ static real pipi(int lam, int mu, int nu, const real *pim, const real *u) {
     return first + second;
 }
 
+/* Vorticity couplings of update_pimn (kernel_IS.cl:349-470): Omega^{mu nu} =
+ * 1/2 Delta^mu_a Delta^nu_b omega^{ab} from omega = (d_t u^x - d_x u^t, d_t u^y - d_y u^t,
+ * d_t u^eta - d_eta u^t, d_x u^y - d_y u^x, d_x u^eta - d_eta u^x, d_y u^eta - d_eta u^y)
+ * (kernel_IS.cl:636-651), and the projected products pi^<mu_lam Omega^nu>lam and
+ * Omega^<mu_lam Omega^nu>lam, each in the reference's operand order. */
+static real omega_proj(int mu, int nu, const real *om, const real *u) {
+    const real ul[4] = {u[0], -u[1], -u[2], -u[3]};
+    real D[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) D[a][b] = (a == b ? RC(1.0f) : RC(0.0f)) - u[a] * ul[b];
+    const real A[4][4] = {{RC(0.0f), om[0], om[1], om[2]}, {-om[0], RC(0.0f), om[3], om[4]},
+                          {-om[1], -om[3], RC(0.0f), om[5]}, {-om[2], -om[4], -om[5], RC(0.0f)}};
+    real temp[4] = {dot4(D[nu], A[0]), dot4(D[nu], A[1]), dot4(D[nu], A[2]), dot4(D[nu], A[3])};
+    return RC(0.5f) * dot4(D[mu], temp);
+}
+static void omega_row(int lam, const real *Om, real B[4]) {
+    if (lam == 0) { B[0] = RC(0.0f); B[1] = Om[0]; B[2] = Om[1]; B[3] = Om[2]; }
+    else if (lam == 1) { B[0] = Om[0]; B[1] = RC(0.0f); B[2] = Om[3]; B[3] = Om[4]; }
+    else if (lam == 2) { B[0] = Om[1]; B[1] = -Om[3]; B[2] = RC(0.0f); B[3] = Om[5]; }
+    else { B[0] = Om[2]; B[1] = -Om[4]; B[2] = -Om[5]; B[3] = RC(0.0f); }
+}
+static real pi_omega(int lam, int mu, int nu, const real *pim, const real *Om, const real *u) {
+    real A[4] = {pim[IDX(0, lam)], -pim[IDX(1, lam)], -pim[IDX(2, lam)], -pim[IDX(3, lam)]};
+    real B[4];
+    omega_row(lam, Om, B);
+    real D[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) D[a][b] = (a == b ? GMN_DIAG[a] : RC(0.0f)) - u[a] * u[b];
+    real first = RC(0.5f) * (dot4(D[mu], A) * dot4(D[nu], B) + dot4(D[mu], B) * dot4(D[nu], A));
+    real temp[4] = {dot4(D[0], A), dot4(D[1], A), dot4(D[2], A), dot4(D[3], A)};
+    real second = RC(-1.0f) / RC(3.0f) * ((mu == nu ? GMN_DIAG[mu] : RC(0.0f)) - u[mu] * u[nu]) * dot4(temp, B);
+    return first + second;
+}
+static real omega_omega(int lam, int mu, int nu, const real *Om, const real *u) {
+    real B[4];
+    omega_row(lam, Om, B);
+    real D[4][4];
+    for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) D[a][b] = (a == b ? GMN_DIAG[a] : RC(0.0f)) - u[a] * u[b];
+    real first = dot4(D[mu], B) * dot4(D[nu], B);
+    real temp[4] = {dot4(D[0], B), dot4(D[1], B), dot4(D[2], B), dot4(D[3], B)};
+    real second = RC(-1.0f) / RC(3.0f) * ((mu == nu ? GMN_DIAG[mu] : RC(0.0f)) - u[mu] * u[nu]) * dot4(temp, B);
+    return first + second;
+}
+/* the six projected vorticity components (Omega^{01}, 02, 03, 12, 13, 23) of a cell */
+static void omega_cell(const real *udt, const real *ux, const real *uy, const real *uz, const real *u, real Om[6]) {
+    real om[6] = {udt[1] - ux[0], udt[2] - uy[0], udt[3] - uz[0], ux[2] - uy[1], ux[3] - uz[1], uy[3] - uz[2]};
+    Om[0] = omega_proj(0, 1, om, u); Om[1] = omega_proj(0, 2, om, u); Om[2] = omega_proj(0, 3, om, u);
+    Om[3] = omega_proj(1, 2, om, u); Om[4] = omega_proj(1, 3, om, u); Om[5] = omega_proj(2, 3, om, u);
+}
+/* test entry: the coupling tensors of one state, out[0..9] = sum over lam of
+ * g_lam,lam pi_omega(lam, mu, nu), out[10..19] the same of omega_omega, out[20..25] Omega */
+void orc_couplings(const real *pim, const real *udt, const real *ux, const real *uy, const real *uz,
+                   const real *u, real *out) {
+    real Om[6];
+    omega_cell(udt, ux, uy, uz, u, Om);
+    for (int mu = 0; mu < 4; mu++)
+        for (int nu = mu; nu < 4; nu++) {
+            int mn = IDX(mu, nu);
+            out[mn] = pi_omega(0, mu, nu, pim, Om, u) - pi_omega(1, mu, nu, pim, Om, u)
+                      - pi_omega(2, mu, nu, pim, Om, u) - pi_omega(3, mu, nu, pim, Om, u);
+            out[10 + mn] = omega_omega(0, mu, nu, Om, u) - omega_omega(1, mu, nu, Om, u)
+                           - omega_omega(2, mu, nu, Om, u) - omega_omega(3, mu, nu, Om, u);
+        }
+    for (int q = 0; q < 6; q++) out[20 + q] = Om[q];
+}
+
 /* update_pimn, kernel_IS.cl:501-709.  With the bulk extension, Pi is relaxed to
  * -zeta*theta with tau_Pi = 3 (zeta/s) hbarc / T and source Pi*theta/3 - Pi*u^tau/tau
  * (the conservative form of tau_Pi D Pi + Pi = -zeta theta - (2/3) tau_Pi Pi theta). */
@@ -1028,6 +1096,8 @@ void orc_update_pimn(const orc_params *p, real *pinew, real *goodcell, const rea
         }
         real pi2[10];
         for (int m = 0; m < 10; m++) pi2[m] = pistep[10 * I + m];
+        real Om[6];
+        if (p->couplings) omega_cell(udt, ux, uy, uz, u, Om);
         real max_pimn_abs = RC(0.0f);
         real out[10];
         for (int mu = 0; mu < 4; mu++) {
@@ -1047,6 +1117,12 @@ void orc_update_pimn(const orc_params *p, real *pinew, real *goodcell, const rea
                     src -= one_over_taupi * coef_pipi * (pipi(0, mu, nu, pi2, u) - pipi(1, mu, nu, pi2, u)
                                                          - pipi(2, mu, nu, pi2, u) - pipi(3, mu, nu, pi2, u));
                 }
+                if (p->couplings & 1)
+                    src += RC(2.0f) * (pi_omega(0, mu, nu, pi2, Om, u) - pi_omega(1, mu, nu, pi2, Om, u)
+                                       - pi_omega(2, mu, nu, pi2, Om, u) - pi_omega(3, mu, nu, pi2, Om, u));
+                if (p->couplings & 2)
+                    src -= RC(1.0f) * one_over_taupi * (omega_omega(0, mu, nu, Om, u) - omega_omega(1, mu, nu, Om, u)
+                                                        - omega_omega(2, mu, nu, Om, u) - omega_omega(3, mu, nu, Om, u));
                 Src[10 * I + mn] += src;
                 pi_old = DIV(pi_old + DIV(Src[10 * I + mn] * DT, (real)step) + DT * one_over_taupi * piNS,
                              u_new[0] + DT * one_over_taupi);

@@ -37,6 +37,7 @@ class OrcParams(C.Structure):
         ('fp64_root', C.c_int32), ('fp64_root_vmin', C.c_float),
         ('nthreads', C.c_int32),
         ('tau0_cfg', C.c_double), ('dt_cfg', C.c_double),
+        ('couplings', C.c_int32),
     ]
 
 
@@ -74,7 +75,7 @@ def _ptr(a):
 
 def make_params(nx, ny, nz, dt, dx, dy, dz, tau0, eos_table, etaos_xmin=0.18, etaos_ymin=0.08,
                 etaos_left_slop=0.0, etaos_right_slop=0.0, lam1=-10.0, riemann_test=False,
-                gubser_visc_test=False, bulk=None, fp64_root_vmin=None, nthreads=0):
+                gubser_visc_test=False, bulk=None, fp64_root_vmin=None, nthreads=0, couplings=0):
     """Build an OrcParams.  ``eos_table`` is a clvisc_amd.eos.EosTable; ``bulk`` is
     None or (zetaos_max, zetaos_tpeak, zetaos_width)."""
     p = OrcParams()
@@ -84,6 +85,7 @@ def make_params(nx, ny, nz, dt, dx, dy, dz, tau0, eos_table, etaos_xmin=0.18, et
     p.etaos_xmin, p.etaos_ymin = etaos_xmin, etaos_ymin
     p.etaos_left_slop, p.etaos_right_slop, p.lam1 = etaos_left_slop, etaos_right_slop, lam1
     p.riemann_test, p.gubser_visc_test = int(bool(riemann_test)), int(bool(gubser_visc_test))
+    p.couplings = int(couplings)     # 1: PIMUNU_OMEGA_COUPLING, 2: OMEGA_OMEGA_COUPLING
     t = eos_table
     p.eos_kind = t.kind_id
     p.eos_nrow, p.eos_ncol, p.eos_num_ed = t.nrow, t.ncol, t.num_ed
@@ -410,6 +412,20 @@ def vorticity(params, ev_old, ev_new, tau, precision='f32'):
     out = np.zeros((a.shape[0], 32), dt)
     L.orc_vorticity.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, rt, C.c_void_p]
     L.orc_vorticity(C.byref(params), _ptr(a), _ptr(b), tau, _ptr(out))
+    return out
+
+
+def couplings(pim, udt, ux, uy, uz, u, precision='f32'):
+    """The IS vorticity couplings of one state (orc_couplings, kernel_IS.cl:354-470):
+    (out[0:10] = sum_lam g_lam,lam pi_omega(lam, mu, nu), out[10:20] the same of omega_omega,
+    out[20:26] = Omega^{01, 02, 03, 12, 13, 23}); pim in the 10-component upper-triangle
+    order, the derivative rows and u as 4-vectors."""
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    a = [np.ascontiguousarray(np.asarray(x, dt)) for x in (pim, udt, ux, uy, uz, u)]
+    out = np.zeros(26, dt)
+    L.orc_couplings.argtypes = [C.c_void_p] * 7
+    L.orc_couplings(*[_ptr(x) for x in a], _ptr(out))
     return out
 
 

@@ -36,7 +36,9 @@ def orc_params(cvp, table, nthreads=0):
                          gubser_visc_test=bool(cvp.flags & _lib.CV_FLAG_GUBSER_VISC_TEST),
                          bulk=bulk,
                          fp64_root_vmin=cvp.fp64_root_vmin if cvp.flags & _lib.CV_FLAG_FP64_ROOT else None,
-                         nthreads=nthreads)
+                         nthreads=nthreads,
+                         couplings=(1 if cvp.flags & _lib.CV_FLAG_PIMN_OMEGA else 0)
+                         | (2 if cvp.flags & _lib.CV_FLAG_OMEGA_OMEGA else 0))
 
 
 def max_rel(a, b, floor_frac=1e-3):

@@ -296,6 +296,47 @@ def test_visc_steps_bulk_3d():
     _visc_compare(p, tab, ev, 20, 'viscous+bulk hotspots')
 
 
+@pytest.mark.parametrize('kernel', [(0, 0, 0), (1, 0, 0), (2, 0, 0)])
+@pytest.mark.parametrize('flags', [_lib.CV_FLAG_PIMN_OMEGA, _lib.CV_FLAG_OMEGA_OMEGA,
+                                   _lib.CV_FLAG_PIMN_OMEGA | _lib.CV_FLAG_OMEGA_OMEGA])
+def test_visc_steps_vorticity_couplings(kernel, flags):
+    """The IS vorticity couplings (pimn_omega_coupling / omega_omega_coupling, kernel_IS.cl:636-683)
+    on a viscous + bulk hotspot event: the per-cell kernel and the tiled kernels' test-branch
+    instance equal the oracle with the same terms bit for bit, and the terms move pi."""
+    p, tab, ev = _visc_case(29, 31, 11, bulk=(0.05, 0.155, 0.02))
+    p.flags |= flags
+    _say('case: vorticity couplings 0x%x, 29x31x11 viscous+bulk, 20 steps, kernel %s' % (flags, kernel))
+    h = _lib.Handle(p, tab, viscous=True)
+    h.set_stage_kernel(*kernel)
+    h.visc_initialize(p.tau0)
+    h.ideal_stage(1, np.float32(p.tau0))
+    h.set('ev1', ev)
+    h.visc_steps(0, 20)
+    o = O.Oracle(orc_params(p, tab, nthreads=16), 'f32', viscous=True)
+    o.load_ini(ev)
+    o.visc_steps(20)
+    for name, r in (('ev1', o.ev[1]), ('pi1', o.pi[1]), ('bulk1', o.bulk[1]), ('udiff', o.udiff)):
+        g = h.get(name)
+        _say('  %-6s bit-equal %.8f' % (name, bit_equal_frac(g, r)))
+        np.testing.assert_array_equal(g, r, err_msg=name)
+    h.close()
+    p0 = _lib.CvParams.from_buffer_copy(p)
+    p0.flags &= ~(_lib.CV_FLAG_PIMN_OMEGA | _lib.CV_FLAG_OMEGA_OMEGA)
+    o0 = O.Oracle(orc_params(p0, tab, nthreads=16), 'f32', viscous=True)
+    o0.load_ini(ev)
+    o0.visc_steps(20)
+    d = np.abs(o.pi[1].astype(np.float64) - o0.pi[1]).max() / np.abs(o0.pi[1]).max()
+    _say('  max |pi(couplings) - pi(none)| / max|pi| = %.3g' % d)
+    assert d > 1e-6
+
+
+def test_batch_rejects_vorticity_couplings():
+    p, tab, _ = _visc_case(13, 11, 5)
+    p.flags |= _lib.CV_FLAG_PIMN_OMEGA
+    with pytest.raises(RuntimeError):
+        _lib.Batch(p, tab, [(13, 11, 5), (9, 9, 3)])
+
+
 def test_visc_stage_api_matches_reference_sequence():
     """cv_is_stage / cv_visc_stage / cv_update_udiff called in the order of
     CLVisc.evolve (pyvisc/visc.py:527-544) give the fused step's result."""
