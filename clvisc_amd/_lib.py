@@ -120,6 +120,7 @@ def lib():
             'cv_last_elapsed_ms': [vp, C.POINTER(f32)],
             'cv_profile_visc_step': [vp, i64, C.POINTER(f32), C.c_int, C.POINTER(C.c_int32)],
             'cv_set_stage_kernel': [vp, C.c_int, C.c_int, C.c_int],
+            'cv_set_jet_source': [vp, C.c_int, C.c_float, C.c_int64],
             'cv_get_stage_kernel': [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
             'cv_set_march_chunk': [vp, C.c_int],
             'cv_get_march_chunk': [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)],
@@ -173,7 +174,7 @@ def exported_symbols():
             'cv_get_field', 'cv_copy_field', 'cv_gather', 'cv_visc_initialize', 'cv_ideal_stage', 'cv_is_stage',
             'cv_visc_stage', 'cv_update_udiff', 'cv_ideal_steps_async', 'cv_visc_steps_async',
             'cv_max_ed', 'cv_sync', 'cv_last_elapsed_ms', 'cv_profile_visc_step',
-            'cv_set_stage_kernel', 'cv_get_stage_kernel', 'cv_set_march_chunk', 'cv_get_march_chunk',
+            'cv_set_stage_kernel', 'cv_get_stage_kernel', 'cv_set_jet_source', 'cv_set_march_chunk', 'cv_get_march_chunk',
             'cv_hypersf_reserve', 'cv_hypersf_snapshot',
             'cv_hypersf_find', 'cv_hypersf_count', 'cv_hypersf_get', 'cv_hypersf_reset',
             'cv_batch_create', 'cv_batch_destroy', 'cv_batch_handle', 'cv_batch_set_active',
@@ -356,6 +357,15 @@ class Handle(object):
         nl = C.c_int32()
         check(self.L.cv_profile_visc_step(self.h, loop0, ms, 16, C.byref(nl)))
         return [ms[i] for i in range(nl.value)]
+
+    def set_jet_source(self, jet_eloss_src=None):
+        """jet_eloss_src = {'switch_on', 'start_pos_index', 'direction'} (pyvisc/ideal.py:211) or
+        None: the deposit of kernel_jet_eloss.cl added to every later stage (cv_set_jet_source)."""
+        if not jet_eloss_src or not jet_eloss_src.get('switch_on', False):
+            check(self.L.cv_set_jet_source(self.h, 0, 0.0, 0))
+        else:
+            check(self.L.cv_set_jet_source(self.h, 1, float(np.float32(jet_eloss_src.get('direction', 0.0))),
+                                           int(jet_eloss_src.get('start_pos_index', 0))))
 
     def set_stage_kernel(self, kind, ty=0, tz=0):
         check(self.L.cv_set_stage_kernel(self.h, kind, ty, tz))

@@ -55,6 +55,7 @@ static int fail(int code, const char *fmt, ...) {
 
 struct cv_handle {
     cv_params prm;
+    JetP jet = {};       // jet energy-loss source of the stage kernels (on = 0: none)
     int device;
     int viscous;
     int bulk;
@@ -451,7 +452,7 @@ static int launch_ideal(cv_handle *h, int step, float tau, float *ev_st, float *
     // x-marching tiles with the marching stage kernel's tile and chunks; the test flags, the
     // FP64 root and the iteration counts take the per-cell kernel
     if (h->kernel == 2 && h->phys.nb && !(h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_FP64_ROOT)) &&
-        !h->track_iters) {
+        !h->track_iters && !h->jet.on) {
         dim3 g((unsigned)h->tile.nwork), b(CV_ITILE_THREADS);
         if (h->eos.kind == 1)
             hipLaunchKernelGGL(k_ideal_tiled<1>, g, b, 0, h->stream, h->phys, h->eos, st, h->tile, ev_st, ev1,
@@ -463,7 +464,7 @@ static int launch_ideal(cv_handle *h, int step, float tau, float *ev_st, float *
         return CV_OK;
     }
     hipLaunchKernelGGL(k_ideal_stage, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, st,
-                       ev_st, ev1, h->Src, ev_out, nullptr, nullptr, h->track_iters ? h->iters : nullptr);
+                       ev_st, ev1, h->Src, ev_out, nullptr, nullptr, h->track_iters ? h->iters : nullptr, h->jet);
     HIP_TRY(hipGetLastError());
     return CV_OK;
 }
@@ -489,10 +490,10 @@ static int launch_kt(cv_handle *h, int step, float tau, float *ev_st, float *pi_
     StageP st = stage_params(h, step, tau);
     if (h->bulk)
         hipLaunchKernelGGL(k_kt_stage<true>, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, st,
-                           ev_st, pi_st, bk_st, ev1, pim0, bk0, pinew, bknew, h->Src, ev_out, udiff_out);
+                           ev_st, pi_st, bk_st, ev1, pim0, bk0, pinew, bknew, h->Src, ev_out, udiff_out, h->jet);
     else
         hipLaunchKernelGGL(k_kt_stage<false>, grid1d(h->n), dim3(256), 0, h->stream, h->phys, h->eos, st,
-                           ev_st, pi_st, bk_st, ev1, pim0, bk0, pinew, bknew, h->Src, ev_out, udiff_out);
+                           ev_st, pi_st, bk_st, ev1, pim0, bk0, pinew, bknew, h->Src, ev_out, udiff_out, h->jet);
     HIP_TRY(hipGetLastError());
     return CV_OK;
 }
@@ -532,7 +533,8 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
                         float *ev1, float *ev2, float *pi1, float *bk1, float *pi_out, float *bk_out,
                         float *ev_out, float *udiff_out) {
     StageP st = stage_params(h, step, tau);
-    if (h->kernel == 1 || h->kernel == 2) {
+    // the jet source runs in the per-cell kernel
+    if ((h->kernel == 1 || h->kernel == 2) && !h->jet.on) {
         const bool march = h->kernel == 2;
         dim3 g(march ? (unsigned)h->tile.nwork : (unsigned)h->tile_grid), b(CV_TILE_THREADS);
         const bool split = h->root_split;
@@ -585,10 +587,11 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
         }
         return CV_OK;
     }
+    if (h->root_split && h->jet.on) return fail(CV_ERR_UNSUPPORTED, "the jet source does not run with the split root solve");
     if (h->root_split) return fail(CV_ERR_UNSUPPORTED, "split root solve needs a tiled stage kernel (kind 1 or 2)");
     dim3 g = grid1d(h->n), b(256);
 #define CV_STAGE_ARGS h->phys, h->eos, st, ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src, \
-                      h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out
+                      h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out, h->jet
     if (h->bulk) {
         if (step == 1) hipLaunchKernelGGL((k_visc_stage<true, 1>), g, b, 0, h->stream, CV_STAGE_ARGS);
         else hipLaunchKernelGGL((k_visc_stage<true, 2>), g, b, 0, h->stream, CV_STAGE_ARGS);
@@ -723,6 +726,28 @@ extern "C" int cv_profile_visc_step(cv_handle *h, int64_t loop0, float *ms, int 
     }
     (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2);
     return rc;
+}
+
+extern "C" int cv_set_jet_source(cv_handle *h, int on, float direction, int64_t start_pos_index) {
+    if (!h) return fail(CV_ERR_ARG, "null handle");
+    if (!on) { h->jet = JetP{}; return CV_OK; }
+    if (start_pos_index < 0 || start_pos_index >= h->n) return fail(CV_ERR_ARG, "start_pos_index outside the grid");
+    if (h->root_split) return fail(CV_ERR_UNSUPPORTED, "the jet source does not run with the split root solve");
+    const int64_t nyz = (int64_t)h->prm.ny * h->prm.nz;
+    const int i0 = (int)(start_pos_index / nyz);
+    const int j0 = (int)((start_pos_index - i0 * nyz) / h->prm.nz);
+    const int k0 = (int)(start_pos_index - i0 * nyz - (int64_t)j0 * h->prm.nz);
+    JetP j;
+    j.on = 1;
+    j.c = (float)cos((double)direction);
+    j.sn = (float)sin((double)direction);
+    j.x0 = (float)(i0 - h->prm.nx / 2) * h->phys.dx;
+    j.y0 = (float)(j0 - h->prm.ny / 2) * h->phys.dy;
+    j.z0 = (float)(k0 - h->prm.nz / 2) * h->phys.dz;
+    j.c3 = (float)pow(sqrt(2.0 * M_PI), 3.0);
+    j.tau0 = (float)h->prm.tau0;
+    h->jet = j;
+    return CV_OK;
 }
 
 extern "C" int cv_set_stage_kernel(cv_handle *h, int kind, int ty, int tz) {
@@ -1782,13 +1807,13 @@ static int launch_ideal_seeded(cv_handle *h, const StageP &st, float *ev_st, flo
     const int64_t n = h->n;
     float *cons = h->seed_buf, *X = cons + 3 * n, *Y = cons + 7 * n, *seed = cons + 9 * n;
     hipLaunchKernelGGL(k_ideal_stage, grid1d(n), dim3(256), 0, h->stream, h->phys, h->eos, st, ev_st, ev1, h->Src,
-                       ev_out, cons, nullptr, nullptr);
+                       ev_out, cons, nullptr, nullptr, h->jet);
     hipLaunchKernelGGL(k_seed_features, grid1d(n), dim3(256), 0, h->stream, cons, cons + n, cons + 2 * n, n, X);
     const int64_t nblk = (n + MLP_SAMPLES_PER_BLOCK - 1) / MLP_SAMPLES_PER_BLOCK;
     hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)nblk), dim3(64 * MLP_WAVES), 0, h->stream, h->seed_mlp->P, X, n, Y);
     hipLaunchKernelGGL(k_take_col, grid1d(n), dim3(256), 0, h->stream, Y, n, 2, 1, seed);
     hipLaunchKernelGGL(k_ideal_stage, grid1d(n), dim3(256), 0, h->stream, h->phys, h->eos, st, ev_st, ev1, h->Src,
-                       ev_out, nullptr, seed, h->track_iters ? h->iters : nullptr);
+                       ev_out, nullptr, seed, h->track_iters ? h->iters : nullptr, h->jet);
     HIP_TRY(hipGetLastError());
     return CV_OK;
 }
@@ -1818,6 +1843,7 @@ extern "C" int cv_set_root_split(cv_handle *h, int on) {
     HIP_TRY(hipSetDevice(h->device));
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (on && !h->seed_buf) HIP_TRY(hipMalloc((void **)&h->seed_buf, sizeof(float) * 10 * h->n));
+    if (on && h->jet.on) return fail(CV_ERR_UNSUPPORTED, "the jet source does not run with the split root solve");
     h->root_split = on != 0;
     return CV_OK;
 }

@@ -52,6 +52,13 @@ struct StageP {
     int step;
 };
 
+// jet energy-loss source (kernel_jet_eloss.cl:8-69): on = 0 adds nothing.  c, sn: cos, sin of
+// the direction; x0, y0, z0: coordinates of the start cell; c3 = (2 pi)^(3/2); tau0: TAU0.
+struct JetP {
+    int on;
+    float c, sn, x0, y0, z0, c3, tau0;
+};
+
 // ---------------------------------------------------------------------------
 // Elementary operations
 // ---------------------------------------------------------------------------

@@ -157,6 +157,35 @@ __device__ __forceinline__ void ideal_axis(const PhysP &p, const Eos &eos, const
     }
 }
 
+// jet_eloss_src (kernel_jet_eloss.cl:8-69), added to the stage source after the KT sweeps
+// (ideal.py:240-246, visc.py:265-271): a Gaussian of width 0.3 fm around the jet, which moves
+// from the start cell at the speed of light; dE/dx = 14 GeV/fm (T/0.3 GeV)^3 with T at the jet's
+// cell of ev_st.  exp and the cube are taken in FP64 and rounded once (as the oracle does).
+__device__ __noinline__ void jet_src(const PhysP &p, const Eos &eos, const JetP &j, float tau,
+                                     const float *__restrict__ ev_st, const Cell &cl, float *s) {
+    const float sig = 0.3f;
+    const float dtau = tau - j.tau0;
+    const float px = j.x0 + j.c * dtau, py = j.y0 + j.sn * dtau, pz = j.z0 + 0.0f * dtau;
+    const int li = (int)(px * p.r_dx + (float)(p.nx / 2));
+    const int lj = (int)(py * p.r_dy + (float)(p.ny / 2));
+    const int lk = (int)(pz * p.r_dz + (float)(p.nz / 2));
+    if (!(li >= 0 && li < p.nx && lj >= 0 && lj < p.ny && lk >= 0 && lk < p.nz)) return;
+    const float x = (float)(cl.i - p.nx / 2) * p.dx;
+    const float y = (float)(cl.j - p.ny / 2) * p.dy;
+    const float z = (float)(cl.k - p.nz / 2) * p.dz;
+    const float norm = (float)(1.0 / (double)(j.c3 * sig * sig * sig * tau));
+    const float r2 = rcp(2.0f * sig * sig);
+    const float g = (-(x - px)) * (x - px) * r2 - (y - py) * (y - py) * r2 - (z - pz) * (z - pz) * r2;
+    const float f = norm * (float)exp((double)g);
+    const float coef[4] = {tau * f, tau * f, tau * f, (tau * tau) * f};
+    const float e = ev_st[(int64_t)li * p.sx + (int64_t)lj * p.sy + lk];
+    const double tr = (double)(eos.T(e) * rcp(0.3f));
+    const float dEdx = 14.0f * (float)(tr * tr * tr);
+    const float dE[4] = {dEdx, dEdx * j.c, dEdx * j.sn, 0.0f};
+#pragma unroll
+    for (int q = 0; q < 4; q++) s[q] = s[q] + coef[q] * dE[q];
+}
+
 // ---------------------------------------------------------------------------
 // CLIdeal.stepUpdate(step): christoffel + 3 sweeps + update_ev, one cell per thread
 // ---------------------------------------------------------------------------
@@ -166,8 +195,8 @@ __device__ __forceinline__ void ideal_axis(const PhysP &p, const Eos &eos, const
 __global__ __launch_bounds__(256) void k_ideal_stage(PhysP p, EosP E, StageP st,
                                                      const float *__restrict__ ev_st,
                                                      const float *ev1, float *Src, float *ev_out,
-                                                     float *cons_out = nullptr, const float *seed_v = nullptr,
-                                                     int *iters_out = nullptr) {
+                                                     float *cons_out, const float *seed_v,
+                                                     int *iters_out, JetP jet) {
     int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= p.n) return;
     Eos eos(E, E.tab);
@@ -188,6 +217,7 @@ __global__ __launch_bounds__(256) void k_ideal_stage(PhysP p, EosP E, StageP st,
     ideal_axis<0>(p, eos, st, cl, ev_st, s);
     ideal_axis<1>(p, eos, st, cl, ev_st, s);
     ideal_axis<2>(p, eos, st, cl, ev_st, s);
+    if (jet.on) jet_src(p, eos, jet, st.tau, ev_st, cl, s);
     if (!cons_out) st4(Src, p.n, I, make_float4(s[0], s[1], s[2], s[3]));
     // update_ev (kernel_ideal.cl:165-231)
     float4 e = ld4(ev1, p.n, I);
@@ -639,7 +669,7 @@ template <bool BULK>
 __global__ __launch_bounds__(256) void k_kt_stage(PhysP p, EosP E, StageP st,
         const float *__restrict__ ev_st, const float *__restrict__ pi_st, const float *__restrict__ bk_st,
         const float *ev1, const float *pim0, const float *bk0, const float *pinew, const float *bknew,
-        float *Src, float *ev_out, float *udiff) {
+        float *Src, float *ev_out, float *udiff, JetP jet) {
     int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= p.n) return;
     Eos eos(E, E.tab);
@@ -667,6 +697,7 @@ __global__ __launch_bounds__(256) void k_kt_stage(PhysP p, EosP E, StageP st,
     kt_axis<0, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s);
     kt_axis<1, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s);
     kt_axis<2, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s);
+    if (jet.on) jet_src(p, eos, jet, st.tau, ev_st, cl, s);
     st4(Src, p.n, I, make_float4(s[0], s[1], s[2], s[3]));
     // update_ev (kernel_visc.cl:322-393)
     float4 e = ld4(ev1, p.n, I);
@@ -830,7 +861,7 @@ __global__ __launch_bounds__(256, CV_STAGE_WAVES) void k_visc_stage(PhysP p, Eos
         const float *__restrict__ ev_st, const float *__restrict__ pi_st, const float *__restrict__ bk_st,
         const float *ev1, const float *ev2, const float *pi1, const float *bk1,
         const float *__restrict__ udiff_in, float *IS_src, float *bsrc, float *Src,
-        float *pi_out, float *bk_out, float *ev_out, float *good, float *udiff_out) {
+        float *pi_out, float *bk_out, float *ev_out, float *good, float *udiff_out, JetP jet) {
     int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= p.n) return;
     Eos eos(E, E.tab);
@@ -887,6 +918,7 @@ __global__ __launch_bounds__(256, CV_STAGE_WAVES) void k_visc_stage(PhysP p, Eos
     CV_PHASE_FENCE();
     visc_axis<2, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s, bs, t, udz);
     CV_PHASE_FENCE();
+    if (jet.on) jet_src(p, eos, jet, st.tau, ev_st, cl, t);
 
     // ---- update_pimn (kernel_IS.cl:501-709) ----
     const float4 e1 = STEP == 1 ? ec : ld4(ev1, n, I);

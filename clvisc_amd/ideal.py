@@ -133,8 +133,7 @@ class CLIdeal(object):
 
     def stepUpdate(self, step, jet_eloss_src=None):
         """One RK2 stage at self.tau (pyvisc/ideal.py:211-255)."""
-        if jet_eloss_src is not None and jet_eloss_src.get('switch_on', False):
-            raise NotImplementedError('the jet energy-loss source is not part of this hydro path')
+        self.handle.set_jet_source(jet_eloss_src)
         self.handle.ideal_stage(step, self.cfg.real(self.tau))
 
     def max_energy_density(self):
@@ -171,8 +170,7 @@ class CLIdeal(object):
                jet_eloss_src=None):
         """Main loop (pyvisc/ideal.py:322-349): max e every step, surface every ntskip,
         bulk info every ntskip, then both RK2 stages."""
-        if jet_eloss_src is not None and jet_eloss_src.get('switch_on', False):
-            raise NotImplementedError('the jet energy-loss source is not part of this hydro path')
+        self.handle.set_jet_source(jet_eloss_src)
         for n in range(max_loops):
             t0 = time.time()
             self.edmax = self.max_energy_density()

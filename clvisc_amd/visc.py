@@ -59,8 +59,7 @@ class CLVisc(object):
     def visc_stepUpdate(self, step, jet_eloss_src=None):
         """KT source with pi and the primitive update into d_ev[3-step]
         (pyvisc/visc.py:276-314)."""
-        if jet_eloss_src is not None and jet_eloss_src.get('switch_on', False):
-            raise NotImplementedError('the jet energy-loss source is not part of this hydro path')
+        self.handle.set_jet_source(jet_eloss_src)
         self.handle.visc_stage(step, self.cfg.real(self.ideal.tau))
 
     def update_udiff(self, d_ev0, d_ev1):
@@ -165,12 +164,12 @@ class CLVisc(object):
         until max e < e_frz, as the reference does (max_loops is not consulted)."""
         if save_vorticity:
             self.handle.vorticity_enable()
-        if jet_eloss_src is not None and jet_eloss_src.get('switch_on', False):
-            raise NotImplementedError('the jet energy-loss source is not part of this hydro path')
+        self.handle.set_jet_source(jet_eloss_src)
         cfg = self.cfg
         if cfg.ETAOS_YMIN < 1.0E-6 and cfg.ETAOS_LEFT_SLOP < 1.0E-6 and cfg.ETAOS_RIGHT_SLOP < 1.0E-6 \
                 and not self.bulk_on:
-            self.ideal.evolve(max_loops, save_hypersf, save_bulk, plot_bulk, force_run_to_maxloop)
+            self.ideal.evolve(max_loops, save_hypersf, save_bulk, plot_bulk, force_run_to_maxloop,
+                              jet_eloss_src)
             self.save_pimn_sf(set_to_zero=True)
             return
         if save_pi:

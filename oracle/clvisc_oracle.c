@@ -100,6 +100,9 @@ typedef struct orc_params {
     int32_t nthreads;          /* OpenMP threads, 0 = runtime default */
     double tau0_cfg, dt_cfg;   /* TAU0, DT as the Python config holds them (double) */
     int32_t couplings;         /* 1: -D PIMUNU_OMEGA_COUPLING, 2: -D OMEGA_OMEGA_COUPLING */
+    int32_t jet_on;            /* jet_eloss_src['switch_on'] */
+    float jet_direction;       /* jet_eloss_src['direction'] (cfg.real) */
+    int64_t jet_start;         /* jet_eloss_src['start_pos_index'] */
 } orc_params;
 
 static int64_t ncells(const orc_params *p) { return (int64_t)p->nx * p->ny * p->nz; }
@@ -1196,10 +1199,50 @@ typedef struct orc_state {
     int32_t *iters;   /* optional: root-solve iteration counts of the last update_ev */
 } orc_state;
 
+/* jet_eloss_src, kernel_jet_eloss.cl:8-69 (called at ideal.py:240-246, visc.py:265-271 after
+ * the KT sources): Src += (tau, tau, tau, tau^2) N exp(-|x - X(tau)|^2 / (2 sigma^2))
+ * (dE/dx)(1, cos phi, sin phi, 0), sigma = 0.3 fm, N = 1 / ((2 pi)^(3/2) sigma^3 tau),
+ * X(tau) = X(start cell) + (tau - TAU0)(cos phi, sin phi, 0), dE/dx = 14 (T/0.3)^3 with T at the
+ * cell of X(tau) in ev; nothing when X(tau) is off the grid.  Divisions by constants through the
+ * reciprocal; exp and the cube in double, rounded once. */
+void orc_jet_src(const orc_params *p, real *Src, const real *ev, real tau) {
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const int64_t nyz = (int64_t)ny * nz, st = p->jet_start;
+    const int i0 = (int)(st / nyz), j0 = (int)((st - i0 * nyz) / nz), k0 = (int)(st - i0 * nyz - (int64_t)j0 * nz);
+    const real c = (real)cos((double)p->jet_direction), sn = (real)sin((double)p->jet_direction);
+    const real c3 = (real)(float)pow(sqrt(2.0 * M_PI), 3.0);
+    const real sig = RC(0.3f), dx = RC(p->dx), dy = RC(p->dy), dz = RC(p->dz);
+    const real dtau = tau - RC(p->tau0);
+    const real px = (real)(float)((float)(i0 - nx / 2) * p->dx) + c * dtau;
+    const real py = (real)(float)((float)(j0 - ny / 2) * p->dy) + sn * dtau;
+    const real pz = (real)(float)((float)(k0 - nz / 2) * p->dz) + RC(0.0f) * dtau;
+    const int li = (int)(px * (RC(1.0f) / dx) + (real)(nx / 2));
+    const int lj = (int)(py * (RC(1.0f) / dy) + (real)(ny / 2));
+    const int lk = (int)(pz * (RC(1.0f) / dz) + (real)(nz / 2));
+    if (!(li >= 0 && li < nx && lj >= 0 && lj < ny && lk >= 0 && lk < nz)) return;
+    const real e = ev[4 * ((int64_t)li * nyz + (int64_t)lj * nz + lk)];
+    const double tr = (double)(eos_T(p, e) * (RC(1.0f) / RC(0.3f)));
+    const real dEdx = RC(14.0f) * (real)(tr * tr * tr);
+    const real dE[4] = {dEdx, dEdx * c, dEdx * sn, RC(0.0f)};
+    const real norm = (real)(1.0 / (double)(c3 * sig * sig * sig * tau));
+    const real r2 = RC(1.0f) / (RC(2.0f) * sig * sig);
+    set_threads(p);
+    #pragma omp parallel for schedule(static)
+    for (int64_t I = 0; I < ncells(p); I++) {
+        const int i = (int)(I / nyz), j = (int)((I - i * nyz) / nz), k = (int)(I - i * nyz - (int64_t)j * nz);
+        const real x = (real)(i - nx / 2) * dx, y = (real)(j - ny / 2) * dy, z = (real)(k - nz / 2) * dz;
+        const real g = (-(x - px)) * (x - px) * r2 - (y - py) * (y - py) * r2 - (z - pz) * (z - pz) * r2;
+        const real f = norm * (real)exp((double)g);
+        const real coef[4] = {tau * f, tau * f, tau * f, (tau * tau) * f};
+        for (int q = 0; q < 4; q++) Src[4 * I + q] = Src[4 * I + q] + coef[q] * dE[q];
+    }
+}
+
 /* CLIdeal.stepUpdate(step), ideal.py:211-255 */
 void orc_ideal_stage(const orc_params *p, orc_state *s, int step, real tau) {
     orc_ideal_christoffel(p, s->Src, s->ev[step], tau, step);
     for (int a = 0; a < 3; a++) orc_ideal_along(p, a, s->Src, s->ev[step], tau);
+    if (p->jet_on) orc_jet_src(p, s->Src, s->ev[step], tau);
     orc_ideal_update_ev(p, s->ev[3 - step], s->ev[1], s->Src, tau, step, s->iters);
 }
 
@@ -1224,6 +1267,7 @@ void orc_visc_stage(const orc_params *p, orc_state *s, int step, real tau) {
     orc_visc_christoffel(p, s->Src, s->ev[step], s->pi[step], bulk ? s->bulk[step] : NULL, tau, step);
     for (int a = 0; a < 3; a++)
         orc_visc_along(p, a, s->Src, s->ev[step], s->pi[step], bulk ? s->bulk[step] : NULL, tau);
+    if (p->jet_on) orc_jet_src(p, s->Src, s->ev[step], tau);
     orc_visc_update_ev(p, s->ev[3 - step], s->ev[1], s->pi[0], s->pi[3 - step],
                        bulk ? s->bulk[0] : NULL, bulk ? s->bulk[3 - step] : NULL,
                        s->Src, tau, step, s->iters);

@@ -38,6 +38,7 @@ class OrcParams(C.Structure):
         ('nthreads', C.c_int32),
         ('tau0_cfg', C.c_double), ('dt_cfg', C.c_double),
         ('couplings', C.c_int32),
+        ('jet_on', C.c_int32), ('jet_direction', C.c_float), ('jet_start', C.c_int64),
     ]
 
 
@@ -75,7 +76,8 @@ def _ptr(a):
 
 def make_params(nx, ny, nz, dt, dx, dy, dz, tau0, eos_table, etaos_xmin=0.18, etaos_ymin=0.08,
                 etaos_left_slop=0.0, etaos_right_slop=0.0, lam1=-10.0, riemann_test=False,
-                gubser_visc_test=False, bulk=None, fp64_root_vmin=None, nthreads=0, couplings=0):
+                gubser_visc_test=False, bulk=None, fp64_root_vmin=None, nthreads=0, couplings=0,
+                jet=None):
     """Build an OrcParams.  ``eos_table`` is a clvisc_amd.eos.EosTable; ``bulk`` is
     None or (zetaos_max, zetaos_tpeak, zetaos_width)."""
     p = OrcParams()
@@ -86,6 +88,8 @@ def make_params(nx, ny, nz, dt, dx, dy, dz, tau0, eos_table, etaos_xmin=0.18, et
     p.etaos_left_slop, p.etaos_right_slop, p.lam1 = etaos_left_slop, etaos_right_slop, lam1
     p.riemann_test, p.gubser_visc_test = int(bool(riemann_test)), int(bool(gubser_visc_test))
     p.couplings = int(couplings)     # 1: PIMUNU_OMEGA_COUPLING, 2: OMEGA_OMEGA_COUPLING
+    if jet is not None:              # (direction, start_pos_index) of jet_eloss_src
+        p.jet_on, p.jet_direction, p.jet_start = 1, float(jet[0]), int(jet[1])
     t = eos_table
     p.eos_kind = t.kind_id
     p.eos_nrow, p.eos_ncol, p.eos_num_ed = t.nrow, t.ncol, t.num_ed
@@ -413,6 +417,18 @@ def vorticity(params, ev_old, ev_new, tau, precision='f32'):
     L.orc_vorticity.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, rt, C.c_void_p]
     L.orc_vorticity(C.byref(params), _ptr(a), _ptr(b), tau, _ptr(out))
     return out
+
+
+def jet_src(params, Src, ev, tau, precision='f32'):
+    """Src (n, 4) plus the jet_eloss_src deposit at tau (orc_jet_src, kernel_jet_eloss.cl:8-69)."""
+    L = lib(precision)
+    dt = np.float32 if precision == 'f32' else np.float64
+    rt = C.c_float if precision == 'f32' else C.c_double
+    s = np.ascontiguousarray(np.array(Src, dt).reshape(-1, 4))
+    e = np.ascontiguousarray(np.asarray(ev, dt).reshape(-1, 4))
+    L.orc_jet_src.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, rt]
+    L.orc_jet_src(C.byref(params), _ptr(s), _ptr(e), tau)
+    return s
 
 
 def couplings(pim, udt, ux, uy, uz, u, precision='f32'):

@@ -330,6 +330,47 @@ def test_visc_steps_vorticity_couplings(kernel, flags):
     assert d > 1e-6
 
 
+@pytest.mark.parametrize('viscous', [False, True])
+@pytest.mark.parametrize('kind', [0, 2])
+def test_jet_source_matches_oracle(viscous, kind):
+    """jet_eloss_src (kernel_jet_eloss.cl:8-69) switched on for every stage: ideal and viscous +
+    bulk steps (the x-marching handle runs jet stages per cell) equal the oracle with the same
+    deposit bit for bit, and the deposit moves the fields."""
+    nx, ny, nz, nsteps = 29, 31, 11, 15
+    p, tab, ev = _visc_case(nx, ny, nz, bulk=(0.05, 0.155, 0.02) if viscous else None)
+    direction, start = np.float32(0.7), (14 * ny + 15) * nz + 5
+    _say('case: jet source %s, kind %d, %dx%dx%d, %d steps, start %d direction %.2f'
+         % ('viscous+bulk' if viscous else 'ideal', kind, nx, ny, nz, nsteps, start, direction))
+    if not viscous:
+        p.flags &= ~_lib.CV_FLAG_BULK
+    h = _lib.Handle(p, tab, viscous=viscous)
+    h.set_stage_kernel(kind, 0, 0)
+    jet = {'switch_on': True, 'start_pos_index': start, 'direction': direction}
+    if viscous:
+        h.visc_initialize(p.tau0)
+        h.ideal_stage(1, np.float32(p.tau0))
+    h.set('ev1', ev)
+    h.set_jet_source(jet)
+    (h.visc_steps if viscous else h.ideal_steps)(0, nsteps)
+    prm = orc_params(p, tab, nthreads=16)
+    prm.jet_on, prm.jet_direction, prm.jet_start = 1, float(direction), start
+    o = O.Oracle(prm, 'f32', viscous=viscous)
+    o.load_ini(ev)
+    (o.visc_steps if viscous else o.ideal_steps)(nsteps)
+    pairs = [('ev1', o.ev[1])] + ([('pi1', o.pi[1]), ('bulk1', o.bulk[1])] if viscous else [])
+    for name, r in pairs:
+        g = h.get(name)
+        _say('  %-6s bit-equal %.8f' % (name, bit_equal_frac(g, r)))
+        np.testing.assert_array_equal(g, r, err_msg=name)
+    h.close()
+    o0 = O.Oracle(orc_params(p, tab, nthreads=16), 'f32', viscous=viscous)
+    o0.load_ini(ev)
+    (o0.visc_steps if viscous else o0.ideal_steps)(nsteps)
+    d = np.abs(o.ev[1][:, 0].astype(np.float64) - o0.ev[1][:, 0]).max() / np.abs(o0.ev[1][:, 0]).max()
+    _say('  max |e(jet) - e(none)| / max e = %.3g' % d)
+    assert d > 1e-4
+
+
 def test_batch_rejects_vorticity_couplings():
     p, tab, _ = _visc_case(13, 11, 5)
     p.flags |= _lib.CV_FLAG_PIMN_OMEGA
