@@ -358,13 +358,14 @@ class Handle(object):
         check(self.L.cv_profile_visc_step(self.h, loop0, ms, 16, C.byref(nl)))
         return [ms[i] for i in range(nl.value)]
 
-    def set_jet_source(self, jet_eloss_src=None):
+    def set_jet_source(self, jet_eloss_src=None, stages=2):
         """jet_eloss_src = {'switch_on', 'start_pos_index', 'direction'} (pyvisc/ideal.py:211) or
-        None: the deposit of kernel_jet_eloss.cl added to every later stage (cv_set_jet_source)."""
+        None: the deposit of kernel_jet_eloss.cl added to the later RK stages in the mask
+        ``stages`` (1: stage 1, 2: stage 2; evolve passes it to stage 2 only) (cv_set_jet_source)."""
         if not jet_eloss_src or not jet_eloss_src.get('switch_on', False):
             check(self.L.cv_set_jet_source(self.h, 0, 0.0, 0))
         else:
-            check(self.L.cv_set_jet_source(self.h, 1, float(np.float32(jet_eloss_src.get('direction', 0.0))),
+            check(self.L.cv_set_jet_source(self.h, int(stages), float(np.float32(jet_eloss_src.get('direction', 0.0))),
                                            int(jet_eloss_src.get('start_pos_index', 0))))
 
     def set_stage_kernel(self, kind, ty=0, tz=0):

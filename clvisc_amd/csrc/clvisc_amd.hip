@@ -731,6 +731,7 @@ extern "C" int cv_profile_visc_step(cv_handle *h, int64_t loop0, float *ms, int 
 extern "C" int cv_set_jet_source(cv_handle *h, int on, float direction, int64_t start_pos_index) {
     if (!h) return fail(CV_ERR_ARG, "null handle");
     if (!on) { h->jet = JetP{}; return CV_OK; }
+    if (on < 0 || on > 3) return fail(CV_ERR_ARG, "on must be a mask of the RK stages (1, 2 or 3)");
     if (start_pos_index < 0 || start_pos_index >= h->n) return fail(CV_ERR_ARG, "start_pos_index outside the grid");
     if (h->root_split) return fail(CV_ERR_UNSUPPORTED, "the jet source does not run with the split root solve");
     const int64_t nyz = (int64_t)h->prm.ny * h->prm.nz;
@@ -738,7 +739,7 @@ extern "C" int cv_set_jet_source(cv_handle *h, int on, float direction, int64_t 
     const int j0 = (int)((start_pos_index - i0 * nyz) / h->prm.nz);
     const int k0 = (int)(start_pos_index - i0 * nyz - (int64_t)j0 * h->prm.nz);
     JetP j;
-    j.on = 1;
+    j.on = on;
     j.c = (float)cos((double)direction);
     j.sn = (float)sin((double)direction);
     j.x0 = (float)(i0 - h->prm.nx / 2) * h->phys.dx;

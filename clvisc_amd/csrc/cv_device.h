@@ -52,7 +52,8 @@ struct StageP {
     int step;
 };
 
-// jet energy-loss source (kernel_jet_eloss.cl:8-69): on = 0 adds nothing.  c, sn: cos, sin of
+// jet energy-loss source (kernel_jet_eloss.cl:8-69): on = mask of the RK stages that add it
+// (1: stage 1, 2: stage 2), 0 adds nothing.  c, sn: cos, sin of
 // the direction; x0, y0, z0: coordinates of the start cell; c3 = (2 pi)^(3/2); tau0: TAU0.
 struct JetP {
     int on;

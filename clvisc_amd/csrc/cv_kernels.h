@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void k_ideal_stage(PhysP p, EosP E, StageP st,
     ideal_axis<0>(p, eos, st, cl, ev_st, s);
     ideal_axis<1>(p, eos, st, cl, ev_st, s);
     ideal_axis<2>(p, eos, st, cl, ev_st, s);
-    if (jet.on) jet_src(p, eos, jet, st.tau, ev_st, cl, s);
+    if (jet.on & st.step) jet_src(p, eos, jet, st.tau, ev_st, cl, s);
     if (!cons_out) st4(Src, p.n, I, make_float4(s[0], s[1], s[2], s[3]));
     // update_ev (kernel_ideal.cl:165-231)
     float4 e = ld4(ev1, p.n, I);
@@ -697,7 +697,7 @@ __global__ __launch_bounds__(256) void k_kt_stage(PhysP p, EosP E, StageP st,
     kt_axis<0, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s);
     kt_axis<1, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s);
     kt_axis<2, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s);
-    if (jet.on) jet_src(p, eos, jet, st.tau, ev_st, cl, s);
+    if (jet.on & st.step) jet_src(p, eos, jet, st.tau, ev_st, cl, s);
     st4(Src, p.n, I, make_float4(s[0], s[1], s[2], s[3]));
     // update_ev (kernel_visc.cl:322-393)
     float4 e = ld4(ev1, p.n, I);
@@ -918,7 +918,7 @@ __global__ __launch_bounds__(256, CV_STAGE_WAVES) void k_visc_stage(PhysP p, Eos
     CV_PHASE_FENCE();
     visc_axis<2, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s, bs, t, udz);
     CV_PHASE_FENCE();
-    if (jet.on) jet_src(p, eos, jet, st.tau, ev_st, cl, t);
+    if (jet.on & st.step) jet_src(p, eos, jet, st.tau, ev_st, cl, t);
 
     // ---- update_pimn (kernel_IS.cl:501-709) ----
     const float4 e1 = STEP == 1 ? ec : ld4(ev1, n, I);

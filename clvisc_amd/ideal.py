@@ -133,7 +133,7 @@ class CLIdeal(object):
 
     def stepUpdate(self, step, jet_eloss_src=None):
         """One RK2 stage at self.tau (pyvisc/ideal.py:211-255)."""
-        self.handle.set_jet_source(jet_eloss_src)
+        self.handle.set_jet_source(jet_eloss_src, stages=3)
         self.handle.ideal_stage(step, self.cfg.real(self.tau))
 
     def max_energy_density(self):

@@ -59,7 +59,7 @@ class CLVisc(object):
     def visc_stepUpdate(self, step, jet_eloss_src=None):
         """KT source with pi and the primitive update into d_ev[3-step]
         (pyvisc/visc.py:276-314)."""
-        self.handle.set_jet_source(jet_eloss_src)
+        self.handle.set_jet_source(jet_eloss_src, stages=3)
         self.handle.visc_stage(step, self.cfg.real(self.ideal.tau))
 
     def update_udiff(self, d_ev0, d_ev1):

@@ -252,11 +252,13 @@ int cv_profile_visc_step(cv_handle *h, int64_t loop0, float *ms, int nms, int32_
 int cv_set_stage_kernel(cv_handle *h, int kind, int ty, int tz);
 /* Jet energy-loss source (jet_eloss_src of pyvisc/kernel/kernel_jet_eloss.cl:8-69, passed as
  * the dict {'switch_on', 'direction', 'start_pos_index'} to CLIdeal.stepUpdate / CLVisc.evolve,
- * ideal.py:211-246, visc.py:232-271): with on != 0 every later stage adds, after its KT
+ * ideal.py:211-246, visc.py:232-271): on is a mask of the RK stages that add it (1: stage 1,
+ * 2: stage 2, 3: both; 0: none).  The reference's evolve passes it to stage 2 only
+ * (ideal.py:344, visc.py:543): on = 2 for the steps calls.  Each such stage adds, after its KT
  * sources, a Gaussian (0.3 fm) deposit of dE/dx = 14 GeV/fm (T/0.3 GeV)^3 around a point that
  * leaves cell start_pos_index (i NY NZ + j NZ + k) at tau0 along (cos direction, sin direction, 0)
- * at the speed of light.  Stages with it run in the per-cell kernels.  CV_ERR_ARG for a start
- * outside the grid, CV_ERR_UNSUPPORTED with cv_set_root_split. */
+ * at the speed of light.  With a jet set, stages run in the per-cell kernels.  CV_ERR_ARG for a
+ * start outside the grid or a mask above 3, CV_ERR_UNSUPPORTED with cv_set_root_split. */
 int cv_set_jet_source(cv_handle *h, int on, float direction, int64_t start_pos_index);
 int cv_get_stage_kernel(cv_handle *h, int32_t *kind, int32_t *ty, int32_t *tz);
 /* x planes per march chunk of kind 2 (0: automatic, about 32 work items per CU). */

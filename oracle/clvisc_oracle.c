@@ -100,7 +100,7 @@ typedef struct orc_params {
     int32_t nthreads;          /* OpenMP threads, 0 = runtime default */
     double tau0_cfg, dt_cfg;   /* TAU0, DT as the Python config holds them (double) */
     int32_t couplings;         /* 1: -D PIMUNU_OMEGA_COUPLING, 2: -D OMEGA_OMEGA_COUPLING */
-    int32_t jet_on;            /* jet_eloss_src['switch_on'] */
+    int32_t jet_on;            /* RK stages given jet_eloss_src (1: stage 1, 2: stage 2) */
     float jet_direction;       /* jet_eloss_src['direction'] (cfg.real) */
     int64_t jet_start;         /* jet_eloss_src['start_pos_index'] */
 } orc_params;
@@ -1242,7 +1242,7 @@ void orc_jet_src(const orc_params *p, real *Src, const real *ev, real tau) {
 void orc_ideal_stage(const orc_params *p, orc_state *s, int step, real tau) {
     orc_ideal_christoffel(p, s->Src, s->ev[step], tau, step);
     for (int a = 0; a < 3; a++) orc_ideal_along(p, a, s->Src, s->ev[step], tau);
-    if (p->jet_on) orc_jet_src(p, s->Src, s->ev[step], tau);
+    if (p->jet_on & step) orc_jet_src(p, s->Src, s->ev[step], tau);
     orc_ideal_update_ev(p, s->ev[3 - step], s->ev[1], s->Src, tau, step, s->iters);
 }
 
@@ -1267,7 +1267,7 @@ void orc_visc_stage(const orc_params *p, orc_state *s, int step, real tau) {
     orc_visc_christoffel(p, s->Src, s->ev[step], s->pi[step], bulk ? s->bulk[step] : NULL, tau, step);
     for (int a = 0; a < 3; a++)
         orc_visc_along(p, a, s->Src, s->ev[step], s->pi[step], bulk ? s->bulk[step] : NULL, tau);
-    if (p->jet_on) orc_jet_src(p, s->Src, s->ev[step], tau);
+    if (p->jet_on & step) orc_jet_src(p, s->Src, s->ev[step], tau);
     orc_visc_update_ev(p, s->ev[3 - step], s->ev[1], s->pi[0], s->pi[3 - step],
                        bulk ? s->bulk[0] : NULL, bulk ? s->bulk[3 - step] : NULL,
                        s->Src, tau, step, s->iters);

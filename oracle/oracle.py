@@ -88,8 +88,8 @@ def make_params(nx, ny, nz, dt, dx, dy, dz, tau0, eos_table, etaos_xmin=0.18, et
     p.etaos_left_slop, p.etaos_right_slop, p.lam1 = etaos_left_slop, etaos_right_slop, lam1
     p.riemann_test, p.gubser_visc_test = int(bool(riemann_test)), int(bool(gubser_visc_test))
     p.couplings = int(couplings)     # 1: PIMUNU_OMEGA_COUPLING, 2: OMEGA_OMEGA_COUPLING
-    if jet is not None:              # (direction, start_pos_index) of jet_eloss_src
-        p.jet_on, p.jet_direction, p.jet_start = 1, float(jet[0]), int(jet[1])
+    if jet is not None:              # (direction, start_pos_index[, stage mask, default 2]) of jet_eloss_src
+        p.jet_on, p.jet_direction, p.jet_start = (int(jet[2]) if len(jet) > 2 else 2), float(jet[0]), int(jet[1])
     t = eos_table
     p.eos_kind = t.kind_id
     p.eos_nrow, p.eos_ncol, p.eos_num_ed = t.nrow, t.ncol, t.num_ed

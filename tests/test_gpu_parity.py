@@ -331,16 +331,17 @@ def test_visc_steps_vorticity_couplings(kernel, flags):
 
 
 @pytest.mark.parametrize('viscous', [False, True])
-@pytest.mark.parametrize('kind', [0, 2])
-def test_jet_source_matches_oracle(viscous, kind):
-    """jet_eloss_src (kernel_jet_eloss.cl:8-69) switched on for every stage: ideal and viscous +
-    bulk steps (the x-marching handle runs jet stages per cell) equal the oracle with the same
-    deposit bit for bit, and the deposit moves the fields."""
+@pytest.mark.parametrize('kind,stages', [(0, 2), (2, 2), (0, 3)])
+def test_jet_source_matches_oracle(viscous, kind, stages):
+    """jet_eloss_src (kernel_jet_eloss.cl:8-69) in the RK stages of the mask (2: stage 2 only, as
+    evolve passes it, ideal.py:344, visc.py:543; 3: both): ideal and viscous + bulk steps (the
+    x-marching handle runs jet stages per cell) equal the oracle with the same deposit bit for
+    bit, and the deposit moves the fields."""
     nx, ny, nz, nsteps = 29, 31, 11, 15
     p, tab, ev = _visc_case(nx, ny, nz, bulk=(0.05, 0.155, 0.02) if viscous else None)
     direction, start = np.float32(0.7), (14 * ny + 15) * nz + 5
-    _say('case: jet source %s, kind %d, %dx%dx%d, %d steps, start %d direction %.2f'
-         % ('viscous+bulk' if viscous else 'ideal', kind, nx, ny, nz, nsteps, start, direction))
+    _say('case: jet source %s, kind %d, stages %d, %dx%dx%d, %d steps, start %d direction %.2f'
+         % ('viscous+bulk' if viscous else 'ideal', kind, stages, nx, ny, nz, nsteps, start, direction))
     if not viscous:
         p.flags &= ~_lib.CV_FLAG_BULK
     h = _lib.Handle(p, tab, viscous=viscous)
@@ -350,10 +351,10 @@ def test_jet_source_matches_oracle(viscous, kind):
         h.visc_initialize(p.tau0)
         h.ideal_stage(1, np.float32(p.tau0))
     h.set('ev1', ev)
-    h.set_jet_source(jet)
+    h.set_jet_source(jet, stages=stages)
     (h.visc_steps if viscous else h.ideal_steps)(0, nsteps)
     prm = orc_params(p, tab, nthreads=16)
-    prm.jet_on, prm.jet_direction, prm.jet_start = 1, float(direction), start
+    prm.jet_on, prm.jet_direction, prm.jet_start = stages, float(direction), start
     o = O.Oracle(prm, 'f32', viscous=viscous)
     o.load_ini(ev)
     (o.visc_steps if viscous else o.ideal_steps)(nsteps)
