@@ -483,3 +483,35 @@ def test_full_size_benchmark_grid_bitwise():
         _say('  %-6s bit-equal %.8f' % (name, frac))
         assert max_rel(g, r).max() <= TOL
     h.close()
+
+
+def test_clvisc_evolve_with_jet_matches_oracle(tmp_path):
+    """CLVisc.evolve(jet_eloss_src=...) (pyvisc/visc.py:473-544: the deposit in stage 2 of each
+    step) against the oracle's viscous steps with the jet in stage 2: identical fields."""
+    from clvisc_amd.config import read_config
+    from clvisc_amd import CLVisc
+    nx, ny, nz, nsteps = 29, 31, 11, 12
+    cfg = read_config(argv=[])
+    cfg.NX, cfg.NY, cfg.NZ = nx, ny, nz
+    cfg.DX, cfg.DY, cfg.DZ, cfg.DT, cfg.TAU0 = 0.08, 0.08, 0.15, 0.02, 0.6
+    cfg.ETAOS_XMIN, cfg.ETAOS_YMIN, cfg.ETAOS_LEFT_SLOP, cfg.ETAOS_RIGHT_SLOP = 0.0, 0.08, 0.0, 0.0
+    cfg.eos_type = 'crossover'
+    cfg.fPathOut = str(tmp_path)
+    start, direction = (14 * ny + 15) * nz + 5, 0.7
+    _say('case: CLVisc.evolve with jet_eloss_src, %dx%dx%d, %d steps' % (nx, ny, nz, nsteps))
+    visc = CLVisc(cfg)
+    pos = ic.woods_saxon_positions(12, 3, R=0.6, a=0.1)
+    ev = ic.hotspots(nx, ny, nz, 0.08, 0.08, 0.15, positions=pos, emax=50.0)
+    visc.ideal.load_ini(ev)
+    visc.evolve(max_loops=nsteps, save_hypersf=False, save_bulk=False, save_pi=False, force_run_to_maxloop=True,
+                jet_eloss_src={'switch_on': True, 'start_pos_index': start, 'direction': direction})
+    prm = orc_params(visc.handle.params, visc.eos_table, nthreads=16)
+    prm.jet_on, prm.jet_direction, prm.jet_start = 2, float(np.float32(direction)), start
+    o = O.Oracle(prm, 'f32', viscous=True)
+    o.load_ini(ev)
+    o.visc_steps(nsteps)
+    for name, r in (('ev1', o.ev[1]), ('pi1', o.pi[1])):
+        g = visc.handle.get(name)
+        m = max_rel(g, r).max()
+        _say('  %-4s max rel %.3g bit-equal %.8f' % (name, m, bit_equal_frac(g, r)))
+        assert m <= TOL, name
