@@ -32,6 +32,8 @@
  *                     the row/col table
  *   bulk_on           bulk viscous pressure Pi with zeta/s(T) (absent in the reference)
  *   fp64_root         conserved-to-primitive solve in double where M > vmin*T00
+ *   plant             tests only: a mistake planted in the eta branches (ORC_PLANT_*),
+ *                     which the eta-direction pins must detect; 0 everywhere else
  *
  * Layouts are the reference's AoS: ev[N][4] = (e, vx, vy, veta) with veta = tau*deta/dtau,
  * pi[N][10] in idx() order (00,01,02,03,11,12,13,22,23,33) with eta components
@@ -103,7 +105,14 @@ typedef struct orc_params {
     int32_t jet_on;            /* RK stages given jet_eloss_src (1: stage 1, 2: stage 2) */
     float jet_direction;       /* jet_eloss_src['direction'] (cfg.real) */
     int64_t jet_start;         /* jet_eloss_src['start_pos_index'] */
+    int32_t plant;             /* tests only: mistakes planted to show that the eta pins fail
+                                  (ORC_PLANT_*); 0 in every other use */
 } orc_params;
+
+/* Planted mistakes (tests/test_eta_oracle.py shows that each fails the eta-direction pins):
+ * the eta sweeps divide by DZ instead of tau*DZ; udz without its Christoffel term. */
+#define ORC_PLANT_ETA_DZ 1
+#define ORC_PLANT_UDZ_NO_CHRISTOFFEL 2
 
 static int64_t ncells(const orc_params *p) { return (int64_t)p->nx * p->ny * p->nz; }
 
@@ -582,7 +591,7 @@ void orc_ideal_along(const orc_params *p, int axis, real *Src, const real *ev, r
             for (int m = 0; m < 4; m++) {
                 if (p->riemann_test) S[m] = S[m] - DIV(k[m], DX);
                 else if (axis < 2) S[m] = S[m] - DIV(tau * k[m], DX);
-                else S[m] = S[m] - DIV(tau * k[m], tau * DX);
+                else S[m] = S[m] - DIV(tau * k[m], (p->plant & ORC_PLANT_ETA_DZ) ? DX : tau * DX);
             }
         }
     }
@@ -764,7 +773,7 @@ void orc_visc_along(const orc_params *p, int axis, real *Src, const real *ev, co
             real *S = Src + 4 * (base + (int64_t)t * stride);
             for (int m = 0; m < 4; m++) {
                 if (axis < 2) S[m] = S[m] - DIV(k[m], DX);
-                else S[m] = S[m] - DIV(k[m], tau * DX);
+                else S[m] = S[m] - DIV(k[m], (p->plant & ORC_PLANT_ETA_DZ) ? DX : tau * DX);
             }
         }
     }
@@ -944,7 +953,8 @@ void orc_is_along(const orc_params *p, int axis, real *Src, real *ud, const real
                 if (axis < 2) ud[4 * IND + c] = DIV(d, DX);
                 else {
                     real chr = DIV((c == 0 ? e[2][3] : (c == 3 ? RC(1.0f) : RC(0.0f))) * u0[2], tau);
-                    ud[4 * IND + c] = DIV(d, DX * tau) + chr;
+                    if (p->plant & ORC_PLANT_UDZ_NO_CHRISTOFFEL) chr = RC(0.0f);
+                    ud[4 * IND + c] = DIV(d, (p->plant & ORC_PLANT_ETA_DZ) ? DX : DX * tau) + chr;
                 }
             }
             for (int mn = 0; mn < 10; mn++) {
@@ -952,13 +962,13 @@ void orc_is_along(const orc_params *p, int axis, real *Src, real *ud, const real
                                    u0[2] * pi1[10 * J[2] + mn], u0[3] * pi1[10 * J[3] + mn],
                                    u0[4] * pi1[10 * J[4] + mn], v_mh, v_ph, lam_mh, lam_ph);
                 if (axis < 2) Src[10 * IND + mn] = Src[10 * IND + mn] - DIV(k, DX);
-                else Src[10 * IND + mn] = Src[10 * IND + mn] - DIV(k, tau * DX);
+                else Src[10 * IND + mn] = Src[10 * IND + mn] - DIV(k, (p->plant & ORC_PLANT_ETA_DZ) ? DX : tau * DX);
             }
             if (bulk && bulk_src) {
                 real k = kt1d_real(u0[0] * bulk[J[0]], u0[1] * bulk[J[1]], u0[2] * bulk[J[2]],
                                    u0[3] * bulk[J[3]], u0[4] * bulk[J[4]], v_mh, v_ph, lam_mh, lam_ph);
                 if (axis < 2) bulk_src[IND] = bulk_src[IND] - DIV(k, DX);
-                else bulk_src[IND] = bulk_src[IND] - DIV(k, tau * DX);
+                else bulk_src[IND] = bulk_src[IND] - DIV(k, (p->plant & ORC_PLANT_ETA_DZ) ? DX : tau * DX);
             }
         }
     }

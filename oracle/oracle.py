@@ -39,7 +39,13 @@ class OrcParams(C.Structure):
         ('tau0_cfg', C.c_double), ('dt_cfg', C.c_double),
         ('couplings', C.c_int32),
         ('jet_on', C.c_int32), ('jet_direction', C.c_float), ('jet_start', C.c_int64),
+        ('plant', C.c_int32),
     ]
+
+
+# mistakes the tests plant in the oracle's eta branches (clvisc_oracle.c ORC_PLANT_*)
+PLANT_ETA_DZ = 1
+PLANT_UDZ_NO_CHRISTOFFEL = 2
 
 
 class OrcState(C.Structure):
