@@ -85,29 +85,125 @@ def make_workload(bulk=True, ic=True):
     return p, tab, ev
 
 
-def cpu_baseline(p, tab, ev, nthreads):
-    """FP32 oracle on the host cores: the central 29 eta slices of the same initial
-    condition, 12 RK2 steps (about 10 s on 16 cores), reference-ordered unfused kernels."""
+def host_info():
+    """CPU model and counts of the host (lscpu), the CPUs this process may use and the
+    OpenMP thread setting."""
+    import subprocess
+    info = {'cpus': os.cpu_count(), 'affinity_cpus': len(os.sched_getaffinity(0)),
+            'OMP_NUM_THREADS': os.environ.get('OMP_NUM_THREADS')}
+    keys = {'Model name': 'model', 'Socket(s)': 'sockets', 'Core(s) per socket': 'cores_per_socket',
+            'Thread(s) per core': 'threads_per_core'}
+    try:
+        out = subprocess.run(['lscpu'], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(':')
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_threads_default():
+    """The host's CPU share: OMP_NUM_THREADS when set (16 per GPU on the GPU hosts), else
+    the CPUs this process may run on."""
+    try:
+        return max(1, int(os.environ.get('OMP_NUM_THREADS', '')))
+    except ValueError:
+        return len(os.sched_getaffinity(0))
+
+
+def _oracle_visc_rate(p, tab, ev, nz, k0, nthreads, warmup, steps):
+    """FP32 oracle viscous steps on the eta slices k0 .. k0+nz-1 of the benchmark field:
+    (cell-updates/s over the timed steps, seconds)."""
     from oracle import oracle as O
     from clvisc_amd import _lib
-    nz = 29
-    k0 = NZ // 2 - nz // 2
     sub = ev.reshape(NX, NY, NZ, 4)[:, :, k0:k0 + nz, :].reshape(-1, 4)
     bulk = (p.zetaos_max, p.zetaos_tpeak, p.zetaos_width) if p.flags & _lib.CV_FLAG_BULK else None
     op = O.make_params(NX, NY, nz, DT, DX, DY, DZ, TAU0, tab, etaos_xmin=0.0, etaos_ymin=0.08,
                        bulk=bulk, nthreads=nthreads)
     o = O.Oracle(op, 'f32', viscous=True)
     o.load_ini(sub)
-    steps = 12
+    o.visc_steps(warmup)
     t0 = time.perf_counter()
     o.visc_steps(steps)
     dt = time.perf_counter() - t0
-    cells = NX * NY * nz
-    return {'value': cells * steps / dt, 'unit': 'cell-updates/s', 'cores': nthreads,
-            'kind': 'port',
-            'sample': '%dx%dx%d central eta slab of the same initial condition, %d RK2 steps, '
-                      'FP32 oracle (unfused reference-ordered kernels, OpenMP), %.1f s'
-                      % (NX, NY, nz, steps, dt)}
+    del o
+    return NX * NY * nz * steps / dt, dt
+
+
+def cpu_baseline(p, tab, ev, nthreads):
+    """The FP32 oracle (unfused, reference-ordered kernels, OpenMP) on the host: the full
+    385x385x115 grid of the same initial condition, 2 warm-up and 10 timed RK2 steps on the
+    host's CPU share, and one thread on a 385x385x5 central eta slab (1 warm-up, 3 timed)."""
+    rate, dt = _oracle_visc_rate(p, tab, ev, NZ, 0, nthreads, 2, 10)
+    nz1 = 5
+    rate1, dt1 = _oracle_visc_rate(p, tab, ev, nz1, NZ // 2 - nz1 // 2, 1, 1, 3)
+    return {'value': rate, 'unit': 'cell-updates/s', 'cores': nthreads, 'kind': 'port',
+            'ms_per_step': 1000.0 * dt / 10,
+            'sample': '%dx%dx%d (the whole workload), 2 warm-up + 10 timed RK2 steps in %.1f s, FP32 oracle '
+                      '(unfused reference-ordered kernels, OpenMP, %d threads)' % (NX, NY, NZ, dt, nthreads),
+            'one_thread': {'value': rate1, 'unit': 'cell-updates/s', 'cores': 1,
+                           'sample': '%dx%dx%d central eta slab, 1 warm-up + 3 timed RK2 steps in %.1f s'
+                                     % (NX, NY, nz1, dt1)},
+            'host': host_info(),
+            'reference_published': {'value': 2.64e6, 'unit': 'cell-updates/s', 'ms_per_step': 6450.0,
+                                    'note': 'reference README, viscous 385x385x115 on an OpenCL CPU device '
+                                            '(BSZ=16), hardware not stated; shear only'}}
+
+
+C1_N, C1_STEPS = 201, 200
+
+
+def c1_gubser(nthreads, with_cpu=True):
+    """Config 1: ideal Gubser flow (q = 1/fm, conformal ideal-gas EOS, e0 at T = 0.3 GeV,
+    tau0 = 1 fm) on 201x201x1, DX = 0.08 fm, DT = 0.02 fm, 200 steps: GPU ms/step and
+    cell-updates/s, the FP32 oracle on the host at nthreads and at 1 thread, and the L1 error
+    of e against the analytic flow (r < 4 fm) at the end."""
+    from clvisc_amd import _lib, ic as ic_mod
+    from clvisc_amd.eos import Eos
+    eos = Eos('ideal_gas')
+    tab = eos.create_table()
+    p = _lib.CvParams()
+    p.nx, p.ny, p.nz = C1_N, C1_N, 1
+    p.tau0, p.dt, p.dx, p.dy, p.dz = 1.0, 0.02, 0.08, 0.08, 0.15
+    p.nxskip = p.nyskip = p.nzskip = 1
+    e0 = float(eos.f_ed(0.3))
+    ev, norm = ic_mod.gubser_ideal(C1_N, C1_N, 1, 0.08, 0.08, 0.15, 1.0, q=1.0, e0=e0)
+    ev = ev.astype(np.float32)
+    h = _lib.Handle(p, tab, viscous=False)
+    h.set('ev1', ev)
+    h.ideal_steps(0, 10)
+    h.set('ev1', ev)
+    h.sync()
+    t0 = time.perf_counter()
+    h.ideal_steps(0, C1_STEPS, sync=False)
+    h.sync()
+    wall = time.perf_counter() - t0
+    g = h.get('ev1')
+    h.close()
+    tau_end = float(np.float32(1.0 + C1_STEPS * 0.02))
+    ea, _ = ic_mod.gubser_ideal(C1_N, C1_N, 1, 0.08, 0.08, 0.15, tau_end, q=1.0)
+    ea[:, 0] *= norm
+    x = (np.arange(C1_N) - C1_N // 2) * 0.08
+    X, Y = np.meshgrid(x, x, indexing='ij')
+    m = np.sqrt(X * X + Y * Y).ravel() < 4.0
+    l1 = float(np.abs(g[m, 0] - ea[m, 0]).sum() / ea[m, 0].sum())
+    ncell = C1_N * C1_N
+    out = {'workload': 'ideal Gubser flow 201x201x1, 200 RK2 steps, FP32', 'gpu_ms_per_step': 1000.0 * wall / C1_STEPS,
+           'gpu_value': ncell * C1_STEPS / wall, 'unit': 'cell-updates/s', 'l1_e_vs_analytic': l1}
+    if with_cpu:
+        from oracle import oracle as O
+        for key, nth in (('cpu', nthreads), ('cpu_one_thread', 1)):
+            op = O.make_params(C1_N, C1_N, 1, 0.02, 0.08, 0.08, 0.15, 1.0, tab, nthreads=nth)
+            o = O.Oracle(op, 'f32', viscous=False)
+            o.load_ini(ev)
+            t0 = time.perf_counter()
+            o.ideal_steps(C1_STEPS)
+            dt = time.perf_counter() - t0
+            out[key] = {'ms_per_step': 1000.0 * dt / C1_STEPS, 'value': ncell * C1_STEPS / dt, 'threads': nth,
+                        'bit_equal_gpu': bool(np.array_equal(o.ev[1], g))}
+    return out
 
 
 def ensemble_events():
@@ -624,7 +720,9 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-learned', action='store_true',
                     help='single: skip the learned (network-seeded root solve) form of the run')
-    ap.add_argument('--cpu-threads', type=int, default=0)
+    ap.add_argument('--cpu-threads', type=int, default=0,
+                    help='threads of the CPU baseline (default: OMP_NUM_THREADS, else the usable CPUs)')
+    ap.add_argument('--no-c1', action='store_true', help='single: skip the config-1 (ideal Gubser 201x201) line')
     ap.add_argument('--events', type=int, default=16, help='combined: number of 385x385x115 events')
     ap.add_argument('--max-loops', type=int, default=3000, help='combined: step limit')
     ap.add_argument('--sf-capacity', type=int, default=6_000_000, help='combined: surface elements per event')
@@ -717,11 +815,22 @@ def main():
         learned = run_learned(args, p, tab, ev, h, loop)
         learned['learned_over_plain'] = learned['value'] / value
     cpu = None
+    nth = args.cpu_threads or cpu_threads_default()
     if not args.no_cpu_baseline:
-        nth = args.cpu_threads or min(16, os.cpu_count() or 1)
         t0 = time.time()
         cpu = cpu_baseline(p, tab, ev, nth)
-        log('cpu baseline: %.4g cell-updates/s on %d threads (%.1fs)' % (cpu['value'], nth, time.time() - t0))
+        log('cpu baseline: %.4g cell-updates/s on %d threads (%.1f ms/step), %.4g on 1 thread (slab) (%.1fs)'
+            % (cpu['value'], nth, cpu['ms_per_step'], cpu['one_thread']['value'], time.time() - t0))
+        log('host: %s' % json.dumps(cpu['host']))
+    c1 = None
+    if not args.no_c1:
+        t0 = time.time()
+        c1 = c1_gubser(nth, with_cpu=not args.no_cpu_baseline)
+        log('C1 ideal Gubser 201x201: GPU %.4f ms/step (%.4g cell-updates/s), L1(e) vs analytic %.4f%s (%.1fs)'
+            % (c1['gpu_ms_per_step'], c1['gpu_value'], c1['l1_e_vs_analytic'],
+               '' if 'cpu' not in c1 else '; CPU %.3f ms/step on %d threads, %.3f on 1 (bit-equal to the GPU: %s)'
+               % (c1['cpu']['ms_per_step'], nth, c1['cpu_one_thread']['ms_per_step'], c1['cpu']['bit_equal_gpu']),
+               time.time() - t0))
     out = {
         'metric': 'cell-updates/s, viscous KT+IS 385x385x115; learned vs plain at matched error',
         'value': value, 'unit': 'cell-updates/s', 'n_gpus': 1,
@@ -737,6 +846,7 @@ def main():
         'roofline': roof,
         'learned': learned,
         'cpu_baseline': cpu,
+        'c1_gubser': c1,
     }
     print(json.dumps(out))
     h.close()

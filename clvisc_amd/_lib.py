@@ -70,8 +70,21 @@ class CvEos(C.Structure):
     ]
 
 
+# status codes of the C-ABI (include/clvisc_amd.h)
+CV_OK = 0
+CV_ERR_ARG = -1
+CV_ERR_HIP = -2
+CV_ERR_STATE = -3
+CV_ERR_CAPACITY = -4
+CV_ERR_UNSUPPORTED = -5
+
+
 class CvError(RuntimeError):
-    pass
+    """A failed C-ABI call (or a missing library); ``code`` is the CV_ERR_* status, or None."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 _lib = None
@@ -165,7 +178,7 @@ def lib():
 
 def check(rc):
     if rc != 0:
-        raise CvError('clvisc_amd error %d: %s' % (rc, lib().cv_last_error().decode()))
+        raise CvError('clvisc_amd error %d: %s' % (rc, lib().cv_last_error().decode()), rc)
 
 
 def exported_symbols():

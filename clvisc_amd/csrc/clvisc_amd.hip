@@ -1491,6 +1491,12 @@ static int batch_build(cv_batch *b) {
 extern "C" int cv_batch_steps_async(cv_batch *b, int64_t nsteps) {
     if (!b) return fail(CV_ERR_ARG, "null batch");
     if (nsteps < 0) return fail(CV_ERR_ARG, "negative step count");
+    // the batched stage kernel carries no jet source: refuse rather than drop a jet set
+    // on a member handle (cv_set_jet_source through cv_batch_handle)
+    for (size_t e = 0; e < b->ev.size(); e++)
+        if (b->active[e] && b->ev[e]->jet.on)
+            return fail(CV_ERR_UNSUPPORTED, "cv_batch_steps: event %d has a jet source; jets are single-event only",
+                        (int)e);
     HIP_TRY(hipSetDevice(b->device));
     for (cv_handle *h : b->ev) HIP_TRY(hipStreamSynchronize(h->stream));
     if (b->dirty) {

@@ -161,15 +161,19 @@ __device__ __forceinline__ void ideal_axis(const PhysP &p, const Eos &eos, const
 // (ideal.py:240-246, visc.py:265-271): a Gaussian of width 0.3 fm around the jet, which moves
 // from the start cell at the speed of light; dE/dx = 14 GeV/fm (T/0.3 GeV)^3 with T at the jet's
 // cell of ev_st.  exp and the cube are taken in FP64 and rounded once (as the oracle does).
-__device__ __noinline__ void jet_src(const PhysP &p, const Eos &eos, const JetP &j, float tau,
-                                     const float *__restrict__ ev_st, const Cell &cl, float *s) {
+// Returns the four terms coef[q] * dE[q] that the caller adds to its source (by value and
+// inlined: a private array or struct whose address reached a call would live in scratch
+// memory in every launch of the kernel, jet or not).
+__device__ __forceinline__ float4 jet_deposit(const PhysP &p, const Eos &eos, const JetP &j, float tau,
+                                              const float *__restrict__ ev_st, const Cell &cl) {
     const float sig = 0.3f;
     const float dtau = tau - j.tau0;
     const float px = j.x0 + j.c * dtau, py = j.y0 + j.sn * dtau, pz = j.z0 + 0.0f * dtau;
     const int li = (int)(px * p.r_dx + (float)(p.nx / 2));
     const int lj = (int)(py * p.r_dy + (float)(p.ny / 2));
     const int lk = (int)(pz * p.r_dz + (float)(p.nz / 2));
-    if (!(li >= 0 && li < p.nx && lj >= 0 && lj < p.ny && lk >= 0 && lk < p.nz)) return;
+    if (!(li >= 0 && li < p.nx && lj >= 0 && lj < p.ny && lk >= 0 && lk < p.nz))
+        return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const float x = (float)(cl.i - p.nx / 2) * p.dx;
     const float y = (float)(cl.j - p.ny / 2) * p.dy;
     const float z = (float)(cl.k - p.nz / 2) * p.dz;
@@ -182,8 +186,11 @@ __device__ __noinline__ void jet_src(const PhysP &p, const Eos &eos, const JetP 
     const double tr = (double)(eos.T(e) * rcp(0.3f));
     const float dEdx = 14.0f * (float)(tr * tr * tr);
     const float dE[4] = {dEdx, dEdx * j.c, dEdx * j.sn, 0.0f};
-#pragma unroll
-    for (int q = 0; q < 4; q++) s[q] = s[q] + coef[q] * dE[q];
+    return make_float4(coef[0] * dE[0], coef[1] * dE[1], coef[2] * dE[2], coef[3] * dE[3]);
+}
+// s[q] + deposit[q], the order of the reference's d_Src + jet source
+__device__ __forceinline__ void jet_add(float *s, float4 d) {
+    s[0] = s[0] + d.x; s[1] = s[1] + d.y; s[2] = s[2] + d.z; s[3] = s[3] + d.w;
 }
 
 // ---------------------------------------------------------------------------
@@ -217,7 +224,7 @@ __global__ __launch_bounds__(256) void k_ideal_stage(PhysP p, EosP E, StageP st,
     ideal_axis<0>(p, eos, st, cl, ev_st, s);
     ideal_axis<1>(p, eos, st, cl, ev_st, s);
     ideal_axis<2>(p, eos, st, cl, ev_st, s);
-    if (jet.on & st.step) jet_src(p, eos, jet, st.tau, ev_st, cl, s);
+    if (jet.on & st.step) jet_add(s, jet_deposit(p, eos, jet, st.tau, ev_st, cl));
     if (!cons_out) st4(Src, p.n, I, make_float4(s[0], s[1], s[2], s[3]));
     // update_ev (kernel_ideal.cl:165-231)
     float4 e = ld4(ev1, p.n, I);
@@ -697,7 +704,7 @@ __global__ __launch_bounds__(256) void k_kt_stage(PhysP p, EosP E, StageP st,
     kt_axis<0, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s);
     kt_axis<1, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s);
     kt_axis<2, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s);
-    if (jet.on & st.step) jet_src(p, eos, jet, st.tau, ev_st, cl, s);
+    if (jet.on & st.step) jet_add(s, jet_deposit(p, eos, jet, st.tau, ev_st, cl));
     st4(Src, p.n, I, make_float4(s[0], s[1], s[2], s[3]));
     // update_ev (kernel_visc.cl:322-393)
     float4 e = ld4(ev1, p.n, I);
@@ -918,7 +925,7 @@ __global__ __launch_bounds__(256, CV_STAGE_WAVES) void k_visc_stage(PhysP p, Eos
     CV_PHASE_FENCE();
     visc_axis<2, BULK>(p, eos, st, cl, ev_st, pi_st, bk_st, s, bs, t, udz);
     CV_PHASE_FENCE();
-    if (jet.on & st.step) jet_src(p, eos, jet, st.tau, ev_st, cl, t);
+    if (jet.on & st.step) jet_add(t, jet_deposit(p, eos, jet, st.tau, ev_st, cl));
 
     // ---- update_pimn (kernel_IS.cl:501-709) ----
     const float4 e1 = STEP == 1 ? ec : ld4(ev1, n, I);

@@ -133,8 +133,14 @@ class CLIdeal(object):
 
     def stepUpdate(self, step, jet_eloss_src=None):
         """One RK2 stage at self.tau (pyvisc/ideal.py:211-255)."""
+        if not jet_eloss_src or not jet_eloss_src.get('switch_on', False):
+            self.handle.ideal_stage(step, self.cfg.real(self.tau))
+            return
         self.handle.set_jet_source(jet_eloss_src, stages=3)
-        self.handle.ideal_stage(step, self.cfg.real(self.tau))
+        try:
+            self.handle.ideal_stage(step, self.cfg.real(self.tau))
+        finally:
+            self.handle.set_jet_source(None)
 
     def max_energy_density(self):
         """max e over d_ev[1] (pyvisc/ideal.py:257-262)."""
@@ -170,7 +176,14 @@ class CLIdeal(object):
                jet_eloss_src=None):
         """Main loop (pyvisc/ideal.py:322-349): max e every step, surface every ntskip,
         bulk info every ntskip, then both RK2 stages."""
+        # the reference passes jet_eloss_src per call: the source is on for this loop only
         self.handle.set_jet_source(jet_eloss_src)
+        try:
+            self._evolve_loop(max_loops, save_hypersf, save_bulk, plot_bulk, to_maxloop)
+        finally:
+            self.handle.set_jet_source(None)
+
+    def _evolve_loop(self, max_loops, save_hypersf, save_bulk, plot_bulk, to_maxloop):
         for n in range(max_loops):
             t0 = time.time()
             self.edmax = self.max_energy_density()

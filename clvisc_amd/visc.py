@@ -59,8 +59,14 @@ class CLVisc(object):
     def visc_stepUpdate(self, step, jet_eloss_src=None):
         """KT source with pi and the primitive update into d_ev[3-step]
         (pyvisc/visc.py:276-314)."""
+        if not jet_eloss_src or not jet_eloss_src.get('switch_on', False):
+            self.handle.visc_stage(step, self.cfg.real(self.ideal.tau))
+            return
         self.handle.set_jet_source(jet_eloss_src, stages=3)
-        self.handle.visc_stage(step, self.cfg.real(self.ideal.tau))
+        try:
+            self.handle.visc_stage(step, self.cfg.real(self.ideal.tau))
+        finally:
+            self.handle.set_jet_source(None)
 
     def update_udiff(self, d_ev0, d_ev1):
         """udiff = u(d_ev1) - u(d_ev0) (pyvisc/visc.py:352-357)."""
@@ -164,14 +170,26 @@ class CLVisc(object):
         until max e < e_frz, as the reference does (max_loops is not consulted)."""
         if save_vorticity:
             self.handle.vorticity_enable()
-        self.handle.set_jet_source(jet_eloss_src)
         cfg = self.cfg
+        # the ideal shortcut of pyvisc/visc.py:483-489 (not taken with save_vorticity, whose
+        # tensors the viscous loop computes; nor with bulk viscosity, which is viscous)
         if cfg.ETAOS_YMIN < 1.0E-6 and cfg.ETAOS_LEFT_SLOP < 1.0E-6 and cfg.ETAOS_RIGHT_SLOP < 1.0E-6 \
-                and not self.bulk_on:
+                and not save_vorticity and not self.bulk_on:
             self.ideal.evolve(max_loops, save_hypersf, save_bulk, plot_bulk, force_run_to_maxloop,
                               jet_eloss_src)
             self.save_pimn_sf(set_to_zero=True)
             return
+        # the reference passes jet_eloss_src per call: the source is on for this loop only
+        self.handle.set_jet_source(jet_eloss_src)
+        try:
+            self._evolve_loop(max_loops, save_hypersf, save_bulk, plot_bulk, save_pi, force_run_to_maxloop,
+                              save_vorticity, debug)
+        finally:
+            self.handle.set_jet_source(None)
+
+    def _evolve_loop(self, max_loops, save_hypersf, save_bulk, plot_bulk, save_pi, force_run_to_maxloop,
+                     save_vorticity, debug):
+        cfg = self.cfg
         if save_pi:
             from .pimninfo import PimnInfo
             self.pimn_info = PimnInfo(cfg)

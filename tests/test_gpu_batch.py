@@ -113,3 +113,22 @@ def test_batch_fp64_root_matches_single_events():
     for e, hs in enumerate(singles):
         _compare(batch.handle(e), hs, 'event %d %s' % (e, dims[e]))
     batch.close()
+
+
+def test_batch_refuses_a_jet_source():
+    """The batched stage kernel has no jet source: a jet set on a member handle makes
+    cv_batch_steps fail with CV_ERR_UNSUPPORTED instead of being dropped; cleared, it runs."""
+    tab = Eos('crossover').create_table()
+    p = cv_params(1, 1, 1, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08)
+    dims = DIMS[:2]
+    _say('case: batch of %d events, jet on event 1' % len(dims))
+    batch = _lib.Batch(p, tab, dims)
+    for e, d in enumerate(dims):
+        _init(batch.handle(e), _ic(d, e), p.tau0)
+    batch.handle(1).set_jet_source({'switch_on': True, 'start_pos_index': 0, 'direction': 0.0})
+    with pytest.raises(_lib.CvError) as err:
+        batch.steps(1)
+    assert err.value.code == _lib.CV_ERR_UNSUPPORTED
+    batch.handle(1).set_jet_source(None)
+    batch.steps(1)
+    batch.close()

@@ -515,3 +515,9 @@ def test_clvisc_evolve_with_jet_matches_oracle(tmp_path):
         m = max_rel(g, r).max()
         _say('  %-4s max rel %.3g bit-equal %.8f' % (name, m, bit_equal_frac(g, r)))
         assert m <= TOL, name
+    # the reference passes jet_eloss_src per call: after evolve returns the handle steps
+    # without the jet (two more steps equal the oracle's steps with jet_on = 0)
+    visc.handle.visc_steps(nsteps, 2)
+    o.p.jet_on = 0
+    o.visc_steps(2)
+    np.testing.assert_array_equal(visc.handle.get('ev1'), o.ev[1])

@@ -88,6 +88,34 @@ def test_vorticity_on_the_surface():
     h.close()
 
 
+def test_clvisc_evolve_zero_viscosity_with_vorticity_runs_viscous_loop(tmp_path):
+    """With eta/s = 0 the reference takes the ideal shortcut only without save_vorticity
+    (pyvisc/visc.py:483-484): with save_vorticity=True it runs the viscous loop, which writes
+    the vorticity on the surface."""
+    from clvisc_amd.config import read_config
+    from clvisc_amd import CLVisc
+    nx, ny, nz = 29, 31, 11
+    cfg = read_config(argv=[])
+    cfg.NX, cfg.NY, cfg.NZ = nx, ny, nz
+    cfg.DX, cfg.DY, cfg.DZ, cfg.DT, cfg.TAU0 = 0.08, 0.08, 0.15, 0.02, 0.6
+    cfg.ETAOS_XMIN, cfg.ETAOS_YMIN, cfg.ETAOS_LEFT_SLOP, cfg.ETAOS_RIGHT_SLOP = 0.0, 0.0, 0.0, 0.0
+    cfg.ntskip, cfg.nxskip, cfg.nyskip, cfg.nzskip = 5, 1, 1, 1
+    cfg.eos_type = 'crossover'
+    cfg.TFRZ = 0.30
+    cfg.fPathOut = str(tmp_path)
+    _say('case: CLVisc.evolve(save_vorticity=True) with eta/s = 0, %dx%dx%d, 20 steps' % (nx, ny, nz))
+    visc = CLVisc(cfg)
+    pos = ic.woods_saxon_positions(12, 3, R=0.6, a=0.1)
+    visc.ideal.load_ini(ic.hotspots(nx, ny, nz, 0.08, 0.08, 0.15, positions=pos, emax=50.0))
+    visc.evolve(max_loops=20, save_hypersf=True, save_bulk=False, save_pi=False, force_run_to_maxloop=True,
+                save_vorticity=True)
+    n = visc.handle.hypersf_count()[0]
+    assert n > 0
+    for name in ('omegamu_sf.dat', 'omegamu_shear1_sf.dat', 'omegamu_shear2_sf.dat', 'omegamu_accT_sf.dat'):
+        assert np.loadtxt(tmp_path / name, ndmin=2).shape[0] == n, name
+    assert np.abs(np.loadtxt(tmp_path / 'omegamu_sf.dat')).max() > 0
+
+
 def test_clvisc_evolve_saves_vorticity(tmp_path):
     from clvisc_amd.config import read_config
     from clvisc_amd import CLVisc
