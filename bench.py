@@ -395,17 +395,16 @@ def run_learned(args, p, tab, ev, h_plain, loop_end):
             'learned_over_plain': None}
 
 
-def run_combined(args, p, tab, name):
-    """Combined workload: `events` batched 385x385x115 viscous + bulk events (400 hotspots each,
-    their own seeds), FP64 root solve for cells with |T^ti| > 0.99 T^tt, evolved until every
-    event's max e drops below e(T = 0.137 GeV) (checked every ntskip = 20 steps, as
-    CLVisc.evolve does), freeze-out surface found every ntskip steps (nxskip = nyskip = 3,
-    nzskip = 1), FP64 energy balance of one step in every ntskip.  Finished events leave
-    the batch.  The plain GPU form (the same events stepped one after another) is timed
-    over a window of steps."""
+def combined_run(p, tab, nev, max_loops=3000, sf_capacity=6_000_000, sf_out='', check_event=None):
+    """`nev` batched 385x385x115 viscous + bulk events (400 hotspots each, their own seeds), FP64
+    root solve for cells with |T^ti| > 0.99 T^tt, evolved until every event's max e drops below
+    e(T = 0.137 GeV) (checked every ntskip = 20 steps, as CLVisc.evolve does), freeze-out surface
+    found every ntskip steps (nxskip = nyskip = 3, nzskip = 1), FP64 energy balance of one step in
+    every ntskip.  Finished events leave the batch.  The plain GPU form (the same events stepped
+    one after another) is timed over a window of steps.  check_event: an event index whose
+    first ntskip steps are also run on its own handle and compared bit for bit.  Returns a dict."""
     from clvisc_amd import _lib
     from clvisc_amd import ic as ic_mod
-    nev = args.events
     ntskip = 20
     p.flags |= _lib.CV_FLAG_FP64_ROOT
     p.fp64_root_vmin = 0.99
@@ -417,14 +416,20 @@ def run_combined(args, p, tab, name):
     batch = _lib.Batch(p, tab, dims)
     hs = [batch.handle(e) for e in range(nev)]
     seeds = [20240601 + e for e in range(nev)]
+    single = None
     for e, h in enumerate(hs):
         pos = ic_mod.woods_saxon_positions(400, seeds[e])
         ev = ic_mod.hotspots(NX, NY, NZ, DX, DY, DZ, positions=pos, emax=50.0)
         h.visc_initialize(TAU0)
         h.ideal_stage(1, np.float32(TAU0))
         h.set('ev1', ev)
-        h.hypersf_reserve(args.sf_capacity)
+        h.hypersf_reserve(sf_capacity)
         h.hypersf_snapshot(np.float32(TAU0))
+        if check_event == e:
+            single = _lib.Handle(p, tab, viscous=True)
+            single.visc_initialize(TAU0)
+            single.ideal_stage(1, np.float32(TAU0))
+            single.set('ev1', ev)
     del ev
     log('combined: %d events of %dx%dx%d set up in %.1fs (e_frz %.4g GeV/fm^3)'
         % (nev, NX, NY, NZ, time.perf_counter() - t_start, efrz))
@@ -434,11 +439,12 @@ def run_combined(args, p, tab, name):
     finish_tau = [None] * nev
     balance = []                       # (event, loop, relative residual)
     energy = [[] for _ in range(nev)]  # (tau, E) per event
+    check = None
     loop = 0
     dev_ms = 0.0
     t_loop = time.perf_counter()
     t_last = t_loop
-    while sum(active) and loop < args.max_loops:
+    while sum(active) and loop < max_loops:
         # one step with the FP64 energy balance of the active events around it
         pre = {e: hs[e].energy_sum(tau_of(loop), 1) for e in range(nev) if active[e]}
         batch.steps(1)
@@ -450,6 +456,14 @@ def run_combined(args, p, tab, name):
             energy[e].append((tau_of(loop), E0))
         batch.steps(ntskip - 1)
         dev_ms += batch.last_elapsed_ms()
+        if single is not None:
+            single.visc_steps(loop, ntskip)
+            h = hs[check_event]
+            check = {f: bool(np.array_equal(h.get(f), single.get(f))) for f in ('ev1', 'pi1', 'bulk1', 'udiff')}
+            check['loops'] = loop + ntskip
+            log('combined: event %d after %d steps equals its own handle: %s' % (check_event, loop + ntskip, check))
+            single.close()
+            single = None
         loop += ntskip
         changed = False
         for e in range(nev):
@@ -483,9 +497,9 @@ def run_combined(args, p, tab, name):
             continue
         sf, tx, pisf, bksf = h.hypersf_get()
         dsum += float(np.abs(sf[:, :4].astype(np.float64)).sum())
-        if args.sf_out:
+        if sf_out:
             from clvisc_amd import hypersf as hsf
-            d = os.path.join(args.sf_out, 'event%d' % e)
+            d = os.path.join(sf_out, 'event%d' % e)
             os.makedirs(d, exist_ok=True)
             hsf.write_hypersf(d, sf, tx, 0.137)
     t_emit = time.perf_counter() - t0
@@ -493,6 +507,7 @@ def run_combined(args, p, tab, name):
     log('combined: %d loops, %.1fs wall (%.1fs device steps) -> %.4g cell-updates/s; %d surface elements '
         '(copied in %.1fs); energy balance per step |res| max %.3g median %.3g'
         % (loop, wall, dev_ms / 1e3, value, nsf, t_emit, np.abs(res).max(), np.median(np.abs(res))))
+    edmax_end = [h.max_ed(1) for h in hs]
     # the plain GPU form: the same events one after another, a window of steps each
     for e in range(nev):
         active[e] = 0
@@ -504,20 +519,34 @@ def run_combined(args, p, tab, name):
     for h in hs:
         h.sync()
     seq = float(ncell) * nev * nwin / (time.perf_counter() - t0)
-    step_value = updates / (dev_ms / 1e3)
+    batch.close()
+    return {'value': value, 'wall': wall, 'dev_ms': dev_ms, 'loops': loop, 'steps_done': steps_done,
+            'finish_tau': finish_tau, 'seeds': seeds, 'surface_elements': nsf, 'surface_counts': counts,
+            'surface_overflow_events': overflow, 'surface_abs_dsigma_sum': dsum, 'balance': res,
+            'energy': energy, 'edmax_end': edmax_end, 'check': check,
+            'step_value': updates / (dev_ms / 1e3), 'plain_sequential_value': seq}
+
+
+def run_combined(args, p, tab, name):
+    """Combined workload (combined_run) as one JSON line."""
+    nev = args.events
+    r = combined_run(p, tab, nev, args.max_loops, args.sf_capacity, args.sf_out)
+    value, wall, loop, res = r['value'], r['wall'], r['loops'], r['balance']
+    step_value, seq, energy, seeds = r['step_value'], r['plain_sequential_value'], r['energy'], r['seeds']
     abytes = sum(ALGO_BYTES[True])
     out = {
         'metric': 'cell-updates/s, combined: %d batched 385x385x115 viscous+bulk events to freeze-out' % nev,
         'value': value, 'unit': 'cell-updates/s', 'n_gpus': 1,
-        'steps': int(sum(steps_done)), 'warmup': 0, 'ms_per_step': 1000.0 * wall / max(1, loop),
+        'steps': int(sum(r['steps_done'])), 'warmup': 0, 'ms_per_step': 1000.0 * wall / max(1, loop),
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': value / REF_CELL_UPDATES,
         'dtype': 'f32 (f64 root for |T^ti| > 0.99 T^tt, f64 energy sums)', 'data': 'synthetic',
         'config': {'workload': '%d events 385x385x115, 400 Woods-Saxon hotspots each (seeds %d..%d), '
                                'eta/s 0.08, bulk zeta/s peak 0.05, crossover EOS, until max T < 0.137 GeV, '
                                'surface every 20 steps (skip 3,3,1)' % (nev, seeds[0], seeds[-1]),
-                   'device': name, 'events': nev, 'cells_per_event': ncell, 'loops': loop,
-                   'finish_tau': finish_tau, 'surface_elements': nsf, 'surface_overflow_events': overflow,
-                   'surface_abs_dsigma_sum': dsum,
+                   'device': name, 'events': nev, 'cells_per_event': NX * NY * NZ, 'loops': loop,
+                   'finish_tau': r['finish_tau'], 'surface_elements': r['surface_elements'],
+                   'surface_overflow_events': r['surface_overflow_events'],
+                   'surface_abs_dsigma_sum': r['surface_abs_dsigma_sum'],
                    'energy_balance_max_abs': float(np.abs(res).max()),
                    'energy_balance_median_abs': float(np.median(np.abs(res))),
                    'energy_first_last': [[energy[e][0][1], energy[e][-1][1]] for e in range(min(nev, 4))],
@@ -530,7 +559,6 @@ def run_combined(args, p, tab, name):
         'cpu_baseline': None,
     }
     print(json.dumps(out))
-    batch.close()
 
 
 def run_surrogate(args, name):

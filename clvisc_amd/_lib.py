@@ -161,6 +161,7 @@ def lib():
             'cv_mlp_set_data': [vp, vp, vp, i64],
             'cv_mlp_train': [vp, C.c_int, C.c_int, f32, vp],
             'cv_mlp_forward': [vp, vp, i64, vp],
+            'cv_mlp_features': [vp, vp, vp, vp, i64, vp],
             'cv_root_eval': [vp, vp, vp, vp, vp, i64, vp, vp, vp],
             'cv_set_root_seed': [vp, vp, C.c_int],
             'cv_set_root_split': [vp, C.c_int],
@@ -193,7 +194,7 @@ def exported_symbols():
             'cv_batch_create', 'cv_batch_destroy', 'cv_batch_handle', 'cv_batch_set_active',
             'cv_batch_steps_async', 'cv_batch_set_loop', 'cv_batch_set_xcd_map', 'cv_batch_sync', 'cv_batch_last_elapsed_ms',
             'cv_batch_info', 'cv_mlp_nparam', 'cv_mlp_create', 'cv_mlp_destroy', 'cv_mlp_get_params',
-            'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward',
+            'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward', 'cv_mlp_features',
             'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums', 'cv_glauber_ini',
             'cv_glauber_nbinary', 'cv_smear_partons', 'cv_longitudinal_profile',
             'cv_sd_to_ed', 'cv_vorticity_enable', 'cv_vorticity', 'cv_vorticity_mu', 'cv_hypersf_get_vorticity',
@@ -569,6 +570,15 @@ class Mlp(object):
         Y = np.empty((X.shape[0], 2), np.float32)
         check(self.L.cv_mlp_forward(self.m, X.ctypes.data, X.shape[0], Y.ctypes.data))
         return Y
+
+    def features(self, T00, M, T03):
+        """The network input of (T^tt, |T^ti|, T^t eta) as the device forms it for the seeded
+        root solve (n x 4): forward() of it gives the seeds cv_root_eval starts from."""
+        a = [np.ascontiguousarray(x, np.float32).reshape(-1) for x in (T00, M, T03)]
+        X = np.empty((a[0].size, 4), np.float32)
+        check(self.L.cv_mlp_features(self.m, a[0].ctypes.data, a[1].ctypes.data, a[2].ctypes.data, a[0].size,
+                                     X.ctypes.data))
+        return X
 
     def close(self):
         if getattr(self, 'm', None):

@@ -1760,6 +1760,26 @@ extern "C" int cv_mlp_forward(cv_mlp *m, const float *X, int64_t n, float *Y) {
     return CV_OK;
 }
 
+extern "C" int cv_mlp_features(cv_mlp *m, const float *T00, const float *M, const float *T03, int64_t n, float *X) {
+    if (!m || !T00 || !M || !T03 || !X || n < 1) return fail(CV_ERR_ARG, "cv_mlp_features: bad argument");
+    HIP_TRY(hipSetDevice(m->device));
+    float *buf = nullptr;
+    HIP_TRY(hipMalloc((void **)&buf, sizeof(float) * n * (3 + MLP_IN)));
+    float *dT = buf, *dM = buf + n, *d3 = buf + 2 * n, *dX = buf + 3 * n;
+    hipError_t e = hipMemcpy(dT, T00, sizeof(float) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dM, M, sizeof(float) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d3, T03, sizeof(float) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_seed_features, grid1d(n), dim3(256), 0, m->stream, dT, dM, d3, n, dX);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+    if (e == hipSuccess) e = hipMemcpy(X, dX, sizeof(float) * MLP_IN * n, hipMemcpyDeviceToHost);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_mlp_features: %s", hipGetErrorString(e));
+    return CV_OK;
+}
+
 // root solve of host states (T^tt, |T^ti|, T^t eta) with the handle's EOS: plain
 // (m == NULL) or from the network's |v| seed; outputs e, |v|, iterations
 extern "C" int cv_root_eval(cv_handle *h, cv_mlp *m, const float *T00, const float *M, const float *T03, int64_t n,

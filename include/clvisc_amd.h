@@ -377,6 +377,10 @@ int cv_mlp_grad(cv_mlp *m, const float *X, const float *T, int64_t n, float *gra
 int cv_mlp_set_data(cv_mlp *m, const float *X, const float *T, int64_t n);
 int cv_mlp_train(cv_mlp *m, int steps, int batch, float lr, float *loss_hist);
 int cv_mlp_forward(cv_mlp *m, const float *X, int64_t n, float *Y);
+/* The network's input features of given states (T^tt, |T^ti|, T^t eta) as the device forms
+ * them for the seeded root solve (n x 4 into X): cv_mlp_forward of these gives the seeds
+ * that cv_root_eval and the seeded stages start from. */
+int cv_mlp_features(cv_mlp *m, const float *T00, const float *M, const float *T03, int64_t n, float *X);
 /* Root solve of given states with the handle's EOS, plain (m NULL) or seeded: e, |v|,
  * iterations. */
 int cv_root_eval(cv_handle *h, cv_mlp *m, const float *T00, const float *M, const float *T03, int64_t n,

@@ -99,7 +99,13 @@ def test_training_and_seeded_root_solve():
     er, itr = O.root_solve(op, T00, M)
     np.testing.assert_array_equal(e0, np.maximum(er, 0))
     np.testing.assert_array_equal(it0, itr)
-    Y = mlp.forward(surrogate.features(T00, M, T03))
+    # the device's features equal the host's restatement (log2 within 2 ulp, the rest exactly)
+    Xd, Xh = mlp.features(T00, M, T03), surrogate.features(T00, M, T03)
+    np.testing.assert_array_equal(Xd[:, 1:], Xh[:, 1:])
+    np.testing.assert_allclose(Xd[:, 0], Xh[:, 0], rtol=3e-7, atol=1e-9)
+    # the seeds the device started from: its own features (log2f may differ from the host's
+    # log2 by an ulp) through the network
+    Y = mlp.forward(mlp.features(T00, M, T03))
     er1, itr1 = O.root_solve_from(op, T00, M, Y[:, 1])
     np.testing.assert_array_equal(e1, np.maximum(er1, 0))
     np.testing.assert_array_equal(it1, itr1)
