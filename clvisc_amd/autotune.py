@@ -17,8 +17,8 @@ import numpy as np
 from . import _lib
 
 TILE_THREADS = 256
-MAX_HALO_CELLS = 405
-MAX_FACES = 288
+MAX_HALO_CELLS = 378
+MAX_FACES = 256
 
 
 def _valid(ny, nz, ty, tz):

@@ -105,6 +105,10 @@ struct cv_handle {
     int64_t sf_cap = 0;
 };
 
+// The x-marching kernels hold the pressure column of an octave EOS table in LDS: they run
+// for row/column tables and for octave tables of at most CV_PTAB_LDS nodes.
+static bool march_ok(const cv_handle *h) { return h->eos.kind != 1 || h->eos.nlast + 1 <= CV_PTAB_LDS; }
+
 // Tile of the tiled stage kernel: ty x tz cells of one x-plane with a 2-cell halo in y and
 // z (none along an axis of length 1).  Checks the LDS capacities of cv_tiled.h.
 static int set_tile(cv_handle *h, int ty, int tz) {
@@ -299,6 +303,7 @@ extern "C" int cv_create(const cv_params *p, const cv_eos *eos, int device, int 
         if (hipGetDeviceProperties(&prop, device) == hipSuccess) h->cus = prop.multiProcessorCount;
     }
     if (!ph.nb) h->kernel = 0;     // the tiled kernel needs 32-bit plane offsets
+    else if (!march_ok(h)) h->kernel = 1;
     if (ok && (rc = default_tile(h)) != CV_OK) {
         cv_destroy(h);
         return rc;
@@ -756,6 +761,9 @@ extern "C" int cv_set_stage_kernel(cv_handle *h, int kind, int ty, int tz) {
     if (kind < 0 || kind > 2) return fail(CV_ERR_ARG, "kernel kind must be 0 (per-cell), 1 (tiled) or 2 (tiled x-march)");
     if (kind >= 1 && !h->phys.nb)
         return fail(CV_ERR_UNSUPPORTED, "tiled kernel needs 40 * cells < 2^32 (%lld cells)", (long long)h->n);
+    if (kind == 2 && !march_ok(h))
+        return fail(CV_ERR_UNSUPPORTED, "x-march kernel: octave EOS tables of at most %d nodes (this one: %d)",
+                    CV_PTAB_LDS, h->eos.nlast + 1);
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (kind >= 1) {
         int rc = (ty > 0 && tz > 0) ? set_tile(h, ty, tz) : default_tile(h);
@@ -1311,7 +1319,9 @@ extern "C" int cv_batch_create(const cv_params *base, const cv_eos *eos, int nev
         p.nx = dims[3 * e]; p.ny = dims[3 * e + 1]; p.nz = dims[3 * e + 2];
         cv_handle *h = nullptr;
         int rc = cv_create(&p, eos, device, 1, &h);
-        if (rc == CV_OK && h->kernel != 2) rc = fail(CV_ERR_UNSUPPORTED, "event %d: grid too large for the batch", e);
+        if (rc == CV_OK && h->kernel != 2)
+            rc = fail(CV_ERR_UNSUPPORTED, "event %d: the batch needs the x-march kernel (grid below 2^32 / 40 cells, "
+                                          "octave EOS of at most %d nodes)", e, CV_PTAB_LDS);
         if (rc != CV_OK) {
             if (h) cv_destroy(h);
             for (cv_handle *g : b->ev) cv_destroy(g);

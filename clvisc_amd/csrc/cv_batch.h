@@ -43,6 +43,12 @@ __global__ void k_batch_advance(BatchState *bs) {
     if (threadIdx.x == 0 && blockIdx.x == 0) bs->step = bs->step + 1;
 }
 
+template <int EK>
+__device__ __forceinline__ auto make_batch_eos(const EosP &E, const float *pt) {
+    if constexpr (EK == 1) return EosOctP(E, E.tab, pt);
+    else return EosT<EK>(E, E.tab);
+}
+
 // SPEC: 0 or CV_SPEC_FP64 (the FP64 root-solve branch for fast cells, CV_FLAG_FP64_ROOT).
 template <bool BULK, int STEP, int EK, int SPEC>
 __global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchState *__restrict__ bs,
@@ -51,7 +57,11 @@ __global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchS
     __shared__ float s_c[CF_N][CV_TILE_MAXC];
     __shared__ float s_f[15][CV_TILE_MAXF];
     __shared__ int s_work;
-    EosT<EK> eos(E, E.tab);
+    // octave table: its pressure column in LDS, as in the single-event marching kernel
+    __shared__ float s_pt[EK == 1 ? CV_PTAB_LDS : 1];
+    if constexpr (EK == 1)
+        for (int k = threadIdx.x; k <= E.nlast; k += CV_TILE_THREADS) s_pt[k] = E.tab[k].y;
+    const auto eos = make_batch_eos<EK>(E, s_pt);
     const int64_t n = bs->step;
     const int par = (int)(n & 1);
     // tau of this stage as cv_visc_steps_async computes it on the host (tau_at)

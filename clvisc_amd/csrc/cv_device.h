@@ -196,6 +196,39 @@ template <int K> struct EosT {
 };
 using Eos = EosT<-1>;
 
+// Octave table whose pressure column (P at the nodes) is held in LDS (pt, nlast + 1 floats):
+// P() reads the column, the other functions the whole table through the caches.  P is the
+// .y component of eos_octave's lerp bit for bit (same node, weight and operations), so the
+// results are those of EosT<1>; only the latency of the pressure lookups changes.
+struct EosOctP {
+    const EosP &E;
+    const float4 *tab;
+    const float *pt;
+    __device__ EosOctP(const EosP &e, const float4 *t, const float *p_) : E(e), tab(t), pt(p_) {}
+    __device__ __forceinline__ bool octave() const { return true; }
+    __device__ __forceinline__ float4 raw(float eps) const { return eos_octave(E, tab, eps); }
+    __device__ __forceinline__ bool in_table(float) const { return true; }
+    __device__ __forceinline__ float P(float eps) const {
+        if (!(eps >= E.lo_e)) {
+            float r = eps > 0.0f ? eps * E.r_lo : 0.0f;
+            return r * pt[0];
+        }
+        if (eps >= E.hi_e) return (eps * E.r_hi) * pt[E.nlast];
+        int ex;
+        float fr = frexpf(eps, &ex);
+        float f = 2.0f * fr - 1.0f;
+        float tt = f * (float)(1 << E.lj);
+        int j = (int)tt;
+        float r = tt - (float)j;
+        int n = ((ex - 1 - E.kmin) << E.lj) + j;
+        float s = 1.0f - r;
+        return s * pt[n] + r * pt[n + 1];
+    }
+    __device__ __forceinline__ float S(float eps) const { return raw(eps).w; }
+    __device__ __forceinline__ float T(float eps) const { return raw(eps).z; }
+    __device__ __forceinline__ float CS2(float eps) const { return raw(eps).x; }
+};
+
 // ---------------------------------------------------------------------------
 // Hydro helpers (helper.h)
 // ---------------------------------------------------------------------------

@@ -14,7 +14,9 @@
 // (kernel_visc.cl:86-124, helper.h:275-302): both halves are the same function of the
 // cells i-1, i, i+1, i+2, so evaluating it once gives the reference's value.
 // Boundary ghosts (kernel_ideal.cl:57-62) are the clamped cell indices.
-// The octave EOS table is copied into LDS once per workgroup.
+// EOS: the x-marching kernel keeps the pressure column of the octave table in LDS (the
+// pressure is looked up about ten times per cell and stage, the root solve's iterations in a
+// dependent chain); the plane-tiled kernel copies the whole octave table to LDS.
 #pragma once
 #include "cv_kernels.h"
 
@@ -24,11 +26,14 @@ namespace cv {
 enum { CF_E = 0, CF_VX, CF_VY, CF_VZ, CF_U0, CF_P, CF_Q0, CF_Q1, CF_Q2, CF_Q3, CF_PI0, CF_BK = CF_PI0 + 10, CF_N };
 
 #define CV_TILE_THREADS 256
-#define CV_TILE_MAXC 405       // cells of tile + halo held in LDS (3 marching workgroups per CU)
-#define CV_TILE_MAXF 288       // faces of one axis held in LDS
-#define CV_EOS_LDS 1600        // octave-table entries held in LDS
-// The octave table is copied to LDS by the plane-tiled kernel; the marching kernel reads it
-// through the caches and spends the LDS on a third workgroup per CU instead (3 x 51 KB).
+// LDS of a marching workgroup: the cells of tile + halo (21 floats each), the faces of one
+// axis (15 floats each) and the pressure column of the octave table, 53.3 KB, so that 3
+// workgroups fit a CU's 160 KB.  The capacities are those of the default 10 x 23 tile of the
+// benchmark grid (14 x 27 = 378 cells, 11 x 23 = 253 faces).
+#define CV_TILE_MAXC 378       // cells of tile + halo held in LDS
+#define CV_TILE_MAXF 256       // faces of one axis held in LDS
+#define CV_EOS_LDS 1600        // octave-table entries held in LDS by the plane-tiled kernel
+#define CV_PTAB_LDS 1540       // octave-table pressures held in LDS by the marching kernels
 #ifndef CV_TILE_EOS_IN_LDS
 #define CV_TILE_EOS_IN_LDS 1
 #endif
@@ -162,8 +167,8 @@ struct StagePtrs {
 // workgroup walks the chunk in increasing x and each thread carries the x face F(i-1/2),
 // the slopes of its cell i and u^mu of cell i-1 from one plane to the next, so every x face
 // and every x slope is evaluated once (the chunk's first plane starts from the full stencil).
-template <bool BULK, int STEP, int EK, int SPEC, bool MARCH>
-__device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, const StageP &st, const TileP &tp,
+template <bool BULK, int STEP, int EK, int SPEC, bool MARCH, class EOS>
+__device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const StageP &st, const TileP &tp,
                                           const StagePtrs &q_, int64_t work, float (*s_c)[CV_TILE_MAXC],
                                           float (*s_f)[CV_TILE_MAXF]) {
     const float *__restrict__ ev_st = q_.ev_st;
@@ -173,8 +178,6 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EosT<EK> &eos, c
     const float *__restrict__ udiff_in = q_.udiff_in;
     float *IS_src = q_.IS_src, *bsrc = q_.bsrc, *Src = q_.Src, *pi_out = q_.pi_out, *bk_out = q_.bk_out;
     float *ev_out = q_.ev_out, *good = q_.good, *udiff_out = q_.udiff_out;
-    const EosP &E = eos.E;
-    const float4 *tab = eos.tab;
     const int tid = threadIdx.x;
     const uint32_t nb = p.nb;
     const float tau = st.tau;
@@ -646,23 +649,38 @@ __global__ __launch_bounds__(CV_TILE_THREADS, MARCH ? 3 : 2) void k_visc_tiled(P
                                                                               StagePtrs q) {
     __shared__ float s_c[CF_N][CV_TILE_MAXC];
     __shared__ float s_f[15][CV_TILE_MAXF];
-    __shared__ float4 s_eos[CV_EOS_LDS];
     const int tid = threadIdx.x;
-    // EOS table into LDS (octave table small enough), else global
-    const float4 *tab = E.tab;
-    if (CV_TILE_EOS_IN_LDS && !MARCH && EK == 1 && E.nlast + 1 <= CV_EOS_LDS) {
-        for (int k = tid; k <= E.nlast; k += CV_TILE_THREADS) s_eos[k] = E.tab[k];
-        tab = s_eos;
-    }
-    __syncthreads();
-    EosT<EK> eos(E, tab);
     const int64_t total = MARCH ? tp.nwork : tp.ntiles;
-    if (tp.xcd > 0 && (int64_t)gridDim.x == total) {
-        tile_work<BULK, STEP, EK, SPEC, MARCH>(p, eos, st, tp, q, xcd_work(blockIdx.x, total, tp.xcd), s_c, s_f);
-        return;
+    if constexpr (MARCH && EK == 1) {
+        // the pressure column of the octave table (the host launches this kernel only for
+        // tables of at most CV_PTAB_LDS nodes)
+        __shared__ float s_pt[CV_PTAB_LDS];
+        for (int k = tid; k <= E.nlast; k += CV_TILE_THREADS) s_pt[k] = E.tab[k].y;
+        __syncthreads();
+        EosOctP eos(E, E.tab, s_pt);
+        if (tp.xcd > 0 && (int64_t)gridDim.x == total) {
+            tile_work<BULK, STEP, EK, SPEC, MARCH>(p, eos, st, tp, q, xcd_work(blockIdx.x, total, tp.xcd), s_c, s_f);
+            return;
+        }
+        for (int64_t work = blockIdx.x; work < total; work += gridDim.x)
+            tile_work<BULK, STEP, EK, SPEC, MARCH>(p, eos, st, tp, q, work, s_c, s_f);
+    } else {
+        // EOS table into LDS (octave table small enough), else global
+        __shared__ float4 s_eos[(!MARCH && EK == 1) ? CV_EOS_LDS : 1];
+        const float4 *tab = E.tab;
+        if (CV_TILE_EOS_IN_LDS && !MARCH && EK == 1 && E.nlast + 1 <= CV_EOS_LDS) {
+            for (int k = tid; k <= E.nlast; k += CV_TILE_THREADS) s_eos[k] = E.tab[k];
+            tab = s_eos;
+        }
+        __syncthreads();
+        EosT<EK> eos(E, tab);
+        if (tp.xcd > 0 && (int64_t)gridDim.x == total) {
+            tile_work<BULK, STEP, EK, SPEC, MARCH>(p, eos, st, tp, q, xcd_work(blockIdx.x, total, tp.xcd), s_c, s_f);
+            return;
+        }
+        for (int64_t work = blockIdx.x; work < total; work += gridDim.x)
+            tile_work<BULK, STEP, EK, SPEC, MARCH>(p, eos, st, tp, q, work, s_c, s_f);
     }
-    for (int64_t work = blockIdx.x; work < total; work += gridDim.x)
-        tile_work<BULK, STEP, EK, SPEC, MARCH>(p, eos, st, tp, q, work, s_c, s_f);
 }
 
 // The end of update_ev for the split stage (kernel_visc.cl:322-393 from the clamp on):

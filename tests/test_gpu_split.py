@@ -43,7 +43,7 @@ def _start(p, tab, ev, kernel=None):
 
 
 @pytest.mark.parametrize('kernel,bulk', [((2, 0, 0), None), ((2, 0, 0), (0.05, 0.155, 0.02)),
-                                         ((1, 0, 0), (0.05, 0.155, 0.02)), ((2, 7, 32), None)])
+                                         ((1, 0, 0), (0.05, 0.155, 0.02)), ((2, 7, 30), None)])
 def test_split_without_seed_equals_fused(kernel, bulk):
     p, tab, ev = _case(29, 31, 11, bulk=bulk)
     _say('case: split vs fused stage, 29x31x11, kernel %s, bulk %s, 20 steps' % (kernel, bulk is not None))
