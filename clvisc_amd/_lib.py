@@ -118,6 +118,7 @@ def lib():
             'cv_max_ed': [vp, C.c_int, C.POINTER(f32)],
             'cv_energy_sum': [vp, C.c_int, C.c_double, vp],
             'cv_slice_sums': [vp, C.c_int, vp],
+            'cv_slice_sums_noalloc': [vp, C.c_int, vp],
             'cv_glauber_ini': [vp, vp, C.c_int],
             'cv_glauber_nbinary': [vp, vp, f32, f32, f32, f32, C.c_int, C.c_int, vp],
             'cv_smear_partons': [vp, vp, i64, f32, f32, f32, C.c_int],
@@ -195,7 +196,7 @@ def exported_symbols():
             'cv_batch_steps_async', 'cv_batch_set_loop', 'cv_batch_set_xcd_map', 'cv_batch_sync', 'cv_batch_last_elapsed_ms',
             'cv_batch_info', 'cv_mlp_nparam', 'cv_mlp_create', 'cv_mlp_destroy', 'cv_mlp_get_params',
             'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward', 'cv_mlp_features',
-            'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums', 'cv_glauber_ini',
+            'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums', 'cv_slice_sums_noalloc', 'cv_glauber_ini',
             'cv_glauber_nbinary', 'cv_smear_partons', 'cv_longitudinal_profile',
             'cv_sd_to_ed', 'cv_vorticity_enable', 'cv_vorticity', 'cv_vorticity_mu', 'cv_hypersf_get_vorticity',
             'cv_polarization', 'cv_set_xcd_map']
@@ -308,6 +309,12 @@ class Handle(object):
         out = np.zeros((self.params.nz, 10), np.float64)
         check(self.L.cv_slice_sums(self.h, which, out.ctypes.data))
         return out
+
+    def slice_sums_noalloc(self, d_out, which=1):
+        """cv_slice_sums into caller-owned device memory (d_out: device address of NZ * 10
+        doubles) with the library's static scratch array: allocates nothing, returns without
+        waiting (sync() before reading d_out)."""
+        check(self.L.cv_slice_sums_noalloc(self.h, which, C.c_void_p(int(d_out))))
 
     def glauber_ini(self, glauber, which=1):
         """Optical Glauber initial condition into slot `which` (cv_glauber_ini)."""

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 
 #include "../../include/clvisc_amd.h"
 #include "cv_kernels.h"
@@ -839,28 +840,85 @@ extern "C" int cv_energy_sum(cv_handle *h, int which, double tau, double *out) {
     return CV_OK;
 }
 
+// Blocks of the slice-sum pass: one per group of (i, j) rows, at most 1024.
+static int slice_blocks(const cv_handle *h) {
+    return (int)std::min<int64_t>(1024, (int64_t)h->prm.nx * h->prm.ny);
+}
+
+// k_slice_sums into part (nb x nz x CV_SLICE_Q doubles), k_slice_final into sums (nz x
+// CV_SLICE_Q doubles), on the handle's stream
+static void launch_slice_sums(cv_handle *h, int which, double *part, double *sums) {
+    const int nz = h->prm.nz;
+    const int64_t nrow = (int64_t)h->prm.nx * h->prm.ny;
+    const int nb = slice_blocks(h);
+    const int64_t rpb = (nrow + nb - 1) / nb;
+    const size_t nq = (size_t)nz * CV_SLICE_Q;
+    hipLaunchKernelGGL(k_slice_sums, dim3(nb), dim3(128), 0, h->stream, h->phys, h->eos, h->ev[which], rpb, part);
+    hipLaunchKernelGGL(k_slice_final, dim3((unsigned)nq), dim3(256), 0, h->stream, part, nb, nz, sums);
+}
+
 extern "C" int cv_slice_sums(cv_handle *h, int which, double *out) {
     if (!h || !out) return fail(CV_ERR_ARG, "null argument");
     if (which < 0 || which > 2) return fail(CV_ERR_ARG, "which must be 0, 1 or 2");
     HIP_TRY(hipSetDevice(h->device));
-    const int nz = h->prm.nz;
-    const int64_t nrow = (int64_t)h->prm.nx * h->prm.ny;
-    const int nb = (int)std::min<int64_t>(1024, nrow);
-    const int64_t rpb = (nrow + nb - 1) / nb;
-    const size_t nq = (size_t)nz * CV_SLICE_Q;
+    const int nb = slice_blocks(h);
+    const size_t nq = (size_t)h->prm.nz * CV_SLICE_Q;
     // nb and nz are fixed by the grid, so one buffer per handle serves every call
     if (!h->d_slice && hipMalloc((void **)&h->d_slice, sizeof(double) * (nq * (size_t)nb + nq)) != hipSuccess) {
         h->d_slice = nullptr;
         return fail(CV_ERR_HIP, "cv_slice_sums: allocation failed");
     }
     double *d = h->d_slice;
-    hipLaunchKernelGGL(k_slice_sums, dim3(nb), dim3(128), 0, h->stream, h->phys, h->eos, h->ev[which], rpb, d);
-    hipLaunchKernelGGL(k_slice_final, dim3((unsigned)nq), dim3(256), 0, h->stream, d, nb, nz, d + nq * (size_t)nb);
+    launch_slice_sums(h, which, d, d + nq * (size_t)nb);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out, d + nq * (size_t)nb, sizeof(double) * nq, hipMemcpyDeviceToHost,
                                             h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_slice_sums: %s", hipGetErrorString(e));
+    return CV_OK;
+}
+
+// ---- cv_slice_sums_noalloc: the slice sums with a static scratch array ------------------
+// The partial sums of k_slice_sums (at most 1024 blocks x CV_NOALLOC_NZ slices x CV_SLICE_Q
+// doubles, 10.5 MB) live in this file-scope device array, one copy per GPU for the life of
+// the process, so the entry point allocates nothing.  Host code must not pass the array's
+// name to a kernel (it is a host-side stand-in): its device address, per device, comes from
+// hipGetSymbolAddress with that device current, taken once per device and kept below.
+#define CV_NOALLOC_NZ 128
+#define CV_NOALLOC_SCRATCH ((size_t)1024 * CV_NOALLOC_NZ * CV_SLICE_Q)
+static __device__ __align__(256) double scratch[CV_NOALLOC_SCRATCH];
+#define CV_MAX_DEVICES 64
+static double *g_scratch_ptr[CV_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// device address of `scratch` on the current device h->device (set up on the first call)
+static int scratch_ptr(int device, double **out) {
+    if (device < 0 || device >= CV_MAX_DEVICES) return fail(CV_ERR_ARG, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(CV_ERR_HIP, "cv_slice_sums_noalloc: hipGetSymbolAddress(scratch) on device %d: %s", device,
+                        hipGetErrorString(e));
+        g_scratch_ptr[device] = (double *)ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return CV_OK;
+}
+
+extern "C" int cv_slice_sums_noalloc(cv_handle *h, int which, double *d_out) {
+    if (!h || !d_out) return fail(CV_ERR_ARG, "null argument");
+    if (which < 0 || which > 2) return fail(CV_ERR_ARG, "which must be 0, 1 or 2");
+    if (h->prm.nz > CV_NOALLOC_NZ)
+        return fail(CV_ERR_CAPACITY, "cv_slice_sums_noalloc: %d eta slices exceed the %d of its static scratch array "
+                                     "(use cv_slice_sums)", h->prm.nz, CV_NOALLOC_NZ);
+    HIP_TRY(hipSetDevice(h->device));
+    double *part = nullptr;
+    int rc = scratch_ptr(h->device, &part);
+    if (rc) return rc;
+    launch_slice_sums(h, which, part, d_out);
+    HIP_TRY(hipGetLastError());
     return CV_OK;
 }
 

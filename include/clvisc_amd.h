@@ -183,6 +183,15 @@ int cv_energy_sum(cv_handle *h, int which, double tau, double *out);
  * slice's lab-frame energy density sum is cosh(eta_k) (q6 - q8) + sinh(eta_k) q7.  Double
  * sums, fixed order. */
 int cv_slice_sums(cv_handle *h, int which, double *out);
+/* No counterpart in the reference.  cv_slice_sums for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the same kernels and sums, with the partial sums in a static device array of
+ * the library (at most 128 eta slices; 10.5 MB per GPU, held until the process ends) instead
+ * of a buffer allocated on the first call.  d_out: caller-owned device memory of nz * 10
+ * doubles, written on the handle's stream; the call returns without waiting (cv_sync).
+ * Grids of more than 128 eta slices fail with CV_ERR_CAPACITY.  Calls that overlap on one GPU
+ * share the array and race. */
+int cv_slice_sums_noalloc(cv_handle *h, int which, double *d_out);
 
 /* ---- initial conditions on the device ------------------------------------------------
  * glauber_ini (kernel_glauber.cl:82-102, Glauber.__init__ pyvisc/glauber.py:63-70): slot
