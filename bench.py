@@ -395,14 +395,16 @@ def run_learned(args, p, tab, ev, h_plain, loop_end):
             'learned_over_plain': None}
 
 
-def combined_run(p, tab, nev, max_loops=3000, sf_capacity=6_000_000, sf_out='', check_event=None):
+def combined_run(p, tab, nev, max_loops=3000, sf_capacity=6_000_000, sf_out='', check_event=None, seed_mlp=None):
     """`nev` batched 385x385x115 viscous + bulk events (400 hotspots each, their own seeds), FP64
     root solve for cells with |T^ti| > 0.99 T^tt, evolved until every event's max e drops below
     e(T = 0.137 GeV) (checked every ntskip = 20 steps, as CLVisc.evolve does), freeze-out surface
     found every ntskip steps (nxskip = nyskip = 3, nzskip = 1), FP64 energy balance of one step in
     every ntskip.  Finished events leave the batch.  The plain GPU form (the same events stepped
     one after another) is timed over a window of steps.  check_event: an event index whose
-    first ntskip steps are also run on its own handle and compared bit for bit.  Returns a dict."""
+    first ntskip steps are also run on its own handle and compared bit for bit.  seed_mlp: the
+    learned + batched form (cv_batch_set_root_seed: split stages, the network's seeds for the
+    root solve, FP64 for the fast cells).  Returns a dict."""
     from clvisc_amd import _lib
     from clvisc_amd import ic as ic_mod
     ntskip = 20
@@ -431,6 +433,8 @@ def combined_run(p, tab, nev, max_loops=3000, sf_capacity=6_000_000, sf_out='', 
             single.ideal_stage(1, np.float32(TAU0))
             single.set('ev1', ev)
     del ev
+    if seed_mlp is not None:
+        batch.set_root_seed(seed_mlp)
     log('combined: %d events of %dx%dx%d set up in %.1fs (e_frz %.4g GeV/fm^3)'
         % (nev, NX, NY, NZ, time.perf_counter() - t_start, efrz))
     tau_of = lambda n: float(np.float32(TAU0 + n * DT))
@@ -527,10 +531,48 @@ def combined_run(p, tab, nev, max_loops=3000, sf_capacity=6_000_000, sf_out='', 
             'step_value': updates / (dev_ms / 1e3), 'plain_sequential_value': seq}
 
 
+def train_combined_seed(p, tab):
+    """The seed network of the learned + batched combined run: 2000 x 4096 Adam steps on the
+    states of the first six split steps of the first event (seed 20240601)."""
+    from clvisc_amd import _lib, surrogate
+    from clvisc_amd import ic as ic_mod
+    h = _lib.Handle(p, tab, viscous=True)
+    pos = ic_mod.woods_saxon_positions(400, 20240601)
+    h.visc_initialize(TAU0)
+    h.ideal_stage(1, np.float32(TAU0))
+    h.set('ev1', ic_mod.hotspots(NX, NY, NZ, DX, DY, DZ, positions=pos, emax=50.0))
+    t0 = time.perf_counter()
+    states = surrogate.visc_run_states(h, 0, 6, every=3)
+    h.close()
+    mlp, hist = surrogate.train_seed(states, steps=2000, batch=4096, lr=3e-3, nmax=5_000_000)
+    info = {'states': int(states[0].size), 'seconds': time.perf_counter() - t0,
+            'loss_first': float(hist[:20].mean()), 'loss_last': float(hist[-20:].mean())}
+    log('combined learned: seed trained on %d states of 6 split steps of event 0 in %.1fs, loss %.4g -> %.4g'
+        % (info['states'], info['seconds'], info['loss_first'], info['loss_last']))
+    return mlp, info
+
+
 def run_combined(args, p, tab, name):
-    """Combined workload (combined_run) as one JSON line."""
+    """Combined workload (combined_run) as one JSON line; with --learned also the learned +
+    batched form of the same run (the seed network trained first), reported against it."""
     nev = args.events
+    learned = None
+    if args.learned:
+        mlp, info = train_combined_seed(p, tab)
+        rl = combined_run(p, tab, nev, args.max_loops, args.sf_capacity, '', seed_mlp=mlp)
+        mlp.close()
+        learned = {'value': rl['value'], 'step_only_value': rl['step_value'], 'wall_s': rl['wall'],
+                   'loops': rl['loops'], 'finish_tau': rl['finish_tau'], 'surface_elements': rl['surface_elements'],
+                   'energy_balance_max_abs': float(np.abs(rl['balance']).max()),
+                   'energy_balance_median_abs': float(np.median(np.abs(rl['balance']))),
+                   'training': info}
     r = combined_run(p, tab, nev, args.max_loops, args.sf_capacity, args.sf_out)
+    if learned is not None:
+        learned['learned_over_plain'] = learned['value'] / r['value']
+        learned['step_only_learned_over_plain'] = learned['step_only_value'] / r['step_value']
+        log('combined learned + batched: %.4g cell-updates/s (steps only %.4g) against plain batched %.4g (%.4g): '
+            '%.3fx' % (learned['value'], learned['step_only_value'], r['value'], r['step_value'],
+                       learned['learned_over_plain']))
     value, wall, loop, res = r['value'], r['wall'], r['loops'], r['balance']
     step_value, seq, energy, seeds = r['step_value'], r['plain_sequential_value'], r['energy'], r['seeds']
     abytes = sum(ALGO_BYTES[True])
@@ -551,7 +593,7 @@ def run_combined(args, p, tab, name):
                    'energy_balance_median_abs': float(np.median(np.abs(res))),
                    'energy_first_last': [[energy[e][0][1], energy[e][-1][1]] for e in range(min(nev, 4))],
                    'step_only_value': step_value, 'plain_sequential_value': seq,
-                   'batched_over_plain': step_value / seq},
+                   'batched_over_plain': step_value / seq, 'learned_batched': learned},
         'roofline': {'bound': 'hbm', 'kernel': 'batch_stage (both stages)',
                      'achieved': round(abytes * step_value / 1e9, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(abytes * step_value / 1e9 / HBM_PEAK_GBS, 5), 'traffic': None,
@@ -758,6 +800,8 @@ def main():
     ap.add_argument('--workload', choices=['single', 'ensemble', 'surrogate', 'combined', 'aux'], default='single',
                     help='single: one 385x385x115 event (the headline); ensemble: 64 uneven events batched; '
                          'surrogate: learned root-solve seed on the ideal Gubser workload')
+    ap.add_argument('--learned', action='store_true',
+                    help='combined: also the learned + batched form (seeded root solve) of the run')
     ap.add_argument('--autotune', action='store_true', help='ensemble: tune tiles and chunks per grid shape')
     ap.add_argument('--autotune-cache', default='', help='ensemble: JSON file of tuned shapes')
     ap.add_argument('--counter-pass', action='store_true',

@@ -163,6 +163,7 @@ def lib():
             'cv_mlp_train': [vp, C.c_int, C.c_int, f32, vp],
             'cv_mlp_forward': [vp, vp, i64, vp],
             'cv_mlp_features': [vp, vp, vp, vp, i64, vp],
+            'cv_batch_set_root_seed': [vp, vp],
             'cv_root_eval': [vp, vp, vp, vp, vp, i64, vp, vp, vp],
             'cv_set_root_seed': [vp, vp, C.c_int],
             'cv_set_root_split': [vp, C.c_int],
@@ -195,7 +196,7 @@ def exported_symbols():
             'cv_batch_create', 'cv_batch_destroy', 'cv_batch_handle', 'cv_batch_set_active',
             'cv_batch_steps_async', 'cv_batch_set_loop', 'cv_batch_set_xcd_map', 'cv_batch_sync', 'cv_batch_last_elapsed_ms',
             'cv_batch_info', 'cv_mlp_nparam', 'cv_mlp_create', 'cv_mlp_destroy', 'cv_mlp_get_params',
-            'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward', 'cv_mlp_features',
+            'cv_mlp_set_params', 'cv_mlp_grad', 'cv_mlp_set_data', 'cv_mlp_train', 'cv_mlp_forward', 'cv_mlp_features', 'cv_batch_set_root_seed',
             'cv_root_eval', 'cv_set_root_seed', 'cv_get_root_iters', 'cv_set_root_split', 'cv_energy_sum', 'cv_slice_sums', 'cv_slice_sums_noalloc', 'cv_glauber_ini',
             'cv_glauber_nbinary', 'cv_smear_partons', 'cv_longitudinal_profile',
             'cv_sd_to_ed', 'cv_vorticity_enable', 'cv_vorticity', 'cv_vorticity_mu', 'cv_hypersf_get_vorticity',
@@ -503,6 +504,12 @@ class Batch(object):
 
     def set_loop(self, loop):
         check(self.L.cv_batch_set_loop(self.b, int(loop)))
+
+    def set_root_seed(self, mlp):
+        """Seeded batch stages (cv_batch_set_root_seed): the split form with the network's
+        seeds (FP64 root for fast cells under CV_FLAG_FP64_ROOT); None: the fused stages."""
+        check(self.L.cv_batch_set_root_seed(self.b, mlp.m if mlp is not None else None))
+        self._seed = mlp            # keep the network alive while the batch uses it
 
     def set_xcd_map(self, nxcd):
         check(self.L.cv_batch_set_xcd_map(self.b, int(nxcd)))

@@ -1352,6 +1352,7 @@ struct cv_batch {
     int grid = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    cv_mlp *seed = nullptr;                // seeded batches: the stages run split (cv_batch_set_root_seed)
 };
 
 static void batch_drop_graph(cv_batch *b) {
@@ -1493,6 +1494,8 @@ static int batch_build(cv_batch *b) {
         for (int k = 0; k < 3; k++) { t.ev[k] = h->ev[k]; t.pi[k] = h->pi[k]; t.bk[k] = h->bk[k]; }
         t.Src = h->Src; t.IS_src = h->IS_src; t.bsrc = h->bsrc; t.udiff = h->udiff; t.good = h->good;
         t.fix = h->d_fix; t.fix_ctr = h->d_fix_ctr;
+        t.cons = b->seed ? h->seed_buf : nullptr;
+        t.seed = b->seed ? h->seed_buf + 9 * h->n : nullptr;
         w += t.tp.nwork;
     }
     if (w >= ((int64_t)1 << 31)) return fail(CV_ERR_CAPACITY, "batch: %lld work items exceed 2^31", (long long)w);
@@ -1522,9 +1525,14 @@ static int batch_build(cv_batch *b) {
 #define CV_BATCH_LAUNCH2(B, S, K, X) \
     hipLaunchKernelGGL((k_batch_stage<B, S, K, X>), g, blk, 0, b->stream, b->d_state, b->d_events, b->ev[0]->eos, \
                        b->d_ctr)
+    // seeded: split stages (the root solve's input to each event's cons), then the seed network
+    // and the root solve over all events; FP64 cells are solved there, not deferred
+    const bool seeded = b->seed != nullptr;
+    const bool defer = b->fp64_root && !seeded;
 #define CV_BATCH_LAUNCH(B, S)                                                              \
     do {                                                                                   \
-        if (b->ek == 1) { if (b->fp64_root) CV_BATCH_LAUNCH2(B, S, 1, CV_SPEC_FP64); else CV_BATCH_LAUNCH2(B, S, 1, 0); } \
+        if (seeded) { if (b->ek == 1) CV_BATCH_LAUNCH2(B, S, 1, CV_SPEC_SPLIT); else CV_BATCH_LAUNCH2(B, S, 0, CV_SPEC_SPLIT); } \
+        else if (b->ek == 1) { if (b->fp64_root) CV_BATCH_LAUNCH2(B, S, 1, CV_SPEC_FP64); else CV_BATCH_LAUNCH2(B, S, 1, 0); } \
         else { if (b->fp64_root) CV_BATCH_LAUNCH2(B, S, 0, CV_SPEC_FP64); else CV_BATCH_LAUNCH2(B, S, 0, 0); }            \
     } while (0)
     // deferred FP64 root solves of each stage, per event
@@ -1532,19 +1540,34 @@ static int batch_build(cv_batch *b) {
     dim3 fg(std::max(1, 4 * (b->ev[0]->cus > 0 ? b->ev[0]->cus : 256) / std::max(1, nact)), nact), fb(256);
 #define CV_BFIX_LAUNCH(B, S, K) hipLaunchKernelGGL((k_batch_fp64_fix<B, S, K>), fg, fb, 0, b->stream, b->d_state, \
                                                    b->d_events, b->ev[0]->eos)
+    // seeded: the network over every event's cells, then the root solve (grids sized by the
+    // largest event; blocks beyond an event's cells return at once)
+    int64_t nmax = 0;
+    for (int e = 0; e < (int)b->ev.size(); e++)
+        if (b->active[e]) nmax = std::max<int64_t>(nmax, b->ev[e]->n);
+    const int cus = b->ev[0]->cus > 0 ? b->ev[0]->cus : 256;
+    dim3 sg((unsigned)std::min<int64_t>((nmax + MLP_SEED_PER_BLOCK - 1) / MLP_SEED_PER_BLOCK,
+                                        std::max<int64_t>(1, 8 * cus / nact)), nact), sb(64 * MLP_WAVES);
+    dim3 rg((unsigned)std::min<int64_t>((nmax + 255) / 256, std::max<int64_t>(1, 8 * cus / nact)), nact), rb(256);
+#define CV_BROOT_LAUNCH(B, S, K) hipLaunchKernelGGL((k_batch_visc_root<B, S, K>), rg, rb, 0, b->stream, b->d_state, \
+                                                    b->d_events, b->ev[0]->eos)
 #define CV_BATCH_FIX(B, S)                                                                                    \
     do {                                                                                                      \
-        if (b->fp64_root) { if (b->ek == 1) CV_BFIX_LAUNCH(B, S, 1); else CV_BFIX_LAUNCH(B, S, 0); }          \
+        if (seeded) {                                                                                         \
+            hipLaunchKernelGGL(k_batch_seed, sg, sb, 0, b->stream, b->d_state, b->d_events, b->seed->P);      \
+            if (b->ek == 1) CV_BROOT_LAUNCH(B, S, 1); else CV_BROOT_LAUNCH(B, S, 0);                          \
+        } else if (defer) { if (b->ek == 1) CV_BFIX_LAUNCH(B, S, 1); else CV_BFIX_LAUNCH(B, S, 0); }          \
     } while (0)
-    if (b->fp64_root)
+    if (defer)
         hipLaunchKernelGGL(k_batch_fix_reset, dim3((nact + 63) / 64), dim3(64), 0, b->stream, b->d_state, b->d_events);
     if (b->bulk) { CV_BATCH_LAUNCH(true, 1); CV_BATCH_FIX(true, 1); }
     else { CV_BATCH_LAUNCH(false, 1); CV_BATCH_FIX(false, 1); }
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_ctr);
-    if (b->fp64_root)
+    if (defer)
         hipLaunchKernelGGL(k_batch_fix_reset, dim3((nact + 63) / 64), dim3(64), 0, b->stream, b->d_state, b->d_events);
     if (b->bulk) { CV_BATCH_LAUNCH(true, 2); CV_BATCH_FIX(true, 2); }
     else { CV_BATCH_LAUNCH(false, 2); CV_BATCH_FIX(false, 2); }
+#undef CV_BROOT_LAUNCH
 #undef CV_BATCH_FIX
 #undef CV_BFIX_LAUNCH
 #undef CV_BATCH_LAUNCH
@@ -1587,6 +1610,25 @@ extern "C" int cv_batch_steps_async(cv_batch *b, int64_t nsteps) {
     // the device step counter advanced with the host loop, and its buffer parity with the
     // host slot swaps above: the same graph serves the next call
     b->loop += nsteps;
+    return CV_OK;
+}
+
+extern "C" int cv_batch_set_root_seed(cv_batch *b, cv_mlp *m) {
+    if (!b) return fail(CV_ERR_ARG, "null batch");
+    int rc = batch_settle(b);
+    if (rc) return rc;
+    // every event's own handle takes the network too (split stages, the same buffers), so
+    // that stepping an event alone gives what the batch gives
+    for (cv_handle *h : b->ev) {
+        rc = m ? cv_set_root_seed(h, m, 0) : CV_OK;
+        if (!m && h->seed_mlp) {
+            h->seed_mlp = nullptr;
+            h->root_split = false;
+        }
+        if (rc) return rc;
+    }
+    b->seed = m;
+    b->dirty = true;
     return CV_OK;
 }
 
@@ -1915,8 +1957,8 @@ static int launch_ideal_seeded(cv_handle *h, const StageP &st, float *ev_st, flo
 
 extern "C" int cv_set_root_seed(cv_handle *h, cv_mlp *m, int track_iters) {
     if (!h) return fail(CV_ERR_ARG, "null handle");
-    if (m && (h->prm.flags & CV_FLAG_FP64_ROOT))
-        return fail(CV_ERR_UNSUPPORTED, "cv_set_root_seed: the seed is for the FP32 root solve");
+    if (m && !h->viscous && (h->prm.flags & CV_FLAG_FP64_ROOT))
+        return fail(CV_ERR_UNSUPPORTED, "cv_set_root_seed: the seeded ideal stage has no FP64 root solve");
     HIP_TRY(hipSetDevice(h->device));
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (m && !h->seed_buf) HIP_TRY(hipMalloc((void **)&h->seed_buf, sizeof(float) * 10 * h->n));

@@ -11,6 +11,7 @@
 // as the single-event path.
 #pragma once
 #include "cv_tiled.h"
+#include "cv_mlp.h"
 
 namespace cv {
 
@@ -22,6 +23,7 @@ struct BatchEvent {
     float *Src, *IS_src, *bsrc, *udiff, *good;
     FixEntry *fix;                     // deferred FP64 root solves (CV_FLAG_FP64_ROOT), n entries
     unsigned *fix_ctr;
+    float *cons, *seed;                // seeded batches: the root solve's input (4 n) and the seeds (n)
 };
 
 struct BatchState {
@@ -49,7 +51,9 @@ __device__ __forceinline__ auto make_batch_eos(const EosP &E, const float *pt) {
     else return EosT<EK>(E, E.tab);
 }
 
-// SPEC: 0 or CV_SPEC_FP64 (the FP64 root-solve branch for fast cells, CV_FLAG_FP64_ROOT).
+// SPEC: 0, CV_SPEC_FP64 (the FP64 root-solve branch for fast cells, CV_FLAG_FP64_ROOT) or
+// CV_SPEC_SPLIT (seeded batches: the stage writes the root solve's input to each event's cons
+// and k_batch_seed / k_batch_visc_root finish the cells).
 template <bool BULK, int STEP, int EK, int SPEC>
 __global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchState *__restrict__ bs,
                                                                      const BatchEvent *__restrict__ evt, EosP E,
@@ -109,12 +113,13 @@ __global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchS
         float *P0 = ev.pi[par], *P1 = ev.pi[1 - par], *P2 = ev.pi[2];
         float *B0 = ev.bk[par], *B1 = ev.bk[1 - par], *B2 = ev.bk[2];
         StagePtrs q;
+        float *cons = (SPEC & CV_SPEC_SPLIT) ? ev.cons : nullptr;
         if (STEP == 1) {
             q = StagePtrs{E1, P1, B1, E1, E2, P1, B1, ev.udiff, ev.IS_src, ev.bsrc, ev.Src, P2, B2, E2, ev.good,
-                          nullptr, nullptr, ev.fix, ev.fix_ctr};
+                          nullptr, cons, ev.fix, ev.fix_ctr};
         } else {
             q = StagePtrs{E2, P2, B2, E1, E2, P1, B1, ev.udiff, ev.IS_src, ev.bsrc, ev.Src, P0, B0, E0, ev.good,
-                          ev.udiff, nullptr, ev.fix, ev.fix_ctr};
+                          ev.udiff, cons, ev.fix, ev.fix_ctr};
         }
         tile_work<BULK, STEP, EK, SPEC, true>(ev.p, eos, st, ev.tp, q, w - ev.work0, s_c, s_f);
     }
@@ -138,6 +143,41 @@ __global__ __launch_bounds__(256) void k_batch_fp64_fix(const BatchState *__rest
 __global__ void k_batch_fix_reset(const BatchState *__restrict__ bs, const BatchEvent *__restrict__ evt) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e < bs->nevents) *evt[e].fix_ctr = 0u;
+}
+
+}  // namespace cv
+
+namespace cv {
+
+// Seed network over the root-solve input of every active event (blockIdx.y = entry of the
+// event table): the seeds of the single-event k_mlp_seed_cons, sample by sample.
+__global__ __launch_bounds__(64 * MLP_WAVES) void k_batch_seed(const BatchState *__restrict__ bs,
+                                                             const BatchEvent *__restrict__ evt,
+                                                             const float *__restrict__ P) {
+    __shared__ float s_h[MLP_WAVES][16][MLP_H + 1];
+    if ((int)blockIdx.y >= bs->nevents) return;
+    const BatchEvent &ev = evt[blockIdx.y];
+    const int64_t n = ev.p.n;
+    const int64_t nblk = (n + MLP_SEED_PER_BLOCK - 1) / MLP_SEED_PER_BLOCK;
+    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) seed_cons_block(P, ev.cons, n, ev.seed, blk, s_h);
+}
+
+// The end of update_ev for every cell of every active event (blockIdx.y = entry of the event
+// table), with the slots of this step's parity as k_batch_stage uses them: FP64 for fast
+// cells (CV_FLAG_FP64_ROOT), else from the event's seeds.
+template <bool BULK, int STEP, int EK>
+__global__ __launch_bounds__(256) void k_batch_visc_root(const BatchState *__restrict__ bs,
+                                                         const BatchEvent *__restrict__ evt, EosP E) {
+    if ((int)blockIdx.y >= bs->nevents) return;
+    const BatchEvent &ev = evt[blockIdx.y];
+    const int par = (int)(bs->step & 1);
+    float *E0 = ev.ev[par], *E1 = ev.ev[1 - par], *E2 = ev.ev[2];
+    const float *Bout = STEP == 1 ? ev.bk[2] : ev.bk[par];
+    float *out = STEP == 1 ? E2 : E0;
+    const int64_t n = ev.p.n;
+    for (int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; I < n; I += (int64_t)gridDim.x * blockDim.x)
+        visc_root_cell<BULK, STEP, EK>(ev.p, E, I, ev.cons, Bout, E1, ev.seed, out, STEP == 2 ? ev.udiff : nullptr,
+                                       nullptr);
 }
 
 }  // namespace cv

@@ -363,10 +363,12 @@ __device__ __forceinline__ mlp_bf16x4 pack4_hw(float a, float b, float c, float 
     return r;
 }
 
-__global__ __launch_bounds__(64 * MLP_WAVES) void k_mlp_seed_cons(const float *__restrict__ P,
-                                                                const float *__restrict__ cons, int64_t n,
-                                                                float *seed_v) {
-    __shared__ float s_h[MLP_WAVES][16][MLP_H + 1];
+// One block (MLP_WAVES waves, 16 MLP_WAVES MLP_SEED_TILES samples) of the seed pass over the
+// samples of block blk; all threads of the block call it (it synchronises).  s_h: the
+// block's LDS staging of the hidden layer.  A sample's seed depends on its own features only.
+__device__ __forceinline__ void seed_cons_block(const float *__restrict__ P, const float *__restrict__ cons,
+                                                int64_t n, float *seed_v, int64_t blk,
+                                                float (*s_h)[16][MLP_H + 1]) {
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int c0 = l & 15, rg = l >> 4;
     // this lane's weights: layer-1 columns 16 n + c0, the 64x64 layer's B operands, layer 3 (|v|)
@@ -387,7 +389,7 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_mlp_seed_cons(const float *_
         }
     }
     const float b3 = P[MLP_OFF_B3 + 1];
-    const int64_t tile0 = ((int64_t)blockIdx.x * MLP_WAVES + wv) * MLP_SEED_TILES;
+    const int64_t tile0 = (blk * MLP_WAVES + wv) * MLP_SEED_TILES;
     for (int t = 0; t < MLP_SEED_TILES; t++) {
         const int64_t base = (tile0 + t) * 16;
         // features of sample base + (l & 15), broadcast to the rows of this lane by shuffles
@@ -445,6 +447,15 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_mlp_seed_cons(const float *_
             if (c0 == 0 && r < n) seed_v[r] = y;
         }
     }
+}
+
+#define MLP_SEED_PER_BLOCK (16 * MLP_WAVES * MLP_SEED_TILES)
+
+__global__ __launch_bounds__(64 * MLP_WAVES) void k_mlp_seed_cons(const float *__restrict__ P,
+                                                                const float *__restrict__ cons, int64_t n,
+                                                                float *seed_v) {
+    __shared__ float s_h[MLP_WAVES][16][MLP_H + 1];
+    seed_cons_block(P, cons, n, seed_v, blockIdx.x, s_h);
 }
 
 __global__ void k_seed_features(const float *__restrict__ T00, const float *__restrict__ M,

@@ -683,18 +683,17 @@ __global__ __launch_bounds__(CV_TILE_THREADS, MARCH ? 3 : 2) void k_visc_tiled(P
     }
 }
 
-// The end of update_ev for the split stage (kernel_visc.cl:322-393 from the clamp on):
-// reads the conserved input the stage wrote to cons (four planes), Pi_new (bk_out) and,
-// at STEP 2, d_ev[1] for udiff; solves for e from seed_v (when given) or from the
+// The end of update_ev for the split stage (kernel_visc.cl:322-393 from the clamp on), one
+// cell: reads the conserved input the stage wrote to cons (four planes of n), Pi_new (bk_out)
+// and, at STEP 2, d_ev[1] for udiff; solves for e in FP64 when the cell is fast
+// (CV_FLAG_FP64_ROOT, M > fp64_vmin T^tt), else from seed_v (when given) or from the
 // reference's start; writes ev_out, udiff_out (STEP 2) and the iteration count.  The same
 // FP32 operations as the fused stage, so the split stage without a seed is bit-identical.
 template <bool BULK, int STEP, int EK>
-__global__ __launch_bounds__(256) void k_visc_root(PhysP p, EosP E, const float *__restrict__ cons,
-                                                   const float *__restrict__ bk_out, const float *__restrict__ ev1,
-                                                   const float *__restrict__ seed_v, float *ev_out, float *udiff_out,
-                                                   int *iters_out) {
-    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (I >= p.n) return;
+__device__ __forceinline__ void visc_root_cell(const PhysP &p, const EosP &E, int64_t I, const float *__restrict__ cons,
+                                               const float *__restrict__ bk_out, const float *__restrict__ ev1,
+                                               const float *__restrict__ seed_v, float *ev_out, float *udiff_out,
+                                               int *iters_out) {
     const int64_t n = p.n;
     EosT<EK> eos(E, E.tab);
     float T00 = cons[I], T01 = cons[n + I], T02 = cons[2 * n + I], T03 = cons[3 * n + I];
@@ -707,10 +706,10 @@ __global__ __launch_bounds__(256) void k_visc_root(PhysP p, EosP E, const float 
     }
     float ed_find;
     int it = 0;
-    if (seed_v)
-        ed_find = root_newton_from(eos, T00, M, bout, BULK, fminf(fmaxf(seed_v[I], 0.0f), 0.9999f), &it);
-    else if ((p.flags & 8) && M > p.fp64_vmin * T00)
+    if ((p.flags & 8) && M > p.fp64_vmin * T00)
         ed_find = (float)root_newton_d(E, E.tab, (double)T00, (double)M, (double)bout, BULK);
+    else if (seed_v)
+        ed_find = root_newton_from(eos, T00, M, bout, BULK, fminf(fmaxf(seed_v[I], 0.0f), 0.9999f), &it);
     else
         ed_find = root_newton_from(eos, T00, M, bout, BULK, 0.5f * (0.0f + 1.0f), &it);
     if (iters_out) iters_out[I] = it;
@@ -733,6 +732,16 @@ __global__ __launch_bounds__(256) void k_visc_root(PhysP p, EosP E, const float 
 #pragma unroll
         for (int c = 0; c < 4; c++) udiff_out[c * n + I] = un[c] - u_old[c];
     }
+}
+
+template <bool BULK, int STEP, int EK>
+__global__ __launch_bounds__(256) void k_visc_root(PhysP p, EosP E, const float *__restrict__ cons,
+                                                   const float *__restrict__ bk_out, const float *__restrict__ ev1,
+                                                   const float *__restrict__ seed_v, float *ev_out, float *udiff_out,
+                                                   int *iters_out) {
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= p.n) return;
+    visc_root_cell<BULK, STEP, EK>(p, E, I, cons, bk_out, ev1, seed_v, ev_out, udiff_out, iters_out);
 }
 
 // The end of update_ev for one deferred cell: FP64 root solve, then the same FP32 tail as

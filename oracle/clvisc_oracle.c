@@ -791,8 +791,10 @@ static void visc_finish(const orc_params *p, real T00, real T01, real T02, real 
         T03 *= DIV(SC * T00, M);
         M = SC * T00;
     }
-    real ed_find = v0 ? root_newton_v0(p, T00, M, bnew, p->bulk_on, *v0, it)
-                      : solve_ed(p, T00, M, bnew, p->bulk_on, it);
+    /* the FP64 root of fast cells (fp64_root) comes before a seed */
+    const int fast = p->fp64_root && M > RC(p->fp64_root_vmin) * T00;
+    real ed_find = (v0 && !fast) ? root_newton_v0(p, T00, M, bnew, p->bulk_on, *v0, it)
+                                 : solve_ed(p, T00, M, bnew, p->bulk_on, it);
     if (ed_find < ACU) { T01 = RC(0.0f); T02 = RC(0.0f); T03 = RC(0.0f); }
     ed_find = r_max(RC(0.0f), ed_find);
     real pr = eos_P(p, ed_find) + bnew;
@@ -836,7 +838,8 @@ void orc_visc_update_ev(const orc_params *p, real *evnew, const real *ev1, const
 
 /* The end of update_ev (visc) for given conserved inputs (T^tt, T^ti as update_ev has them
  * before the clamp, 4 per cell) and Pi_new (NULL: no bulk): the split form of the stage.
- * v0 (NULL: the reference's start) seeds the root solve, clamped to [0, 0.9999]. */
+ * v0 (NULL: the reference's start) seeds the root solve, clamped to [0, 0.9999]; cells that
+ * take the FP64 root (fp64_root, M > vmin T00) take it with or without a seed. */
 void orc_visc_root_from(const orc_params *p, const real *cons, const real *bulk, const real *v0, int64_t n,
                         real *ev_out, int32_t *iters) {
     for (int64_t I = 0; I < n; I++) {

@@ -132,3 +132,58 @@ def test_batch_refuses_a_jet_source():
     batch.handle(1).set_jet_source(None)
     batch.steps(1)
     batch.close()
+
+
+@pytest.mark.parametrize('fp64', [False, True])
+def test_seeded_batch_matches_seeded_single_events(fp64):
+    """cv_batch_set_root_seed: the batch's split stages, seed network and root solve over all
+    events in one graph give each event's fields exactly as its own seeded handle does (FP64
+    root for fast cells when CV_FLAG_FP64_ROOT is set); deactivating an event keeps it so."""
+    tab = Eos('crossover').create_table()
+    flags = _lib.CV_FLAG_FP64_ROOT if fp64 else 0
+    p = cv_params(1, 1, 1, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08,
+                  bulk=(0.05, 0.155, 0.02), flags=flags, fp64_root_vmin=0.3)
+    dims = DIMS[:4]
+    _say('case: seeded batch of %d events, FP64 root %s' % (len(dims), fp64))
+    rng = np.random.default_rng(31)
+    mlp = _lib.Mlp()
+    mlp.set_params((rng.standard_normal(mlp.nparam) * 0.3).astype(np.float32))
+    batch = _lib.Batch(p, tab, dims)
+    singles = []
+    for e, d in enumerate(dims):
+        ev = _ic(d, e)
+        _init(batch.handle(e), ev, p.tau0)
+        q = _lib.CvParams.from_buffer_copy(p)
+        q.nx, q.ny, q.nz = d
+        hs = _lib.Handle(q, tab, viscous=True)
+        _init(hs, ev, p.tau0)
+        hs.set_root_seed(mlp)
+        singles.append(hs)
+    batch.set_root_seed(mlp)
+    batch.steps(3)
+    for hs in singles:
+        hs.visc_steps(0, 3)
+    for e, hs in enumerate(singles):
+        _compare(batch.handle(e), hs, 'event %d %s' % (e, dims[e]))
+    mask = [1, 1, 0, 1]
+    batch.set_active(mask)
+    batch.steps(2)
+    for e, hs in enumerate(singles):
+        if mask[e]:
+            hs.visc_steps(3, 2)
+    for e, hs in enumerate(singles):
+        _compare(batch.handle(e), hs, 'event %d (mask) %s' % (e, dims[e]))
+    # back to the fused stages: equal to unseeded single events from here on
+    batch.set_root_seed(None)
+    for hs in singles:
+        hs.set_root_seed(None)
+        hs.set_root_split(False)
+    batch.set_active([1, 1, 1, 1])
+    batch.steps(1)
+    for e, hs in enumerate(singles):
+        hs.visc_steps(5 if mask[e] else 3, 1)
+    for e, hs in enumerate(singles):
+        if mask[e]:
+            _compare(batch.handle(e), hs, 'event %d (unseeded) %s' % (e, dims[e]))
+    batch.close()
+    mlp.close()

@@ -105,6 +105,34 @@ def test_split_seeded_root_matches_oracle():
     m.close()
 
 
+def test_split_seeded_fp64_root_matches_oracle():
+    """Seed and CV_FLAG_FP64_ROOT together (vmin 0.02, so that many cells are fast): the fast
+    cells take the FP64 root, the others start from the seed; the stage equals the oracle's
+    split root with the same seeds and the same precedence, bit for bit."""
+    p, tab, ev = _case(33, 29, 13, bulk=(0.05, 0.155, 0.02))
+    p.flags |= _lib.CV_FLAG_FP64_ROOT
+    p.fp64_root_vmin = 0.02
+    _say('case: seeded split stage with the FP64 root (vmin 0.02) vs oracle, 33x29x13 viscous+bulk')
+    h = _start(p, tab, ev)
+    h.visc_steps(0, 20)
+    rng = np.random.default_rng(22)
+    m = _lib.Mlp()
+    m.set_params((rng.standard_normal(MO.NPARAM) * 0.3).astype(np.float32))
+    h.set_root_seed(m, track_iters=True)
+    h.visc_steps(20, 1)
+    cons, seed, bk, g, it = h.get('root_cons'), h.get('root_seed'), h.get('bulk1'), h.get('ev1'), h.root_iters()
+    T00, M, _ = surrogate.clamp_cons(cons)
+    fast = M > np.float32(0.02) * T00
+    op = orc_params(p, tab)
+    r, itr = O.visc_root_from(op, cons, bk, seed)
+    _say('  %d fast cells of %d; ev1 bit-equal %.8f' % (fast.sum(), fast.size, bit_equal_frac(g, r)))
+    assert fast.sum() > 100
+    np.testing.assert_array_equal(g, r)
+    np.testing.assert_array_equal(it[~fast], itr[~fast])
+    h.close()
+    m.close()
+
+
 def test_trained_seed_viscous_run_close_to_plain():
     """Seed trained on the split stage's own states; the seeded run needs fewer iterations
     and stays within the root solve's tolerance of the plain run."""
