@@ -252,6 +252,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
         float s[10], bs = 0.0f, t[4], udx[4], udy[4], udz[4];
         float q[10];
         float4 ec = make_float4(0.f, 0.f, 0.f, 0.f);
+        float T_pre = 0.0f, s_pre = 0.0f, e2x_pre = 0.0f, u0_own = 1.0f, bk_own = 0.0f;
         if (active) {
             if (STEP == 1) {
 #pragma unroll
@@ -265,6 +266,16 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
                 t[0] = qq.x; t[1] = qq.y; t[2] = qq.z; t[3] = qq.w;
             }
             ec = make_float4(s_c[CF_E][cc], s_c[CF_VX][cc], s_c[CF_VY][cc], s_c[CF_VZ][cc]);
+            u0_own = s_c[CF_U0][cc];
+            if (BULK) bk_own = s_c[CF_BK][cc];
+            // the update's EOS lookup (T, s at the stage's own e) and, at STEP 1, the stale e
+            // of d_ev[2] are issued here, so that their latency overlaps the face passes
+            {
+                const float4 cpTs0 = eos.raw(ec.x);
+                T_pre = cpTs0.z;
+                s_pre = cpTs0.w;
+            }
+            if (STEP == 1) e2x_pre = ldo(ev2, bo);
 #pragma unroll
             for (int m = 0; m < 10; m++) q[m] = s_c[CF_PI0 + m][cc];
             // visc_src_christoffel (kernel_IS.cl:41-67)
@@ -408,6 +419,18 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
                 up1 = c == 0 ? s_c[CF_U0][cp] * 1.0f : s_c[CF_U0][cp] * s_c[CF_E + c][cp];
                 udy[c] = slope(um1, ui, up1) * p.r_dy;
             }
+            // z gradient of u^mu (cells only) here, so that no cell value is read after the
+            // z faces: the next plane may then overwrite the cells without a barrier of its own
+            const float u0c = u0_own;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int cm = tp.hz > 0 ? cc - 1 : cc, cp = tp.hz > 0 ? cc + 1 : cc;
+                float um1 = c == 0 ? s_c[CF_U0][cm] * 1.0f : s_c[CF_U0][cm] * s_c[CF_E + c][cm];
+                float ui = c == 0 ? u0c * 1.0f : u0c * s_c[CF_E + c][cc];
+                float up1 = c == 0 ? s_c[CF_U0][cp] * 1.0f : s_c[CF_U0][cp] * s_c[CF_E + c][cp];
+                float chr = ((c == 0 ? ec.w : (c == 3 ? 1.0f : 0.0f)) * u0c) * st.r_tau;
+                udz[c] = slope(um1, ui, up1) * st.r_taudz + chr;
+            }
         }
         __syncthreads();
         // ---- 4. z faces: face f in [0, tz] lies between tile columns f-1 and f ----
@@ -451,23 +474,13 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
                 for (int m = 0; m < 10; m++) s[m] = s[m] - 0.0f * st.r_taudz;
                 if (BULK) bs = bs - 0.0f * st.r_taudz;
             }
-            const float u0c = s_c[CF_U0][cc];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const int cm = tp.hz > 0 ? cc - 1 : cc, cp = tp.hz > 0 ? cc + 1 : cc;
-                float um1 = c == 0 ? s_c[CF_U0][cm] * 1.0f : s_c[CF_U0][cm] * s_c[CF_E + c][cm];
-                float ui = c == 0 ? u0c * 1.0f : u0c * s_c[CF_E + c][cc];
-                float up1 = c == 0 ? s_c[CF_U0][cp] * 1.0f : s_c[CF_U0][cp] * s_c[CF_E + c][cp];
-                float chr = ((c == 0 ? ec.w : (c == 3 ? 1.0f : 0.0f)) * u0c) * st.r_tau;
-                udz[c] = slope(um1, ui, up1) * st.r_taudz + chr;
-            }
         }
 
         // ---- 5. update_pimn (kernel_IS.cl:501-709) and update_ev (kernel_visc.cl:322-393) ----
         if (active) {
             const float4 e1 = STEP == 1 ? ec : e1_pre;
-            const float4 e2 = STEP == 2 ? ec : ld4p(ev2, bo, nb);
-            float g1 = STEP == 1 ? s_c[CF_U0][cc] : gamma3(e1.y, e1.z, e1.w);
+            const float4 e2 = STEP == 2 ? ec : make_float4(e2x_pre, 0.0f, 0.0f, 0.0f);
+            float g1 = STEP == 1 ? u0_own : gamma3(e1.y, e1.z, e1.w);
             float u_old[4] = {g1 * 1.0f, g1 * e1.y, g1 * e1.z, g1 * e1.w};
             float u_new[4], udt[4];
             if (STEP == 1) {
@@ -478,7 +491,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
                     u_new[c] = u_old[c] + ud_;
                 }
             } else {
-                float g2 = s_c[CF_U0][cc];
+                float g2 = u0_own;
                 u_new[0] = g2 * 1.0f; u_new[1] = g2 * e2.y; u_new[2] = g2 * e2.z; u_new[3] = g2 * e2.w;
 #pragma unroll
                 for (int c = 0; c < 4; c++) udt[c] = (u_new[c] - u_old[c]) * p.r_dt;
@@ -493,7 +506,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
             for (int c = 0; c < 4; c++) DU[c] = dot4(u, da[c]);
             const float ed_step = STEP == 1 ? e1.x : e2.x;
             const float theta = udt[0] + udx[1] + udy[2] + udz[3];
-            const float4 cpTs = eos.raw(ed_step);
+            const float4 cpTs = make_float4(0.0f, 0.0f, T_pre, s_pre);   // eos.raw(ed_step), ed_step = ec.x
             const float temperature = cpTs.z;
             const float local_etaos = etaos(p, temperature);
             float etav = local_etaos * cpTs.w * CV_HBARC;
@@ -547,7 +560,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
                 float zs = zetaos(p, temperature);
                 float zetav = zs * cpTs.w * CV_HBARC;
                 float oot_b = temperature * rcp(3.0f * rmax(CV_ACU, zs) * CV_HBARC);
-                float b2 = s_c[CF_BK][cc];
+                float b2 = bk_own;
                 float bsrc_ = (b2 * theta) * p.r_3 - (b2 * u[0]) * st.r_tau;
                 bs += bsrc_;
                 b_start = STEP == 1 ? b2 : bk1_pre;
@@ -639,7 +652,8 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
             }
             }
         }
-        __syncthreads();   // LDS reused by the next plane / tile
+        // no barrier here: after the z faces no cell value is read, and the next plane's
+        // face passes (which overwrite the faces) come after that plane's first barrier
       }
     }
 }
