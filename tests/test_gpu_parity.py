@@ -521,3 +521,36 @@ def test_clvisc_evolve_with_jet_matches_oracle(tmp_path):
     o.p.jet_on = 0
     o.visc_steps(2)
     np.testing.assert_array_equal(visc.handle.get('ev1'), o.ev[1])
+
+
+def test_large_octave_table_runs_the_plane_tiled_kernel():
+    """The x-marching kernels hold the pressure column of an octave table in LDS (at most 1540
+    nodes): a finer table (3073 nodes) gets the plane-tiled kernel by default, kind 2 and
+    batches refuse it, and its steps equal the oracle's."""
+    tab = Eos('crossover', log2j=7).create_table()
+    nx, ny, nz = 29, 31, 11
+    p = cv_params(nx, ny, nz, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08,
+                  bulk=(0.05, 0.155, 0.02))
+    _say('case: octave table of %d nodes, %dx%dx%d, 8 steps' % (tab.num_ed, nx, ny, nz))
+    h = _lib.Handle(p, tab, viscous=True)
+    assert h.stage_kernel()[0] == 1
+    with pytest.raises(_lib.CvError) as err:
+        h.set_stage_kernel(2, 0, 0)
+    assert err.value.code == _lib.CV_ERR_UNSUPPORTED
+    with pytest.raises(_lib.CvError) as err:
+        _lib.Batch(p, tab, [(nx, ny, nz)])
+    assert err.value.code == _lib.CV_ERR_UNSUPPORTED
+    pos = ic.woods_saxon_positions(12, 3, R=0.6, a=0.1)
+    ev = ic.hotspots(nx, ny, nz, 0.08, 0.08, 0.15, positions=pos, emax=50.0)
+    h.visc_initialize(p.tau0)
+    h.ideal_stage(1, np.float32(p.tau0))
+    h.set('ev1', ev)
+    h.visc_steps(0, 8)
+    o = O.Oracle(orc_params(p, tab), 'f32', viscous=True)
+    o.load_ini(ev)
+    o.visc_steps(8)
+    for name, r in (('ev1', o.ev[1]), ('pi1', o.pi[1])):
+        g = h.get(name)
+        _say('  %-4s bit-equal %.8f' % (name, bit_equal_frac(g, r)))
+        np.testing.assert_array_equal(g, r)
+    h.close()
