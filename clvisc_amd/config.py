@@ -80,6 +80,8 @@ def read_config(path=None, argv=None):
 
     g('path', 'fPathIni', str, 'The absolute path for initial conditions')
     g('path', 'fPathOut', str, 'The absolute path for output directory')
+    g('path', 'eos_table_dir', str, 'directory of the s95p EOS tables (the reference checkout\'s '
+      'pyvisc/eos/eos_table); chemical_potential.dat is written when it is set', '')
     g('glauber', 'Edmax', float, 'maximum energy density for most central collisions')
     g('glauber', 'NumOfNucleons', float, 'Number of nucleons, A=197 for Au; A=208 for Pb')
     g('glauber', 'SQRTS', float, 'Beam energy in units of GeV/n')

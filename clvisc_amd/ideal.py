@@ -13,7 +13,7 @@ import time
 
 import numpy as np
 
-from . import _lib
+from . import _lib, chemical_potential
 from .eos import Eos
 from .hypersf import HyperSurface
 
@@ -92,6 +92,7 @@ class CLIdeal(object):
         self.tau = self.cfg.real(self.cfg.TAU0)
         self.compile_options = self.__compile_options()
         self.eos = Eos(self.cfg.eos_type) if handcrafted_eos is None else handcrafted_eos
+        self.__chemical_potential(path, handcrafted_eos)
         self.eos_table = self.eos.create_table(None, self.compile_options)
         self.efrz = float(self.eos.f_ed(self.cfg.TFRZ))
         from .bulkinfo import BulkInfo
@@ -107,6 +108,23 @@ class CLIdeal(object):
         self.history = []
         self.edmax = 0.0
         self.num_of_sf = 0
+
+    def __chemical_potential(self, path, handcrafted_eos):
+        """chemical_potential.dat in the output directory (pyvisc/ideal.py:65), when the s95p
+        tables are at hand (cfg.eos_table_dir or $CLVISC_EOS_TABLE_DIR).  e_frz comes from the
+        EOS named by cfg.eos_type, as in the reference, or from the handcrafted EOS when that
+        type has no table here."""
+        table_dir = getattr(self.cfg, 'eos_table_dir', None) or os.environ.get(chemical_potential.TABLE_DIR_ENV)
+        if not table_dir:
+            return None
+        eos = self.eos
+        if handcrafted_eos is not None:
+            try:
+                eos = Eos(self.cfg.eos_type)
+            except ValueError:
+                pass
+        return chemical_potential.create_table(self.cfg.TFRZ, path, eos_type=self.cfg.eos_type, table_dir=table_dir,
+                                               eos=eos, copy_to=getattr(self.cfg, 'chem_copy_to', ()))
 
     @staticmethod
     def determine_float_size(cfg):
