@@ -116,6 +116,7 @@ def lib():
             'cv_ideal_steps_async': [vp, i64, i64],
             'cv_visc_steps_async': [vp, i64, i64],
             'cv_max_ed': [vp, C.c_int, C.POINTER(f32)],
+            'cv_pizz_o_ep': [vp, C.c_int, vp],
             'cv_energy_sum': [vp, C.c_int, C.c_double, vp],
             'cv_slice_sums': [vp, C.c_int, vp],
             'cv_slice_sums_noalloc': [vp, C.c_int, vp],
@@ -189,7 +190,7 @@ def exported_symbols():
     return ['cv_last_error', 'cv_device_info', 'cv_create', 'cv_destroy', 'cv_set_field',
             'cv_get_field', 'cv_copy_field', 'cv_gather', 'cv_visc_initialize', 'cv_ideal_stage', 'cv_is_stage',
             'cv_visc_stage', 'cv_update_udiff', 'cv_ideal_steps_async', 'cv_visc_steps_async',
-            'cv_max_ed', 'cv_sync', 'cv_last_elapsed_ms', 'cv_profile_visc_step',
+            'cv_max_ed', 'cv_pizz_o_ep', 'cv_sync', 'cv_last_elapsed_ms', 'cv_profile_visc_step',
             'cv_set_stage_kernel', 'cv_get_stage_kernel', 'cv_set_jet_source', 'cv_set_march_chunk', 'cv_get_march_chunk',
             'cv_hypersf_reserve', 'cv_hypersf_snapshot',
             'cv_hypersf_find', 'cv_hypersf_count', 'cv_hypersf_get', 'cv_hypersf_reset',
@@ -370,6 +371,13 @@ class Handle(object):
         out = C.c_float()
         check(self.L.cv_max_ed(self.h, which, C.byref(out)))
         return out.value
+
+    def pizz_o_ep(self, which=1):
+        """Per-eta-slice mean of pi^{eta eta}/(e + P) over the cells with e + P > 0.133
+        (CLVisc.check_pizz)."""
+        out = np.empty(int(self.params.nz), np.float32)
+        check(self.L.cv_pizz_o_ep(self.h, which, out.ctypes.data))
+        return out
 
     def sync(self):
         check(self.L.cv_sync(self.h))

@@ -821,6 +821,24 @@ extern "C" int cv_max_ed(cv_handle *h, int which, float *out) {
     return CV_OK;
 }
 
+extern "C" int cv_pizz_o_ep(cv_handle *h, int which, float *out) {
+    if (!h || !out) return fail(CV_ERR_ARG, "null argument");
+    if (which < 0 || which > 2) return fail(CV_ERR_ARG, "which must be 0, 1 or 2");
+    if (!h->viscous) return fail(CV_ERR_UNSUPPORTED, "cv_pizz_o_ep: the handle has no pi^{mu nu}");
+    HIP_TRY(hipSetDevice(h->device));
+    float *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(float) * h->prm.nz));
+    hipLaunchKernelGGL(k_pizz_o_ep, dim3((unsigned)h->prm.nz), dim3(64), 0, h->stream, h->phys, h->eos, h->ev[which],
+                       h->pi[which], d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, sizeof(float) * h->prm.nz, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipError_t e2 = hipFree(d);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_pizz_o_ep: %s", hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(CV_ERR_HIP, "cv_pizz_o_ep: %s", hipGetErrorString(e2));
+    return CV_OK;
+}
+
 extern "C" int cv_energy_sum(cv_handle *h, int which, double tau, double *out) {
     if (!h || !out) return fail(CV_ERR_ARG, "null argument");
     if (which < 0 || which > 2) return fail(CV_ERR_ARG, "which must be 0, 1 or 2");

@@ -1109,6 +1109,48 @@ __global__ void k_udiff(PhysP p, const float *__restrict__ ev0, const float *__r
     for (int c = 0; c < 4; c++) udiff[c * p.n + I] = ub[c] - ua[c];
 }
 
+// pizz_o_ep (kernel_checkpi.cl:5-27, CLVisc.check_pizz): for eta slice K = blockIdx.x the
+// mean of pi^{eta eta}/(e + P) over the cells with e + P > 0.133.  The 64 lanes evaluate 64
+// consecutive cells of the slice (cell c = i NY + j, so c runs over i, then j) and lane 0 adds
+// their terms in that order, the reference's one-thread sum bit for bit.
+__global__ __launch_bounds__(64) void k_pizz_o_ep(PhysP p, EosP E, const float *__restrict__ ev,
+                                                  const float *__restrict__ pi, float *out) {
+    Eos eos(E, E.tab);
+    const int64_t n = p.n, nij = (int64_t)p.nx * p.ny;
+    const int64_t K = blockIdx.x;
+    __shared__ float s_q[64];
+    __shared__ int s_ok[64];
+    float ratio = 0.0f;
+    int num_of_cells = 0;
+    for (int64_t b = 0; b < nij; b += 64) {
+        const int64_t c = b + threadIdx.x;
+        float q = 0.0f;
+        int ok = 0;
+        if (c < nij) {
+            const int64_t I = c * p.nz + K;
+            const float e = ev[I];
+            const float ep = e + eos.P(e);
+            if (ep > 0.133f) {
+                q = pi[9 * n + I] * rcp(ep);
+                ok = 1;
+            }
+        }
+        s_q[threadIdx.x] = q;
+        s_ok[threadIdx.x] = ok;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int64_t m = nij - b < 64 ? nij - b : 64;
+            for (int64_t k = 0; k < m; k++)
+                if (s_ok[k]) {
+                    ratio += s_q[k];
+                    num_of_cells++;
+                }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[K] = ratio * rcp((float)num_of_cells);
+}
+
 // max of e (reduction_stage1 + host max, kernel_reduction.cl:7-39): block max by
 // wavefront shuffles, one atomic per block on the order-preserving integer image.
 __device__ __forceinline__ unsigned int f2ord(float f) {

@@ -105,6 +105,14 @@ class CLVisc(object):
         if save_pi and hasattr(self, 'pimn_info'):
             self.pimn_info.save()
 
+    def check_pizz(self):
+        """Print <pi^{eta eta}/(e + P)> of the mid-rapidity slice over the cells with
+        e + P > 0.133 (pyvisc/visc.py:200-208, kernel_checkpi.cl) and return the per-slice
+        values."""
+        pizz_o_ep = self.handle.pizz_o_ep(1)
+        print('<pizz/(e+P)> =', pizz_o_ep[self.cfg.NZ // 2])
+        return pizz_o_ep
+
     def check_bad_cell(self):
         """Mid-rapidity masks of e > efrz and good cells (pyvisc/visc.py:282-300)."""
         NX, NY, NZ = self.cfg.NX, self.cfg.NY, self.cfg.NZ

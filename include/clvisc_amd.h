@@ -164,6 +164,12 @@ int cv_visc_steps_async(cv_handle *h, int64_t loop0, int64_t nsteps);
  * ideal.py:257-262). */
 int cv_max_ed(cv_handle *h, int which, float *out);
 
+/* pizz_o_ep of CLVisc.check_pizz (kernel_checkpi.cl:5-27, visc.py:200-208): out[K] (NZ floats,
+ * host memory) = mean over the cells of eta slice K with e + P > 0.133 of pi^{eta eta}/(e + P),
+ * from d_ev[which] and d_pi[which], summed in the reference's cell order.  Viscous handles
+ * only (CV_ERR_UNSUPPORTED otherwise).  Synchronous. */
+int cv_pizz_o_ep(cv_handle *h, int which, float *out);
+
 /* FP64 energy balance of slot `which` at proper time tau (no counterpart in the reference):
  * out[0] = sum over cells of tau (T^tautau + pi^tautau) dx dy deta (the conserved Q^tau of
  * kernel_visc.cl:322-393 summed), out[1] = sum of T~^etaeta = tau^2 T^etaeta (+ pi~^etaeta)

@@ -463,6 +463,26 @@ real orc_max_ed(const orc_params *p, const real *ev) {
     return m;
 }
 
+/* pizz_o_ep (kernel_checkpi.cl:5-27, visc.py:200-208): for each eta slice K the mean of
+ * pi^{eta eta} / (e + P) over the cells with e + P > 0.133, summed over i, then j */
+void orc_pizz_o_ep(const orc_params *p, const real *ev, const real *pi, real *out) {
+    const int64_t NY = p->ny, NZ = p->nz;
+    for (int64_t K = 0; K < NZ; K++) {
+        real ratio = RC(0.0f);
+        int num_of_cells = 0;
+        for (int64_t i = 0; i < p->nx; i++)
+            for (int64_t j = 0; j < NY; j++) {
+                const int64_t I = i * NY * NZ + j * NZ + K;
+                const real ep = ev[4 * I] + eos_P(p, ev[4 * I]);
+                if (ep > RC(0.133f)) {
+                    ratio += DIV(pi[10 * I + 9], ep);
+                    num_of_cells++;
+                }
+            }
+        out[K] = DIV(ratio, (real)num_of_cells);
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Pencil iteration                                                           */
 /* ------------------------------------------------------------------------- */

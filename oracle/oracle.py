@@ -189,6 +189,13 @@ class Oracle(object):
     def max_ed(self, which=1):
         return float(self.L.orc_max_ed(C.byref(self.p), _ptr(self.ev[which])))
 
+    def pizz_o_ep(self, which=1):
+        """Per-eta-slice mean of pi^{eta eta}/(e + P) over cells with e + P > 0.133
+        (CLVisc.check_pizz, kernel_checkpi.cl:5-27)."""
+        out = np.empty(self.p.nz, self.dtype)
+        self.L.orc_pizz_o_ep(C.byref(self.p), _ptr(self.ev[which]), _ptr(self.pi[which]), _ptr(out))
+        return out
+
     def load_ini(self, ev, pi=None, bulk=None):
         self.ev[1][...] = np.asarray(ev, self.dtype).reshape(self.size, 4)
         if pi is not None:
