@@ -182,6 +182,13 @@ class BulkInfo(object):
         vol = tau * self.cfg.DX * self.cfg.DY * self.cfg.DZ
         return float(e_lab.sum() * vol), float((u0 * np.asarray(self.eos.f_S(ed), np.float64)).sum() * vol)
 
+    def ecc_vs_rapidity(self, d_ev):
+        """Append eccp1, eccp2 of every eta slice of d_ev (a device field proxy or an
+        (N, 4) array) to ecc1_vs_rapidity / ecc2_vs_rapidity (bulkinfo.py:247-260)."""
+        NX, NY, NZ = self.cfg.NX, self.cfg.NY, self.cfg.NZ
+        arr = d_ev.get() if hasattr(d_ev, 'get') else np.asarray(d_ev)
+        self._ecc_vs_rapidity(arr.reshape(NX, NY, NZ, 4))
+
     def _ecc_vs_rapidity(self, bulk):
         NZ = self.cfg.NZ
         ecc1 = np.empty(NZ)

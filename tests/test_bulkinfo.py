@@ -74,6 +74,11 @@ def test_get_records_slices_and_ecc_vs_rapidity():
     assert e1.shape == (nz,) and np.ptp(e2) < 1e-12 and np.ptp(e1) < 1e-12
     ref = bi.eccp(ev[:, :, 0, 0], ev[:, :, 0, 1], ev[:, :, 0, 2], ev[:, :, 0, 3])
     assert e2[0] == ref[1] and e1[0] == ref[0]
+    # the public ecc_vs_rapidity(d_ev) of the reference appends the same values
+    bi.ecc_vs_rapidity(_Field(ev.reshape(-1, 4)))
+    assert len(bi.ecc2_vs_rapidity) == 2
+    np.testing.assert_array_equal(bi.ecc2_vs_rapidity[1], e2)
+    np.testing.assert_array_equal(bi.ecc1_vs_rapidity[1], e1)
 
 
 def test_total_energy_and_entropy_closed_forms():
