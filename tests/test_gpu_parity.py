@@ -211,6 +211,42 @@ def test_riemann_mode_matches_oracle(viscous):
     h.close()
 
 
+@pytest.mark.parametrize('viscous', [False, True])
+def test_riemann_rarefaction_along_eta_matches_oracle(viscous):
+    """The Riemann problem of pyvisc/tests/test_riemann.py on its grid (NX = NY = 1, NZ = 401,
+    DZ = 0.05, DT = 0.01, the step along the third axis, vacuum on the right), 400 steps:
+    the device equals the oracle, which follows the analytic rarefaction
+    (tests/test_riemann_oracle.py).  Viscous: eta/s = 0, through the viscous kernels."""
+    nz = 401
+    tab = Eos('ideal_gas').create_table()
+    p = cv_params(1, 1, nz, 0.01, 0.08, 0.08, 0.05, 1.0, etaos_xmin=0.0, etaos_ymin=0.0,
+                  flags=_lib.CV_FLAG_RIEMANN_TEST)
+    z = (np.arange(nz) - nz // 2) * 0.05
+    ev = np.zeros((nz, 4), np.float32)
+    ev[:, 0] = np.where(z < 0, 3.0, 0.0)
+    _say('case: riemann rarefaction 1x1x%d %s, 400 steps' % (nz, 'viscous' if viscous else 'ideal'))
+    h = _lib.Handle(p, tab, viscous=viscous)
+    o = O.Oracle(orc_params(p, tab), 'f32', viscous=viscous)
+    o.load_ini(ev)
+    if viscous:
+        h.visc_initialize(p.tau0)
+        h.ideal_stage(1, np.float32(p.tau0))
+        h.set('ev1', ev)
+        h.visc_steps(0, 400)
+        o.visc_steps(400)
+        pairs = (('ev1', o.ev[1]), ('pi1', o.pi[1]))
+    else:
+        h.set('ev1', ev)
+        h.ideal_steps(0, 400)
+        o.ideal_steps(400)
+        pairs = (('ev1', o.ev[1]),)
+    for name, r in pairs:
+        g = h.get(name)
+        _say('  %-4s bit-equal %.8f' % (name, bit_equal_frac(g, r)))
+        np.testing.assert_array_equal(g, r, err_msg=name)
+    h.close()
+
+
 def _visc_case(nx, ny, nz, bulk=None, nhot=12, seed=3, eos_kind='crossover'):
     eos = Eos(eos_kind)
     tab = eos.create_table()
