@@ -247,6 +247,32 @@ def test_riemann_rarefaction_along_eta_matches_oracle(viscous):
     h.close()
 
 
+def test_longitudinal_expansion_matches_oracle():
+    """The workload of pyvisc/tests/test_longitudinal_expansion.py: NX = NY = 1, NZ = 401,
+    DZ = 0.075, DT = 0.005, e = 30 exp(-eta^2 / (2 * 2^2)) at rest, ideal, 1400 steps (the
+    reference's test only saves the profiles): device equal to the oracle, and the profile
+    stays even in eta with tau v^eta odd."""
+    nz, nsteps = 401, 1400
+    tab = Eos('ideal_gas').create_table()
+    p = cv_params(1, 1, nz, 0.005, 0.08, 0.08, 0.075, 0.6)
+    eta = (np.arange(nz) - nz // 2) * np.float32(0.075)
+    ev = np.zeros((nz, 4), np.float32)
+    ev[:, 0] = 30.0 * np.exp(-eta.astype(np.float64) ** 2 / (2.0 * 2.0 * 2.0))
+    _say('case: longitudinal expansion 1x1x%d, %d steps' % (nz, nsteps))
+    h = _lib.Handle(p, tab, viscous=False)
+    h.set('ev1', ev)
+    h.ideal_steps(0, nsteps)
+    o = O.Oracle(orc_params(p, tab), 'f32', viscous=False)
+    o.load_ini(ev)
+    o.ideal_steps(nsteps)
+    g = h.get('ev1')
+    _say('  ev1 bit-equal %.8f; e(0) %.4g -> %.4g' % (bit_equal_frac(g, o.ev[1]), ev[nz // 2, 0], g[nz // 2, 0]))
+    np.testing.assert_array_equal(g, o.ev[1])
+    np.testing.assert_array_equal(g[::-1, 0], g[:, 0])
+    np.testing.assert_array_equal(-g[::-1, 3], g[:, 3])
+    h.close()
+
+
 def _visc_case(nx, ny, nz, bulk=None, nhot=12, seed=3, eos_kind='crossover'):
     eos = Eos(eos_kind)
     tab = eos.create_table()
