@@ -70,7 +70,13 @@ __global__ __launch_bounds__(CV_ITILE_THREADS) void k_ideal_tiled(PhysP p, EosP 
                                                                   float *ev_out) {
     __shared__ float s_c[CI_N][CV_TILE_MAXC];
     __shared__ float s_f[4][CV_TILE_MAXF];
-    EosT<EK> eos(E, E.tab);
+    // octave table: its pressure column in LDS, as in the viscous marching kernel (the host
+    // launches this kernel only for tables of at most CV_PTAB_LDS nodes)
+    __shared__ float s_pt[EK == 1 ? CV_PTAB_LDS : 1];
+    const auto eos = [&]() {
+        if constexpr (EK == 1) return EosOctP(E, E.tab, s_pt);
+        else return EosT<EK>(E, E.tab);
+    }();
     const int tid = threadIdx.x;
     const uint32_t nb = p.nb;
     const float tau = st.tau;
@@ -80,6 +86,10 @@ __global__ __launch_bounds__(CV_ITILE_THREADS) void k_ideal_tiled(PhysP p, EosP 
     const int64_t work = (tp.xcd > 0 && (int64_t)gridDim.x == tp.nwork) ? xcd_work(blockIdx.x, tp.nwork, tp.xcd)
                                                                          : (int64_t)blockIdx.x;
     if (work >= tp.nwork) return;                      // uniform over the workgroup
+    if constexpr (EK == 1) {
+        for (int n = tid; n <= E.nlast; n += CV_ITILE_THREADS) s_pt[n] = E.tab[n].y;
+        __syncthreads();
+    }
     const int kt = (int)(work % tp.ntk);
     const int64_t r = work / tp.ntk;
     const int jt = (int)(r % tp.ntj);
