@@ -317,10 +317,13 @@ MATCHED_REL = 1e-4     # learned form counts as matched when max rel |e_learned 
 
 
 def run_learned(args, p, tab, ev, h_plain, loop_end):
-    """The learned form of the same workload: the split stage (stage kernel, then network
-    seed and root solve in their own kernels) seeded by the 4-64-64-2 network trained on the
-    split stage's own states of this run.  Same initial state and step count as the plain
-    run that ended at loop_end on h_plain; returns the 'learned' object of the JSON line."""
+    """The learned form of the same workload: the root solve seeded by the 4-64-64-2 network
+    trained on the split stage's own states of this run, with the network inside the stage
+    kernel (fused: each wave evaluates it on its 64 cells before the root solve).  Same
+    initial state and step count as the plain run that ended at loop_end on h_plain.  The
+    split form (stage kernel, then the network and the root solve in their own kernels) is
+    timed over a window of the following steps.  Returns the 'learned' object of the JSON
+    line."""
     from clvisc_amd import _lib, surrogate
     ncell = NX * NY * NZ
     hl = _lib.Handle(p, tab, viscous=True)
@@ -338,6 +341,7 @@ def run_learned(args, p, tab, ev, h_plain, loop_end):
         % (states[0].size, w, t_states, t_train, hist[:20].mean(), hist[-20:].mean()))
     del states
     hl.set_root_seed(mlp, track_iters=False)
+    hl.set_root_split(False)
     nrun = loop_end - w
     hl.sync()
     t0 = time.perf_counter()
@@ -353,21 +357,29 @@ def run_learned(args, p, tab, ev, h_plain, loop_end):
     hl.set_root_seed(mlp, track_iters=True)
     hl.visc_steps(loop_end, 1)
     it_l = hl.root_iters()
+    # the split form of the learned step, over the next steps of the same run
+    nsplit = max(2, min(20, nrun))
+    hl.set_root_seed(mlp, track_iters=False)
+    hl.set_root_split(True)
+    hl.sync()
+    t0 = time.perf_counter()
+    hl.visc_steps(loop_end + 1, nsplit, sync=False)
+    hl.sync()
+    split_learned_value = ncell * nsplit / (time.perf_counter() - t0)
     h_plain.set_root_split(True)
     h_plain.set_root_seed(None, track_iters=True)
     h_plain.visc_steps(loop_end, 1)
     it_p = h_plain.root_iters()
     # the split form without the seed (the network's share of the learned step time)
-    nsplit = max(2, min(20, nrun))
     h_plain.set_root_seed(None, track_iters=False)
     h_plain.sync()
     t0 = time.perf_counter()
     h_plain.visc_steps(loop_end + 1, nsplit, sync=False)
     h_plain.sync()
     split_value = ncell * nsplit / (time.perf_counter() - t0)
-    log('learned: %d steps %.3fs -> %.3f ms/step, %.4g cell-updates/s; split without seed %.4g; '
-        'iterations plain %.3f learned %.3f; max rel e vs plain %.3g'
-        % (nrun, wall, 1000 * wall / nrun, value, split_value, it_p.mean(), it_l.mean(), rel))
+    log('learned (network in the stage kernel): %d steps %.3fs -> %.3f ms/step, %.4g cell-updates/s; '
+        'split form %.4g (without seed %.4g); iterations plain %.3f learned %.3f; max rel e vs plain %.3g'
+        % (nrun, wall, 1000 * wall / nrun, value, split_learned_value, split_value, it_p.mean(), it_l.mean(), rel))
     # FP32 sensitivity of the plain run itself: the same steps from the initial field with
     # e scaled by (1 + 2^-23) (one ulp), compared with the plain run at loop_end
     hl.close()
@@ -386,6 +398,7 @@ def run_learned(args, p, tab, ev, h_plain, loop_end):
     hu.close()
     mlp.close()
     return {'value': value, 'unit': 'cell-updates/s', 'ms_per_step': 1000.0 * wall / nrun, 'steps': nrun,
+            'form': 'fused (the network in the stage kernel)', 'split_value': split_learned_value,
             'max_rel_e_vs_plain': rel, 'matched': bool(rel <= MATCHED_REL), 'matched_tolerance': MATCHED_REL,
             'plain_one_ulp_ic_max_rel_e': rel_ulp,
             'within_one_ulp_sensitivity': bool(rel <= rel_ulp),
@@ -886,6 +899,7 @@ def main():
     if not args.no_learned:
         learned = run_learned(args, p, tab, ev, h, loop)
         learned['learned_over_plain'] = learned['value'] / value
+        learned['split_learned_over_plain'] = learned['split_value'] / value
     cpu = None
     nth = args.cpu_threads or cpu_threads_default()
     if not args.no_cpu_baseline:

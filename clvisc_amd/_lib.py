@@ -426,7 +426,10 @@ class Handle(object):
 
     def set_root_split(self, on=True):
         """Viscous stages in split form (stage, then seed and root solve in their own
-        kernels); the fields root_cons / root_seed hold the last stage's solve input."""
+        kernels); the fields root_cons / root_seed hold the last stage's solve input.
+        on=False with a seed network set: the network runs inside the stage kernel (the same
+        fields bit for bit; root_seed and the iteration counts are kept when iterations are
+        tracked)."""
         check(self.L.cv_set_root_split(self.h, int(on)))
 
     def root_iters(self):

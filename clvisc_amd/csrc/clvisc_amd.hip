@@ -29,6 +29,7 @@ struct cv_mlp {
     int64_t part_blocks = 0;
     float *X = nullptr, *T = nullptr;       // training set
     int64_t ndata = 0;
+    uint32_t *pk = nullptr;                 // packed BF16 operands of the seed pass (k_mlp_pack_t)
     int64_t adam_t = 0;
     hipStream_t stream = nullptr;
 };
@@ -504,6 +505,12 @@ static int launch_kt(cv_handle *h, int step, float tau, float *ev_st, float *pi_
     return CV_OK;
 }
 
+// the seed pass's packed BF16 operands from the network's current parameters (on the stage's
+// stream, so that training between stages is always seen)
+static void mlp_pack(const cv_mlp *m, hipStream_t s) {
+    hipLaunchKernelGGL(k_mlp_pack_t, dim3((MLP_PK_THREADS + 255) / 256), dim3(256), 0, s, m->P, m->pk);
+}
+
 // second half of a split viscous stage: the learned seed (when set) from the conserved
 // input the stage wrote, then the root solve and velocities
 static int launch_visc_root(cv_handle *h, int step, const float *ev1, const float *bk_out, float *ev_out,
@@ -515,8 +522,9 @@ static int launch_visc_root(cv_handle *h, int step, const float *ev1, const floa
         seed = h->seed_buf + 9 * n;
         const int64_t per_blk = 16 * MLP_WAVES * MLP_SEED_TILES;
         const int64_t nblk = (n + per_blk - 1) / per_blk;
+        mlp_pack(h->seed_mlp, h->stream);
         hipLaunchKernelGGL(k_mlp_seed_cons, dim3((unsigned)nblk), dim3(64 * MLP_WAVES), 0, h->stream, h->seed_mlp->P,
-                           cons, n, seed);
+                           h->seed_mlp->pk, cons, n, seed);
     }
     int *it = h->track_iters ? h->iters : nullptr;
 #define CV_ROOT_LAUNCH(B, S, K) \
@@ -543,21 +551,29 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
     if ((h->kernel == 1 || h->kernel == 2) && !h->jet.on) {
         const bool march = h->kernel == 2;
         dim3 g(march ? (unsigned)h->tile.nwork : (unsigned)h->tile_grid), b(CV_TILE_THREADS);
-        const bool split = h->root_split;
-        StagePtrs sp = {ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src,
-                        h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out,
-                        split ? h->seed_buf : nullptr, h->d_fix, h->d_fix_ctr};
-#define CV_TILED_ARGS h->phys, h->eos, st, h->tile, sp
-        // optional branches: none, the FP64 root alone, the split alone (k_visc_root solves the
-        // FP64 cells of a split stage), or all of them (test flags)
         const bool tests = (h->prm.flags & (CV_FLAG_RIEMANN_TEST | CV_FLAG_GUBSER_VISC_TEST | CV_FLAG_PIMN_OMEGA |
                                             CV_FLAG_OMEGA_OMEGA)) != 0;
         const bool fp64 = (h->prm.flags & CV_FLAG_FP64_ROOT) != 0;
+        const int ek = h->eos.kind;
+        // the learned seed runs inside the stage (the whole wave evaluates the network before
+        // the root solve) on the x-marching kernel with an octave table, unless the split form
+        // is asked for; otherwise a seeded stage runs split (the same fields either way)
+        const bool fused_seed = h->seed_mlp && !h->root_split && march && ek == 1 && !tests;
+        const bool split = h->root_split || (h->seed_mlp && !fused_seed);
+        StagePtrs sp = {ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src,
+                        h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out,
+                        split ? h->seed_buf : nullptr, h->d_fix, h->d_fix_ctr,
+                        fused_seed ? h->seed_mlp->P : nullptr, fused_seed ? h->seed_mlp->pk : nullptr,
+                        fused_seed && h->track_iters ? h->seed_buf + 9 * h->n : nullptr,
+                        fused_seed && h->track_iters ? h->iters : nullptr};
+#define CV_TILED_ARGS h->phys, h->eos, st, h->tile, sp
+        // optional branches: none, the FP64 root alone, the split alone (k_visc_root solves the
+        // FP64 cells of a split stage), or all of them (test flags)
         const int spec = tests ? CV_SPEC_ALL : (split ? CV_SPEC_SPLIT : (fp64 ? CV_SPEC_FP64 : 0));
         // FP64 cells are deferred to k_fp64_fix unless the split stage solves every cell later
         const bool defer = fp64 && !split;
         if (defer) hipLaunchKernelGGL(k_fix_reset, dim3(1), dim3(64), 0, h->stream, h->d_fix_ctr);
-        const int ek = h->eos.kind;
+        if (fused_seed) mlp_pack(h->seed_mlp, h->stream);
 #define CV_TILED_LAUNCH2(B, S, K, X, M) hipLaunchKernelGGL((k_visc_tiled<B, S, K, X, M>), g, b, 0, h->stream, CV_TILED_ARGS)
 #define CV_TILED_LAUNCH(B, S, K, X) \
         do { if (march) CV_TILED_LAUNCH2(B, S, K, X, true); else CV_TILED_LAUNCH2(B, S, K, X, false); } while (0)
@@ -568,11 +584,18 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
              else CV_TILED_LAUNCH(B, S, K, 0); } while (0)
 #define CV_TILED_SEL(B, S) \
         if (ek == 1) CV_TILED_SEL_K(B, S, 1); else CV_TILED_SEL_K(B, S, 0);
-        if (h->bulk) {
+#define CV_SEED_LAUNCH(B, S)                                                                                     \
+        do { if (fp64) CV_TILED_LAUNCH2(B, S, 1, CV_SPEC_SEED | CV_SPEC_FP64, true);                              \
+             else CV_TILED_LAUNCH2(B, S, 1, CV_SPEC_SEED, true); } while (0)
+        if (fused_seed) {
+            if (h->bulk) { if (step == 1) CV_SEED_LAUNCH(true, 1); else CV_SEED_LAUNCH(true, 2); }
+            else { if (step == 1) CV_SEED_LAUNCH(false, 1); else CV_SEED_LAUNCH(false, 2); }
+        } else if (h->bulk) {
             if (step == 1) { CV_TILED_SEL(true, 1) } else { CV_TILED_SEL(true, 2) }
         } else {
             if (step == 1) { CV_TILED_SEL(false, 1) } else { CV_TILED_SEL(false, 2) }
         }
+#undef CV_SEED_LAUNCH
 #undef CV_TILED_SEL
 #undef CV_TILED_SEL_K
 #undef CV_TILED_LAUNCH
@@ -593,8 +616,10 @@ static int launch_stage(cv_handle *h, int step, float tau, float *ev_st, float *
         }
         return CV_OK;
     }
-    if (h->root_split && h->jet.on) return fail(CV_ERR_UNSUPPORTED, "the jet source does not run with the split root solve");
-    if (h->root_split) return fail(CV_ERR_UNSUPPORTED, "split root solve needs a tiled stage kernel (kind 1 or 2)");
+    if ((h->root_split || h->seed_mlp) && h->jet.on)
+        return fail(CV_ERR_UNSUPPORTED, "the jet source does not run with the split or seeded root solve");
+    if (h->root_split || h->seed_mlp)
+        return fail(CV_ERR_UNSUPPORTED, "the split or seeded root solve needs a tiled stage kernel (kind 1 or 2)");
     dim3 g = grid1d(h->n), b(256);
 #define CV_STAGE_ARGS h->phys, h->eos, st, ev_st, pi_st, bk_st, ev1, ev2, pi1, bk1, h->udiff, h->IS_src, \
                       h->bsrc, h->Src, pi_out, bk_out, ev_out, h->good, udiff_out, h->jet
@@ -739,7 +764,8 @@ extern "C" int cv_set_jet_source(cv_handle *h, int on, float direction, int64_t 
     if (!on) { h->jet = JetP{}; return CV_OK; }
     if (on < 0 || on > 3) return fail(CV_ERR_ARG, "on must be a mask of the RK stages (1, 2 or 3)");
     if (start_pos_index < 0 || start_pos_index >= h->n) return fail(CV_ERR_ARG, "start_pos_index outside the grid");
-    if (h->root_split) return fail(CV_ERR_UNSUPPORTED, "the jet source does not run with the split root solve");
+    if (h->root_split || (h->viscous && h->seed_mlp))
+        return fail(CV_ERR_UNSUPPORTED, "the jet source does not run with the split or seeded viscous root solve");
     const int64_t nyz = (int64_t)h->prm.ny * h->prm.nz;
     const int i0 = (int)(start_pos_index / nyz);
     const int j0 = (int)((start_pos_index - i0 * nyz) / h->prm.nz);
@@ -1572,10 +1598,12 @@ static int batch_build(cv_batch *b) {
 #define CV_BATCH_FIX(B, S)                                                                                    \
     do {                                                                                                      \
         if (seeded) {                                                                                         \
-            hipLaunchKernelGGL(k_batch_seed, sg, sb, 0, b->stream, b->d_state, b->d_events, b->seed->P);      \
+            hipLaunchKernelGGL(k_batch_seed, sg, sb, 0, b->stream, b->d_state, b->d_events, b->seed->P,       \
+                               b->seed->pk);                                                                  \
             if (b->ek == 1) CV_BROOT_LAUNCH(B, S, 1); else CV_BROOT_LAUNCH(B, S, 0);                          \
         } else if (defer) { if (b->ek == 1) CV_BFIX_LAUNCH(B, S, 1); else CV_BFIX_LAUNCH(B, S, 0); }          \
     } while (0)
+    if (seeded) mlp_pack(b->seed, b->stream);
     if (defer)
         hipLaunchKernelGGL(k_batch_fix_reset, dim3((nact + 63) / 64), dim3(64), 0, b->stream, b->d_state, b->d_events);
     if (b->bulk) { CV_BATCH_LAUNCH(true, 1); CV_BATCH_FIX(true, 1); }
@@ -1719,6 +1747,7 @@ extern "C" int cv_mlp_create(int device, uint64_t seed, cv_mlp **out) {
               hipMalloc((void **)&m->m2, sizeof(float) * MLP_NPARAM) == hipSuccess &&
               hipMalloc((void **)&m->grad, sizeof(float) * MLP_NPARAM) == hipSuccess &&
               hipMalloc((void **)&m->loss, sizeof(float)) == hipSuccess &&
+              hipMalloc((void **)&m->pk, sizeof(uint32_t) * MLP_PK_WORDS) == hipSuccess &&
               hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) == hipSuccess;
     if (!ok) {
         cv_mlp_destroy(m);
@@ -1748,6 +1777,7 @@ extern "C" int cv_mlp_destroy(cv_mlp *m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     (void)hipFree(m->P); (void)hipFree(m->m1); (void)hipFree(m->m2); (void)hipFree(m->grad); (void)hipFree(m->loss);
     (void)hipFree(m->part); (void)hipFree(m->part_loss); (void)hipFree(m->X); (void)hipFree(m->T);
+    (void)hipFree(m->pk);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
     return CV_OK;
@@ -1984,16 +2014,16 @@ extern "C" int cv_set_root_seed(cv_handle *h, cv_mlp *m, int track_iters) {
         HIP_TRY(hipMalloc((void **)&h->iters, sizeof(int) * h->n));
         HIP_TRY(hipMemset(h->iters, 0, sizeof(int) * h->n));
     }
+    // a network set on a viscous handle without one starts in the split form
+    if (h->viscous && m && !h->seed_mlp) h->root_split = true;
     h->seed_mlp = m;
     h->track_iters = track_iters != 0;
-    if (h->viscous && m) h->root_split = true;
     return CV_OK;
 }
 
 extern "C" int cv_set_root_split(cv_handle *h, int on) {
     if (!h) return fail(CV_ERR_ARG, "null handle");
     if (!h->viscous) return fail(CV_ERR_STATE, "cv_set_root_split: viscous handle required");
-    if (!on && h->seed_mlp) return fail(CV_ERR_STATE, "cv_set_root_split: a seed network is set (cv_set_root_seed)");
     if (on && h->kernel == 0) return fail(CV_ERR_UNSUPPORTED, "cv_set_root_split: needs a tiled stage kernel (kind 1 or 2)");
     HIP_TRY(hipSetDevice(h->device));
     HIP_TRY(hipStreamSynchronize(h->stream));

@@ -153,13 +153,13 @@ namespace cv {
 // event table): the seeds of the single-event k_mlp_seed_cons, sample by sample.
 __global__ __launch_bounds__(64 * MLP_WAVES) void k_batch_seed(const BatchState *__restrict__ bs,
                                                              const BatchEvent *__restrict__ evt,
-                                                             const float *__restrict__ P) {
-    __shared__ float s_h[MLP_WAVES][16][MLP_H + 1];
+                                                             const float *__restrict__ P,
+                                                             const uint32_t *__restrict__ pk) {
     if ((int)blockIdx.y >= bs->nevents) return;
     const BatchEvent &ev = evt[blockIdx.y];
     const int64_t n = ev.p.n;
     const int64_t nblk = (n + MLP_SEED_PER_BLOCK - 1) / MLP_SEED_PER_BLOCK;
-    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) seed_cons_block(P, ev.cons, n, ev.seed, blk, s_h);
+    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) seed_cons_block(P, pk, ev.cons, n, ev.seed, blk);
 }
 
 // The end of update_ev for every cell of every active event (blockIdx.y = entry of the event

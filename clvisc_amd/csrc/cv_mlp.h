@@ -339,15 +339,18 @@ __device__ __forceinline__ void seed_features(float T00, float M, float T03, flo
     x[3] = (float)(ex - 1) * (1.0f / 24.0f);
 }
 
-// Seed of the split viscous root solve: the conserved input as the stage wrote it (four
-// planes T^tt, T^tx, T^ty, T^t eta before the |T^ti| <= 0.999 T^tt clamp, as k_visc_root
-// reads them), the clamp, the features, the network; seed_v = the |v| output.
-// Throughput form of the forward pass (only the |v| output): every wave packs its share of
-// the weights once (BF16 B operands of the 64x64 layer in registers, hardware round to
-// nearest even) and runs MLP_SEED_TILES tiles of 16 samples; tanh(z) = 1 - 2/(exp(2z) + 1)
-// with the hardware exp and reciprocal.  The seed only sets where the root solve starts
-// (the solve keeps the reference's bracket and stopping rule), so these approximations
-// change iteration counts, not what the solve converges to.
+// Seed of the viscous root solve (split and fused stages): the clamp, the features, the
+// network; the seed is the |v| output.  Throughput form of the forward pass, with the
+// products transposed: Z^T = W X^T, the weights are the A operands (rows = a layer's units,
+// k = its inputs) and the activations the B operands (k = inputs, columns = samples), so the
+// C tiles of one layer (rows = its units, columns = 16 samples) are directly the B operands
+// of the next and no transposition through LDS is needed.  Lane l = 16 rg + c0 holds sample
+// c0 and units 4 rg + i (i = 0..3) of each 16-unit tile.  Layer 1 is one MFMA per 16 units
+// (its 4 inputs fill k = 0..3, zero above), the biases are the accumulators' initial values,
+// tanh(z) = 1 - 2/(exp(2z) + 1) with the hardware exp and reciprocal, and BF16 conversions
+// are the hardware's round to nearest even.  The seed only sets where the root solve starts
+// (the solve keeps the reference's bracket and stopping rule), so these approximations change
+// iteration counts, not what the solve converges to.
 #define MLP_SEED_TILES 8
 __device__ __forceinline__ float tanh_seed(float z) {
     const float e = __expf(2.0f * z);
@@ -362,100 +365,177 @@ __device__ __forceinline__ mlp_bf16x4 pack4_hw(float a, float b, float c, float 
     r[0] = (short)(ulo & 0xffffu); r[1] = (short)(ulo >> 16); r[2] = (short)(uhi & 0xffffu); r[3] = (short)(uhi >> 16);
     return r;
 }
+__device__ __forceinline__ mlp_bf16x4 as_bf16x4(uint2 u) { return __builtin_bit_cast(mlp_bf16x4, u); }
 
-// One block (MLP_WAVES waves, 16 MLP_WAVES MLP_SEED_TILES samples) of the seed pass over the
-// samples of block blk; all threads of the block call it (it synchronises).  s_h: the
-// block's LDS staging of the hidden layer.  A sample's seed depends on its own features only.
-__device__ __forceinline__ void seed_cons_block(const float *__restrict__ P, const float *__restrict__ cons,
-                                                int64_t n, float *seed_v, int64_t blk,
-                                                float (*s_h)[16][MLP_H + 1]) {
-    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int c0 = l & 15, rg = l >> 4;
-    // this lane's weights: layer-1 columns 16 n + c0, the 64x64 layer's B operands, layer 3 (|v|)
-    float w1[4][MLP_IN], b1[4], b2[4], w3[4];
-    mlp_bf16x4 bp[4][4];
+// Packed operands of the transposed form (MLP_PK_WORDS 32-bit words, k_mlp_pack_t), per lane
+// l = 16 rg + c0 so that one wave's load of a tile is 512 contiguous bytes:
+//   W1 tile m (0..3) at uint2 64 m + l:            W1[16 m + c0][4 rg + i], zero for rg > 0
+//   W2 tile (m, s) at uint2 256 + 64 (4 m + s) + l: W2[16 m + c0][16 s + 4 rg + i]
+//   the |v| row of W3, BF16-rounded, as 64 floats at word MLP_PK_W3
+#define MLP_PK_W3 2560
+#define MLP_PK_WORDS (MLP_PK_W3 + MLP_H)
+
+__global__ void k_mlp_pack_t(const float *__restrict__ P, uint32_t *pk) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < 20 * 64) {
+        const int t = q >> 6, l = q & 63, c0 = l & 15, rg = l >> 4;
+        float w[4];
+        if (t < 4) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) w[i] = rg == 0 ? P[MLP_OFF_W1 + (16 * t + c0) * MLP_IN + i] : 0.0f;
+        } else {
+            const int m = (t - 4) >> 2, s = (t - 4) & 3;
+#pragma unroll
+            for (int i = 0; i < 4; i++) w[i] = P[MLP_OFF_W2 + (16 * m + c0) * MLP_H + 16 * s + 4 * rg + i];
+        }
+        reinterpret_cast<uint2 *>(pk)[q] = __builtin_bit_cast(uint2, pack4_hw(w[0], w[1], w[2], w[3]));
+    } else if (q < 20 * 64 + MLP_H) {
+        const int c = q - 20 * 64;
+        reinterpret_cast<float *>(pk)[MLP_PK_W3 + c] = bf16_hw(P[MLP_OFF_W3 + MLP_H + c]);
+    }
+}
+#define MLP_PK_THREADS (20 * 64 + MLP_H)
+
+// |v| seeds of sample c0 of NG groups of 16 samples (y[g] on every lane with that c0), the
+// groups sharing each weight load.  xT[g]: the samples' BF16 features on the lanes with
+// rg = 0, zero on the others.  A sample's seed depends on its own features only, whatever
+// NG.  All 64 lanes of the wave must execute it.
+template <int NG>
+__device__ __forceinline__ void seed_groups_t(const float *__restrict__ P, const uint32_t *__restrict__ pk,
+                                              const mlp_bf16x4 xT[NG], float y[NG]) {
+    const int l = threadIdx.x & 63, rg = l >> 4;
+    const uint2 *__restrict__ tiles = reinterpret_cast<const uint2 *>(pk);
+    const float *__restrict__ w3 = reinterpret_cast<const float *>(pk) + MLP_PK_W3;
+    mlp_bf16x4 h1[NG][4];
 #pragma unroll
     for (int m = 0; m < 4; m++) {
-        const int col = 16 * m + c0;
+        const float4 b = *reinterpret_cast<const float4 *>(P + MLP_OFF_B1 + 16 * m + 4 * rg);
+        const mlp_bf16x4 w = as_bf16x4(tiles[64 * m + l]);
 #pragma unroll
-        for (int k = 0; k < MLP_IN; k++) w1[m][k] = bf16_hw(P[MLP_OFF_W1 + col * MLP_IN + k]);
-        b1[m] = P[MLP_OFF_B1 + col];
-        b2[m] = P[MLP_OFF_B2 + col];
-        w3[m] = bf16_hw(P[MLP_OFF_W3 + 1 * MLP_H + col]);
+        for (int g = 0; g < NG; g++) {
+            const mlp_f32x4 z = mfma16(w, xT[g], mlp_f32x4{b.x, b.y, b.z, b.w});
+            h1[g][m] = pack4_hw(tanh_seed(z[0]), tanh_seed(z[1]), tanh_seed(z[2]), tanh_seed(z[3]));
+        }
+    }
+    float part[NG];
 #pragma unroll
-        for (int st = 0; st < 4; st++) {
-            const float *w = P + MLP_OFF_W2 + col * MLP_H + 16 * st + 4 * rg;
-            bp[st][m] = pack4_hw(w[0], w[1], w[2], w[3]);
+    for (int g = 0; g < NG; g++) part[g] = 0.0f;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const float4 b = *reinterpret_cast<const float4 *>(P + MLP_OFF_B2 + 16 * m + 4 * rg);
+        mlp_f32x4 z[NG];
+#pragma unroll
+        for (int g = 0; g < NG; g++) z[g] = mlp_f32x4{b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const mlp_bf16x4 w = as_bf16x4(tiles[256 + 64 * (4 * m + s) + l]);
+#pragma unroll
+            for (int g = 0; g < NG; g++) z[g] = mfma16(w, h1[g][s], z[g]);
+        }
+        const float4 w = *reinterpret_cast<const float4 *>(w3 + 16 * m + 4 * rg);
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
+            part[g] = part[g] + bf16_hw(tanh_seed(z[g][0])) * w.x;
+            part[g] = part[g] + bf16_hw(tanh_seed(z[g][1])) * w.y;
+            part[g] = part[g] + bf16_hw(tanh_seed(z[g][2])) * w.z;
+            part[g] = part[g] + bf16_hw(tanh_seed(z[g][3])) * w.w;
         }
     }
     const float b3 = P[MLP_OFF_B3 + 1];
+#pragma unroll
+    for (int g = 0; g < NG; g++) {
+        float v = part[g] + __shfl_xor(part[g], 16, 64);
+        v = v + __shfl_xor(v, 32, 64);
+        y[g] = v + b3;
+    }
+}
+
+// The features of one cell's root-solve input after the |T^ti| <= 0.999 T^tt clamp
+// (kernel_visc.cl:365-372), as BF16 pairs (x0, x1) and (x2, x3).
+__device__ __forceinline__ void seed_features_bf16(float T00, float M, float T03, uint32_t &lo, uint32_t &hi) {
+    float x[MLP_IN];
+    seed_features(T00, M, T03, x);
+    const mlp_bf16x4 b = pack4_hw(x[0], x[1], x[2], x[3]);
+    const uint2 u = __builtin_bit_cast(uint2, b);
+    lo = u.x;
+    hi = u.y;
+}
+
+// Seeds of the 64 cells of a wave (this lane's cell: features lo, hi from
+// seed_features_bf16, zero for a lane without a cell): four groups of 16 samples, group g = lanes 16 g ..
+// 16 g + 15.  All 64 lanes must execute it.
+__device__ __forceinline__ float seed_wave_t(const float *__restrict__ P, const uint32_t *__restrict__ pk,
+                                             uint32_t lo, uint32_t hi) {
+    const int l = threadIdx.x & 63, c0 = l & 15, rg = l >> 4;
+    float v = 0.0f;
+#pragma unroll 1
+    for (int g0 = 0; g0 < 4; g0 += 2) {
+        mlp_bf16x4 xT[2];
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            uint32_t a = (uint32_t)__shfl((int)lo, 16 * (g0 + g) + c0, 64);
+            uint32_t b = (uint32_t)__shfl((int)hi, 16 * (g0 + g) + c0, 64);
+            if (rg != 0) { a = 0u; b = 0u; }
+            xT[g] = as_bf16x4(make_uint2(a, b));
+        }
+        // the weights are read per pair of groups: the empty asm hides that the offset is
+        // zero (and the loads loop invariant), so that the compiler does not hold all 88
+        // weight registers of a lane across the host kernel's loops
+        int zero = 0;
+        asm volatile("" : "+s"(zero));
+        float y[2];
+        seed_groups_t<2>(P + zero, pk + zero, xT, y);
+        if (rg == g0) v = y[0];
+        if (rg == g0 + 1) v = y[1];
+    }
+    return v;
+}
+
+// One block (MLP_WAVES waves, 16 MLP_WAVES MLP_SEED_TILES samples) of the seed pass over the
+// conserved input of a split stage (four planes T^tt, T^tx, T^ty, T^t eta before the clamp,
+// as k_visc_root reads them); seed_v = the seeds.  A sample's seed depends on its own
+// features only, and equals the fused stage's seed of the same cell bit for bit.
+__device__ __forceinline__ void seed_cons_block(const float *__restrict__ P, const uint32_t *__restrict__ pk,
+                                                const float *__restrict__ cons, int64_t n, float *seed_v,
+                                                int64_t blk) {
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int c0 = l & 15, rg = l >> 4;
     const int64_t tile0 = (blk * MLP_WAVES + wv) * MLP_SEED_TILES;
-    for (int t = 0; t < MLP_SEED_TILES; t++) {
-        const int64_t base = (tile0 + t) * 16;
-        // features of sample base + (l & 15), broadcast to the rows of this lane by shuffles
-        float x[MLP_IN] = {0.0f, 0.0f, 0.0f, 0.0f};
-        {
-            const int64_t r = base + c0;
-            if (r < n) {
-                float T00 = cons[r], T01 = cons[n + r], T02 = cons[2 * n + r], T03 = cons[3 * n + r];
+    for (int t = 0; t < MLP_SEED_TILES; t += 2) {
+        // two tiles of 16 samples per pass of the weights
+        mlp_bf16x4 xT[2];
+        int64_t r[2];
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            r[g] = (tile0 + t + g) * 16 + c0;
+            uint32_t lo = 0u, hi = 0u;
+            if (rg == 0 && r[g] < n) {
+                const int64_t q = r[g];
+                float T00 = cons[q], T01 = cons[n + q], T02 = cons[2 * n + q], T03 = cons[3 * n + q];
                 float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
                 if (M > T00) {
                     T03 *= (0.999f * T00) * (1.0f / M);
                     M = 0.999f * T00;
                 }
-                seed_features(T00, M, T03, x);
+                seed_features_bf16(T00, M, T03, lo, hi);
             }
+            xT[g] = as_bf16x4(make_uint2(lo, hi));
         }
-        float xr[4][MLP_IN];
+        float y[2];
+        seed_groups_t<2>(P, pk, xT, y);
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int k = 0; k < MLP_IN; k++) xr[i][k] = bf16_hw(__shfl(x[k], 4 * rg + i, 64));
-        // layer 1 (VALU): C layout, row 4 rg + i, column 16 m + c0; to LDS for the A layout
-#pragma unroll
-        for (int m = 0; m < 4; m++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                float z = xr[i][0] * w1[m][0];
-                z = z + xr[i][1] * w1[m][1];
-                z = z + xr[i][2] * w1[m][2];
-                z = z + xr[i][3] * w1[m][3];
-                s_h[wv][4 * rg + i][16 * m + c0] = tanh_seed(z + b1[m]);
-            }
-        __syncthreads();
-        mlp_f32x4 acc[4];
-#pragma unroll
-        for (int m = 0; m < 4; m++) acc[m] = mlp_f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int st = 0; st < 4; st++) {
-            const int kb = 16 * st + 4 * rg;
-            const mlp_bf16x4 a = pack4_hw(s_h[wv][c0][kb + 0], s_h[wv][c0][kb + 1], s_h[wv][c0][kb + 2],
-                                          s_h[wv][c0][kb + 3]);
-#pragma unroll
-            for (int m = 0; m < 4; m++) acc[m] = mfma16(a, bp[st][m], acc[m]);
-        }
-        __syncthreads();
-        // layer 2 activation and layer 3 (the |v| output only)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            float sum = bf16_hw(tanh_seed(acc[0][i] + b2[0])) * w3[0];
-            sum = sum + bf16_hw(tanh_seed(acc[1][i] + b2[1])) * w3[1];
-            sum = sum + bf16_hw(tanh_seed(acc[2][i] + b2[2])) * w3[2];
-            sum = sum + bf16_hw(tanh_seed(acc[3][i] + b2[3])) * w3[3];
-            const float y = sum16(sum) + b3;
-            const int64_t r = base + 4 * rg + i;
-            if (c0 == 0 && r < n) seed_v[r] = y;
-        }
+        for (int g = 0; g < 2; g++)
+            if (rg == 0 && r[g] < n) seed_v[r[g]] = y[g];
     }
 }
 
 #define MLP_SEED_PER_BLOCK (16 * MLP_WAVES * MLP_SEED_TILES)
 
 __global__ __launch_bounds__(64 * MLP_WAVES) void k_mlp_seed_cons(const float *__restrict__ P,
+                                                                const uint32_t *__restrict__ pk,
                                                                 const float *__restrict__ cons, int64_t n,
                                                                 float *seed_v) {
-    __shared__ float s_h[MLP_WAVES][16][MLP_H + 1];
-    seed_cons_block(P, cons, n, seed_v, blockIdx.x, s_h);
+    seed_cons_block(P, pk, cons, n, seed_v, blockIdx.x);
 }
 
 __global__ void k_seed_features(const float *__restrict__ T00, const float *__restrict__ M,

@@ -19,6 +19,7 @@
 // dependent chain); the plane-tiled kernel copies the whole octave table to LDS.
 #pragma once
 #include "cv_kernels.h"
+#include "cv_mlp.h"
 
 namespace cv {
 
@@ -149,6 +150,13 @@ struct StagePtrs {
     // deferred FP64 root solves (SPEC & CV_SPEC_FP64 kernels): list of n entries and its count
     FixEntry *fix;
     unsigned *fix_ctr;
+    // learned seed in the stage (SPEC & CV_SPEC_SEED kernels): the network's FP32 parameters
+    // and packed operands (k_mlp_pack_t); when set, each cell's seed and its root-solve
+    // iteration count (0 for a deferred FP64 cell) are written to seed_out / iters_out
+    const float *mlp_P;
+    const uint32_t *mlp_pk;
+    float *seed_out;
+    int *iters_out;
 };
 
 // Optional branches of the stage kernels (template mask SPEC; flags and pointers are still
@@ -158,6 +166,31 @@ struct StagePtrs {
 #define CV_SPEC_FP64 2       // FP64 root solve for fast cells
 #define CV_SPEC_SPLIT 4      // split root solve (StagePtrs::cons_out)
 #define CV_SPEC_ALL 7
+#define CV_SPEC_SEED 8       // learned seed of the root solve, run by the whole wave in the stage
+
+// The end of update_ev from the solved e on (kernel_visc.cl:378-392): velocities, the new
+// ev and, at STEP 2, udiff = u(ev_new) - u(ev_1) (kernel_IS.cl:714-724).
+template <bool BULK, int STEP, class EOS>
+__device__ __forceinline__ void update_ev_tail(const EOS &eos, float ed_find, float T00, float T01, float T02,
+                                               float T03, float bout, const float u_old[4], float *ev_out,
+                                               float *udiff_out, uint32_t bo, uint32_t nb) {
+    if (ed_find < CV_ACU) { T01 = 0.0f; T02 = 0.0f; T03 = 0.0f; }
+    ed_find = rmax(0.0f, ed_find);
+    float pr = eos.P(ed_find) + bout;
+    if (BULK) pr = rmax(pr, 0.0f);
+    float repv = rcp(rmax(CV_ACU, T00 + pr));
+    float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
+    stp(ev_out, bo, nb, 0, en.x);
+    stp(ev_out, bo, nb, 1, en.y);
+    stp(ev_out, bo, nb, 2, en.z);
+    stp(ev_out, bo, nb, 3, en.w);
+    if (STEP == 2) {
+        float gn = gamma3(en.y, en.z, en.w);
+        float un[4] = {gn * 1.0f, gn * en.y, gn * en.z, gn * en.w};
+#pragma unroll
+        for (int c = 0; c < 4; c++) stp(udiff_out, bo, nb, c, un[c] - u_old[c]);
+    }
+}
 
 // One work item of the stage.  All threads of the workgroup call it (it synchronises).
 // EK: EOS table kind (0 row/col, 1 octave); SPEC: mask of the optional branches above.
@@ -477,6 +510,12 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
         }
 
         // ---- 5. update_pimn (kernel_IS.cl:501-709) and update_ev (kernel_visc.cl:322-393) ----
+        // SEED: the root-solve input of the cell, carried out of the per-lane branch so that
+        // the whole wave runs the network (sd_go: the cell solves from the seed)
+        float sd_T[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sd_M = 0.0f, sd_b = 0.0f;
+        float sd_u[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        uint32_t sd_lo = 0u, sd_hi = 0u;
+        bool sd_go = false;
         if (active) {
             const float4 e1 = STEP == 1 ? ec : e1_pre;
             const float4 e2 = STEP == 2 ? ec : make_float4(e2x_pre, 0.0f, 0.0f, 0.0f);
@@ -619,6 +658,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
                 T01 *= sc; T02 *= sc; T03 *= sc;
                 M = 0.999f * T00;
             }
+            if (SPEC & CV_SPEC_SEED) seed_features_bf16(T00, M, T03, sd_lo, sd_hi);
             if ((SPEC & CV_SPEC_FP64) && (p.flags & 8) && M > p.fp64_vmin * T00) {
                 // FP64 root solve deferred to k_fp64_fix (kept out of this kernel's registers);
                 // each cell is recorded at most once per stage, so n entries always suffice.
@@ -631,25 +671,31 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
                 base = __shfl(base, leader, 64);
                 const unsigned slot = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
                 q_.fix[slot] = FixEntry{bo >> 2, T00, T01, T02, T03, M, bout, 0.0f};
-            } else {
-            float ed_find = root_newton(eos, T00, M, bout, BULK);
-            if (ed_find < CV_ACU) { T01 = 0.0f; T02 = 0.0f; T03 = 0.0f; }
-            ed_find = rmax(0.0f, ed_find);
-            float pr = eos.P(ed_find) + bout;
-            if (BULK) pr = rmax(pr, 0.0f);
-            float repv = rcp(rmax(CV_ACU, T00 + pr));
-            float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
-            stp(ev_out, bo, nb, 0, en.x);
-            stp(ev_out, bo, nb, 1, en.y);
-            stp(ev_out, bo, nb, 2, en.z);
-            stp(ev_out, bo, nb, 3, en.w);
-            if (STEP == 2) {
-                float gn = gamma3(en.y, en.z, en.w);
-                float un[4] = {gn * 1.0f, gn * en.y, gn * en.z, gn * en.w};
+                if ((SPEC & CV_SPEC_SEED) && q_.iters_out) q_.iters_out[bo >> 2] = 0;
+            } else if (SPEC & CV_SPEC_SEED) {
+                sd_go = true;
+                sd_T[0] = T00; sd_T[1] = T01; sd_T[2] = T02; sd_T[3] = T03;
+                sd_M = M;
+                sd_b = bout;
 #pragma unroll
-                for (int c = 0; c < 4; c++) stp(udiff_out, bo, nb, c, un[c] - u_old[c]);
+                for (int c = 0; c < 4; c++) sd_u[c] = u_old[c];
+            } else {
+                update_ev_tail<BULK, STEP>(eos, root_newton(eos, T00, M, bout, BULK), T00, T01, T02, T03, bout, u_old,
+                                           ev_out, udiff_out, bo, nb);
             }
             }
+        }
+        if constexpr ((SPEC & CV_SPEC_SEED) != 0) {
+            // the network on the wave's 64 cells (lanes without a cell give zero features),
+            // then the root solve from the seed with the reference's bracket and stopping rule
+            const float v0 = seed_wave_t(q_.mlp_P, q_.mlp_pk, sd_lo, sd_hi);
+            if (active && q_.seed_out) *(float *)((char *)q_.seed_out + bo) = v0;
+            if (sd_go) {
+                int it = 0;
+                const float ed_find = root_newton_from(eos, sd_T[0], sd_M, sd_b, BULK, fminf(fmaxf(v0, 0.0f), 0.9999f), &it);
+                if (q_.iters_out) q_.iters_out[bo >> 2] = it;
+                update_ev_tail<BULK, STEP>(eos, ed_find, sd_T[0], sd_T[1], sd_T[2], sd_T[3], sd_b, sd_u, ev_out,
+                                           udiff_out, bo, nb);
             }
         }
         // no barrier here: after the z faces no cell value is read, and the next plane's

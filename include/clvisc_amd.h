@@ -401,7 +401,8 @@ int cv_mlp_features(cv_mlp *m, const float *T00, const float *M, const float *T0
 int cv_root_eval(cv_handle *h, cv_mlp *m, const float *T00, const float *M, const float *T03, int64_t n,
                  float *e_out, float *v_out, int32_t *iters);
 /* Ideal stages of the handle (cv_ideal_stage, cv_ideal_steps_async) seed their root solve
- * with m (NULL: the reference's start); track_iters keeps the iteration count per cell
+ * with m (NULL: the reference's start), and so do the viscous stages (cv_set_root_split
+ * picks the split or the in-stage form); track_iters keeps the iteration count per cell
  * of the last stage for cv_get_root_iters. */
 int cv_set_root_seed(cv_handle *h, cv_mlp *m, int track_iters);
 /* No counterpart in the reference (north-star extension, the learned + batched form).  The
@@ -416,8 +417,12 @@ int cv_batch_set_root_seed(cv_batch *b, cv_mlp *m);
  * (update_ev from the clamp on, kernel_visc.cl:322-393), seeded by the network set with
  * cv_set_root_seed (field CV_ROOT_SEED) or from the reference's start; with
  * CV_FLAG_FP64_ROOT the fast cells take the FP64 root, seed or not.  Without a seed the
- * split step is bit-identical to the fused one.  A network set on a viscous handle turns
- * the split on by itself; on = 0 with a network set is an error. */
+ * split step is bit-identical to the fused one.  A network set on a viscous handle without
+ * one turns the split on by itself.  on = 0 with a network set runs the seed inside the
+ * stage kernel (kind 2 with an octave table, no test flags; other stages stay split): each
+ * wave evaluates the network on its 64 cells before the root solve, with the same seeds,
+ * iteration counts and fields as the split form bit for bit; the seeds (CV_ROOT_SEED) and
+ * iteration counts are written only when cv_set_root_seed tracks iterations. */
 int cv_set_root_split(cv_handle *h, int on);
 int cv_get_root_iters(cv_handle *h, int32_t *host, int64_t n);
 

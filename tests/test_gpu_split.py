@@ -162,3 +162,48 @@ def test_trained_seed_viscous_run_close_to_plain():
     assert rel.max() < 1e-3
     for x in (hp, hs, mlp):
         x.close()
+
+
+@pytest.mark.parametrize('kernel,bulk,fp64', [((2, 0, 0), (0.05, 0.155, 0.02), False),
+                                              ((2, 0, 0), (0.05, 0.155, 0.02), True),
+                                              ((2, 0, 0), None, False),
+                                              ((2, 7, 30), None, True)])
+def test_fused_seed_equals_split_seed(kernel, bulk, fp64):
+    """The learned seed inside the stage kernel (cv_set_root_split(0) with a network set: the
+    whole wave runs the network before the root solve) gives the split stage's seeds,
+    iteration counts and fields bit for bit, with and without the FP64 root for fast cells;
+    the split stage is the one checked against the oracle above."""
+    p, tab, ev = _case(37, 31, 13, bulk=bulk, nhot=14, seed=5)
+    if fp64:
+        p.flags |= _lib.CV_FLAG_FP64_ROOT
+        p.fp64_root_vmin = 0.02
+    _say('case: fused vs split seeded stage, 37x31x13, kernel %s, bulk %s, fp64 %s, 5 + 3 steps'
+         % (kernel, bulk is not None, fp64))
+    rng = np.random.default_rng(23)
+    m = _lib.Mlp()
+    m.set_params((rng.standard_normal(MO.NPARAM) * 0.3).astype(np.float32))
+    hs = _start(p, tab, ev, kernel)
+    hf = _start(p, tab, ev, kernel)
+    hs.visc_steps(0, 5)
+    hf.visc_steps(0, 5)
+    hs.set_root_seed(m, track_iters=True)
+    hf.set_root_seed(m, track_iters=True)
+    hf.set_root_split(False)
+    for n in range(5, 8):
+        hs.visc_steps(n, 1)
+        hf.visc_steps(n, 1)
+        for name in ('root_seed',):
+            a, b = hf.get(name), hs.get(name)
+            _say('  step %d %-9s bit-equal %.8f' % (n, name, bit_equal_frac(a, b)))
+            np.testing.assert_array_equal(a, b, err_msg='%s step %d' % (name, n))
+        np.testing.assert_array_equal(hf.root_iters(), hs.root_iters(), err_msg='iterations step %d' % n)
+    names = FIELDS + (('bulk0', 'bulk1', 'bulk2') if bulk else ())
+    for name in names:
+        a, b = hf.get(name), hs.get(name)
+        _say('  %-8s bit-equal %.8f' % (name, bit_equal_frac(a, b)))
+        np.testing.assert_array_equal(a, b, err_msg=name)
+    it = hf.root_iters()
+    _say('  iterations mean %.3f' % it.mean())
+    hs.close()
+    hf.close()
+    m.close()
