@@ -1566,16 +1566,24 @@ static int batch_build(cv_batch *b) {
     HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     dim3 g(b->grid), blk(CV_TILE_THREADS);
     hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->d_ctr);
+    // seeded with an octave table: the network inside the stage (each wave seeds its cells
+    // before the root solve; FP64 cells deferred as in the plain stages); with a row/col
+    // table: split stages (the root solve's input to each event's cons), then the seed
+    // network and the root solve over all events, FP64 cells solved there
+    const bool seeded = b->seed != nullptr;
+    const bool seed_in_stage = seeded && b->ek == 1;
+    const bool seed_split = seeded && !seed_in_stage;
+    const float *mP = seed_in_stage ? b->seed->P : nullptr;
+    const uint32_t *mpk = seed_in_stage ? b->seed->pk : nullptr;
+    const bool defer = b->fp64_root && !seed_split;
 #define CV_BATCH_LAUNCH2(B, S, K, X) \
     hipLaunchKernelGGL((k_batch_stage<B, S, K, X>), g, blk, 0, b->stream, b->d_state, b->d_events, b->ev[0]->eos, \
-                       b->d_ctr)
-    // seeded: split stages (the root solve's input to each event's cons), then the seed network
-    // and the root solve over all events; FP64 cells are solved there, not deferred
-    const bool seeded = b->seed != nullptr;
-    const bool defer = b->fp64_root && !seeded;
+                       b->d_ctr, mP, mpk)
 #define CV_BATCH_LAUNCH(B, S)                                                              \
     do {                                                                                   \
-        if (seeded) { if (b->ek == 1) CV_BATCH_LAUNCH2(B, S, 1, CV_SPEC_SPLIT); else CV_BATCH_LAUNCH2(B, S, 0, CV_SPEC_SPLIT); } \
+        if (seed_in_stage) { if (b->fp64_root) CV_BATCH_LAUNCH2(B, S, 1, CV_SPEC_SEED | CV_SPEC_FP64); \
+                             else CV_BATCH_LAUNCH2(B, S, 1, CV_SPEC_SEED); }                          \
+        else if (seed_split) CV_BATCH_LAUNCH2(B, S, 0, CV_SPEC_SPLIT);                                \
         else if (b->ek == 1) { if (b->fp64_root) CV_BATCH_LAUNCH2(B, S, 1, CV_SPEC_FP64); else CV_BATCH_LAUNCH2(B, S, 1, 0); } \
         else { if (b->fp64_root) CV_BATCH_LAUNCH2(B, S, 0, CV_SPEC_FP64); else CV_BATCH_LAUNCH2(B, S, 0, 0); }            \
     } while (0)
@@ -1597,10 +1605,10 @@ static int batch_build(cv_batch *b) {
                                                     b->d_events, b->ev[0]->eos)
 #define CV_BATCH_FIX(B, S)                                                                                    \
     do {                                                                                                      \
-        if (seeded) {                                                                                         \
+        if (seed_split) {                                                                                     \
             hipLaunchKernelGGL(k_batch_seed, sg, sb, 0, b->stream, b->d_state, b->d_events, b->seed->P,       \
                                b->seed->pk);                                                                  \
-            if (b->ek == 1) CV_BROOT_LAUNCH(B, S, 1); else CV_BROOT_LAUNCH(B, S, 0);                          \
+            CV_BROOT_LAUNCH(B, S, 0);                                                                         \
         } else if (defer) { if (b->ek == 1) CV_BFIX_LAUNCH(B, S, 1); else CV_BFIX_LAUNCH(B, S, 0); }          \
     } while (0)
     if (seeded) mlp_pack(b->seed, b->stream);

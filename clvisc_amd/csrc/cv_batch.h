@@ -51,13 +51,17 @@ __device__ __forceinline__ auto make_batch_eos(const EosP &E, const float *pt) {
     else return EosT<EK>(E, E.tab);
 }
 
-// SPEC: 0, CV_SPEC_FP64 (the FP64 root-solve branch for fast cells, CV_FLAG_FP64_ROOT) or
-// CV_SPEC_SPLIT (seeded batches: the stage writes the root solve's input to each event's cons
-// and k_batch_seed / k_batch_visc_root finish the cells).
+// SPEC: 0, CV_SPEC_FP64 (the FP64 root-solve branch for fast cells, CV_FLAG_FP64_ROOT),
+// CV_SPEC_SEED (seeded batches with an octave table: the network in the stage, mlp_P and
+// mlp_pk its parameters and packed operands), CV_SPEC_SEED | CV_SPEC_FP64, or CV_SPEC_SPLIT
+// (seeded batches with a row/col table: the stage writes the root solve's input to each
+// event's cons and k_batch_seed / k_batch_visc_root finish the cells).
 template <bool BULK, int STEP, int EK, int SPEC>
 __global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchState *__restrict__ bs,
                                                                      const BatchEvent *__restrict__ evt, EosP E,
-                                                                     unsigned long long *ctr) {
+                                                                     unsigned long long *ctr,
+                                                                     const float *__restrict__ mlp_P,
+                                                                     const uint32_t *__restrict__ mlp_pk) {
     __shared__ float s_c[CF_N][CV_TILE_MAXC];
     __shared__ float s_f[15][CV_TILE_MAXF];
     __shared__ int s_work;
@@ -116,10 +120,10 @@ __global__ __launch_bounds__(CV_TILE_THREADS, 3) void k_batch_stage(const BatchS
         float *cons = (SPEC & CV_SPEC_SPLIT) ? ev.cons : nullptr;
         if (STEP == 1) {
             q = StagePtrs{E1, P1, B1, E1, E2, P1, B1, ev.udiff, ev.IS_src, ev.bsrc, ev.Src, P2, B2, E2, ev.good,
-                          nullptr, cons, ev.fix, ev.fix_ctr};
+                          nullptr, cons, ev.fix, ev.fix_ctr, mlp_P, mlp_pk, nullptr, nullptr};
         } else {
             q = StagePtrs{E2, P2, B2, E1, E2, P1, B1, ev.udiff, ev.IS_src, ev.bsrc, ev.Src, P0, B0, E0, ev.good,
-                          ev.udiff, cons, ev.fix, ev.fix_ctr};
+                          ev.udiff, cons, ev.fix, ev.fix_ctr, mlp_P, mlp_pk, nullptr, nullptr};
         }
         tile_work<BULK, STEP, EK, SPEC, true>(ev.p, eos, st, ev.tp, q, w - ev.work0, s_c, s_f);
     }

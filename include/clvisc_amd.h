@@ -406,11 +406,13 @@ int cv_root_eval(cv_handle *h, cv_mlp *m, const float *T00, const float *M, cons
  * of the last stage for cv_get_root_iters. */
 int cv_set_root_seed(cv_handle *h, cv_mlp *m, int track_iters);
 /* No counterpart in the reference (north-star extension, the learned + batched form).  The
- * batch's stages in split form seeded by m (NULL: back to the fused stages): one persistent
- * stage launch writes every event's root-solve input, then the network and the root solve
- * run over all events inside the same graph; cells with |T^ti| > fp64_root_vmin T^tt take
- * the FP64 root under CV_FLAG_FP64_ROOT, the others start from the seed.  Every event's own
- * handle gets m as with cv_set_root_seed, so an event stepped alone gives the same fields. */
+ * batch's stages seeded by m (NULL: back to the unseeded stages).  With an octave table the
+ * network runs inside the persistent stage launch (each wave seeds its 64 cells before the
+ * root solve); with a row/col table the stages run split: one persistent stage launch writes
+ * every event's root-solve input, then the network and the root solve run over all events
+ * inside the same graph.  Cells with |T^ti| > fp64_root_vmin T^tt take the FP64 root under
+ * CV_FLAG_FP64_ROOT, the others start from the seed.  Every event's own handle gets m as
+ * with cv_set_root_seed, so an event stepped alone gives the same fields. */
 int cv_batch_set_root_seed(cv_batch *b, cv_mlp *m);
 /* Viscous stages of the tiled kernels (kinds 1, 2) in split form: the stage writes the
  * root solve's conserved input (field CV_ROOT_CONS) and k_visc_root finishes the cells
