@@ -346,24 +346,26 @@ __device__ __forceinline__ void seed_features(float T00, float M, float T03, flo
 // C tiles of one layer (rows = its units, columns = 16 samples) are directly the B operands
 // of the next and no transposition through LDS is needed.  Lane l = 16 rg + c0 holds sample
 // c0 and units 4 rg + i (i = 0..3) of each 16-unit tile.  Layer 1 is one MFMA per 16 units
-// (its 4 inputs fill k = 0..3, zero above), the biases are the accumulators' initial values,
-// tanh(z) = 1 - 2/(exp(2z) + 1) with the hardware exp and reciprocal, and BF16 conversions
-// are the hardware's round to nearest even.  The seed only sets where the root solve starts
+// (its 4 inputs fill k = 0..3, zero above), layer 3 (the |v| row) one MFMA per 16 units whose
+// A tile holds the weights in row 0, the biases are the accumulators' initial values,
+// tanh(z) = 1 - 2/(2^(2 z log2 e) + 1) is one multiply, the hardware exp2, one add, the
+// hardware reciprocal and one FMA, and BF16 conversions are the hardware's round to nearest
+// even.  (Folding 2 log2 e into the packed weights would save the multiply but changes their
+// BF16 rounding: the seeds then move 1e-2 away from the network's, against 6e-4.)  The seed only sets where the root solve starts
 // (the solve keeps the reference's bracket and stopping rule), so these approximations change
 // iteration counts, not what the solve converges to.
 #define MLP_SEED_TILES 8
 __device__ __forceinline__ float tanh_seed(float z) {
-    const float e = __expf(2.0f * z);
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+    const float e = __builtin_amdgcn_exp2f(z * 2.88539008177792681f);   // exp(2 z)
+    return fmaf(-2.0f, __builtin_amdgcn_rcpf(e + 1.0f), 1.0f);
 }
 __device__ __forceinline__ float bf16_hw(float x) { return (float)(__bf16)x; }
 __device__ __forceinline__ mlp_bf16x4 pack4_hw(float a, float b, float c, float d) {
     typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
-    const bf2 lo = (bf2){(__bf16)a, (__bf16)b}, hi = (bf2){(__bf16)c, (__bf16)d};
-    const uint32_t ulo = __builtin_bit_cast(uint32_t, lo), uhi = __builtin_bit_cast(uint32_t, hi);
-    mlp_bf16x4 r;
-    r[0] = (short)(ulo & 0xffffu); r[1] = (short)(ulo >> 16); r[2] = (short)(uhi & 0xffffu); r[3] = (short)(uhi >> 16);
-    return r;
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    // one v_cvt_pk_bf16_f32 per pair
+    const bf2 lo = __builtin_convertvector((f2){a, b}, bf2), hi = __builtin_convertvector((f2){c, d}, bf2);
+    return __builtin_bit_cast(mlp_bf16x4, make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi)));
 }
 __device__ __forceinline__ mlp_bf16x4 as_bf16x4(uint2 u) { return __builtin_bit_cast(mlp_bf16x4, u); }
 
@@ -371,32 +373,32 @@ __device__ __forceinline__ mlp_bf16x4 as_bf16x4(uint2 u) { return __builtin_bit_
 // l = 16 rg + c0 so that one wave's load of a tile is 512 contiguous bytes:
 //   W1 tile m (0..3) at uint2 64 m + l:            W1[16 m + c0][4 rg + i], zero for rg > 0
 //   W2 tile (m, s) at uint2 256 + 64 (4 m + s) + l: W2[16 m + c0][16 s + 4 rg + i]
-//   the |v| row of W3, BF16-rounded, as 64 floats at word MLP_PK_W3
-#define MLP_PK_W3 2560
-#define MLP_PK_WORDS (MLP_PK_W3 + MLP_H)
+//   W3 tile m at uint2 1280 + 64 m + l:            W3[1][16 m + 4 rg + i] (the |v| row) for
+//                                                  c0 = 0, zero for the other rows
+#define MLP_PK_WORDS (2 * 64 * 24)
 
 __global__ void k_mlp_pack_t(const float *__restrict__ P, uint32_t *pk) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < 20 * 64) {
+    if (q < 24 * 64) {
         const int t = q >> 6, l = q & 63, c0 = l & 15, rg = l >> 4;
         float w[4];
         if (t < 4) {
 #pragma unroll
             for (int i = 0; i < 4; i++) w[i] = rg == 0 ? P[MLP_OFF_W1 + (16 * t + c0) * MLP_IN + i] : 0.0f;
-        } else {
+        } else if (t < 20) {
             const int m = (t - 4) >> 2, s = (t - 4) & 3;
 #pragma unroll
             for (int i = 0; i < 4; i++) w[i] = P[MLP_OFF_W2 + (16 * m + c0) * MLP_H + 16 * s + 4 * rg + i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) w[i] = c0 == 0 ? P[MLP_OFF_W3 + MLP_H + 16 * (t - 20) + 4 * rg + i] : 0.0f;
         }
         reinterpret_cast<uint2 *>(pk)[q] = __builtin_bit_cast(uint2, pack4_hw(w[0], w[1], w[2], w[3]));
-    } else if (q < 20 * 64 + MLP_H) {
-        const int c = q - 20 * 64;
-        reinterpret_cast<float *>(pk)[MLP_PK_W3 + c] = bf16_hw(P[MLP_OFF_W3 + MLP_H + c]);
     }
 }
-#define MLP_PK_THREADS (20 * 64 + MLP_H)
+#define MLP_PK_THREADS (24 * 64)
 
-// |v| seeds of sample c0 of NG groups of 16 samples (y[g] on every lane with that c0), the
+// |v| seeds of sample c0 of NG groups of 16 samples (y[g] on the lanes with rg = 0), the
 // groups sharing each weight load.  xT[g]: the samples' BF16 features on the lanes with
 // rg = 0, zero on the others.  A sample's seed depends on its own features only, whatever
 // NG.  All 64 lanes of the wave must execute it.
@@ -405,7 +407,6 @@ __device__ __forceinline__ void seed_groups_t(const float *__restrict__ P, const
                                               const mlp_bf16x4 xT[NG], float y[NG]) {
     const int l = threadIdx.x & 63, rg = l >> 4;
     const uint2 *__restrict__ tiles = reinterpret_cast<const uint2 *>(pk);
-    const float *__restrict__ w3 = reinterpret_cast<const float *>(pk) + MLP_PK_W3;
     mlp_bf16x4 h1[NG][4];
 #pragma unroll
     for (int m = 0; m < 4; m++) {
@@ -417,9 +418,10 @@ __device__ __forceinline__ void seed_groups_t(const float *__restrict__ P, const
             h1[g][m] = pack4_hw(tanh_seed(z[0]), tanh_seed(z[1]), tanh_seed(z[2]), tanh_seed(z[3]));
         }
     }
-    float part[NG];
+    const float b3 = P[MLP_OFF_B3 + 1];
+    mlp_f32x4 out[NG];
 #pragma unroll
-    for (int g = 0; g < NG; g++) part[g] = 0.0f;
+    for (int g = 0; g < NG; g++) out[g] = mlp_f32x4{b3, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int m = 0; m < 4; m++) {
         const float4 b = *reinterpret_cast<const float4 *>(P + MLP_OFF_B2 + 16 * m + 4 * rg);
@@ -432,22 +434,15 @@ __device__ __forceinline__ void seed_groups_t(const float *__restrict__ P, const
 #pragma unroll
             for (int g = 0; g < NG; g++) z[g] = mfma16(w, h1[g][s], z[g]);
         }
-        const float4 w = *reinterpret_cast<const float4 *>(w3 + 16 * m + 4 * rg);
+        // layer 3: row 0 of the output tile (column = sample) accumulates the |v| row
+        const mlp_bf16x4 w3 = as_bf16x4(tiles[1280 + 64 * m + l]);
 #pragma unroll
-        for (int g = 0; g < NG; g++) {
-            part[g] = part[g] + bf16_hw(tanh_seed(z[g][0])) * w.x;
-            part[g] = part[g] + bf16_hw(tanh_seed(z[g][1])) * w.y;
-            part[g] = part[g] + bf16_hw(tanh_seed(z[g][2])) * w.z;
-            part[g] = part[g] + bf16_hw(tanh_seed(z[g][3])) * w.w;
-        }
+        for (int g = 0; g < NG; g++)
+            out[g] = mfma16(w3, pack4_hw(tanh_seed(z[g][0]), tanh_seed(z[g][1]), tanh_seed(z[g][2]),
+                                         tanh_seed(z[g][3])), out[g]);
     }
-    const float b3 = P[MLP_OFF_B3 + 1];
 #pragma unroll
-    for (int g = 0; g < NG; g++) {
-        float v = part[g] + __shfl_xor(part[g], 16, 64);
-        v = v + __shfl_xor(v, 32, 64);
-        y[g] = v + b3;
-    }
+    for (int g = 0; g < NG; g++) y[g] = out[g][0];
 }
 
 // The features of one cell's root-solve input after the |T^ti| <= 0.999 T^tt clamp
@@ -485,8 +480,10 @@ __device__ __forceinline__ float seed_wave_t(const float *__restrict__ P, const 
         asm volatile("" : "+s"(zero));
         float y[2];
         seed_groups_t<2>(P + zero, pk + zero, xT, y);
-        if (rg == g0) v = y[0];
-        if (rg == g0 + 1) v = y[1];
+        // sample c0's seed is on lane c0 (rg = 0); lane 16 g + c0 holds that cell
+        const float ya = __shfl(y[0], c0, 64), yb = __shfl(y[1], c0, 64);
+        if (rg == g0) v = ya;
+        if (rg == g0 + 1) v = yb;
     }
     return v;
 }
