@@ -197,6 +197,40 @@ extern "C" int cv_device_info(int device, char *name, int name_len, int32_t *cus
     return CV_OK;
 }
 
+// every float with biased exponent 1..252 (2^-126 <= |b| < 2^126), both signs: rcpn(b)
+// against the correctly rounded 1/b
+#define CV_RCPN_PER_SIGN (252ull << 23)
+__global__ void k_selftest_rcpn(unsigned long long *bad) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long cnt = 0;
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < 2 * CV_RCPN_PER_SIGN; u += stride) {
+        const uint32_t sign = u >= CV_RCPN_PER_SIGN ? 0x80000000u : 0u;
+        const uint32_t bits = sign | (uint32_t)(u % CV_RCPN_PER_SIGN + (1ull << 23));
+        const float b = __uint_as_float(bits);
+        if (__float_as_uint(rcpn(b)) != __float_as_uint(rcp(b))) cnt++;
+    }
+    if (cnt) atomicAdd(bad, cnt);
+}
+
+extern "C" int cv_selftest_rcpn(int device, uint64_t *mismatches, uint64_t *checked) {
+    if (!mismatches || !checked) return fail(CV_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(device));
+    unsigned long long *d = nullptr;
+    HIP_TRY(hipMalloc((void **)&d, sizeof(unsigned long long)));
+    unsigned long long h = 0;
+    hipError_t e = hipMemcpy(d, &h, sizeof(h), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_selftest_rcpn, dim3(16384), dim3(256), 0, 0, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_selftest_rcpn: %s", hipGetErrorString(e));
+    *mismatches = h;
+    *checked = 2 * CV_RCPN_PER_SIGN;
+    return CV_OK;
+}
+
 static StageP stage_params(const cv_handle *h, int step, float tau) {
     StageP s;
     s.step = step;

@@ -64,6 +64,14 @@ struct JetP {
 // Elementary operations
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float rcp(float b) { return 1.0f / b; }     // correctly rounded
+// 1/b for 2^-126 <= |b| < 2^126: the hardware reciprocal refined by one FMA Newton step
+// (3 instructions against the 10 of the correctly rounded division).  On that whole range it
+// equals rcp(b) for every float (cv_selftest_rcpn runs the check on the device); its callers
+// pass only operands that lie in that range by construction.
+__device__ __forceinline__ float rcpn(float b) {
+    const float r0 = __builtin_amdgcn_rcpf(b);
+    return fmaf(fmaf(-b, r0, 1.0f), r0, r0);
+}
 __device__ __forceinline__ float rmax(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float rmin(float a, float b) { return b < a ? b : a; }
 // Regulation of the updated bulk pressure (the bulk Pi extension): |Pi| <= P(e)/2 and
@@ -110,7 +118,7 @@ __device__ __forceinline__ float slope(float a, float b, float c) {
     return __builtin_amdgcn_fmed3f(0.5f * (c - a), inner, 0.0f);
 }
 __device__ __forceinline__ float gamma3(float vx, float vy, float vz) {
-    return rcp(sqrtf(rmax(1.0f - vx * vx - vy * vy - vz * vz, CV_ACU)));
+    return rcpn(sqrtf(rmax(1.0f - vx * vx - vy * vy - vz * vz, CV_ACU)));   // operand in [3.1e-4, 1]
 }
 
 // ---------------------------------------------------------------------------
@@ -270,7 +278,10 @@ __device__ __forceinline__ float max_prop_speed(const PhysP &p, const EOS &eos, 
 // a negative effective pressure would allow |v| > 1 in v = M / (T00 + P + Pi)).
 // v0: start of the iteration (the reference starts at the middle of [0, 1]; a learned
 // seed starts elsewhere, the bracket and the stopping rule stay the same).
-template <class EOS>
+// RN: the Newton step divides with rcpn.  Valid for the stages' own states: T00 >= 1e-7 / tau
+// (the acu floor), M <= 0.999 T00 and P + Pi >= 0, so df >= (2/3) T00 is in rcpn's range;
+// states given from outside (cv_root_eval) take the correctly rounded division.
+template <bool RN = true, class EOS>
 __device__ __forceinline__ float root_newton_from(const EOS &eos, float T00, float M, float bulk, bool clamp0,
                                                   float v0, int *iters) {
     float vl = 0.0f, vh = 1.0f;
@@ -291,7 +302,7 @@ __device__ __forceinline__ float root_newton_from(const EOS &eos, float T00, flo
             v = vl + dv;
         } else {
             dvold = dv;
-            dv = f * rcp(df);
+            dv = f * (RN ? rcpn(df) : rcp(df));
             v -= dv;
         }
         i++;

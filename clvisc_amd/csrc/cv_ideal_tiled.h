@@ -239,7 +239,7 @@ __global__ __launch_bounds__(CV_ITILE_THREADS) void k_ideal_tiled(PhysP p, EosP 
             float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
             const float SC = 0.999f;
             if (M > T00) {
-                const float sc = (SC * T00) * rcp(M);
+                const float sc = (SC * T00) * rcpn(M);      // M > T00 >= 1e-7 / tau
                 T01 *= sc; T02 *= sc; T03 *= sc;
                 M = SC * T00;
             }
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(CV_ITILE_THREADS) void k_ideal_tiled(PhysP p, EosP 
             float ed_find = root_newton_from(eos, T00, M, 0.0f, false, 0.5f * (0.0f + 1.0f), &it);
             ed_find = rmax(0.0f, ed_find);
             const float pr = eos.P(ed_find);
-            const float repv = rcp(rmax(CV_ACU, T00 + pr));
+            const float repv = rcpn(rmax(CV_ACU, T00 + pr));   // >= 1e-7
             stp(ev_out, bo, nb, 0, ed_find);
             stp(ev_out, bo, nb, 1, T01 * repv);
             stp(ev_out, bo, nb, 2, T02 * repv);

@@ -289,7 +289,7 @@ __global__ __launch_bounds__(256) void k_root_eval(EosP E, const float *__restri
     const float t = T00[q], m = M[q];
     int it = 0;
     const float v0 = seed_v ? fminf(fmaxf(seed_v[q], 0.0f), 0.9999f) : 0.5f * (0.0f + 1.0f);
-    float e = rmax(0.0f, root_newton_from(eos, t, m, 0.0f, false, v0, &it));
+    float e = rmax(0.0f, root_newton_from<false>(eos, t, m, 0.0f, false, v0, &it));
     e_out[q] = e;
     v_out[q] = m * rcp(rmax(CV_ACU, t + eos.P(e)));
     iters[q] = it;

@@ -178,7 +178,7 @@ __device__ __forceinline__ void update_ev_tail(const EOS &eos, float ed_find, fl
     ed_find = rmax(0.0f, ed_find);
     float pr = eos.P(ed_find) + bout;
     if (BULK) pr = rmax(pr, 0.0f);
-    float repv = rcp(rmax(CV_ACU, T00 + pr));
+    float repv = rcpn(rmax(CV_ACU, T00 + pr));         // T00 + pr >= 1e-7 (rcpn's range)
     float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
     stp(ev_out, bo, nb, 0, en.x);
     stp(ev_out, bo, nb, 1, en.y);
@@ -549,7 +549,8 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
             const float temperature = cpTs.z;
             const float local_etaos = etaos(p, temperature);
             float etav = local_etaos * cpTs.w * CV_HBARC;
-            float one_over_taupi = temperature * rcp(3.0f * rmax(CV_ACU, local_etaos) * CV_HBARC);
+            // rcpn: the operands below are >= 5.9e-8 (acu floor) and >= 1 (u^tau + ...)
+            float one_over_taupi = temperature * rcpn(3.0f * rmax(CV_ACU, local_etaos) * CV_HBARC);
             float coef_pipi = 0.0f;
             const bool gubser = (SPEC & CV_SPEC_TESTS) && (p.flags & 2);
             // vorticity couplings (test-branch instances only; production kernels carry neither)
@@ -568,7 +569,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
             const float gd[4] = {1.0f, -1.0f, -1.0f, -1.0f};
             float max_abs = 0.0f;
             float out[10];
-            const float rden = rcp(u_new[0] + p.dt * one_over_taupi);
+            const float rden = rcpn(u_new[0] + p.dt * one_over_taupi);
 #pragma unroll
             for (int mu = 0; mu < 4; mu++) {
 #pragma unroll
@@ -598,13 +599,13 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
             if (BULK) {
                 float zs = zetaos(p, temperature);
                 float zetav = zs * cpTs.w * CV_HBARC;
-                float oot_b = temperature * rcp(3.0f * rmax(CV_ACU, zs) * CV_HBARC);
+                float oot_b = temperature * rcpn(3.0f * rmax(CV_ACU, zs) * CV_HBARC);
                 float b2 = bk_own;
                 float bsrc_ = (b2 * theta) * p.r_3 - (b2 * u[0]) * st.r_tau;
                 bs += bsrc_;
                 b_start = STEP == 1 ? b2 : bk1_pre;
                 float b_old = b_start * u_old[0];
-                bout = (b_old + (bs * p.dt) * st.r_step - p.dt * oot_b * zetav * theta) * rcp(u_new[0] + p.dt * oot_b);
+                bout = (b_old + (bs * p.dt) * st.r_step - p.dt * oot_b * zetav * theta) * rcpn(u_new[0] + p.dt * oot_b);
                 bout = bulk_regulate(bout, eos.P(e2.x), e2.x);
                 if (fabsf(bout) > max_abs) max_abs = fabsf(bout);
             }
@@ -654,7 +655,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
             } else {
             float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
             if (M > T00) {
-                float sc = (0.999f * T00) * rcp(M);
+                float sc = (0.999f * T00) * rcpn(M);            // M > T00 >= 1e-7 / tau
                 T01 *= sc; T02 *= sc; T03 *= sc;
                 M = 0.999f * T00;
             }
@@ -760,7 +761,7 @@ __device__ __forceinline__ void visc_root_cell(const PhysP &p, const EosP &E, in
     const float bout = BULK ? bk_out[I] : 0.0f;
     float M = sqrtf(T01 * T01 + T02 * T02 + T03 * T03);
     if (M > T00) {
-        float sc = (0.999f * T00) * rcp(M);
+        float sc = (0.999f * T00) * rcpn(M);                // M > T00 >= 1e-7 / tau
         T01 *= sc; T02 *= sc; T03 *= sc;
         M = 0.999f * T00;
     }
@@ -777,7 +778,7 @@ __device__ __forceinline__ void visc_root_cell(const PhysP &p, const EosP &E, in
     ed_find = rmax(0.0f, ed_find);
     float pr = eos.P(ed_find) + bout;
     if (BULK) pr = rmax(pr, 0.0f);
-    float repv = rcp(rmax(CV_ACU, T00 + pr));
+    float repv = rcpn(rmax(CV_ACU, T00 + pr));         // T00 + pr >= 1e-7 (rcpn's range)
     float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
     ev_out[I] = en.x;
     ev_out[n + I] = en.y;
@@ -817,7 +818,7 @@ __device__ __forceinline__ void fp64_fix_cell(const PhysP &p, const EosP &E, con
     ed_find = rmax(0.0f, ed_find);
     float pr = eos.P(ed_find) + f.bulk;
     if (BULK) pr = rmax(pr, 0.0f);
-    float repv = rcp(rmax(CV_ACU, f.T00 + pr));
+    float repv = rcpn(rmax(CV_ACU, f.T00 + pr));
     float4 en = make_float4(ed_find, T01 * repv, T02 * repv, T03 * repv);
     ev_out[I] = en.x;
     ev_out[n + I] = en.y;

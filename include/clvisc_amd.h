@@ -118,6 +118,13 @@ const char *cv_last_error(void);
 /* Library / device info: fills name (>= 64 bytes) and compute units. */
 int cv_device_info(int device, char *name, int name_len, int32_t *cus);
 
+/* No counterpart in the reference.  Device check of the reciprocal the stage kernels use
+ * where the operand is known to lie in [2^-126, 2^126) (the hardware reciprocal refined by one
+ * FMA Newton step): for every float b with 2^-126 <= |b| < 2^126 it is compared bit for bit
+ * with the correctly rounded 1/b; *checked = the floats compared, *mismatches = those that
+ * differ (0 is required for the stage kernels' results to equal the oracle's). */
+int cv_selftest_rcpn(int device, uint64_t *mismatches, uint64_t *checked);
+
 /* Construction = CLIdeal.__init__ / CLVisc.__init__ buffer setup (ideal.py:30-119,
  * visc.py:24-104) on HIP device `device`.  Allocates all fields zeroed; viscous != 0
  * also allocates pi / IS_src / udiff / goodcell.  Does not run any kernel. */
