@@ -102,7 +102,7 @@ def lib():
         vp, i32, i64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_float
         sig = {
             'cv_device_info': [C.c_int, C.c_char_p, C.c_int, C.POINTER(C.c_int32)],
-            'cv_selftest_rcpn': [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
+            'cv_selftest_fastmath': [C.c_int, C.POINTER(C.c_uint64)],
             'cv_create': [C.POINTER(CvParams), C.POINTER(CvEos), C.c_int, C.c_int, C.POINTER(vp)],
             'cv_destroy': [vp],
             'cv_set_field': [vp, C.c_int, vp, i64],
@@ -188,7 +188,7 @@ def check(rc):
 
 def exported_symbols():
     """Names of the C-ABI entry points this binding expects."""
-    return ['cv_last_error', 'cv_device_info', 'cv_selftest_rcpn', 'cv_create', 'cv_destroy', 'cv_set_field',
+    return ['cv_last_error', 'cv_device_info', 'cv_selftest_fastmath', 'cv_create', 'cv_destroy', 'cv_set_field',
             'cv_get_field', 'cv_copy_field', 'cv_gather', 'cv_visc_initialize', 'cv_ideal_stage', 'cv_is_stage',
             'cv_visc_stage', 'cv_update_udiff', 'cv_ideal_steps_async', 'cv_visc_steps_async',
             'cv_max_ed', 'cv_pizz_o_ep', 'cv_sync', 'cv_last_elapsed_ms', 'cv_profile_visc_step',
@@ -618,12 +618,13 @@ class Mlp(object):
             pass
 
 
-def selftest_rcpn(device=0):
-    """(mismatches, floats checked) of the stage kernels' reciprocal against the correctly
-    rounded 1/b over every float with 2^-126 <= |b| < 2^126 (cv_selftest_rcpn)."""
-    bad, n = C.c_uint64(), C.c_uint64()
-    check(lib().cv_selftest_rcpn(device, C.byref(bad), C.byref(n)))
-    return bad.value, n.value
+def selftest_fastmath(device=0):
+    """cv_selftest_fastmath: ((mismatches, floats checked) of the stage kernels' reciprocal
+    against 1/b on 2^-126 <= |b| < 2^126, the same of their square root against sqrtf on
+    [2^-24, 1])."""
+    out = (C.c_uint64 * 4)()
+    check(lib().cv_selftest_fastmath(device, out))
+    return (out[0], out[1]), (out[2], out[3])
 
 
 def device_info(device=0):

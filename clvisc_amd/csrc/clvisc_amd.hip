@@ -197,37 +197,47 @@ extern "C" int cv_device_info(int device, char *name, int name_len, int32_t *cus
     return CV_OK;
 }
 
-// every float with biased exponent 1..252 (2^-126 <= |b| < 2^126), both signs: rcpn(b)
-// against the correctly rounded 1/b
+// rcpn: every float with biased exponent 1..252 (2^-126 <= |b| < 2^126), both signs, against
+// the correctly rounded 1/b; sqrtn: every float of [2^-24, 1] against sqrtf
 #define CV_RCPN_PER_SIGN (252ull << 23)
-__global__ void k_selftest_rcpn(unsigned long long *bad) {
+#define CV_SQRTN_LO 0x33800000u           // 2^-24
+#define CV_SQRTN_HI 0x3f800000u           // 1
+__global__ void k_selftest_fastmath(unsigned long long *bad) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    unsigned long long cnt = 0;
-    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < 2 * CV_RCPN_PER_SIGN; u += stride) {
-        const uint32_t sign = u >= CV_RCPN_PER_SIGN ? 0x80000000u : 0u;
-        const uint32_t bits = sign | (uint32_t)(u % CV_RCPN_PER_SIGN + (1ull << 23));
-        const float b = __uint_as_float(bits);
-        if (__float_as_uint(rcpn(b)) != __float_as_uint(rcp(b))) cnt++;
+    const uint64_t nr = 2 * CV_RCPN_PER_SIGN, ns = (uint64_t)(CV_SQRTN_HI - CV_SQRTN_LO) + 1;
+    unsigned long long cr = 0, cs = 0;
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < nr + ns; u += stride) {
+        if (u < nr) {
+            const uint32_t sign = u >= CV_RCPN_PER_SIGN ? 0x80000000u : 0u;
+            const float b = __uint_as_float(sign | (uint32_t)(u % CV_RCPN_PER_SIGN + (1ull << 23)));
+            cr += __float_as_uint(rcpn(b)) != __float_as_uint(rcp(b));
+        } else {
+            const float y = __uint_as_float(CV_SQRTN_LO + (uint32_t)(u - nr));
+            cs += __float_as_uint(sqrtn(y)) != __float_as_uint(sqrtf(y));
+        }
     }
-    if (cnt) atomicAdd(bad, cnt);
+    if (cr) atomicAdd(&bad[0], cr);
+    if (cs) atomicAdd(&bad[1], cs);
 }
 
-extern "C" int cv_selftest_rcpn(int device, uint64_t *mismatches, uint64_t *checked) {
-    if (!mismatches || !checked) return fail(CV_ERR_ARG, "null argument");
+extern "C" int cv_selftest_fastmath(int device, uint64_t out[4]) {
+    if (!out) return fail(CV_ERR_ARG, "null argument");
     HIP_TRY(hipSetDevice(device));
     unsigned long long *d = nullptr;
-    HIP_TRY(hipMalloc((void **)&d, sizeof(unsigned long long)));
-    unsigned long long h = 0;
-    hipError_t e = hipMemcpy(d, &h, sizeof(h), hipMemcpyHostToDevice);
+    HIP_TRY(hipMalloc((void **)&d, 2 * sizeof(unsigned long long)));
+    unsigned long long h[2] = {0, 0};
+    hipError_t e = hipMemcpy(d, h, sizeof(h), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_selftest_rcpn, dim3(16384), dim3(256), 0, 0, d);
+        hipLaunchKernelGGL(k_selftest_fastmath, dim3(16384), dim3(256), 0, 0, d);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
     (void)hipFree(d);
-    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_selftest_rcpn: %s", hipGetErrorString(e));
-    *mismatches = h;
-    *checked = 2 * CV_RCPN_PER_SIGN;
+    if (e != hipSuccess) return fail(CV_ERR_HIP, "cv_selftest_fastmath: %s", hipGetErrorString(e));
+    out[0] = h[0];
+    out[1] = 2 * CV_RCPN_PER_SIGN;
+    out[2] = h[1];
+    out[3] = (uint64_t)(CV_SQRTN_HI - CV_SQRTN_LO) + 1;
     return CV_OK;
 }
 

@@ -66,11 +66,20 @@ struct JetP {
 __device__ __forceinline__ float rcp(float b) { return 1.0f / b; }     // correctly rounded
 // 1/b for 2^-126 <= |b| < 2^126: the hardware reciprocal refined by one FMA Newton step
 // (3 instructions against the 10 of the correctly rounded division).  On that whole range it
-// equals rcp(b) for every float (cv_selftest_rcpn runs the check on the device); its callers
-// pass only operands that lie in that range by construction.
+// equals rcp(b) for every float (cv_selftest_fastmath runs the check on the device); its
+// callers pass only operands that lie in that range by construction.
 __device__ __forceinline__ float rcpn(float b) {
     const float r0 = __builtin_amdgcn_rcpf(b);
     return fmaf(fmaf(-b, r0, 1.0f), r0, r0);
+}
+// sqrt(y) for 2^-24 <= y <= 1: the hardware reciprocal square root r, s = y r and one FMA
+// Newton step (1 transcendental and 4 instructions against about 12 for the correctly
+// rounded sqrtf); on that whole range it equals sqrtf(y) for every float (device check as
+// for rcpn).  The bare hardware square root differs from sqrtf on about 15 % of all floats.
+__device__ __forceinline__ float sqrtn(float y) {
+    const float r = __builtin_amdgcn_rsqf(y);
+    const float s = y * r;
+    return fmaf(fmaf(-s, s, y), 0.5f * r, s);
 }
 __device__ __forceinline__ float rmax(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float rmin(float a, float b) { return b < a ? b : a; }
@@ -118,7 +127,8 @@ __device__ __forceinline__ float slope(float a, float b, float c) {
     return __builtin_amdgcn_fmed3f(0.5f * (c - a), inner, 0.0f);
 }
 __device__ __forceinline__ float gamma3(float vx, float vy, float vz) {
-    return rcpn(sqrtf(rmax(1.0f - vx * vx - vy * vy - vz * vz, CV_ACU)));   // operand in [3.1e-4, 1]
+    // max(1 - v^2, acu) lies in [1e-7, 1] (sqrtn's range), its square root in [3.1e-4, 1] (rcpn's)
+    return rcpn(sqrtn(rmax(1.0f - vx * vx - vy * vy - vz * vz, CV_ACU)));
 }
 
 // ---------------------------------------------------------------------------

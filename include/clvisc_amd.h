@@ -118,12 +118,15 @@ const char *cv_last_error(void);
 /* Library / device info: fills name (>= 64 bytes) and compute units. */
 int cv_device_info(int device, char *name, int name_len, int32_t *cus);
 
-/* No counterpart in the reference.  Device check of the reciprocal the stage kernels use
- * where the operand is known to lie in [2^-126, 2^126) (the hardware reciprocal refined by one
- * FMA Newton step): for every float b with 2^-126 <= |b| < 2^126 it is compared bit for bit
- * with the correctly rounded 1/b; *checked = the floats compared, *mismatches = those that
- * differ (0 is required for the stage kernels' results to equal the oracle's). */
-int cv_selftest_rcpn(int device, uint64_t *mismatches, uint64_t *checked);
+/* No counterpart in the reference.  Device check of the two short forms the stage kernels use
+ * where the operand is known to lie in their range: the reciprocal (hardware reciprocal + one
+ * FMA Newton step) against the correctly rounded 1/b for every float with
+ * 2^-126 <= |b| < 2^126, and the square root (hardware reciprocal square root + one FMA
+ * Newton step) against the correctly rounded sqrtf for every float of [2^-24, 1], bit for
+ * bit.  out[0] / out[1]: reciprocal mismatches / floats compared; out[2] / out[3]: square
+ * root mismatches / floats compared (zero mismatches are required for the stage kernels'
+ * results to equal the oracle's). */
+int cv_selftest_fastmath(int device, uint64_t out[4]);
 
 /* Construction = CLIdeal.__init__ / CLVisc.__init__ buffer setup (ideal.py:30-119,
  * visc.py:24-104) on HIP device `device`.  Allocates all fields zeroed; viscous != 0
