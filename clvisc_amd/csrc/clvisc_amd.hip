@@ -293,6 +293,7 @@ extern "C" int cv_create(const cv_params *p, const cv_eos *eos, int device, int 
     ph.etaos_xmin = p->etaos_xmin; ph.etaos_ymin = p->etaos_ymin;
     ph.etaos_ls = p->etaos_left_slop; ph.etaos_rs = p->etaos_right_slop; ph.lam1 = p->lam1;
     ph.zmax = p->zetaos_max; ph.ztpeak = p->zetaos_tpeak; ph.zwidth = p->zetaos_width;
+    ph.r_zwidth = 1.0f / p->zetaos_width;
     ph.fp64_vmin = p->fp64_root_vmin;
     ph.flags = p->flags;
     // 32-bit plane offsets of the tiled kernel: 10 planes of 4 n bytes below 2^32

@@ -40,7 +40,7 @@ struct PhysP {
     float dx, dy, dz, dt;
     float r_dx, r_dy, r_dz, r_dt, r_3;
     float etaos_xmin, etaos_ymin, etaos_ls, etaos_rs, lam1;
-    float zmax, ztpeak, zwidth;
+    float zmax, ztpeak, zwidth, r_zwidth;   // r_zwidth = RN(1 / zwidth), from the host
     float fp64_vmin;
     int flags;
     int lam_fast;           // 1 when the table's cs2 <= 0.5 (max(lambda,1) shortcut valid)
@@ -255,7 +255,7 @@ __device__ __forceinline__ float etaos(const PhysP &p, float t) {
                               : (p.etaos_rs * (t - p.etaos_xmin) + p.etaos_ymin);
 }
 __device__ __forceinline__ float zetaos(const PhysP &p, float t) {
-    float d = (t - p.ztpeak) * rcp(p.zwidth);
+    float d = (t - p.ztpeak) * p.r_zwidth;
     return p.zmax * rcp(1.0f + d * d);
 }
 
