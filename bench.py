@@ -226,7 +226,11 @@ def ensemble_ic(d, nhot, seed):
 
 def run_ensemble(args, p, tab, name):
     """Batched event-by-event workload: one persistent launch per RK stage over all
-    events (graph-replayed steps) vs the same events stepped one after another."""
+    events (graph-replayed steps) vs the same events stepped one after another.  With
+    --learned also the learned + batched form: the same events in a second batch whose root
+    solves are seeded by the network (inside the persistent stage launch), trained first on
+    the split states of one 161x161x41 event of the ensemble; reported against both plain
+    forms, with the largest relative e difference from the plain batch after the same steps."""
     from clvisc_amd import _lib
     from clvisc_amd.autotune import Autotuner
     dims, nhot = ensemble_events()
@@ -296,6 +300,9 @@ def run_ensemble(args, p, tab, name):
     log('one event after another: %.4g cell-updates/s (%d steps)' % (seq, nseq))
     for hs in singles:
         hs.close()
+    learned = None
+    if args.learned:
+        learned = ensemble_learned(args, p, tab, dims, evs, batch, value, seq)
     out = {
         'metric': 'cell-updates/s, batched event-by-event viscous KT+IS (64 uneven events)',
         'value': value, 'unit': 'cell-updates/s', 'n_gpus': 1,
@@ -309,8 +316,87 @@ def run_ensemble(args, p, tab, name):
                    'sequential_value': seq, 'batched_over_sequential': value / seq,
                    'physical': bool(np.isfinite(edmax) and 0.0 < edmax <= 50.0)},
     }
+    if learned is not None:
+        out['learned_batched'] = learned
     print(json.dumps(out))
     batch.close()
+
+
+def ensemble_learned(args, p, tab, dims, evs, batch, plain_value, seq_value):
+    """The learned + batched form of the ensemble (see run_ensemble); batch is the plain batch
+    after its warm-up and timed steps."""
+    from clvisc_amd import _lib, surrogate
+    e3 = next(e for e, d in enumerate(dims) if d[2] > 1)
+    q = _lib.CvParams.from_buffer_copy(p)
+    q.nx, q.ny, q.nz = dims[e3]
+    h = _lib.Handle(q, tab, viscous=True)
+    h.visc_initialize(TAU0)
+    h.ideal_stage(1, np.float32(TAU0))
+    h.set('ev1', evs[e3])
+    t0 = time.perf_counter()
+    states = surrogate.visc_run_states(h, 0, 6, every=2)
+    h.close()
+    mlp, hist = surrogate.train_seed(states, steps=2000, batch=4096, lr=3e-3, nmax=5_000_000)
+    t_train = time.perf_counter() - t0
+    log('ensemble learned: seed trained on %d states of event %d (%s) in %.1fs, loss %.4g -> %.4g'
+        % (states[0].size, e3, 'x'.join(map(str, dims[e3])), t_train, hist[:20].mean(), hist[-20:].mean()))
+    del states
+    bl = _lib.Batch(p, tab, dims)
+    for e in range(len(dims)):
+        hh = bl.handle(e)
+        if args.autotune:                       # the plain batch's tuned tiles and chunks
+            ref = batch.handle(e)
+            hh.set_stage_kernel(*ref.stage_kernel())
+            hh.set_march_chunk(ref.march_chunk()[0])
+        hh.visc_initialize(TAU0)
+        hh.ideal_stage(1, np.float32(TAU0))
+        hh.set('ev1', evs[e])
+    bl.set_root_seed(mlp)
+    ncell = sum(a * b * c for a, b, c in dims)
+    bl.steps(args.warmup)
+    bl.sync()
+    t0 = time.perf_counter()
+    bl.steps(args.steps, sync=False)
+    bl.sync()
+    wall = time.perf_counter() - t0
+    value = ncell * args.steps / wall
+    rel = 0.0
+    for e in range(len(dims)):
+        ep, el = batch.handle(e).get('ev1')[:, 0], bl.handle(e).get('ev1')[:, 0]
+        big = ep > 1e-3 * ep.max()
+        rel = max(rel, float((np.abs(el[big] - ep[big]) / ep[big]).max()))
+    log('learned + batched: %d steps %.3fs -> %.4g cell-updates/s: %.3fx the plain batch, %.3fx the events '
+        'one after another; max rel e vs the plain batch %.3g' % (args.steps, wall, value, value / plain_value,
+                                                                    value / seq_value, rel))
+    bl.close()
+    mlp.close()
+    # FP32 sensitivity of the plain batch itself: the same steps from the initial fields with e
+    # scaled by (1 + 2^-23) (one ulp), compared with the plain batch
+    bu = _lib.Batch(p, tab, dims)
+    for e in range(len(dims)):
+        hh = bu.handle(e)
+        if args.autotune:
+            ref = batch.handle(e)
+            hh.set_stage_kernel(*ref.stage_kernel())
+            hh.set_march_chunk(ref.march_chunk()[0])
+        hh.visc_initialize(TAU0)
+        hh.ideal_stage(1, np.float32(TAU0))
+        evp = np.array(evs[e], np.float32, copy=True)
+        evp[:, 0] *= np.float32(1.0 + 2.0 ** -23)
+        hh.set('ev1', evp)
+    bu.steps(args.warmup + args.steps)
+    rel_ulp = 0.0
+    for e in range(len(dims)):
+        ep, eu = batch.handle(e).get('ev1')[:, 0], bu.handle(e).get('ev1')[:, 0]
+        big = ep > 1e-3 * ep.max()
+        rel_ulp = max(rel_ulp, float((np.abs(eu[big] - ep[big]) / ep[big]).max()))
+    bu.close()
+    log('plain batch from the initial fields perturbed by one ulp: max rel e vs the plain batch %.3g' % rel_ulp)
+    return {'value': value, 'unit': 'cell-updates/s', 'ms_per_step': 1000.0 * wall / args.steps,
+            'over_plain_batched': value / plain_value, 'over_plain_sequential': value / seq_value,
+            'max_rel_e_vs_plain_batched': rel, 'matched': bool(rel <= MATCHED_REL),
+            'plain_one_ulp_ic_max_rel_e': rel_ulp, 'within_one_ulp_sensitivity': bool(rel <= rel_ulp),
+            'train_seconds': t_train, 'loss_last': float(hist[-20:].mean())}
 
 
 MATCHED_REL = 1e-4     # learned form counts as matched when max rel |e_learned - e_plain| <= this
@@ -814,7 +900,7 @@ def main():
                     help='single: one 385x385x115 event (the headline); ensemble: 64 uneven events batched; '
                          'surrogate: learned root-solve seed on the ideal Gubser workload')
     ap.add_argument('--learned', action='store_true',
-                    help='combined: also the learned + batched form (seeded root solve) of the run')
+                    help='ensemble, combined: also the learned + batched form (seeded root solve) of the run')
     ap.add_argument('--autotune', action='store_true', help='ensemble: tune tiles and chunks per grid shape')
     ap.add_argument('--autotune-cache', default='', help='ensemble: JSON file of tuned shapes')
     ap.add_argument('--counter-pass', action='store_true',
