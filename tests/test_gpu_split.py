@@ -207,3 +207,28 @@ def test_fused_seed_equals_split_seed(kernel, bulk, fp64):
     hs.close()
     hf.close()
     m.close()
+
+
+def test_fused_seed_request_on_plane_tiled_kernel_runs_split():
+    """The in-stage seed needs the x-marching kernel; asked for on the plane-tiled kernel
+    (kind 1) the stage runs split, with the split form's fields and seeds."""
+    p, tab, ev = _case(29, 27, 11, bulk=(0.05, 0.155, 0.02), nhot=10, seed=7)
+    _say('case: fused seed requested on kind 1, 29x27x11, 3 + 2 steps')
+    rng = np.random.default_rng(24)
+    m = _lib.Mlp()
+    m.set_params((rng.standard_normal(MO.NPARAM) * 0.3).astype(np.float32))
+    hs = _start(p, tab, ev, (1, 0, 0))
+    hf = _start(p, tab, ev, (1, 0, 0))
+    hs.visc_steps(0, 3)
+    hf.visc_steps(0, 3)
+    hs.set_root_seed(m, track_iters=True)
+    hf.set_root_seed(m, track_iters=True)
+    hf.set_root_split(False)
+    hs.visc_steps(3, 2)
+    hf.visc_steps(3, 2)
+    for name in ('ev1', 'pi1', 'bulk1', 'root_seed'):
+        np.testing.assert_array_equal(hf.get(name), hs.get(name), err_msg=name)
+    np.testing.assert_array_equal(hf.root_iters(), hs.root_iters())
+    hs.close()
+    hf.close()
+    m.close()
