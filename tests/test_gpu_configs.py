@@ -206,3 +206,47 @@ def test_c5_combined_run_to_freeze_out():
     assert all(c[0] > 0 for c in r['surface_counts'])
     assert np.median(np.abs(r['balance'])) < 1e-6 and np.abs(r['balance']).max() < 1e-3
     assert r['check'] is not None and all(r['check'][f] for f in ('ev1', 'pi1', 'bulk1', 'udiff')), r['check']
+
+
+def test_c2_full_grid_seed_in_stage_equals_split():
+    """The learned form on the benchmark grid itself (385x385x115, default 10x23 tiles: partial
+    tiles along y, 26 idle lanes per workgroup): 5 plain steps, then 2 seeded steps with the
+    network inside the stage kernel and, on a second handle, in split form, with and without
+    the FP64 root for fast cells.  Seeds, iteration counts and fields are bit-equal."""
+    from oracle import mlp_oracle as MO
+    p, tab, ev = bench.make_workload(True)
+    for fp64 in (False, True):
+        q = _lib.CvParams.from_buffer_copy(p)
+        if fp64:
+            q.flags |= _lib.CV_FLAG_FP64_ROOT
+            q.fp64_root_vmin = 0.02
+        _say('case: 385x385x115 seeded in the stage vs split, 5 + 2 steps, fp64 %s' % fp64)
+        rng = np.random.default_rng(25)
+        m = _lib.Mlp()
+        m.set_params((rng.standard_normal(MO.NPARAM) * 0.3).astype(np.float32))
+        hs, hf = [], []
+        for lst in (hs, hf):
+            h = _lib.Handle(q, tab, viscous=True)
+            h.visc_initialize(bench.TAU0)
+            h.ideal_stage(1, np.float32(bench.TAU0))
+            h.set('ev1', ev)
+            h.visc_steps(0, 5)
+            h.set_root_seed(m, track_iters=True)
+            lst.append(h)
+        hs, hf = hs[0], hf[0]
+        hf.set_root_split(False)
+        hs.visc_steps(5, 2)
+        hf.visc_steps(5, 2)
+        for name in ('root_seed', 'ev1', 'pi1', 'bulk1', 'udiff'):
+            a, b = hf.get(name), hs.get(name)
+            _say('  %-9s bit-equal %.8f' % (name, bit_equal_frac(a, b)))
+            np.testing.assert_array_equal(a, b, err_msg=name)
+        it = hf.root_iters()
+        np.testing.assert_array_equal(it, hs.root_iters())
+        nfix = int((it == 0).sum())
+        _say('  iterations mean %.3f (zero: %d deferred FP64 cells)' % (it.mean(), nfix))
+        if fp64:
+            assert nfix > 1000
+        hs.close()
+        hf.close()
+        m.close()
