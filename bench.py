@@ -40,18 +40,27 @@ LAUNCH_NAMES = ['visc_stage1', 'visc_stage2']
 PROFILE = os.path.join(HERE, 'profiles', 'k_visc_tiled_visc_bulk_385x385x115.md')
 
 
-def profiled_traffic(kernel):
-    """HBM-side bytes per dispatch of a stage kernel from the committed rocprofv3 counter
-    summary (FETCH_SIZE x 2.00 + WRITE_SIZE, calibrated on a device copy), or None."""
+def profiled_value(key, kernel):
+    """A per-dispatch figure of a stage kernel from the committed rocprofv3 counter summary
+    (lines 'key kernel value'), or None."""
     try:
         with open(PROFILE) as f:
             for line in f:
                 w = line.split()
-                if len(w) == 3 and w[0] == 'traffic_bytes' and w[1] == kernel:
+                if len(w) == 3 and w[0] == key and w[1] == kernel:
                     return float(w[2])
     except OSError:
         pass
     return None
+
+
+def profiled_traffic(kernel):
+    """HBM-side bytes per dispatch of a stage kernel (FETCH_SIZE x 2.00 + WRITE_SIZE,
+    calibrated on a device copy), or None."""
+    return profiled_value('traffic_bytes', kernel)
+
+
+VALU_ISSUE_PER_S = 1024 * 2.4e9 / 4     # wave64 VALU instructions per second: 1024 SIMDs, 4 cycles each
 
 
 def log(*a):
@@ -981,6 +990,11 @@ def main():
             'traffic': profiled_traffic(LAUNCH_NAMES[dom]) if bulk else None,
             'step_algorithmic_bytes_per_cell': sum(abytes),
             'step_frac': round(sum(abytes) * value / 1e9 / HBM_PEAK_GBS, 5)}
+    # what bounds the kernel: its VALU instructions per dispatch (SQ_INSTS_VALU of the
+    # committed profile) issued in this run's dispatch time, against the VALU issue rate
+    valu = profiled_value('valu_insts', LAUNCH_NAMES[dom]) if bulk else None
+    if valu:
+        roof['valu_issue_frac'] = round(valu / VALU_ISSUE_PER_S / (prof[dom] * 1e-3), 4)
     learned = None
     if not args.no_learned:
         learned = run_learned(args, p, tab, ev, h, loop)
