@@ -134,17 +134,20 @@ def test_batch_refuses_a_jet_source():
     batch.close()
 
 
-@pytest.mark.parametrize('fp64', [False, True])
-def test_seeded_batch_matches_seeded_single_events(fp64):
-    """cv_batch_set_root_seed: the batch's split stages, seed network and root solve over all
-    events in one graph give each event's fields exactly as its own seeded handle does (FP64
-    root for fast cells when CV_FLAG_FP64_ROOT is set); deactivating an event keeps it so."""
-    tab = Eos('crossover').create_table()
+@pytest.mark.parametrize('fp64,eos_name', [(False, 'crossover'), (True, 'crossover'), (False, 'ideal_gas'),
+                                            (True, 'ideal_gas')])
+def test_seeded_batch_matches_seeded_single_events(fp64, eos_name):
+    """cv_batch_set_root_seed gives each event's fields exactly as its own seeded handle does
+    (FP64 root for fast cells when CV_FLAG_FP64_ROOT is set): with the octave table (crossover)
+    the network runs inside the persistent stage launch, with the row/col table (ideal gas)
+    the stages run split, then the seed network and the root solve over all events, in one
+    graph.  Deactivating an event keeps it so."""
+    tab = Eos(eos_name).create_table()
     flags = _lib.CV_FLAG_FP64_ROOT if fp64 else 0
     p = cv_params(1, 1, 1, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08,
                   bulk=(0.05, 0.155, 0.02), flags=flags, fp64_root_vmin=0.3)
     dims = DIMS[:4]
-    _say('case: seeded batch of %d events, FP64 root %s' % (len(dims), fp64))
+    _say('case: seeded batch of %d events, FP64 root %s, %s EOS' % (len(dims), fp64, eos_name))
     rng = np.random.default_rng(31)
     mlp = _lib.Mlp()
     mlp.set_params((rng.standard_normal(mlp.nparam) * 0.3).astype(np.float32))
