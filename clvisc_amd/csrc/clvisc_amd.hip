@@ -298,10 +298,16 @@ extern "C" int cv_create(const cv_params *p, const cv_eos *eos, int device, int 
     ph.flags = p->flags;
     // 32-bit plane offsets of the tiled kernel: 10 planes of 4 n bytes below 2^32
     ph.nb = (40 * n < ((int64_t)1 << 32)) ? (uint32_t)(4 * n) : 0u;
-    // the max(lambda, 1) shortcut needs every cs2 the lookup can return <= 0.5
-    float cs2max = -1e30f;
-    for (int64_t i = 0; i < eos->n_entries; i++) cs2max = eos->rgba[4 * i] > cs2max ? eos->rgba[4 * i] : cs2max;
-    ph.lam_fast = cs2max <= 0.5f ? 1 : 0;
+    // the max(lambda, 1) shortcut needs every cs2 the lookup can return in [0, 0.5]: the
+    // table's nodes (a lookup interpolates between two of them or scales none) and the
+    // 1/3 of the fit above a row/col table
+    float cs2max = -1e30f, cs2min = 1e30f;
+    for (int64_t i = 0; i < eos->n_entries; i++) {
+        const float c = eos->rgba[4 * i];
+        cs2max = c > cs2max ? c : cs2max;
+        cs2min = c < cs2min ? c : cs2min;
+    }
+    ph.lam_fast = (cs2max <= 0.5f && cs2min >= 0.0f) ? 1 : 0;
     EosP &E = h->eos;
     memset(&E, 0, sizeof(E));
     E.kind = eos->kind;

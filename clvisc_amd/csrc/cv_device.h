@@ -43,7 +43,7 @@ struct PhysP {
     float zmax, ztpeak, zwidth, r_zwidth;   // r_zwidth = RN(1 / zwidth), from the host
     float fp64_vmin;
     int flags;
-    int lam_fast;           // 1 when the table's cs2 <= 0.5 (max(lambda,1) shortcut valid)
+    int lam_fast;           // 1 when the table's cs2 lie in [0, 0.5] (max(lambda,1) shortcut valid)
     uint32_t nb;            // bytes of one field plane (4 n) when 10 planes fit 32-bit offsets, else 0
 };
 
@@ -267,12 +267,18 @@ __device__ __forceinline__ float4 t0m(float4 ev, float pr) {
                        a * (ev.z * u0) - 0.0f * pr, a * (ev.w * u0) - 0.0f * pr);
 }
 
-// maxPropagationSpeed (helper.h:94-105) = max(lambda_KT, 1).  When the table's cs2 is
-// at most 0.5 and v^2 <= 0.81, lambda_KT <= (|v| + cs)/(1 + |v| cs) < 0.983, far below
-// 1 minus its FP32 rounding error, so the maximum is exactly 1.0f.
+// maxPropagationSpeed (helper.h:94-105) = max(lambda_KT, 1).  Take X = ut^2 >= 1 as an
+// input of its own (gamma3 never returns less than 1, whatever v is), a = 1 - cs2, c = cs:
+//   1 - lambda_KT = X a (1 - |vk|)^2 / (X a (1 - |vk|) + cs2 + c sqrt(X (1 - vk^2) a + cs2)),
+// which is >= 0 for every X >= 0, |vk| <= 1 and 0 <= cs2 <= 1.  With 0 <= cs2 <= 0.5 and
+// vk^2 <= 0.98 it is at least 5.0e-5 (839 FP32 ulps of 1, the minimum at X = 1,
+// cs2 = 0.5), while the FP32 evaluation of the formula (every operation rounded once) is
+// off by a few ulps whatever X is (at most 6.3 ulps over 1.2e8 samples with X up to 1e7).
+// So the maximum is exactly 1.0f there, and only the face velocity component is tested.
+// lam_fast: every cs2 the lookup can return lies in [0, 0.5] (checked on the table).
 template <class EOS>
 __device__ __forceinline__ float max_prop_speed(const PhysP &p, const EOS &eos, float4 edv, float vk) {
-    if (p.lam_fast && (edv.y * edv.y + edv.z * edv.z + edv.w * edv.w) <= 0.81f) return 1.0f;
+    if (p.lam_fast && vk * vk <= 0.98f) return 1.0f;
     float ut = gamma3(edv.y, edv.z, edv.w);
     float uk = ut * vk;
     float ut2 = ut * ut;

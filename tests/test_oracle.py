@@ -10,6 +10,8 @@ Reference known-answer tests restated here:
   1st-order viscous Bjorken   pyvisc/tests/test_bjorken_visc.py:74-80
   2nd-order viscous Gubser    pyvisc/tests/test_gubser_visc.py:11-27 (GUBSER_VISC_TEST)
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -177,6 +179,86 @@ def test_max_prop_speed_is_one_for_subluminal(ideal_tab):
     ev = ev[(ev[:, 1:] ** 2).sum(1) < 0.999]
     lam = O.max_prop_speed(params(ideal_tab), ev, ev[:, 1])
     np.testing.assert_array_equal(lam, 1.0)
+
+
+@pytest.mark.parametrize('cs2', ['crossover', 0.5, 0.0])
+def test_max_prop_speed_is_one_below_the_face_velocity_bound(cs2, cross):
+    """The device returns max(lambda, 1) = 1.0f without evaluating lambda when the face
+    component vk^2 <= 0.98 and the table's cs2 lie in [0, 0.5]: lambda <= 1 - 5e-5 in exact
+    arithmetic there, whatever the other components (gamma is clamped at 1 - v^2 = 1e-7).
+    The reference's FP32 formula (the oracle's max_prop_speed) gives exactly 1.0 on such
+    states: near the bound, at |v| -> 1 and beyond (the clamp), at NaN transverse
+    components, over the table's whole energy range and beyond it, and on tables whose cs2 is
+    0.5 or 0 at every node.  Beyond the bound (vk^2 -> 1) the formula does exceed 1."""
+    _, tab = cross
+    if cs2 != 'crossover':
+        rgba = tab.rgba.copy()
+        rgba[:, 0] = cs2
+        tab = dataclasses.replace(tab, rgba=rgba, cs2_max=float(cs2), cs2_min=float(cs2))
+    p = params(tab)
+    rng = np.random.default_rng(17)
+    n = 400_000
+    vk = np.sqrt(0.98) * (1.0 - rng.uniform(0, 1, n) ** 3 * 0.5) * rng.choice([-1.0, 1.0], n)
+    vk = vk.astype(np.float32)
+    vk = vk[vk * vk <= np.float32(0.98)]
+    n = vk.size
+    # transverse part: from none up to |v| = 1 - 1e-9, |v| > 1 (gamma clamped), NaN
+    room = np.sqrt(np.maximum(1.0 - vk.astype(np.float64) ** 2, 0.0))
+    frac = np.where(rng.uniform(0, 1, n) < 0.5, 1.0 - 10.0 ** rng.uniform(-9, 0, n), rng.uniform(0, 1.5, n))
+    phi = rng.uniform(0, 2 * np.pi, n)
+    vt1 = (room * frac * np.cos(phi)).astype(np.float32)
+    vt2 = (room * frac * np.sin(phi)).astype(np.float32)
+    vt1[rng.uniform(0, 1, n) < 0.01] = np.nan
+    e = (10.0 ** rng.uniform(-9, 4, n)).astype(np.float32)
+    ev = np.zeros((n, 4), np.float32)
+    ev[:, 0] = e
+    for axis in range(3):
+        ev[:, 1 + axis] = vk
+        ev[:, 1 + (axis + 1) % 3] = vt1
+        ev[:, 1 + (axis + 2) % 3] = vt2
+        lam = O.max_prop_speed(p, ev, vk)
+        np.testing.assert_array_equal(lam, np.float32(1.0))
+    if cs2 != 'crossover':
+        return
+    # the bound is not vacuous: along the face direction close to the light cone the FP32
+    # formula leaves 1 on the crossover table (the maximum is not 1.0f there)
+    vk_hi = np.float32(1.0) - rng.uniform(0, 3e-7, 20000).astype(np.float32)
+    ev_hi = np.zeros((vk_hi.size, 4), np.float32)
+    ev_hi[:, 0] = (10.0 ** rng.uniform(-3, 2, vk_hi.size)).astype(np.float32)
+    ev_hi[:, 1] = vk_hi
+    assert (O.max_prop_speed(p, ev_hi, vk_hi) > 1.0).any()
+
+
+def test_max_prop_speed_margin_exceeds_fp32_error():
+    """The margin behind the device's lambda shortcut, with u^tau as a free input X >= 1:
+    1 - lambda_KT in exact arithmetic stays >= 5.0e-5 for vk^2 <= 0.98 and cs2 in [0, 0.5],
+    while helper.h:94-105's FP32 operation sequence (every operation rounded once, as the
+    oracle and the device evaluate it) stays within a few ulps of the exact value for X from
+    1 to 1e7, i.e. far below the margin.  The closed form of the margin is checked against
+    the direct evaluation in float64."""
+    f = np.float32
+    rng = np.random.default_rng(23)
+    n = 2_000_000
+    ut = np.where(rng.uniform(0, 1, n) < 0.5, np.exp(rng.uniform(0, np.log(np.sqrt(1e7)), n)),
+                  1.0 + rng.uniform(0, 1e-2, n)).astype(f)
+    vk = (np.sqrt(0.98) * (1 - rng.uniform(0, 1, n) ** 4 * 0.3) * rng.choice([-1, 1], n)).astype(f)
+    cs2 = (0.5 * (1 - rng.uniform(0, 1, n) ** 3)).astype(f)
+    keep = vk * vk <= f(0.98)
+    ut, vk, cs2 = ut[keep], vk[keep], cs2[keep]
+    with np.errstate(all='ignore'):
+        uk = ut * vk; ut2 = ut * ut; uk2 = uk * uk
+        d = ut2 - uk2
+        lam32 = (np.abs(ut * uk * (f(1) - cs2)) + np.sqrt((d - (d - f(1)) * cs2) * cs2)) \
+            * (f(1) / (ut2 - (ut2 - f(1)) * cs2))
+    U, V, C2 = ut.astype(np.float64), vk.astype(np.float64), cs2.astype(np.float64)
+    X, a, c = U * U, 1 - C2, np.sqrt(C2)
+    lam = (X * np.abs(V) * a + c * np.sqrt(X * (1 - V * V) * a + C2)) / (X * a + C2)
+    margin = X * a * (1 - np.abs(V)) ** 2 / (X * a * (1 - np.abs(V)) + C2 + c * np.sqrt(X * (1 - V * V) * a + C2))
+    np.testing.assert_allclose(1 - lam, margin, rtol=1e-6, atol=1e-12)
+    assert margin.min() >= 5.0e-5
+    err_ulps = np.abs(lam32.astype(np.float64) - lam) / 2.0 ** -24
+    assert err_ulps.max() < 16
+    assert lam32.max() < 1.0
 
 
 # --- reduction -----------------------------------------------------------------------
