@@ -100,6 +100,69 @@ def test_ideal_steps_tiled_3d(eos_kind, dims, kernel, chunk):
     h.close()
 
 
+@pytest.mark.parametrize('viscous,kernel', [(False, (0, 0, 0)), (False, (2, 0, 0)), (True, (0, 0, 0)),
+                                            (True, (1, 0, 0)), (True, (2, 0, 0))])
+def test_steps_with_the_full_lambda_formula(viscous, kernel):
+    """A crossover table whose cs2 column is scaled past 0.5 (1.6x, only lambda reads it)
+    turns off the device's max(lambda_KT, 1) = 1 shortcut, so every face evaluates the
+    reference's formula (gamma, cs2 lookup, square root, division), as the oracle always does:
+    identical fields."""
+    import dataclasses
+    tab0 = Eos('crossover').create_table()
+    rgba = tab0.rgba.copy()
+    rgba[:, 0] *= np.float32(1.6)
+    tab = dataclasses.replace(tab0, rgba=rgba, cs2_max=float(rgba[:, 0].max()), cs2_min=float(rgba[:, 0].min()))
+    assert rgba[:, 0].max() > 0.5
+    nx, ny, nz = 29, 31, 11
+    p = cv_params(nx, ny, nz, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08,
+                  bulk=(0.05, 0.155, 0.02) if viscous else None)
+    pos = ic.woods_saxon_positions(12, 3, R=0.6, a=0.1)
+    ev = ic.hotspots(nx, ny, nz, 0.08, 0.08, 0.15, positions=pos, emax=50.0)
+    if viscous:
+        _visc_compare(p, tab, ev, 20, 'viscous + bulk, full lambda formula', kernel)
+        return
+    _say('case: ideal hotspots %dx%dx%d, full lambda formula, 25 steps, kernel %s' % (nx, ny, nz, kernel))
+    h = _lib.Handle(p, tab, viscous=False)
+    h.set_stage_kernel(*kernel)
+    h.set('ev1', ev)
+    h.ideal_steps(0, 25)
+    o = O.Oracle(orc_params(p, tab, nthreads=16), 'f32', viscous=False)
+    o.load_ini(ev)
+    o.ideal_steps(25)
+    for name, g, r in (('ev1', h.get('ev1'), o.ev[1]), ('ev2', h.get('ev2'), o.ev[2])):
+        _say('  %-4s bit-equal %.8f' % (name, bit_equal_frac(g, r)))
+        np.testing.assert_array_equal(g, r, err_msg=name)
+    h.close()
+
+
+@pytest.mark.parametrize('kernel', [(0, 0, 0), (2, 0, 0)])
+def test_fast_periphery_matches_oracle(kernel):
+    """A small hot blob expanding into a wide vacuum margin for 100 steps (viscous + bulk):
+    its periphery moves at |v| > 0.9, where the device's max(lambda_KT, 1) = 1 shortcut
+    applies on the face component alone (0.81 < vk^2 <= 0.98, outside the |v|^2 <= 0.81 of
+    the earlier criterion).  Identical fields to the oracle, which evaluates the formula at
+    every face.  (Faces with vk^2 > 0.98 are rare in such flows; the scaled-cs2 case above
+    runs the formula at every face.)"""
+    nx, ny, nz = 101, 101, 9
+    p = cv_params(nx, ny, nz, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08,
+                  bulk=(0.05, 0.155, 0.02))
+    tab = Eos('crossover').create_table()
+    pos = ic.woods_saxon_positions(5, 7, R=0.3, a=0.05)
+    ev = ic.hotspots(nx, ny, nz, 0.08, 0.08, 0.15, positions=pos, emax=50.0)
+    h = _lib.Handle(p, tab, viscous=True)
+    h.set_stage_kernel(*kernel)
+    h.visc_initialize(p.tau0)
+    h.ideal_stage(1, np.float32(p.tau0))
+    h.set('ev1', ev)
+    h.visc_steps(0, 100)
+    vk2 = h.get('ev1')[:, 1:] ** 2
+    h.close()
+    mid = ((vk2 > 0.81) & (vk2 <= 0.98)).any(1).mean()
+    _say('case: fast periphery, cells with a component 0.81 < vk^2 <= 0.98 after 100 steps: %.4f' % mid)
+    assert mid > 0.05
+    _visc_compare(p, tab, ev, 100, 'fast periphery', kernel)
+
+
 @pytest.mark.parametrize('dims', [(33, 1, 17), (1, 29, 13), (1, 1, 41), (27, 25, 2), (2, 3, 9)])
 @pytest.mark.parametrize('viscous', [True, False])
 def test_steps_thin_grids(dims, viscous):
