@@ -83,6 +83,7 @@ struct cv_handle {
     int tile_grid = 0;
     int march_xc = 0;         // x planes per march chunk (0: automatic)
     int xcd = 8;              // XCD-contiguous work mapping of the tiled kernels (MI355X: 8 XCDs; cv_set_xcd_map)
+    bool slab_chunks = true;  // with it, march chunks per XCD slab (set_tile)
     // learned root-solve seed of the ideal stage (cv_set_root_seed)
     struct cv_mlp *seed_mlp = nullptr;
     bool track_iters = false;
@@ -138,8 +139,18 @@ static int set_tile(cv_handle *h, int ty, int tz) {
     t.xc = (int)((p.nx + nch - 1) / nch);
     if (h->march_xc > 0) t.xc = h->march_xc < p.nx ? h->march_xc : p.nx;
     t.nchunks = (p.nx + t.xc - 1) / t.xc;
-    t.nwork = cols * t.nchunks;
     t.xcd = h->xcd;
+    // with the XCD-contiguous mapping: one x slab per XCD, each cut into chunks of xc planes
+    // with its short remainder last, so that the last work items an XCD runs are short ones
+    // (the dispatch drains sooner); an XCD's range is then exactly its slab
+    t.slabs = 0; t.cps = 0;
+    if (h->slab_chunks && t.xcd > 1 && p.nx >= 2 * t.xcd) {
+        const int maxl = (p.nx + t.xcd - 1) / t.xcd;
+        t.slabs = t.xcd;
+        t.cps = (maxl + t.xc - 1) / t.xc;
+        t.nchunks = t.slabs * t.cps;
+    }
+    t.nwork = cols * t.nchunks;
     h->tile = t;
     int64_t g = (int64_t)(h->cus > 0 ? h->cus : 256) * 2 * 4;
     h->tile_grid = (int)(t.ntiles < g ? t.ntiles : g);
@@ -865,8 +876,7 @@ extern "C" int cv_set_xcd_map(cv_handle *h, int nxcd) {
     HIP_TRY(hipSetDevice(h->device));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->xcd = nxcd;
-    h->tile.xcd = nxcd;
-    return CV_OK;
+    return set_tile(h, h->tile.ty, h->tile.tz);
 }
 
 extern "C" int cv_get_march_chunk(cv_handle *h, int32_t *xc, int32_t *nchunks) {
@@ -1580,11 +1590,11 @@ static int batch_build(cv_batch *b) {
         BatchEvent &t = tab[q];
         t.p = h->phys;
         t.tp = h->tile;
-        if (h->march_xc == 0) {
-            t.tp.xc = std::min(xcb, h->prm.nx);
-            t.tp.nchunks = (h->prm.nx + t.tp.xc - 1) / t.tp.xc;
-            t.tp.nwork = (int64_t)t.tp.ntj * t.tp.ntk * t.tp.nchunks;
-        }
+        // the batch has its own work ranges over the whole list: plain chunks along x
+        t.tp.slabs = 0; t.tp.cps = 0;
+        if (h->march_xc == 0) t.tp.xc = std::min(xcb, h->prm.nx);
+        t.tp.nchunks = (h->prm.nx + t.tp.xc - 1) / t.tp.xc;
+        t.tp.nwork = (int64_t)t.tp.ntj * t.tp.ntk * t.tp.nchunks;
         t.work0 = w;
         for (int k = 0; k < 3; k++) { t.ev[k] = h->ev[k]; t.pi[k] = h->pi[k]; t.bk[k] = h->bk[k]; }
         t.Src = h->Src; t.IS_src = h->IS_src; t.bsrc = h->bsrc; t.udiff = h->udiff; t.good = h->good;

@@ -93,8 +93,9 @@ __global__ __launch_bounds__(CV_ITILE_THREADS) void k_ideal_tiled(PhysP p, EosP 
     const int kt = (int)(work % tp.ntk);
     const int64_t r = work / tp.ntk;
     const int jt = (int)(r % tp.ntj);
-    const int ibeg = (int)(r / tp.ntj) * tp.xc;
-    const int iend = ibeg + tp.xc < p.nx ? ibeg + tp.xc : p.nx;
+    int ibeg, iend;
+    march_planes(tp, p.nx, (int)(r / tp.ntj), ibeg, iend);
+    if (ibeg >= iend) return;                          // an empty last chunk of a slab (uniform)
     const int j0 = jt * ty, k0 = kt * tz;
     const int j = j0 + my, k = k0 + mz;
     const bool active = lane_in_tile && j < p.ny && k < p.nz;

@@ -48,7 +48,23 @@ struct TileP {
     int xc, nchunks;           // march: x planes per chunk, chunks along x
     int64_t nwork;             // march: ntj * ntk * nchunks
     int xcd;                   // > 0: spread work items over this many XCDs in contiguous ranges
+    int slabs, cps;            // march, slabs > 0: x cut into `slabs` slabs of cps chunks each
 };
+
+// x planes [ibeg, iend) of march chunk xr.  With slabs, slab s holds the planes
+// [s nx / slabs, (s + 1) nx / slabs) in cps chunks of xc planes, the last one shorter (or
+// empty): one slab is the work range of one XCD, so the last items each XCD runs are short.
+__device__ __forceinline__ void march_planes(const TileP &tp, int nx, int xr, int &ibeg, int &iend) {
+    if (tp.slabs > 0) {
+        const int s = xr / tp.cps, c = xr - s * tp.cps;
+        const int x1 = (int)(((int64_t)(s + 1) * nx) / tp.slabs);
+        ibeg = (int)(((int64_t)s * nx) / tp.slabs) + c * tp.xc;
+        iend = ibeg + tp.xc < x1 ? ibeg + tp.xc : x1;
+    } else {
+        ibeg = xr * tp.xc;
+        iend = ibeg + tp.xc < nx ? ibeg + tp.xc : nx;
+    }
+}
 
 // Work item of workgroup b when the grid has one workgroup per item: the hardware hands
 // workgroups to the XCDs round-robin (b % nx), so XCD x gets the contiguous range of items
@@ -222,8 +238,9 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
         const int64_t r = work / tp.ntk;
         const int jt = (int)(r % tp.ntj);
         const int xr = (int)(r / tp.ntj);            // MARCH: chunk, else the x-plane
-        const int ibeg = MARCH ? xr * tp.xc : xr;
-        const int iend = MARCH ? (ibeg + tp.xc < p.nx ? ibeg + tp.xc : p.nx) : xr + 1;
+        int ibeg = xr, iend = xr + 1;
+        if (MARCH) march_planes(tp, p.nx, xr, ibeg, iend);
+        if (ibeg >= iend) return;                    // an empty last chunk of a slab (uniform)
         const int j0 = jt * ty, k0 = kt * tz;
         // state carried along x by MARCH: F(i-1/2), slopes of cell i, u^mu of cell i-1, and
         // u^tau, P + Pi of cell i+1 (evaluated as cell i+2 of the previous plane)

@@ -74,7 +74,8 @@ def test_ideal_steps_gubser_201():
 @pytest.mark.parametrize('eos_kind,dims,kernel,chunk', [
     ('crossover', (29, 31, 11), (2, 0, 0), None), ('crossover', (29, 31, 11), (2, 7, 30), 3),
     ('crossover', (29, 31, 11), (2, 11, 20), 5), ('ideal_gas', (27, 25, 9), (2, 0, 0), None),
-    ('ideal_gas', (27, 25, 9), (0, 0, 0), None), ('crossover', (45, 41, 1), (2, 64, 1), 4)])
+    ('ideal_gas', (27, 25, 9), (0, 0, 0), None), ('crossover', (45, 41, 1), (2, 64, 1), 4),
+    ('crossover', (29, 31, 11), (2, 0, 0), 1)])
 def test_ideal_steps_tiled_3d(eos_kind, dims, kernel, chunk):
     """The ideal stage on x-marching tiles (kind 2; kind 0 is the per-cell kernel) against
     the oracle's ideal steps: identical fields."""
@@ -398,10 +399,13 @@ def test_visc_steps_2d_boost_invariant(kernel, chunk):
 
 
 @pytest.mark.parametrize('kernel,chunk,xcd', [((1, 0, 0), None, 0), ((1, 8, 27), None, 3),
-                                              ((2, 0, 0), None, 0), ((2, 7, 30), 3, 3), ((2, 11, 20), 5, 16)])
+                                              ((2, 0, 0), None, 0), ((2, 7, 30), 3, 3), ((2, 11, 20), 5, 16),
+                                              ((2, 0, 0), 1, 8), ((2, 0, 0), 4, 8), ((2, 0, 0), 2, 5)])
 def test_visc_steps_xcd_map(kernel, chunk, xcd):
     """The tiled kernels' workgroup -> work-item mapping (off, and XCD counts that do not
-    divide the work) changes only the schedule: the oracle's fields bit for bit."""
+    divide the work) changes only the schedule: the oracle's fields bit for bit.  With the
+    map on and nx >= 2 nxcd the march chunks are cut per XCD slab of x planes (29 planes:
+    slabs of 3 and 4, or 5 and 6): one chunk per slab, and slabs whose last chunk is empty."""
     p, tab, ev = _visc_case(29, 31, 11, bulk=(0.05, 0.155, 0.02))
     _visc_compare(p, tab, ev, 20, 'viscous+bulk hotspots', kernel, chunk, xcd)
 
