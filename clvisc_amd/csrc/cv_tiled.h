@@ -48,18 +48,27 @@ struct TileP {
     int xc, nchunks;           // march: x planes per chunk, chunks along x
     int64_t nwork;             // march: ntj * ntk * nchunks
     int xcd;                   // > 0: spread work items over this many XCDs in contiguous ranges
-    int slabs, cps;            // march, slabs > 0: x cut into `slabs` slabs of cps chunks each
+    int slabs, cps, xt;        // march, slabs > 0: x cut into `slabs` slabs of cps chunks each,
+                               // the last xt planes of a slab its last chunk (xt 0: remainder)
 };
 
 // x planes [ibeg, iend) of march chunk xr.  With slabs, slab s holds the planes
-// [s nx / slabs, (s + 1) nx / slabs) in cps chunks of xc planes, the last one shorter (or
-// empty): one slab is the work range of one XCD, so the last items each XCD runs are short.
+// [s nx / slabs, (s + 1) nx / slabs); its last chunk is its last xt planes and the chunks
+// before it hold xc planes each (the last of them shorter, or empty), or, with xt = 0,
+// its chunks hold xc planes with the remainder last.  One slab is the work range of one
+// XCD, so the last items each XCD runs are short ones.
 __device__ __forceinline__ void march_planes(const TileP &tp, int nx, int xr, int &ibeg, int &iend) {
     if (tp.slabs > 0) {
         const int s = xr / tp.cps, c = xr - s * tp.cps;
+        const int x0 = (int)(((int64_t)s * nx) / tp.slabs);
         const int x1 = (int)(((int64_t)(s + 1) * nx) / tp.slabs);
-        ibeg = (int)(((int64_t)s * nx) / tp.slabs) + c * tp.xc;
-        iend = ibeg + tp.xc < x1 ? ibeg + tp.xc : x1;
+        const int xm = x1 - tp.xt;                    // end of the main chunks
+        if (tp.xt > 0 && c == tp.cps - 1) {
+            ibeg = xm; iend = x1;
+        } else {
+            ibeg = x0 + c * tp.xc;
+            iend = ibeg + tp.xc < xm ? ibeg + tp.xc : xm;
+        }
     } else {
         ibeg = xr * tp.xc;
         iend = ibeg + tp.xc < nx ? ibeg + tp.xc : nx;

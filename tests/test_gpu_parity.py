@@ -410,6 +410,35 @@ def test_visc_steps_xcd_map(kernel, chunk, xcd):
     _visc_compare(p, tab, ev, 20, 'viscous+bulk hotspots', kernel, chunk, xcd)
 
 
+@pytest.mark.parametrize('viscous', [False, True])
+def test_automatic_slab_chunks_with_a_tail(viscous):
+    """Automatic march chunks on a grid with many tile columns (192x101x151, 10x23 tiles: 77
+    columns): each XCD's slab of 24 planes is cut into chunks of 3 planes, a shorter one and a
+    2-plane tail chunk last.  Identical fields to the oracle, ideal and viscous + bulk."""
+    nx, ny, nz = 192, 101, 151
+    p = cv_params(nx, ny, nz, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08,
+                  bulk=(0.05, 0.155, 0.02) if viscous else None)
+    tab = Eos('crossover').create_table()
+    pos = ic.woods_saxon_positions(40, 11, R=4.0, a=0.5)
+    ev = ic.hotspots(nx, ny, nz, 0.08, 0.08, 0.15, positions=pos, emax=50.0)
+    if viscous:
+        _visc_compare(p, tab, ev, 4, 'viscous + bulk, automatic slab chunks', (2, 10, 23))
+        return
+    h = _lib.Handle(p, tab, viscous=False)
+    h.set_stage_kernel(2, 10, 23)
+    _say('case: ideal %dx%dx%d, automatic slab chunks %s, 8 steps' % (nx, ny, nz, h.march_chunk()))
+    assert h.march_chunk() == (3, 72)
+    h.set('ev1', ev)
+    h.ideal_steps(0, 8)
+    o = O.Oracle(orc_params(p, tab, nthreads=16), 'f32', viscous=False)
+    o.load_ini(ev)
+    o.ideal_steps(8)
+    for name, g, r in (('ev1', h.get('ev1'), o.ev[1]), ('ev2', h.get('ev2'), o.ev[2])):
+        _say('  %-4s bit-equal %.8f' % (name, bit_equal_frac(g, r)))
+        np.testing.assert_array_equal(g, r, err_msg=name)
+    h.close()
+
+
 @pytest.mark.parametrize('kernel', [(0, 0, 0), (2, 0, 0)])
 def test_visc_steps_fp64_root(kernel):
     """CV_FLAG_FP64_ROOT with a low threshold (M > 0.3 T^tt) so that many cells take the
