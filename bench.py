@@ -973,6 +973,7 @@ def main():
     ms_per_step = 1000.0 * wall / args.steps
     value = ncell * args.steps / wall
     edmax = h.max_ed(1)
+    kind, chunk = h.stage_kernel(), h.march_chunk()
     # the hotspot field only decays (peak 50 GeV/fm^3 at tau0): a larger or non-finite max
     # means the evolution went unstable and the timing is not of the physical workload
     ok = bool(np.isfinite(edmax) and 0.0 < edmax <= 50.0)
@@ -1028,7 +1029,9 @@ def main():
                                'dtau=0.02 fm, 400 Woods-Saxon hotspots, crossover EOS (octave table)'
                                % (' + bulk (zeta/s peak 0.05 at 0.155 GeV)' if bulk else ''),
                    'grid': [NX, NY, NZ], 'device': name, 'physical': ok,
-                   'edmax_end': float(edmax)},
+                   'edmax_end': float(edmax),
+                   'stage_kernel': {'kind': kind[0], 'tile': [kind[1], kind[2]],
+                                    'march_chunk': chunk[0], 'chunks': chunk[1]}},
         'roofline': roof,
         'learned': learned,
         'cpu_baseline': cpu,

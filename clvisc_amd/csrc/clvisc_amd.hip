@@ -139,34 +139,31 @@ static int set_tile(cv_handle *h, int ty, int tz) {
     if (h->march_xc > 0) t.xc = h->march_xc < p.nx ? h->march_xc : p.nx;
     t.nchunks = (p.nx + t.xc - 1) / t.xc;
     t.xcd = h->xcd;
-    // with the XCD-contiguous mapping: one x slab per XCD (an XCD's range is then exactly
-    // its slab), whose last chunk is a short one, so that the last work items an XCD runs
-    // are short and the dispatch drains sooner.  Automatic chunks: about 24 work items per
-    // CU, i.e. `rows` chunks per slab, the last one xt planes (a twelfth of the slab, 1 to
-    // 4) and the others sharing the rest (385 planes, 195 tile columns: 15, 15, 15, 4; at
-    // 9-plane chunks the step took 1.2 % longer, with a full-length last chunk 2 % longer);
-    // a chunk length set by cv_set_march_chunk keeps its length, the remainder last.
+    // automatic chunks with the XCD-contiguous mapping: one x slab per XCD (an XCD's range
+    // is then exactly its slab) whose last chunk is short, so that the last work items an
+    // XCD runs are short and the dispatch drains sooner: about 24 work items per CU, i.e.
+    // `rows` chunks per slab, the last one xt planes (a twelfth of the slab, 1 to 4) and the
+    // others sharing the rest (385 planes, 195 tile columns: 15, 15, 15, 4; at 9-plane
+    // chunks the step took 1.2 % longer, with a full-length last chunk 2 % longer).  Where
+    // the main chunks would be shorter than twice the tail (few tile columns: the items are
+    // short anyway), and for a chunk length set by cv_set_march_chunk, the chunks run along
+    // the whole x axis, as before.
     t.slabs = 0; t.cps = 0; t.xt = 0;
-    if (t.xcd > 1 && p.nx >= 2 * t.xcd) {
+    if (h->march_xc == 0 && t.xcd > 1 && p.nx >= 2 * t.xcd) {
         const int maxl = (p.nx + t.xcd - 1) / t.xcd;
-        t.slabs = t.xcd;
-        if (h->march_xc > 0) {
-            t.cps = (maxl + t.xc - 1) / t.xc;
-        } else {
-            const int xt = std::min(4, std::max(1, maxl / 12));
-            const int64_t items = (int64_t)(h->cus > 0 ? h->cus : 256) * 24;
-            int64_t rows = (items / std::max<int64_t>(1, cols) + t.slabs / 2) / t.slabs;
-            rows = std::max<int64_t>(2, std::min<int64_t>(rows, maxl - xt + 1));
-            const int xcm = (maxl - xt + (int)rows - 2) / ((int)rows - 1);
-            if (xcm > xt) {
-                t.xt = xt;
-                t.xc = xcm;
-                t.cps = (maxl - xt + xcm - 1) / xcm + 1;
-            } else {
-                t.cps = (maxl + t.xc - 1) / t.xc;     // chunks as short as the tail: no tail
-            }
+        const int xt = std::min(4, std::max(1, maxl / 12));
+        const int64_t items = (int64_t)(h->cus > 0 ? h->cus : 256) * 24;
+        int64_t rows = (items / std::max<int64_t>(1, cols) + t.xcd / 2) / t.xcd;
+        rows = std::max<int64_t>(2, std::min<int64_t>(rows, maxl - xt));   // no empty chunk
+        const int nm = (int)rows - 1;
+        const int xcm = (maxl - xt + nm - 1) / nm;                          // longest main chunk
+        if (maxl - xt >= 2 && (maxl - xt) / nm >= 2 * xt) {
+            t.slabs = t.xcd;
+            t.xt = xt;
+            t.xc = xcm;
+            t.cps = nm + 1;
+            t.nchunks = t.slabs * t.cps;
         }
-        t.nchunks = t.slabs * t.cps;
     }
     t.nwork = cols * t.nchunks;
     h->tile = t;

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(CV_ITILE_THREADS) void k_ideal_tiled(PhysP p, EosP 
     const int jt = (int)(r % tp.ntj);
     int ibeg, iend;
     march_planes(tp, p.nx, (int)(r / tp.ntj), ibeg, iend);
-    if (ibeg >= iend) return;                          // an empty last chunk of a slab (uniform)
+    if (ibeg >= iend) return;                          // no planes (uniform over the workgroup)
     const int j0 = jt * ty, k0 = kt * tz;
     const int j = j0 + my, k = k0 + mz;
     const bool active = lane_in_tile && j < p.ny && k < p.nz;

@@ -49,25 +49,24 @@ struct TileP {
     int64_t nwork;             // march: ntj * ntk * nchunks
     int xcd;                   // > 0: spread work items over this many XCDs in contiguous ranges
     int slabs, cps, xt;        // march, slabs > 0: x cut into `slabs` slabs of cps chunks each,
-                               // the last xt planes of a slab its last chunk (xt 0: remainder)
+                               // the last one the slab's last xt planes
 };
 
 // x planes [ibeg, iend) of march chunk xr.  With slabs, slab s holds the planes
-// [s nx / slabs, (s + 1) nx / slabs); its last chunk is its last xt planes and the chunks
-// before it hold xc planes each (the last of them shorter, or empty), or, with xt = 0,
-// its chunks hold xc planes with the remainder last.  One slab is the work range of one
-// XCD, so the last items each XCD runs are short ones.
+// [s nx / slabs, (s + 1) nx / slabs): its last chunk is its last xt planes, and the planes
+// before them are shared by its cps - 1 other chunks (lengths differing by at most one).
+// One slab is the work range of one XCD, so the last items each XCD runs are short ones.
 __device__ __forceinline__ void march_planes(const TileP &tp, int nx, int xr, int &ibeg, int &iend) {
     if (tp.slabs > 0) {
-        const int s = xr / tp.cps, c = xr - s * tp.cps;
+        const int s = xr / tp.cps, c = xr - s * tp.cps, nm = tp.cps - 1;
         const int x0 = (int)(((int64_t)s * nx) / tp.slabs);
         const int x1 = (int)(((int64_t)(s + 1) * nx) / tp.slabs);
         const int xm = x1 - tp.xt;                    // end of the main chunks
-        if (tp.xt > 0 && c == tp.cps - 1) {
+        if (c == nm) {
             ibeg = xm; iend = x1;
         } else {
-            ibeg = x0 + c * tp.xc;
-            iend = ibeg + tp.xc < xm ? ibeg + tp.xc : xm;
+            ibeg = x0 + (xm - x0) * c / nm;
+            iend = x0 + (xm - x0) * (c + 1) / nm;
         }
     } else {
         ibeg = xr * tp.xc;
@@ -249,7 +248,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
         const int xr = (int)(r / tp.ntj);            // MARCH: chunk, else the x-plane
         int ibeg = xr, iend = xr + 1;
         if (MARCH) march_planes(tp, p.nx, xr, ibeg, iend);
-        if (ibeg >= iend) return;                    // an empty last chunk of a slab (uniform)
+        if (ibeg >= iend) return;                    // no planes (uniform over the workgroup)
         const int j0 = jt * ty, k0 = kt * tz;
         // state carried along x by MARCH: F(i-1/2), slopes of cell i, u^mu of cell i-1, and
         // u^tau, P + Pi of cell i+1 (evaluated as cell i+2 of the previous plane)

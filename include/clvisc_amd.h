@@ -288,17 +288,16 @@ int cv_set_jet_source(cv_handle *h, int on, float direction, int64_t start_pos_i
 int cv_get_stage_kernel(cv_handle *h, int32_t *kind, int32_t *ty, int32_t *tz);
 /* x planes per march chunk of kind 2 (0: automatic, about 32 work items per CU). */
 int cv_set_march_chunk(cv_handle *h, int xc);
-/* xc and the number of chunks along x (with the XCD map and nx >= 2 nxcd: nxcd slabs of
- * equally many chunks, see cv_set_xcd_map). */
+/* xc (the longest chunk) and the number of chunks along x (automatic chunks cut per XCD
+ * slab: nxcd slabs of equally many chunks, see cv_set_xcd_map). */
 int cv_get_march_chunk(cv_handle *h, int32_t *xc, int32_t *nchunks);
 /* Work mapping of the tiled stage kernels (no counterpart in the reference): nxcd > 0 gives
  * each of the nxcd XCDs, to which the hardware hands workgroups round-robin, a contiguous
  * range of work items (neighbouring tiles share an L2); 0 keeps workgroup b on item b.
- * With nxcd > 1 and nx >= 2 nxcd, the range of an XCD is one slab of x planes, cut into
- * march chunks with a short chunk last (automatic chunks: a tail of 1 to 4 planes after
- * chunks of xc; a chunk length set by cv_set_march_chunk: its remainder), so that the items
- * an XCD runs last are short ones.  Default 8 (MI355X has 8 XCDs).  Results are the same
- * either way. */
+ * With automatic march chunks of at least twice their tail, nxcd > 1 and nx >= 2 nxcd, the
+ * range of an XCD is one slab of x planes, cut into chunks with a tail of 1 to 4 planes
+ * last, so that the items an XCD runs last are short ones.  Default 8 (MI355X has 8 XCDs).
+ * Results are the same either way. */
 int cv_set_xcd_map(cv_handle *h, int nxcd);
 
 /* Wait for all queued work. */

@@ -403,18 +403,16 @@ def test_visc_steps_2d_boost_invariant(kernel, chunk):
                                               ((2, 0, 0), 1, 8), ((2, 0, 0), 4, 8), ((2, 0, 0), 2, 5)])
 def test_visc_steps_xcd_map(kernel, chunk, xcd):
     """The tiled kernels' workgroup -> work-item mapping (off, and XCD counts that do not
-    divide the work) changes only the schedule: the oracle's fields bit for bit.  With the
-    map on and nx >= 2 nxcd the march chunks are cut per XCD slab of x planes (29 planes:
-    slabs of 3 and 4, or 5 and 6): one chunk per slab, and slabs whose last chunk is empty."""
+    divide the work) changes only the schedule: the oracle's fields bit for bit."""
     p, tab, ev = _visc_case(29, 31, 11, bulk=(0.05, 0.155, 0.02))
     _visc_compare(p, tab, ev, 20, 'viscous+bulk hotspots', kernel, chunk, xcd)
 
 
 @pytest.mark.parametrize('viscous', [False, True])
 def test_automatic_slab_chunks_with_a_tail(viscous):
-    """Automatic march chunks on a grid with many tile columns (192x101x151, 10x23 tiles: 77
-    columns): each XCD's slab of 24 planes is cut into chunks of 3 planes, a shorter one and a
-    2-plane tail chunk last.  Identical fields to the oracle, ideal and viscous + bulk."""
+    """Automatic march chunks on a grid with many tile columns (192x101x151, 4x16 tiles: 260
+    columns): each XCD's slab of 24 planes is cut into two 11-plane chunks and a 2-plane tail
+    chunk last.  Identical fields to the oracle, ideal and viscous + bulk."""
     nx, ny, nz = 192, 101, 151
     p = cv_params(nx, ny, nz, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08,
                   bulk=(0.05, 0.155, 0.02) if viscous else None)
@@ -422,12 +420,12 @@ def test_automatic_slab_chunks_with_a_tail(viscous):
     pos = ic.woods_saxon_positions(40, 11, R=4.0, a=0.5)
     ev = ic.hotspots(nx, ny, nz, 0.08, 0.08, 0.15, positions=pos, emax=50.0)
     if viscous:
-        _visc_compare(p, tab, ev, 4, 'viscous + bulk, automatic slab chunks', (2, 10, 23))
+        _visc_compare(p, tab, ev, 4, 'viscous + bulk, automatic slab chunks', (2, 4, 16))
         return
     h = _lib.Handle(p, tab, viscous=False)
-    h.set_stage_kernel(2, 10, 23)
+    h.set_stage_kernel(2, 4, 16)
     _say('case: ideal %dx%dx%d, automatic slab chunks %s, 8 steps' % (nx, ny, nz, h.march_chunk()))
-    assert h.march_chunk() == (3, 72)
+    assert h.march_chunk() == (11, 24)
     h.set('ev1', ev)
     h.ideal_steps(0, 8)
     o = O.Oracle(orc_params(p, tab, nthreads=16), 'f32', viscous=False)
