@@ -838,7 +838,7 @@ __device__ __forceinline__ void fp64_fix_cell(const PhysP &p, const EosP &E, con
     EosT<EK> eos(E, E.tab);
     const int64_t n = p.n, I = f.cell;
     float T01 = f.T01, T02 = f.T02, T03 = f.T03;
-    float ed_find = (float)root_newton_d(E, E.tab, (double)f.T00, (double)f.M, (double)f.bulk, BULK);
+    float ed_find = (float)root_newton_d_inl(E, E.tab, (double)f.T00, (double)f.M, (double)f.bulk, BULK);
     if (ed_find < CV_ACU) { T01 = 0.0f; T02 = 0.0f; T03 = 0.0f; }
     ed_find = rmax(0.0f, ed_find);
     float pr = eos.P(ed_find) + f.bulk;
