@@ -372,9 +372,10 @@ __device__ __forceinline__ double eos_P_d(const EosP &E, const float4 *tab, doub
     return 0.3327 * eps - 0.3223 * pow(eps, 0.4585) - 0.003906 * eps * exp(-0.05697 * eps);
 }
 
-// The kernels that finish deferred cells (k_fp64_fix, k_batch_fp64_fix) inline it; the
-// stage kernels that carry the branch call root_newton_d, out of line: few cells take it
-// there, and inlined its double registers would spill.
+// The kernels that finish deferred cells (k_fp64_fix, k_batch_fp64_fix) and the split root
+// kernels (k_visc_root, k_batch_visc_root) inline it; the per-cell stage kernels that carry
+// the branch call root_newton_d, out of line: few cells take it there, and inlined its
+// double registers would spill.
 __device__ __forceinline__ double root_newton_d_inl(const EosP &E, const float4 *tab, double T00, double M,
                                                     double bulk, bool clamp0 = false) {
     double vl = 0.0, vh = 1.0, dpe = 1.0 / 3.0;

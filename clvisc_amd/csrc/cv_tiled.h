@@ -793,7 +793,7 @@ __device__ __forceinline__ void visc_root_cell(const PhysP &p, const EosP &E, in
     float ed_find;
     int it = 0;
     if ((p.flags & 8) && M > p.fp64_vmin * T00)
-        ed_find = (float)root_newton_d(E, E.tab, (double)T00, (double)M, (double)bout, BULK);
+        ed_find = (float)root_newton_d_inl(E, E.tab, (double)T00, (double)M, (double)bout, BULK);
     else if (seed_v)
         ed_find = root_newton_from(eos, T00, M, bout, BULK, fminf(fmaxf(seed_v[I], 0.0f), 0.9999f), &it);
     else
