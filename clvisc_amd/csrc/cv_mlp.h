@@ -445,6 +445,14 @@ __device__ __forceinline__ void seed_groups_t(const float *__restrict__ P, const
     for (int g = 0; g < NG; g++) y[g] = out[g][0];
 }
 
+// Viscous stages: the network proposes the start of the root solve for cells with
+// T^tt >= MLP_SEED_TMIN [GeV/fm^3]; a more dilute cell starts where the reference starts
+// (v = 0.5, the plain solve converges there in 2 iterations), and a wavefront without a
+// cell above the floor skips the network.  A cell's seed depends on its own T^tt and
+// features only, so the fused and the split stage give the same seeds.
+#define MLP_SEED_TMIN 1.0e-3f
+__device__ __forceinline__ float seed_gate(float T00, float y) { return T00 >= MLP_SEED_TMIN ? y : 0.5f; }
+
 // The features of one cell's root-solve input after the |T^ti| <= 0.999 T^tt clamp
 // (kernel_visc.cl:365-372), as BF16 pairs (x0, x1) and (x2, x3).
 __device__ __forceinline__ void seed_features_bf16(float T00, float M, float T03, uint32_t &lo, uint32_t &hi) {
@@ -522,7 +530,7 @@ __device__ __forceinline__ void seed_cons_block(const float *__restrict__ P, con
         seed_groups_t<2>(P, pk, xT, y);
 #pragma unroll
         for (int g = 0; g < 2; g++)
-            if (rg == 0 && r[g] < n) seed_v[r[g]] = y[g];
+            if (rg == 0 && r[g] < n) seed_v[r[g]] = seed_gate(cons[r[g]], y[g]);
     }
 }
 

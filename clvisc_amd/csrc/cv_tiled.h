@@ -540,6 +540,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
         float sd_T[4] = {0.0f, 0.0f, 0.0f, 0.0f}, sd_M = 0.0f, sd_b = 0.0f;
         float sd_u[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         uint32_t sd_lo = 0u, sd_hi = 0u;
+        float sd_t00 = 0.0f;                                   // T^tt of the cell, for the seed floor
         bool sd_go = false;
         if (active) {
             const float4 e1 = STEP == 1 ? ec : e1_pre;
@@ -684,7 +685,7 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
                 T01 *= sc; T02 *= sc; T03 *= sc;
                 M = 0.999f * T00;
             }
-            if (SPEC & CV_SPEC_SEED) seed_features_bf16(T00, M, T03, sd_lo, sd_hi);
+            if (SPEC & CV_SPEC_SEED) { seed_features_bf16(T00, M, T03, sd_lo, sd_hi); sd_t00 = T00; }
             if ((SPEC & CV_SPEC_FP64) && (p.flags & 8) && M > p.fp64_vmin * T00) {
                 // FP64 root solve deferred to k_fp64_fix (kept out of this kernel's registers);
                 // each cell is recorded at most once per stage, so n entries always suffice.
@@ -712,9 +713,11 @@ __device__ __forceinline__ void tile_work(const PhysP &p, const EOS &eos, const 
             }
         }
         if constexpr ((SPEC & CV_SPEC_SEED) != 0) {
-            // the network on the wave's 64 cells (lanes without a cell give zero features),
-            // then the root solve from the seed with the reference's bracket and stopping rule
-            const float v0 = seed_wave_t(q_.mlp_P, q_.mlp_pk, sd_lo, sd_hi);
+            // the network on the wave's 64 cells (lanes without a cell give zero features) when
+            // one of them is above the seed floor, then the root solve from the seed with the
+            // reference's bracket and stopping rule
+            const bool wave_net = __any(sd_t00 >= MLP_SEED_TMIN);
+            const float v0 = seed_gate(sd_t00, wave_net ? seed_wave_t(q_.mlp_P, q_.mlp_pk, sd_lo, sd_hi) : 0.5f);
             if (active && q_.seed_out) *(float *)((char *)q_.seed_out + bo) = v0;
             if (sd_go) {
                 int it = 0;
