@@ -510,6 +510,7 @@ __device__ __forceinline__ void seed_cons_block(const float *__restrict__ P, con
         // two tiles of 16 samples per pass of the weights
         mlp_bf16x4 xT[2];
         int64_t r[2];
+        bool need = false;
 #pragma unroll
         for (int g = 0; g < 2; g++) {
             r[g] = (tile0 + t + g) * 16 + c0;
@@ -523,11 +524,13 @@ __device__ __forceinline__ void seed_cons_block(const float *__restrict__ P, con
                     M = 0.999f * T00;
                 }
                 seed_features_bf16(T00, M, T03, lo, hi);
+                need = need || T00 >= MLP_SEED_TMIN;
             }
             xT[g] = as_bf16x4(make_uint2(lo, hi));
         }
-        float y[2];
-        seed_groups_t<2>(P, pk, xT, y);
+        // the 32 samples of this pass all below the seed floor: no network (wave-uniform)
+        float y[2] = {0.5f, 0.5f};
+        if (__any(need)) seed_groups_t<2>(P, pk, xT, y);
 #pragma unroll
         for (int g = 0; g < 2; g++)
             if (rg == 0 && r[g] < n) seed_v[r[g]] = seed_gate(cons[r[g]], y[g]);
