@@ -101,19 +101,26 @@ def test_ideal_steps_tiled_3d(eos_kind, dims, kernel, chunk):
     h.close()
 
 
-@pytest.mark.parametrize('viscous,kernel', [(False, (0, 0, 0)), (False, (2, 0, 0)), (True, (0, 0, 0)),
-                                            (True, (1, 0, 0)), (True, (2, 0, 0))])
-def test_steps_with_the_full_lambda_formula(viscous, kernel):
-    """A crossover table whose cs2 column is scaled past 0.5 (1.6x, only lambda reads it)
-    turns off the device's max(lambda_KT, 1) = 1 shortcut, so every face evaluates the
-    reference's formula (gamma, cs2 lookup, square root, division), as the oracle always does:
+@pytest.mark.parametrize('viscous,kernel,cs2_map', [(False, (0, 0, 0), 'scale'), (False, (2, 0, 0), 'scale'),
+                                                    (True, (0, 0, 0), 'scale'), (True, (1, 0, 0), 'scale'),
+                                                    (True, (2, 0, 0), 'scale'), (True, (2, 0, 0), 'shift'),
+                                                    (False, (2, 0, 0), 'shift')])
+def test_steps_with_the_full_lambda_formula(viscous, kernel, cs2_map):
+    """A crossover table whose cs2 column is scaled past 0.5 (1.6x) or shifted below 0 (-0.2;
+    only lambda reads it) turns off the device's max(lambda_KT, 1) = 1 shortcut, so every face
+    evaluates the reference's formula (gamma, cs2 lookup, square root, division), as the
+    oracle always does, negative cs2 (a NaN square root, or lambda_KT above 1) included:
     identical fields."""
     import dataclasses
     tab0 = Eos('crossover').create_table()
     rgba = tab0.rgba.copy()
-    rgba[:, 0] *= np.float32(1.6)
+    if cs2_map == 'scale':
+        rgba[:, 0] *= np.float32(1.6)
+        assert rgba[:, 0].max() > 0.5
+    else:
+        rgba[:, 0] -= np.float32(0.2)
+        assert rgba[:, 0].min() < 0.0
     tab = dataclasses.replace(tab0, rgba=rgba, cs2_max=float(rgba[:, 0].max()), cs2_min=float(rgba[:, 0].min()))
-    assert rgba[:, 0].max() > 0.5
     nx, ny, nz = 29, 31, 11
     p = cv_params(nx, ny, nz, 0.02, 0.08, 0.08, 0.15, 0.6, etaos_xmin=0.0, etaos_ymin=0.08,
                   bulk=(0.05, 0.155, 0.02) if viscous else None)
